@@ -1,0 +1,31 @@
+// Genotype row -> native AST (ast-prefix-v2 JSON) and program text.
+//
+// Reference behaviour: export_native_phase_layout + compact_genome_tables
+// (cpp/src/evolution/repro/gpu/phase_layout.hpp:316-475,
+// cpp/src/evolution/repro/pack.cpp:11-77): the expression-entry envelope
+// PROGRAM, BLOCK_CONS, RETURN, <phase>, BLOCK_NIL; one constant per CONST node
+// in prefix order; names are the grammar inputs (declaration order) that the
+// program uses. Writers: write_ast_json (cpp/src/cli/program_json.cpp:211-232)
+// and ProgramPrinter (program_json.cpp:876-946).
+#pragma once
+#include <string>
+#include <vector>
+#include "grammar.hpp"
+#include "json.hpp"
+
+namespace gagp_amd {
+
+struct AstNode { int kind, i0, i1; };
+struct Ast {
+  std::vector<AstNode> nodes;
+  std::vector<std::string> names;
+  std::vector<Val> consts;
+};
+
+// `origins`/`consts` are one phase row; `phase` is its expression table.
+Ast export_whole_program(const Grammar& g, const PhaseGrammar& phase, const std::vector<int>& origins,
+                         const std::vector<Val>& consts);
+json::Value ast_json(const Ast& ast);
+std::string program_text(const Ast& ast);
+
+}  // namespace gagp_amd

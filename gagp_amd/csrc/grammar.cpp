@@ -1,0 +1,621 @@
+// Grammar front end (host). See grammar.hpp for the reference behaviour
+// each step follows.
+#include "grammar.hpp"
+
+#include <algorithm>
+#include <fstream>
+#include <functional>
+#include <map>
+#include <set>
+#include <sstream>
+
+namespace gagp_amd {
+namespace {
+using json::Value;
+
+struct NodeInfo { int kind; const char* name; int arity; int i0; };  // i0: 0 unused, 1 name, 2 constant
+// Serialized names and prefix arities of the node kinds this engine emits
+// (cpp/src/evolution/node_descriptor.cpp:24-85).
+const NodeInfo kNodes[] = {
+    {NK_PROGRAM, "program", 1, 0}, {NK_BLOCK_NIL, "block_nil", 0, 0}, {NK_BLOCK_CONS, "block_cons", 2, 0},
+    {NK_ASSIGN, "assign", 1, 1}, {NK_IF_STMT, "if_stmt", 3, 0}, {NK_FOR_RANGE, "for_range", 2, 1},
+    {NK_RETURN, "return", 1, 0}, {NK_CONST, "const", 0, 2}, {NK_VAR, "var", 0, 1}, {NK_NEG, "neg", 1, 0},
+    {NK_NOT, "not", 1, 0}, {NK_ADD, "add", 2, 0}, {NK_SUB, "sub", 2, 0}, {NK_MUL, "mul", 2, 0},
+    {NK_DIV, "div", 2, 0}, {NK_MOD, "mod", 2, 0}, {NK_LT, "lt", 2, 0}, {NK_LE, "le", 2, 0},
+    {NK_GT, "gt", 2, 0}, {NK_GE, "ge", 2, 0}, {NK_EQ, "eq", 2, 0}, {NK_NE, "ne", 2, 0},
+    {NK_AND, "and", 2, 0}, {NK_OR, "or", 2, 0}, {NK_IF_EXPR, "if_expr", 3, 0},
+    {NK_CALL_ABS, "call_abs", 1, 0}, {NK_CALL_MIN, "call_min", 2, 0}, {NK_CALL_MAX, "call_max", 2, 0},
+    {NK_CALL_CLIP, "call_clip", 3, 0}, {NK_CALL_IDIV0, "call_idiv0", 2, 0}, {NK_CALL_IMOD0, "call_imod0", 2, 0},
+    {NK_CALL_LEN, "call_len", 1, 0}, {NK_CALL_INDEX, "call_index", 2, 0}, {NK_BOUND_VAR, "bound_var", 0, 1},
+    {NK_LET_REGION, "let_region", 2, 0}, {NK_TRAVERSE, "traverse", 4, 0}, {NK_TRAVERSE_RANGE, "traverse_range", 6, 0},
+    {NK_REGION_VAR, "region_var", 0, 0}, {NK_CHECK_INT, "check_int", 1, 0}, {NK_CHECK_LIST, "check_list", 1, 0},
+    {NK_CALL_TRUNC_DIV0, "call_trunc_div0", 2, 0}, {NK_CALL_TRUNC_MOD0, "call_trunc_mod0", 2, 0},
+    {NK_CALL_INDEX_NONNEG, "call_index_nonnegative", 2, 0}, {NK_CALL_INDEX_CLAMPED, "call_index_clamped_or_zero", 2, 0},
+};
+const NodeInfo& node_info(int kind) {
+  for (const auto& n : kNodes) if (n.kind == kind) return n;
+  throw std::logic_error("unknown node kind " + std::to_string(kind));
+}
+
+void keys(const Value& v, std::initializer_list<const char*> allowed, const char* what) {
+  if (v.kind != Value::Object) throw std::invalid_argument(std::string(what) + " must be an object");
+  const std::set<std::string> names(allowed.begin(), allowed.end());
+  for (const auto& kv : v.obj)
+    if (!names.count(kv.first)) throw std::invalid_argument(std::string(what) + ": unknown key " + kv.first);
+}
+std::uint32_t positive(const Value& v, std::uint32_t maximum, const char* what) {
+  if (v.kind != Value::Number || v.num < 1 || v.num > maximum || std::floor(v.num) != v.num)
+    throw std::invalid_argument(std::string(what) + " must be an integer in 1.." + std::to_string(maximum));
+  return std::uint32_t(v.num);
+}
+std::string identifier(const Value& v) {
+  const std::string id = v.at("id").as_string("id");
+  const auto letter = [](char c) { return (c >= 'a' && c <= 'z') || (c >= 'A' && c <= 'Z') || c == '_'; };
+  if (id.empty() || !letter(id[0]) || !std::all_of(id.begin(), id.end(), [&](char c) {
+        return letter(c) || (c >= '0' && c <= '9') || c == '.' || c == '-'; }))
+    throw std::invalid_argument("invalid grammar identifier: " + id);
+  return id;
+}
+std::vector<Binding> scope_of(const Value& v) {
+  std::vector<Binding> out;
+  if (v.kind == Value::Null) return out;
+  for (const auto& item : v.as_array("scope")) {
+    keys(item, {"name", "type"}, "binding");
+    Binding b{item.at("name").as_string("name"), parse_type_name(item.at("type").as_string("type"))};
+    for (const auto& o : out) if (o.name == b.name) throw std::invalid_argument("duplicate binding: " + b.name);
+    out.push_back(b);
+  }
+  return out;
+}
+Cat category_of(const Value& v) {
+  if (!v.has("category")) return Cat::Expression;
+  const auto& c = v.at("category").as_string("category");
+  if (c == "Expression") return Cat::Expression;
+  if (c == "Program") return Cat::Program;
+  if (c == "Block") return Cat::Block;
+  if (c == "Statement") return Cat::Statement;
+  throw std::invalid_argument("unknown category " + c);
+}
+std::int64_t int_literal(const Value& v) {
+  const auto& text = v.as_string("Int constant (decimal string)");
+  if (text.empty()) throw std::invalid_argument("Int constant must be a canonical decimal string");
+  errno = 0;
+  char* end = nullptr;
+  const long long r = std::strtoll(text.c_str(), &end, 10);
+  if (errno || end != text.c_str() + text.size() || std::to_string(r) != text)
+    throw std::invalid_argument("Int constant must be canonical signed 64-bit decimal string");
+  return r;
+}
+
+struct Sig { std::string op; std::vector<std::uint8_t> args; std::uint8_t result; };
+Sig parse_signature(const std::string& key) {
+  Sig s;
+  const auto open = key.find('('), close = key.find(")->");
+  if (open == std::string::npos || close == std::string::npos || close < open)
+    throw std::invalid_argument("unknown exact primitive overload: " + key);
+  s.op = key.substr(0, open);
+  std::string list = key.substr(open + 1, close - open - 1);
+  std::stringstream ss(list);
+  std::string item;
+  while (!list.empty() && std::getline(ss, item, ',')) s.args.push_back(parse_type_name(item));
+  s.result = parse_type_name(key.substr(close + 3));
+  return s;
+}
+
+// The admitted slice of the primitive catalog (cpp/src/evolution/grammar/catalog.cpp:72-180):
+// Int/Bool scalar operations, list length and indexing.
+void resolve_primitive(const Sig& s, CExpr& e) {
+  const auto I = TAG_INT, B = TAG_BOOL, L = TAG_INT_LIST;
+  const auto& a = s.args;
+  const auto is = [&](std::initializer_list<std::uint8_t> want, std::uint8_t result) {
+    return std::vector<std::uint8_t>(want) == a && s.result == result;
+  };
+  struct Bin { const char* op; int nk; int builtin; };
+  static const Bin ints[] = {{"add", NK_ADD, -1}, {"sub", NK_SUB, -1}, {"mul", NK_MUL, -1}, {"mod", NK_MOD, -1},
+      {"min", NK_CALL_MIN, B_MIN}, {"max", NK_CALL_MAX, B_MAX}, {"idiv0", NK_CALL_IDIV0, B_IDIV0},
+      {"imod0", NK_CALL_IMOD0, B_IMOD0}, {"trunc_div0", NK_CALL_TRUNC_DIV0, B_TRUNC_DIV0},
+      {"trunc_mod0", NK_CALL_TRUNC_MOD0, B_TRUNC_MOD0}};
+  for (const auto& b : ints)
+    if (s.op == b.op && is({I, I}, I)) { e.node_kind = b.nk; e.builtin = b.builtin; return; }
+  static const Bin cmps[] = {{"lt", NK_LT, -1}, {"le", NK_LE, -1}, {"gt", NK_GT, -1}, {"ge", NK_GE, -1}};
+  for (const auto& b : cmps)
+    if (s.op == b.op && is({I, I}, B)) { e.node_kind = b.nk; return; }
+  if ((s.op == "eq" || s.op == "ne") && a.size() == 2 && a[0] == a[1] && (a[0] == I || a[0] == B) && s.result == B) {
+    e.node_kind = s.op == "eq" ? NK_EQ : NK_NE; return;
+  }
+  if (s.op == "neg" && is({I}, I)) { e.node_kind = NK_NEG; return; }
+  if (s.op == "abs" && is({I}, I)) { e.node_kind = NK_CALL_ABS; e.builtin = B_ABS; return; }
+  if (s.op == "clip" && is({I, I, I}, I)) { e.node_kind = NK_CALL_CLIP; e.builtin = B_CLIP; return; }
+  if (s.op == "check_int" && is({I}, I)) { e.node_kind = NK_CHECK_INT; return; }
+  if (s.op == "not" && is({B}, B)) { e.node_kind = NK_NOT; return; }
+  if (s.op == "and" && is({B, B}, B)) { e.node_kind = NK_AND; return; }
+  if (s.op == "or" && is({B, B}, B)) { e.node_kind = NK_OR; return; }
+  if (s.op == "if" && a.size() == 3 && a[0] == B && a[1] == a[2] && a[1] == s.result && (s.result == I || s.result == B)) {
+    e.node_kind = NK_IF_EXPR; return;
+  }
+  if (s.op == "len" && is({L}, I)) { e.node_kind = NK_CALL_LEN; e.builtin = B_LEN; return; }
+  if (s.op == "index" && is({L, I}, I)) { e.node_kind = NK_CALL_INDEX; e.builtin = B_INDEX; return; }
+  if (s.op == "index_nonnegative" && is({L, I}, I)) { e.node_kind = NK_CALL_INDEX_NONNEG; e.builtin = B_INDEX_NONNEG; return; }
+  if (s.op == "index_clamped_or_zero" && is({L, I}, I)) { e.node_kind = NK_CALL_INDEX_CLAMPED; e.builtin = B_INDEX_CLAMPED; return; }
+  throw Unsupported("primitive outside the engine's admitted catalog: " + s.op);
+}
+
+class Compiler {
+ public:
+  Grammar compile(const Value& doc) {
+    keys(doc, {"format_version", "entry", "inputs", "locals", "search_limits", "execution_limits", "nonterminals",
+               "templates", "imports"}, "grammar definition");
+    if (doc.at("format_version").as_string("format_version") != "grammar-definition-v2")
+      throw std::invalid_argument("grammar format_version must be grammar-definition-v2");
+    if (doc.has("imports") && !doc.at("imports").as_array("imports").empty()) throw Unsupported("grammar imports");
+    if (doc.has("templates") && !doc.at("templates").as_array("templates").empty()) throw Unsupported("grammar templates");
+    const auto& search = doc.at("search_limits");
+    keys(search, {"max_nodes", "max_depth"}, "search_limits");
+    g_.max_nodes = positive(search.at("max_nodes"), 65536, "max_nodes");
+    g_.max_depth = positive(search.at("max_depth"), 256, "max_depth");
+    const auto& exec = doc.at("execution_limits");
+    keys(exec, {"fuel"}, "execution_limits");
+    g_.fuel = positive(exec.at("fuel"), 0x7fffffff, "fuel");
+    g_.inputs = scope_of(doc.has("inputs") ? doc.at("inputs") : Value{});
+    g_.locals = scope_of(doc.has("locals") ? doc.at("locals") : Value{});
+    if (!g_.locals.empty()) throw Unsupported("grammar locals (statement programs)");
+    // Resolution order: nonterminals and their alternatives sorted by id.
+    std::vector<const Value*> rules;
+    for (const auto& r : doc.at("nonterminals").as_array("nonterminals")) rules.push_back(&r);
+    if (rules.empty() || rules.size() > 4096) throw std::invalid_argument("grammar requires 1..4096 nonterminals");
+    std::stable_sort(rules.begin(), rules.end(), [](const Value* a, const Value* b) { return identifier(*a) < identifier(*b); });
+    for (const Value* r : rules) {
+      keys(*r, {"id", "type", "scope", "alternatives", "category", "variation"}, "nonterminal");
+      CNt nt;
+      nt.id = identifier(*r);
+      if (ids_.count(nt.id)) throw std::invalid_argument("duplicate nonterminal id " + nt.id);
+      nt.type = parse_type_name(r->at("type").as_string("type"));
+      nt.cat = category_of(*r);
+      if (nt.cat != Cat::Expression) throw Unsupported("statement nonterminals");
+      if (r->has("variation")) {
+        if (r->at("variation").kind != Value::Bool) throw std::invalid_argument("nonterminal variation must be Boolean");
+        nt.variation = r->at("variation").b;
+      }
+      nt.scope = scope_of(r->at("scope"));
+      ids_.emplace(nt.id, std::uint32_t(g_.nts.size()));
+      g_.nts.push_back(std::move(nt));
+    }
+    const auto& entry = doc.at("entry");
+    keys(entry, {"nonterminal", "type", "category"}, "entry");
+    g_.entry = reference(entry.at("nonterminal").as_string("entry nonterminal"));
+    if (g_.nts[g_.entry].type != parse_type_name(entry.at("type").as_string("entry type")))
+      throw std::invalid_argument("entry result type mismatch");
+    if (category_of(entry) != g_.nts[g_.entry].cat) throw std::invalid_argument("entry category mismatch");
+    if (!g_.nts[g_.entry].scope.empty()) throw std::invalid_argument("entry nonterminal requires unavailable lexical bindings");
+    for (std::size_t i = 0; i < rules.size(); ++i) {
+      std::vector<const Value*> alts;
+      for (const auto& a : rules[i]->at("alternatives").as_array("alternatives")) alts.push_back(&a);
+      if (alts.empty()) throw std::invalid_argument("nonterminal has no alternatives");
+      std::stable_sort(alts.begin(), alts.end(), [](const Value* a, const Value* b) { return identifier(*a) < identifier(*b); });
+      std::set<std::string> seen;
+      for (const Value* a : alts) {
+        keys(*a, {"id", "weight", "expression"}, "alternative");
+        const std::string aid = identifier(*a);
+        if (!seen.insert(aid).second) throw std::invalid_argument("duplicate alternative id " + aid);
+        const auto& w = a->at("weight");
+        if (w.kind != Value::Number || !std::isfinite(w.num) || w.num <= 0)
+          throw std::invalid_argument("production weight must be positive and finite");
+        CProd p;
+        p.nt = std::uint32_t(i);
+        p.weight = w.num;
+        p.stable_id = g_.nts[i].id + "/" + aid;
+        p.expression = expression(a->at("expression"), g_.nts[i].scope, 0);
+        const auto& e = g_.exprs[p.expression];
+        if (e.type != g_.nts[i].type || e.cat != g_.nts[i].cat)
+          throw std::invalid_argument("production result type mismatch: " + p.stable_id);
+        g_.nts[i].productions.push_back(std::uint32_t(g_.prods.size()));
+        g_.prods.push_back(std::move(p));
+      }
+    }
+    costs();
+    return std::move(g_);
+  }
+
+ private:
+  std::uint32_t reference(const std::string& id) const {
+    const auto it = ids_.find(id);
+    if (it == ids_.end()) throw std::invalid_argument("unknown nonterminal: " + id);
+    return it->second;
+  }
+  static std::uint32_t binding(const std::vector<Binding>& scope, const std::string& name) {
+    for (std::size_t i = 0; i < scope.size(); ++i) if (scope[i].name == name) return std::uint32_t(i);
+    throw std::invalid_argument("unknown binding: " + name);
+  }
+  std::uint32_t append(CExpr e) {
+    if (g_.exprs.size() >= 65536) throw std::invalid_argument("grammar expression capacity exceeded");
+    g_.exprs.push_back(std::move(e));
+    return std::uint32_t(g_.exprs.size() - 1);
+  }
+  CDomain domain(const Value& v) {
+    keys(v, {"type", "values", "range"}, "constant domain");
+    CDomain d;
+    d.type = parse_type_name(v.at("type").as_string("constant type"));
+    if (v.has("values") == v.has("range")) throw std::invalid_argument("constant domain needs exactly one of values/range");
+    if (v.has("range")) {
+      if (d.type != TAG_INT) throw Unsupported("non-Int constant range");
+      const auto& r = v.at("range").as_array("range");
+      if (r.size() != 2) throw std::invalid_argument("range needs two endpoints");
+      d.integer_range = true;
+      d.minimum = int_literal(r[0]);
+      d.maximum = int_literal(r[1]);
+      if (d.minimum > d.maximum) throw std::invalid_argument("range endpoints must be ascending");
+      return d;
+    }
+    const auto& vals = v.at("values").as_array("values");
+    if (vals.empty()) throw std::invalid_argument("constant values must be nonempty");
+    for (const auto& x : vals) {
+      if (d.type == TAG_INT) d.values.push_back(val_int(int_literal(x)));
+      else if (d.type == TAG_BOOL) {
+        if (x.kind != Value::Bool) throw std::invalid_argument("Bool constant must be boolean");
+        d.values.push_back(val_bool(x.b));
+      } else throw Unsupported(std::string("constant domain of type ") + type_name(d.type));
+    }
+    return d;
+  }
+  std::uint32_t expression(const Value& v, const std::vector<Binding>& env, unsigned depth) {
+    if (depth >= 256) throw std::invalid_argument("grammar expression capacity exceeded");
+    if (v.kind != Value::Object) throw std::invalid_argument("expression must be an object");
+    CExpr e;
+    if (v.has("constant")) {
+      keys(v, {"constant"}, "constant expression");
+      e.kind = EKind::Constant;
+      e.target = std::uint32_t(g_.domains.size());
+      g_.domains.push_back(domain(v.at("constant")));
+      e.type = g_.domains.back().type;
+      e.node_kind = NK_CONST;
+    } else if (v.has("ref")) {
+      keys(v, {"ref"}, "nonterminal reference");
+      e.kind = EKind::Reference;
+      e.target = reference(v.at("ref").as_string("ref"));
+      const auto& t = g_.nts[e.target];
+      e.type = t.type;
+      e.cat = t.cat;
+      for (const auto& req : t.scope) {
+        const auto idx = binding(env, req.name);
+        if (env[idx].type != req.type) throw std::invalid_argument("reference scope type mismatch");
+        e.scope_mapping.push_back(idx);
+      }
+    } else if (v.has("input") || v.has("bound")) {
+      const bool input = v.has("input");
+      keys(v, {input ? "input" : "bound"}, "binding expression");
+      const auto& scope = input ? g_.inputs : env;
+      e.kind = input ? EKind::Input : EKind::Bound;
+      e.target = binding(scope, v.at(input ? "input" : "bound").as_string("binding name"));
+      e.type = scope[e.target].type;
+      e.node_kind = input ? NK_VAR : NK_REGION_VAR;
+      if (!input) throw Unsupported("lexical bound references (let/traverse bodies)");
+    } else if (v.has("signature")) {
+      keys(v, {"signature", "args", "bind"}, "primitive expression");
+      if (v.has("bind")) throw Unsupported("region-binding primitives (let/traverse)");
+      const Sig s = parse_signature(v.at("signature").as_string("signature"));
+      e.kind = EKind::Primitive;
+      resolve_primitive(s, e);
+      e.type = s.result;
+      const auto& args = v.at("args").as_array("args");
+      if (args.size() != s.args.size()) throw std::invalid_argument("primitive argument count mismatch");
+      for (std::size_t i = 0; i < args.size(); ++i) {
+        const auto child = expression(args[i], env, depth + 1);
+        if (g_.exprs[child].type != s.args[i] || g_.exprs[child].cat != Cat::Expression)
+          throw std::invalid_argument("primitive argument type mismatch");
+        e.children.push_back(child);
+      }
+    } else if (v.has("local") || v.has("control")) {
+      throw Unsupported("statement grammars (locals/controls)");
+    } else if (v.has("template") || v.has("hole") || v.has("structured")) {
+      throw Unsupported("templates and structured regions");
+    } else {
+      throw std::invalid_argument("unknown expression form");
+    }
+    return append(std::move(e));
+  }
+  // minimum_expression_nodes without templates (budget.cpp:19-55).
+  std::uint32_t cost(std::uint32_t id, std::uint32_t depth) const {
+    const auto& e = g_.exprs[id];
+    if (e.kind == EKind::Reference) return g_.nts[e.target].minimum_by_depth[depth];
+    if (!depth) return kNoId;
+    std::uint64_t r = 1;
+    for (auto c : e.children) {
+      const auto n = cost(c, depth - 1);
+      if (n == kNoId) return kNoId;
+      r += n;
+      if (r > g_.max_nodes) return kNoId;
+    }
+    return std::uint32_t(r);
+  }
+  void costs() {
+    const auto depths = g_.max_depth;
+    for (auto& nt : g_.nts) nt.minimum_by_depth.assign(depths + 1, kNoId);
+    for (std::uint32_t d = 1; d <= depths; ++d) {
+      bool changed;
+      do {
+        changed = false;
+        for (const auto& p : g_.prods) {
+          const auto n = cost(p.expression, d);
+          auto& t = g_.nts[p.nt].minimum_by_depth[d];
+          if (n < t) { t = n; changed = true; }
+        }
+      } while (changed);
+    }
+    for (const auto& nt : g_.nts) {
+      std::uint32_t best = kNoId;
+      for (auto n : nt.minimum_by_depth) best = std::min(best, n);
+      if (best == kNoId) throw std::invalid_argument("unproductive nonterminal or no derivation within search budget: " + nt.id);
+    }
+  }
+  Grammar g_;
+  std::map<std::string, std::uint32_t> ids_;
+};
+
+std::string read_file(const std::string& path) {
+  std::ifstream in(path, std::ios::binary);
+  if (!in) throw std::runtime_error("cannot read " + path);
+  std::ostringstream s;
+  s << in.rdbuf();
+  return s.str();
+}
+}  // namespace
+
+const char* node_serialized_name(int kind) { return node_info(kind).name; }
+int node_prefix_arity(int kind) { return node_info(kind).arity; }
+bool node_i0_is_name(int kind) { return node_info(kind).i0 == 1; }
+bool node_i0_is_constant(int kind) { return node_info(kind).i0 == 2; }
+
+std::uint8_t parse_type_name(const std::string& n) {
+  static const char* names[] = {"Int", "Float", "Bool", "Char", "String", "IntList", "FloatList", "StringList"};
+  for (int i = 0; i < 8; ++i) if (n == names[i]) return std::uint8_t(i);
+  throw std::invalid_argument("unknown grammar value type: " + n);
+}
+const char* type_name(std::uint8_t t) {
+  static const char* names[] = {"Int", "Float", "Bool", "Char", "String", "IntList", "FloatList", "StringList"};
+  return t < 8 ? names[t] : "Invalid";
+}
+
+Grammar load_grammar_text(const std::string& text) { return Compiler{}.compile(json::parse(text)); }
+Grammar load_grammar_file(const std::string& path) { return load_grammar_text(read_file(path)); }
+
+PhaseGrammar compile_phase(const Grammar& g, unsigned root, unsigned max_depth, unsigned site_depth,
+                           const std::vector<std::string>& input_order) {
+  const auto fail = [](const std::string& why) -> void { throw Unsupported("GPU phase construction: " + why); };
+  if (root >= g.nts.size() || !max_depth || max_depth > 64 || max_depth > g.max_depth || site_depth > 15) fail("invalid root/depth");
+  PhaseGrammar out;
+  out.depth_count = int(max_depth + 1);
+  out.site_depth = site_depth > 0;
+  out.exprs.resize(g.exprs.size());
+  out.nts.resize(g.nts.size());
+  out.domains.resize(g.domains.size());
+  std::vector<bool> seen_nt(g.nts.size()), seen_expr(g.exprs.size()), seen_domain(g.domains.size());
+  using Env = std::vector<int>;
+  std::vector<Env> nt_env(g.nts.size()), expr_env(g.exprs.size());
+  std::function<void(unsigned, const Env&)> visit_expr, visit_nt;
+  visit_nt = [&](unsigned id, const Env& env) {
+    if (seen_nt[id]) {
+      if (nt_env[id] != env) fail("nonterminal entered under two lexical environments");
+      return;
+    }
+    seen_nt[id] = true;
+    nt_env[id] = env;
+    const auto& nt = g.nts[id];
+    if (nt.cat != Cat::Expression || nt.scope.size() > 8) fail("nonterminal capability");
+    auto& t = out.nts[id];
+    t.scope_count = int(nt.scope.size());
+    t.production_offset = int(out.prods.size());
+    for (auto p : nt.productions) {
+      out.prods.push_back({int(g.prods[p].expression), g.prods[p].weight});
+      ++t.production_count;
+    }
+    if (!t.production_count) fail("no production");
+    if (env.size() != nt.scope.size()) fail("nonterminal environment size");
+    for (auto p : nt.productions) visit_expr(g.prods[p].expression, env);
+  };
+  visit_expr = [&](unsigned id, const Env& env) {
+    if (seen_expr[id]) {
+      if (expr_env[id] != env) fail("expression entered under two lexical environments");
+      return;
+    }
+    seen_expr[id] = true;
+    expr_env[id] = env;
+    const auto& e = g.exprs[id];
+    auto& x = out.exprs[id];
+    x.node.type = e.type;
+    if (e.kind == EKind::Reference) {
+      Env inner;
+      for (auto m : e.scope_mapping) inner.push_back(env.at(m));
+      x.kind = PK_REFERENCE;
+      x.target = int(e.target);
+      x.scope_count = int(e.scope_mapping.size());
+      visit_nt(e.target, inner);
+      return;
+    }
+    if (e.children.size() > 3) fail("arity capacity");
+    x.node.arity = int(e.children.size());
+    for (std::size_t i = 0; i < e.children.size(); ++i) x.children[i] = int(e.children[i]);
+    if (e.kind == EKind::Constant) {
+      x.kind = PK_CONSTANT;
+      x.target = int(e.target);
+      x.native_kind = NK_CONST;
+      x.node.kind = TK_CONSTANT;
+      if (!seen_domain[e.target]) {
+        seen_domain[e.target] = true;
+        const auto& d = g.domains[e.target];
+        auto& pd = out.domains[e.target];
+        pd.integer_range = d.integer_range;
+        pd.minimum = d.minimum;
+        pd.maximum = d.maximum;
+        if (!d.integer_range) {
+          pd.value_offset = int(out.values.size());
+          pd.value_count = int(d.values.size());
+          out.values.insert(out.values.end(), d.values.begin(), d.values.end());
+        }
+      }
+    } else if (e.kind == EKind::Input) {
+      const auto& name = g.inputs.at(e.target).name;
+      const auto it = std::find(input_order.begin(), input_order.end(), name);
+      if (it == input_order.end()) fail("input missing from evaluator order");
+      const int position = int(it - input_order.begin());
+      if (position >= 64) fail("input local capacity");
+      x.kind = PK_INPUT;
+      x.target = position;
+      x.native_kind = NK_VAR;
+      x.node.kind = TK_LOCAL;
+      x.node.operand = position;
+    } else if (e.kind == EKind::Primitive) {
+      x.kind = PK_OPERATION;
+      x.native_kind = e.node_kind;
+      switch (e.node_kind) {
+        case NK_IF_EXPR: x.node.kind = TK_IF; break;
+        case NK_AND: x.node.kind = TK_AND; break;
+        case NK_OR: x.node.kind = TK_OR; break;
+        default:
+          x.node.kind = TK_OPERATION;
+          if (e.builtin >= 0) { x.node.opcode = OP_CALL_BUILTIN; x.node.operand = e.builtin; }
+          else {
+            switch (e.node_kind) {
+              case NK_ADD: x.node.opcode = OP_ADD; break;
+              case NK_SUB: x.node.opcode = OP_SUB; break;
+              case NK_MUL: x.node.opcode = OP_MUL; break;
+              case NK_DIV: x.node.opcode = OP_DIV; break;
+              case NK_MOD: x.node.opcode = OP_MOD; break;
+              case NK_NEG: x.node.opcode = OP_NEG; break;
+              case NK_NOT: x.node.opcode = OP_NOT; break;
+              case NK_LT: x.node.opcode = OP_LT; break;
+              case NK_LE: x.node.opcode = OP_LE; break;
+              case NK_GT: x.node.opcode = OP_GT; break;
+              case NK_GE: x.node.opcode = OP_GE; break;
+              case NK_EQ: x.node.opcode = OP_EQ; break;
+              case NK_NE: x.node.opcode = OP_NE; break;
+              case NK_CHECK_INT: x.node.opcode = OP_CHECK_INT; break;
+              case NK_CHECK_LIST: x.node.opcode = OP_CHECK_LIST; break;
+              default: fail("unsupported primitive opcode");
+            }
+          }
+      }
+    } else {
+      fail("expression kind requires native fallback");
+    }
+    for (std::size_t i = 0; i < e.children.size(); ++i) {
+      x.child_scope[i] = static_cast<signed char>(env.size());
+      visit_expr(e.children[i], env);
+    }
+  };
+  visit_nt(root, Env(g.nts[root].scope.size()));
+  {  // identity scope for the root environment (binder numbers 0..n-1)
+  }
+  // Each production root must resolve to a distinct node (site annotation
+  // recovers a production from its first node's origin).
+  std::vector<bool> resolved(out.exprs.size());
+  for (const auto& p : out.prods) {
+    const int e = p.expression;
+    if (out.exprs[e].kind == PK_REFERENCE) fail("root aliases require native fallback");
+    if (resolved[e]) fail("productions sharing a root node require native fallback");
+    resolved[e] = true;
+  }
+  const std::size_t D = out.depth_count;
+  PExpr entry;
+  entry.kind = PK_REFERENCE;
+  entry.target = int(root);
+  entry.scope_count = int(g.nts[root].scope.size());
+  entry.node.type = g.nts[root].type;
+  out.root = int(out.exprs.size());
+  out.exprs.push_back(entry);
+  if (!site_depth) {
+    out.logical_count = 1;
+    out.minimum.assign(out.exprs.size() * D, kNoId);
+    // minimum_expression_nodes (budget.cpp) over the compiled grammar
+    std::function<std::uint32_t(std::uint32_t, std::uint32_t)> cost = [&](std::uint32_t id, std::uint32_t d) -> std::uint32_t {
+      const auto& e = g.exprs[id];
+      if (e.kind == EKind::Reference) return g.nts[e.target].minimum_by_depth.at(d);
+      if (!d) return kNoId;
+      std::uint64_t r = 1;
+      for (auto c : e.children) {
+        const auto n = cost(c, d - 1);
+        if (n == kNoId) return kNoId;
+        r += n;
+        if (r > g.max_nodes) return kNoId;
+      }
+      return std::uint32_t(r);
+    };
+    for (std::size_t i = 0; i < seen_expr.size(); ++i)
+      if (seen_expr[i]) for (unsigned d = 0; d <= max_depth; ++d) out.minimum[i * D + d] = cost(std::uint32_t(i), d);
+    for (unsigned d = 0; d <= max_depth; ++d) out.minimum[std::size_t(out.root) * D + d] = g.nts[root].minimum_by_depth.at(d);
+  } else {
+    out.logical_count = int(site_depth + 1);
+    const std::size_t L = out.logical_count;
+    const std::size_t size = out.exprs.size() * D * L;
+    if (size > 4u * 1024 * 1024) fail("site depth table capacity");
+    out.minimum.assign(size, kNoId);
+    std::vector<char> computed(size);
+    std::function<unsigned(int, int, int)> cost = [&](int id, int d, int l) -> unsigned {
+      const std::size_t off = (std::size_t(id) * D + d) * L + l;
+      if (computed[off]) return out.minimum[off];
+      computed[off] = 1;
+      const auto& e = out.exprs[id];
+      unsigned result = kNoId;
+      if (e.kind == PK_REFERENCE) {
+        if (l) {
+          const auto& nt = out.nts[e.target];
+          for (int i = 0; i < nt.production_count; ++i)
+            result = std::min(result, cost(out.prods[nt.production_offset + i].expression, d, l - 1));
+        }
+      } else if (d > 0) {
+        std::uint64_t total = 1;
+        for (int i = 0; i < e.node.arity; ++i) total += cost(e.children[i], d - 1, l);
+        result = total >= kNoId ? kNoId : unsigned(total);
+      }
+      out.minimum[off] = result;
+      return result;
+    };
+    for (std::size_t i = 0; i < out.exprs.size(); ++i)
+      if (int(i) == out.root || seen_expr[i])
+        for (unsigned d = 0; d <= max_depth; ++d)
+          for (unsigned l = 0; l <= site_depth; ++l) cost(int(i), int(d), int(l));
+  }
+  // Variation flags (phase_variation.hpp:6-14).
+  for (std::size_t i = 0; i < out.nts.size(); ++i)
+    if (out.nts[i].production_count) out.nts[i].fixed = !g.nts[i].variation;
+  // Static weight normalization (phase_grammar.hpp:48-67).
+  const std::size_t DL = std::size_t(out.depth_count) * out.logical_count;
+  for (auto& nt : out.nts) {
+    if (!nt.production_count) continue;
+    double maximum = 0;
+    for (int i = 0; i < nt.production_count; ++i) maximum = std::max(maximum, out.prods[nt.production_offset + i].weight);
+    bool proven = maximum > 0;
+    for (std::size_t k = 0; k < DL && proven; ++k) {
+      unsigned cheapest = kNoId, max_weight_cheapest = kNoId;
+      for (int i = 0; i < nt.production_count; ++i) {
+        const auto& p = out.prods[nt.production_offset + i];
+        const unsigned c = out.minimum.at(std::size_t(p.expression) * DL + k);
+        cheapest = std::min(cheapest, c);
+        if (p.weight == maximum) max_weight_cheapest = std::min(max_weight_cheapest, c);
+      }
+      proven = cheapest == max_weight_cheapest;
+    }
+    if (!proven) continue;
+    for (int i = 0; i < nt.production_count; ++i) out.prods[nt.production_offset + i].weight /= maximum;
+    nt.normalized = 1;
+  }
+  return out;
+}
+
+Layout whole_program_layout(const Grammar& g) {
+  Layout l;
+  const auto& entry = g.nts[g.entry];
+  if (entry.cat != Cat::Expression) throw Unsupported("program-entry grammars");
+  if (g.max_depth <= 3 || g.max_nodes <= 4)
+    throw Unsupported("expression entry needs four envelope nodes and three prefix levels");
+  if (g.max_nodes > 2048 || g.max_depth > 64) throw Unsupported("GPU genotype capacity requires fallback");
+  if (entry.minimum_by_depth.at(g.max_depth - 3) > g.max_nodes - 4)
+    throw Unsupported("generation request has no feasible derivation");
+  l.fixed_nodes = 4;
+  l.slot_max_depth = g.max_depth - 3;
+  l.max_variable_nodes = g.max_nodes - 4;
+  l.entry_nt = g.entry;
+  l.program_terminator = true;
+  return l;
+}
+
+}  // namespace gagp_amd

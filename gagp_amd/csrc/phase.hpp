@@ -1,0 +1,624 @@
+// Per-child operators of the resident phase population on this engine's row
+// layout: a tree row stores, per node, only its origin (phase expression id)
+// and its packed variation site; node fields (kind, opcode, arity, fuel, type)
+// are read from the expression table, constants live in a per-row value list
+// in prefix order.
+//
+// The draw order and every accept/reject rule restate the reference bodies:
+//   generate_phase        cpp/src/evolution/repro/gpu/device/phase_generate.cuh:125-270
+//   subtree end / depths  .../device/phase_variation.cuh:81-142
+//   splice_phase_tree     .../device/phase_variation.cuh:147-207
+//   tournament, sites     cpp/src/evolution/repro/gpu/phase_operators.cuh:46-83
+//   crossover_child       phase_operators.cuh:156-199
+//   mutate_child          phase_operators.cuh:202-245
+//   compare_child         phase_operators.cuh:247-265
+//   compile_phase_tree    .../device/phase_compile.cuh:206-895 (expression subset)
+#pragma once
+#include "core.hpp"
+#include "grammar.hpp"
+
+namespace gagp_amd {
+
+// Semantic phase capacity (nodes per phase row) used by the operator
+// formulas; physical rows are sized to the grammar's node budget, which the
+// operators never exceed (crossover and mutation are bounded by it first).
+constexpr int kPhaseCapacity = 1024;
+constexpr int kPending = 192;
+
+enum Status : int { ST_OK = 0, ST_INVALID = 1, ST_UNSUPPORTED = 2, ST_CAPACITY = 3 };
+
+struct PhaseView {
+  const PExpr* expr; int n_expr;
+  const PProd* prod; int n_prod;
+  const PNt* nt; int n_nt;
+  const PDomain* dom; int n_dom;
+  const Val* values; int n_values;
+  const unsigned* minimum; int depth_count, logical_count; int site_depth;
+  int root;
+};
+
+// Site packing: -1 for a non-site node, else entry expression | parent scope << 17.
+GHD inline int site_pack(int entry, int parent_scope) { return entry | (parent_scope << 17); }
+GHD inline int site_entry(int s) { return s & 0x1ffff; }
+GHD inline int site_parent_scope(int s) { return (s >> 17) & 0xff; }
+GHD inline int site_nt(const PhaseView& g, int s) { return g.expr[site_entry(s)].target; }
+GHD inline int site_scope(const PhaseView& g, int s) {
+  const PExpr& e = g.expr[site_entry(s)];
+  return e.scope_count | (g.nt[e.target].fixed ? kFixedSite : 0);
+}
+
+// A row view: origins, sites, constants of one phase tree.
+struct Row {
+  int* origin;
+  int* site;
+  Val* cval;
+  int n, nc;  // node and constant counts (n == 0: invalid row)
+};
+struct CRow {
+  const int* origin;
+  const int* site;
+  const Val* cval;
+  int n, nc;
+};
+GHD inline CRow as_const(const Row& r) { return {r.origin, r.site, r.cval, r.n, r.nc}; }
+
+struct GenResult { int status, nodes, constants, steps; };
+
+GHD inline unsigned g_minimum(const PhaseView& g, int e, int d, int l) {
+  return (e < 0 || e >= g.n_expr || d < 0 || d >= g.depth_count || l < 0 || l >= g.logical_count)
+             ? 0xffffffffu : g.minimum[(unsigned(e) * g.depth_count + d) * g.logical_count + l];
+}
+GHD inline int child_scope(const PExpr& x, int i, int scope) { return x.child_scope[i] >= 0 ? x.child_scope[i] : scope; }
+
+// Top-down derivation with a minimum-cost reservation and a pending stack.
+// Writes origins (and sites when `sites` is non-null) and constants.
+GHD inline GenResult generate_phase(const PhaseView& g, int root_expression, int scope_count, int depth, int capacity,
+                                    std::uint64_t seed, int* origins, Val* constants, int* sites, int logical_depth) {
+  GenResult r{ST_INVALID, 0, 0, 0};
+  if (root_expression < 0 || root_expression >= g.n_expr || scope_count < 0 || scope_count > 15 || depth <= 0 ||
+      depth >= g.depth_count || depth > 64 || logical_depth < 0 || logical_depth >= g.logical_count || capacity <= 0 ||
+      capacity > kPhaseCapacity || g.n_expr > 131071)
+    return r;
+  struct Task { int e, d, sc, l; };
+  Task pending[kPending];
+  int count = 1;
+  pending[0] = {root_expression, depth, scope_count, logical_depth};
+  unsigned reserved = g_minimum(g, root_expression, depth, logical_depth);
+  if (reserved > unsigned(capacity)) { r.status = ST_CAPACITY; return r; }
+  int next_site = -1;
+  SplitMix random(seed);
+  constexpr int unselected = 131071;
+  while (count) {
+    if (++r.steps > 8192) { r.status = ST_CAPACITY; return r; }
+    const Task task = pending[--count];
+    const unsigned required = g_minimum(g, task.e, task.d, task.l);
+    if (required > reserved) return r;
+    reserved -= required;
+    const PExpr& x = g.expr[task.e];
+    if (x.kind == PK_REFERENCE) {
+      if (x.target < 0 || x.target >= g.n_nt || x.scope_count < 0 || x.scope_count > 8) return r;
+      if (sites) {
+        if (next_site >= 0) return r;
+        next_site = site_pack(task.e, task.sc);
+      }
+      const PNt nt = g.nt[x.target];
+      if (nt.scope_count != x.scope_count || nt.production_offset < 0 || nt.production_count <= 0 ||
+          nt.production_offset > g.n_prod || nt.production_count > g.n_prod - nt.production_offset)
+        return r;
+      if (g.site_depth && task.l == 0) { r.status = ST_CAPACITY; return r; }
+      Task next{unselected, task.d, x.scope_count, task.l - (g.site_depth ? 1 : 0)};
+      if (reserved > unsigned(capacity - r.nodes)) { r.status = ST_CAPACITY; return r; }
+      const unsigned available = unsigned(capacity - r.nodes) - reserved;
+      double maximum = nt.normalized ? 1 : 0, total = 0;
+      if (!nt.normalized)
+        for (int i = 0; i < nt.production_count; ++i) {
+          const PProd p = g.prod[nt.production_offset + i];
+          if (g_minimum(g, p.expression, task.d, next.l) <= available && p.weight > maximum) maximum = p.weight;
+        }
+      if (!(maximum > 0)) { r.status = ST_CAPACITY; return r; }
+      for (int i = 0; i < nt.production_count; ++i) {
+        const PProd p = g.prod[nt.production_offset + i];
+        if (g_minimum(g, p.expression, task.d, next.l) <= available) total += nt.normalized ? p.weight : p.weight / maximum;
+      }
+      double choice = random.unit() * total;
+      for (int i = 0; i < nt.production_count; ++i) {
+        const PProd p = g.prod[nt.production_offset + i];
+        if (g_minimum(g, p.expression, task.d, next.l) > available) continue;
+        if (p.expression < 0 || p.expression >= g.n_expr) return r;
+        next.e = p.expression;
+        choice -= nt.normalized ? p.weight : p.weight / maximum;
+        if (choice < 0) break;
+      }
+      if (next.e == unselected || count >= kPending) return r;
+      reserved += g_minimum(g, next.e, next.d, next.l);
+      pending[count++] = next;
+      continue;
+    }
+    if (x.kind == PK_ALIAS) return r;  // templates are not admitted by the front end
+    if (task.d <= 0 || r.nodes >= capacity) { r.status = ST_CAPACITY; return r; }
+    const int arity = x.node.arity;
+    if (arity < 0 || arity > kMaxArity) return r;
+    if (x.kind == PK_BOUND) {
+      if (arity != 0 || x.node.operand < 0 || x.node.operand >= 16) return r;
+    } else if (x.kind == PK_CONSTANT) {
+      if (x.target < 0 || x.target >= g.n_dom || arity != 0) return r;
+      const PDomain d = g.dom[x.target];
+      Val v;
+      if (d.integer_range) {
+        if (d.minimum > d.maximum) return r;
+        v = val_int(random.integer(d.minimum, d.maximum));
+      } else {
+        if (d.value_offset < 0 || d.value_count <= 0 || d.value_offset > g.n_values || d.value_count > g.n_values - d.value_offset)
+          return r;
+        v = g.values[d.value_offset + random.bounded(std::uint64_t(d.value_count))];
+      }
+      if (v.tag == TAG_INVALID || v.tag == TAG_FALLBACK) return r;
+      constants[r.constants++] = v;
+    } else if (x.kind != PK_OPERATION && x.kind != PK_INPUT) return r;
+    if (sites) { sites[r.nodes] = next_site; next_site = -1; }
+    origins[r.nodes++] = task.e;
+    for (int i = arity - 1; i >= 0; --i) {
+      if (count >= kPending) { r.status = ST_CAPACITY; return r; }
+      const int child = x.children[i];
+      const unsigned need = g_minimum(g, child, task.d - 1, task.l);
+      if (need > unsigned(capacity - r.nodes) || reserved > unsigned(capacity - r.nodes) - need) {
+        r.status = ST_CAPACITY;
+        return r;
+      }
+      Task next = task;
+      next.e = child;
+      --next.d;
+      next.sc = child_scope(x, i, task.sc);
+      pending[count++] = next;
+      reserved += need;
+    }
+  }
+  if (reserved || !r.nodes) return r;
+  r.status = ST_OK;
+  return r;
+}
+
+GHD inline int arity_of(const PhaseView& g, int origin) { return g.expr[origin].node.arity; }
+GHD inline bool is_constant(const PhaseView& g, int origin) { return g.expr[origin].kind == PK_CONSTANT; }
+
+GHD inline int subtree_end(const PhaseView& g, const CRow& t, int start) {
+  if (start < 0 || start >= t.n || t.n > kPhaseCapacity) return -1;
+  int pending = 1;
+  for (int i = start; i < t.n; ++i) {
+    const int a = arity_of(g, t.origin[i]);
+    if (a < 0 || a > kMaxArity) return -1;
+    pending += a - 1;
+    if (!pending) return i + 1;
+  }
+  return -1;
+}
+GHD inline int node_depth(const PhaseView& g, const CRow& t, int position) {
+  if (position < 0 || position >= t.n || t.n > kPhaseCapacity) return -1;
+  int pending[64];
+  int depth = 1;
+  pending[0] = 1;
+  for (int i = 0; i <= position; ++i) {
+    while (depth && !pending[depth - 1]) --depth;
+    if (!depth) return -1;
+    if (i == position) return depth;
+    --pending[depth - 1];
+    const int a = arity_of(g, t.origin[i]);
+    if (a < 0 || a > kMaxArity) return -1;
+    if (a) { if (depth == 64) return -1; pending[depth++] = a; }
+  }
+  return -1;
+}
+// Nested grammar-invocation depth; with `position`, the incoming depth at it.
+GHD inline int site_depth_of(const PhaseView& g, const int* origin, const int* site, int count, int position = -1) {
+  if (count <= 0 || count > kPhaseCapacity || position >= count) return -1;
+  int pending[64], incoming[64];
+  int depth = 1, maximum = 0;
+  pending[0] = 1;
+  incoming[0] = 0;
+  for (int i = 0; i < count; ++i) {
+    while (depth && !pending[depth - 1]) --depth;
+    if (!depth) return -1;
+    if (i == position) return incoming[depth - 1];
+    const int a = arity_of(g, origin[i]);
+    if (a < 0 || a > kMaxArity) return -1;
+    const int value = incoming[depth - 1] + (site[i] >= 0 ? 1 : 0);
+    maximum = value > maximum ? value : maximum;
+    --pending[depth - 1];
+    if (a) { if (depth == 64) return -1; pending[depth] = a; incoming[depth++] = value; }
+  }
+  while (depth && !pending[depth - 1]) --depth;
+  return depth ? -1 : maximum;
+}
+GHD inline int constants_before(const PhaseView& g, const CRow& t, int end) {
+  int c = 0;
+  for (int i = 0; i < end; ++i) c += is_constant(g, t.origin[i]);
+  return c;
+}
+
+// Replace base[destination's subtree] by donor[source's subtree] into `out`
+// (which must alias neither input). Fused depth tracking.
+GHD inline GenResult splice(const PhaseView& g, const CRow& base, int destination, const CRow& donor, int source, Row out,
+                            int out_capacity, int max_depth, int site_limit) {
+  GenResult r{ST_INVALID, 0, 0, 0};
+  if (out_capacity <= 0 || out_capacity > kPhaseCapacity || max_depth <= 0 || max_depth > 64 || base.n <= 0 ||
+      base.n > kPhaseCapacity || donor.n <= 0 || donor.n > kPhaseCapacity || destination < 0 || destination >= base.n ||
+      source < 0 || source >= donor.n)
+    return r;
+  const int a = base.site[destination], b = donor.site[source];
+  if (a < 0 || b < 0 || site_nt(g, a) != site_nt(g, b) || site_scope(g, a) != site_scope(g, b)) return r;
+  const int end = subtree_end(g, base, destination), donor_end = subtree_end(g, donor, source);
+  if (end < 0 || donor_end < 0) return r;
+  const int length = base.n - (end - destination) + donor_end - source;
+  if (length > out_capacity) { r.status = ST_CAPACITY; return r; }
+  int pending[64];
+  int depth = 1, maximum = 1;
+  pending[0] = 1;
+  // Constant cursors of the three copied ranges (constants are in prefix order).
+  int c_base = 0, c_donor = constants_before(g, donor, source), c_tail = constants_before(g, base, end);
+  for (int i = 0; i < length; ++i) {
+    const bool incoming = i >= destination && i < destination + donor_end - source;
+    const CRow& tree = incoming ? donor : base;
+    const int index = incoming ? source + i - destination : i < destination ? i : i + (end - destination) - (donor_end - source);
+    const int o = tree.origin[index];
+    const int ar = arity_of(g, o);
+    while (depth && !pending[depth - 1]) --depth;
+    if (!depth || ar < 0 || ar > kMaxArity) return r;
+    maximum = depth > maximum ? depth : maximum;
+    --pending[depth - 1];
+    if (ar) { if (depth == 64) return r; pending[depth++] = ar; }
+    if (is_constant(g, o)) {
+      int& cursor = incoming ? c_donor : (i < destination ? c_base : c_tail);
+      if (cursor >= tree.nc) return r;
+      out.cval[r.constants++] = tree.cval[cursor++];
+    }
+    out.origin[i] = o;
+    out.site[i] = tree.site[index];
+  }
+  out.site[destination] = a;
+  while (depth && !pending[depth - 1]) --depth;
+  if (depth) return r;
+  if (maximum > max_depth) { r.status = ST_CAPACITY; return r; }
+  if (site_limit) {
+    const int h = site_depth_of(g, out.origin, out.site, length);
+    if (h < 0) return r;
+    if (h > site_limit) { r.status = ST_CAPACITY; return r; }
+  }
+  r.nodes = length;
+  r.status = ST_OK;
+  return r;
+}
+
+struct OpStats { int parent, crossover_attempted, crossover, crossover_rejected, mutation, changed, error, reusable_parent; };
+struct OpPolicy {
+  int counted_sites;
+  int uniform_phase;
+  double crossover_probability;
+  int crossover_attempts;
+  int mutation_max_nodes;
+};
+
+// Member ranking keys: fitness (desc), wrong (asc), nodes (asc), index (asc).
+struct Keys {
+  const double* fitness;
+  const unsigned* wrong;
+  const unsigned* nodes;
+  int has_wrong;
+};
+// Strict total order of the steady-state schedule (cpp/src/evolution/evolve.cpp:358-362).
+GHD inline bool better(const Keys& k, int a, int b) {
+  if (k.fitness[a] != k.fitness[b]) return k.fitness[a] > k.fitness[b];
+  if (k.has_wrong) {
+    if (k.wrong[a] != k.wrong[b]) return k.wrong[a] < k.wrong[b];
+    if (k.nodes[a] != k.nodes[b]) return k.nodes[a] < k.nodes[b];
+  }
+  return a < b;
+}
+// Tournament on selection values. With wrong counts the selection value is
+// P - rank under `better`, so comparing values is comparing by `better`;
+// without, the value is the fitness and ties keep the earlier draw.
+GHD inline bool selected_over(const Keys& k, int candidate, int best) {
+  return k.has_wrong ? better(k, candidate, best) : k.fitness[candidate] > k.fitness[best];
+}
+GHD inline int tournament(const Keys& k, int population, int pressure, SplitMix& random) {
+  int best = int(random.bounded(std::uint64_t(population)));
+  for (int i = 1; i < pressure; ++i) {
+    const int c = int(random.bounded(std::uint64_t(population)));
+    if (selected_over(k, c, best)) best = c;
+  }
+  return best;
+}
+
+GHD inline bool site_matches(const PhaseView& g, const CRow& t, int i, int nt, int scope) {
+  const int s = t.site[i];
+  if (s < 0) return false;
+  const int sc = site_scope(g, s);
+  if (sc & kFixedSite) return false;
+  return nt < 0 || (site_nt(g, s) == nt && sc == scope);
+}
+// Single-phase rows: slot selection is trivially slot 0 (or none).
+GHD inline int choose_site(const PhaseView& g, const CRow& t, SplitMix& random, bool counted, int nt, int scope) {
+  if (t.n <= 0) return -1;
+  if (counted) {
+    unsigned count = 0;
+    for (int i = 0; i < t.n; ++i) count += site_matches(g, t, i, nt, scope);
+    if (!count) return -1;
+    unsigned rank = unsigned(random.bounded(count));
+    for (int i = 0; i < t.n; ++i)
+      if (site_matches(g, t, i, nt, scope)) { if (rank == 0) return i; --rank; }
+    return -1;
+  }
+  int selected = -1;
+  unsigned count = 0;
+  for (int i = 0; i < t.n; ++i)
+    if (site_matches(g, t, i, nt, scope))
+      if (random.bounded(++count) == 0) selected = i;
+  return selected;
+}
+
+GHD inline void copy_row(const CRow& from, Row to, int* to_n, int* to_nc) {
+  for (int i = 0; i < from.n; ++i) { to.origin[i] = from.origin[i]; to.site[i] = from.site[i]; }
+  for (int i = 0; i < from.nc; ++i) to.cval[i] = from.cval[i];
+  *to_n = from.n;
+  *to_nc = from.nc;
+}
+
+struct SlotInfo {
+  int root, scope, max_depth, logical_depth;
+  int max_variable_nodes;
+  int row_capacity;  // physical row size
+};
+
+// Crossover of one child into `out` (a private row per child). `in(m)` gives
+// the current member rows.
+template <class MemberRow>
+GHD inline void crossover_child(const PhaseView& g, const SlotInfo& s, int child, MemberRow in, Row out, int* out_n, int* out_nc,
+                                const Keys& keys, int population, int pressure, std::uint64_t seed, const OpPolicy& pol, OpStats& st) {
+  SplitMix random(seed ^ (0x617089e168ULL + std::uint64_t(child)));
+  st = OpStats{-1, 0, 0, 0, 0, 0, 0, -1};
+  const int parent = tournament(keys, population, pressure, random);
+  const int donor = tournament(keys, population, pressure, random);
+  st.parent = parent;
+  const CRow a = in(parent);
+  if (a.n <= 0) { st.error = 1; return; }
+  const int total = a.n;
+  copy_row(a, out, out_n, out_nc);
+  if (pol.crossover_probability < 1 && random.unit() >= pol.crossover_probability) return;
+  st.crossover_attempted = 1;
+  const int selected_slot = pol.uniform_phase ? int(random.bounded(1)) : -1;
+  (void)selected_slot;
+  const CRow b = in(donor);
+  for (int attempt = 0; attempt < pol.crossover_attempts; ++attempt) {
+    const int dest = choose_site(g, a, random, pol.counted_sites != 0, -1, -1);
+    if (dest < 0) break;
+    const int contract = a.site[dest];
+    // An invalid donor row draws nothing and rejects the attempt.
+    const int src = b.n > 0 ? choose_site(g, b, random, pol.counted_sites != 0, site_nt(g, contract), site_scope(g, contract)) : -1;
+    if (src < 0) { ++st.crossover_rejected; continue; }
+    const int ae = subtree_end(g, a, dest), be = subtree_end(g, b, src);
+    if (ae < 0 || be < 0) { st.error = 3; return; }
+    if (total - (ae - dest) + (be - src) > s.max_variable_nodes) { ++st.crossover_rejected; continue; }
+    const GenResult r = splice(g, a, dest, b, src, out, kPhaseCapacity, s.max_depth, s.logical_depth);
+    if (r.status == ST_OK) {
+      *out_n = r.nodes;
+      *out_nc = r.constants;
+      st.crossover = 1;
+      return;
+    }
+    copy_row(a, out, out_n, out_nc);  // failed speculation restores the phase
+    ++st.crossover_rejected;
+  }
+}
+
+// Mutation of the crossed child row `x` in place; `tmp` holds the generated
+// subtree and `spliced` the candidate (private rows, never aliasing `x`).
+GHD inline void mutate_child(const PhaseView& g, const SlotInfo& s, int child, Row x, int* x_n, int* x_nc, Row tmp, Row spliced,
+                             double probability, std::uint64_t seed, const OpPolicy& pol, OpStats& st) {
+  if (st.error) return;
+  if (x.n <= 0) { st.error = 4; return; }
+  const int total = x.n;
+  SplitMix random(seed ^ (0x124ab512bf01ULL + std::uint64_t(child)));
+  if (random.unit() >= probability) return;
+  if (pol.uniform_phase) (void)random.bounded(1);
+  const CRow a = as_const(x);
+  const int dest = choose_site(g, a, random, pol.counted_sites != 0, -1, -1);
+  if (dest < 0) return;
+  const int end = subtree_end(g, a, dest), depth = node_depth(g, a, dest);
+  if (end < 0 || depth < 0) { st.error = 6; return; }
+  const int site = a.site[dest];
+  const int incoming = s.logical_depth ? site_depth_of(g, a.origin, a.site, a.n, dest) : 0;
+  if (incoming < 0 || incoming > s.logical_depth) { st.error = 6; return; }
+  const int logical = s.logical_depth - incoming;
+  const int replaced = end - dest;
+  const int whole = s.max_variable_nodes - (total - replaced), phase = kPhaseCapacity - (a.n - replaced);
+  const int available = whole < phase ? whole : phase, uncapped = available < kPhaseCapacity ? available : kPhaseCapacity;
+  const bool capped = pol.mutation_max_nodes > 0 && pol.mutation_max_nodes < uncapped;
+  const int bounded = capped ? pol.mutation_max_nodes : uncapped;
+  const GenResult gen = generate_phase(g, site_entry(site), site_parent_scope(site), s.max_depth - depth + 1, bounded,
+                                       random.next(), tmp.origin, tmp.cval, tmp.site, logical);
+  if (capped && gen.status == ST_CAPACITY) return;
+  if (gen.status != ST_OK) { st.error = 7; return; }
+  const CRow sub{tmp.origin, tmp.site, tmp.cval, gen.nodes, gen.constants};
+  const GenResult r = splice(g, a, dest, sub, 0, spliced, kPhaseCapacity, s.max_depth, s.logical_depth);
+  if (r.status != ST_OK) { st.error = 8; return; }
+  const CRow sp{spliced.origin, spliced.site, spliced.cval, r.nodes, r.constants};
+  copy_row(sp, x, x_n, x_nc);
+  st.mutation = 1;
+}
+
+// Is the child identical to its first parent? (constants compared by tag and
+// payload; reuse additionally requires identical raw bits).
+GHD inline void compare_child(const CRow& parent_row, const CRow& child_row, int population, OpStats& st) {
+  if (st.error) return;
+  if (st.parent < 0 || st.parent >= population) { st.error = 9; return; }
+  bool equal = parent_row.n == child_row.n && parent_row.nc == child_row.nc, reusable = true;
+  for (int i = 0; equal && i < parent_row.n; ++i) equal = parent_row.origin[i] == child_row.origin[i];
+  for (int i = 0; equal && i < parent_row.nc; ++i) {
+    const Val x = parent_row.cval[i], y = child_row.cval[i];
+    equal = x.tag == y.tag && (x.tag == TAG_BOOL ? x.b == y.b : x.i == y.i);
+    reusable = reusable && x.i == y.i && x.b == y.b;
+  }
+  st.changed = !equal;
+  st.reusable_parent = equal && reusable ? st.parent : -1;
+}
+
+// Genotype hash (FNV-1a style mix over origins and constant payloads); only
+// a lookup key, equality is always checked in full.
+GHD inline std::uint64_t row_hash(const CRow& r) {
+  std::uint64_t h = 0xcbf29ce484222325ULL;
+  const auto mix = [&](std::uint64_t v) { h ^= v; h *= 0x100000001b3ULL; };
+  mix(std::uint64_t(r.n));
+  for (int i = 0; i < r.n; ++i) mix(std::uint64_t(std::uint32_t(r.origin[i])));
+  for (int i = 0; i < r.nc; ++i) { mix(std::uint64_t(r.cval[i].i)); mix(std::uint64_t(r.cval[i].tag) << 8 | r.cval[i].b); }
+  return h;
+}
+// Genotype identity used for completed-result reuse (parent_fitness_cache.cuh:11-58).
+GHD inline bool rows_equal(const CRow& a, const CRow& b) {
+  if (a.n != b.n || a.nc != b.nc) return false;
+  for (int i = 0; i < a.n; ++i) if (a.origin[i] != b.origin[i]) return false;
+  for (int i = 0; i < a.nc; ++i) {
+    const Val x = a.cval[i], y = b.cval[i];
+    if (x.tag != y.tag || (x.tag == TAG_BOOL ? x.b != y.b : x.i != y.i)) return false;
+  }
+  return true;
+}
+
+// ---- lowering (expression phases) ----
+// Instruction of this engine's VM: opcode, builtin argc, static stack height
+// before the instruction, fuel charge, operand.
+struct Ins {
+  std::uint8_t op;
+  std::uint8_t argc;
+  std::uint8_t sp;
+  std::uint8_t fuel;
+  std::int32_t a;
+};
+static_assert(sizeof(Ins) == 8, "Ins is 8 bytes");
+constexpr int kMaxStack = 64;
+
+struct LowerResult { int status, code, constants, stack_bound, result_type; };
+
+// Post-order lowering of an expression phase plus the program terminator,
+// with the reference's jump shapes for if/and/or. Constants are emitted in
+// first-use order (one pool entry per constant node).
+GHD inline LowerResult lower_phase(const PhaseView& g, const CRow& t, const std::uint8_t* local_types, int local_count,
+                                   int expected_type, Ins* code, int code_capacity, Val* pool, int pool_capacity) {
+  LowerResult out{ST_INVALID, 0, 0, 0, TAG_INVALID};
+  if (t.n <= 0 || t.n > 4096) return out;
+  struct Frame { int origin, seen, first_jump, second_jump, base_stack; std::uint8_t args[3]; };
+  Frame frames[64];
+  int depth = 0, cursor = 0, stack = 0, cconst = 0;
+  int completed = TAG_INVALID;
+  bool have_value = false;
+  const auto emit = [&](int op, int a, int argc, unsigned fuel) -> bool {
+    if (out.code >= code_capacity) { out.status = ST_CAPACITY; return false; }
+    if (fuel > 255 || stack < 0 || stack > kMaxStack) { out.status = ST_UNSUPPORTED; return false; }
+    code[out.code++] = Ins{std::uint8_t(op), std::uint8_t(argc), std::uint8_t(stack), std::uint8_t(fuel), a};
+    return true;
+  };
+  const auto grow = [&](int by) { stack += by; if (stack > out.stack_bound) out.stack_bound = stack; };
+  for (;;) {
+    if (!have_value) {
+      if (cursor >= t.n) return out;
+      const int o = t.origin[cursor++];
+      const PExpr& x = g.expr[o];
+      const PNode& node = x.node;
+      if (node.kind == TK_CONSTANT) {
+        if (cconst >= t.nc) return out;
+        const Val v = t.cval[cconst++];
+        if (v.tag == TAG_INVALID || v.tag == TAG_FALLBACK) { out.status = ST_UNSUPPORTED; return out; }
+        if (out.constants >= pool_capacity) { out.status = ST_CAPACITY; return out; }
+        pool[out.constants] = v;
+        if (!emit(OP_PUSH_CONST, out.constants, 0, node.fuel)) return out;
+        ++out.constants;
+        grow(1);
+        completed = v.tag;
+        have_value = true;
+      } else if (node.kind == TK_LOCAL) {
+        if (node.operand < 0 || node.operand >= local_count) return out;
+        completed = local_types[node.operand];
+        if (completed == TAG_INVALID || completed == TAG_FALLBACK) { out.status = ST_UNSUPPORTED; return out; }
+        if (!emit(OP_LOAD, node.operand, 0, node.fuel)) return out;
+        grow(1);
+        have_value = true;
+      } else {
+        if (node.arity < 1 || node.arity > 3) return out;
+        if (node.kind == TK_IF) { if (node.arity != 3) return out; }
+        else if (node.kind == TK_AND || node.kind == TK_OR) { if (node.arity != 2) return out; }
+        else if (node.kind != TK_OPERATION) { out.status = ST_UNSUPPORTED; return out; }
+        if (depth == 64) { out.status = ST_CAPACITY; return out; }
+        frames[depth++] = Frame{o, 0, -1, -1, stack, {TAG_INVALID, TAG_INVALID, TAG_INVALID}};
+        continue;
+      }
+    }
+    if (depth == 0) {
+      if (cursor != t.n || stack != 1 || completed != expected_type) return out;
+      if (out.stack_bound > kMaxStack) { out.status = ST_CAPACITY; return out; }
+      // program terminator (whole-program expression entry)
+      if (!emit(OP_RETURN, 0, 0, 1)) return out;
+      out.status = ST_OK;
+      out.result_type = completed;
+      return out;
+    }
+    Frame& f = frames[depth - 1];
+    const PNode& node = g.expr[f.origin].node;
+    if (f.seen < 3) f.args[f.seen] = std::uint8_t(completed);
+    ++f.seen;
+    const bool branch = node.kind == TK_IF || node.kind == TK_AND || node.kind == TK_OR;
+    if (branch && f.seen == 1) {
+      if (completed != TAG_BOOL) return out;
+      f.first_jump = out.code;
+      if (!emit(node.kind == TK_OR ? OP_JMP_IF_TRUE : OP_JMP_IF_FALSE, 0, 0, node.fuel)) return out;
+      --stack;
+    } else if (node.kind == TK_IF && f.seen == 2) {
+      f.second_jump = out.code;
+      if (!emit(OP_JMP, 0, 0, node.merge_fuel)) return out;
+      code[f.first_jump].a = out.code;
+      stack = f.base_stack;
+    }
+    if (f.seen < node.arity) { have_value = false; continue; }
+    if (node.kind == TK_IF) {
+      if (f.args[1] != f.args[2]) return out;
+      code[f.second_jump].a = out.code;
+    } else if (node.kind == TK_AND || node.kind == TK_OR) {
+      if (completed != TAG_BOOL) return out;
+      if (!emit(OP_NOT, 0, 0, node.fuel) || !emit(OP_NOT, 0, 0, node.fuel)) return out;
+      const int finish = out.code;
+      if (!emit(OP_JMP, 0, 0, node.fuel)) return out;
+      code[f.first_jump].a = out.code;
+      stack = f.base_stack;
+      // push the short-circuit constant
+      if (out.constants >= pool_capacity) { out.status = ST_CAPACITY; return out; }
+      pool[out.constants] = val_bool(node.kind == TK_OR);
+      if (!emit(OP_PUSH_CONST, out.constants, 0, node.fuel)) return out;
+      ++out.constants;
+      grow(1);
+      code[finish].a = out.code;
+    } else {
+      // operation result type: the certified Int/Bool rules, else the grammar type
+      int rt = TAG_INVALID;
+      const std::uint8_t* args = f.args;
+      switch (node.opcode) {
+        case OP_NEG: case OP_CHECK_INT: rt = args[0] == TAG_INT ? TAG_INT : TAG_INVALID; break;
+        case OP_NOT: rt = args[0] == TAG_BOOL ? TAG_BOOL : TAG_INVALID; break;
+        case OP_ADD: case OP_SUB: case OP_MUL: case OP_MOD:
+          rt = args[0] == TAG_INT && args[1] == TAG_INT ? TAG_INT : TAG_INVALID; break;
+        case OP_LT: case OP_LE: case OP_GT: case OP_GE:
+          rt = args[0] == TAG_INT && args[1] == TAG_INT ? TAG_BOOL : TAG_INVALID; break;
+        case OP_EQ: case OP_NE:
+          rt = args[0] == args[1] && (args[0] == TAG_INT || args[0] == TAG_BOOL) ? TAG_BOOL : TAG_INVALID; break;
+        default: break;
+      }
+      if (rt == TAG_INVALID) rt = node.type;
+      else if (node.type != TAG_INVALID && node.type != rt) return out;
+      if (rt == TAG_INVALID || rt == TAG_FALLBACK) { out.status = ST_UNSUPPORTED; return out; }
+      // operands are on the stack: the instruction sees the stack with them
+      if (node.opcode == OP_CALL_BUILTIN) { if (!emit(OP_CALL_BUILTIN, node.operand, node.arity, node.fuel)) return out; }
+      else if (!emit(node.opcode, 0, 0, node.fuel)) return out;
+      stack -= node.arity - 1;
+      completed = rt;
+    }
+    --depth;
+  }
+}
+
+}  // namespace gagp_amd

@@ -1,0 +1,21 @@
+# Builds the reference's single-threaded C++ CPU engine (`gagp`) straight from
+# its sources with g++ -- no CMake, no CUDA, no OpenSSL (the grammar identity
+# hash then uses the reference's own built-in SHA-256).  Outputs go only to
+# oracle/_ref/.  Used as the checker that writes golden fixtures in the build
+# container, and as the CPU baseline ("kind": "reference") timed by bench.py.
+#
+#   make -f oracle/ref.mk -j8            (from the repository root)
+REF ?= /root/reference/cpp
+OUT := oracle/_ref
+SRCS := $(filter-out $(REF)/src/runtime/gpu/%,$(shell find $(REF)/src -name '*.cpp' 2>/dev/null | sort))
+OBJS := $(patsubst $(REF)/src/%.cpp,$(OUT)/obj/%.o,$(SRCS))
+CXXFLAGS ?= -std=c++17 -O3 -DNDEBUG -ffp-contract=off
+all: $(OUT)/gagp
+$(OUT)/obj/%.o: $(REF)/src/%.cpp
+	@mkdir -p $(dir $@)
+	g++ $(CXXFLAGS) -I$(REF)/include -c $< -o $@
+$(OUT)/gagp: $(OBJS)
+	g++ $(CXXFLAGS) $(OBJS) -o $@ -lpthread
+clean:
+	rm -rf $(OUT)
+.PHONY: all clean
