@@ -1,0 +1,1291 @@
+// Device-resident search engine for gfx950 (see engine.hpp).
+//
+// One steady-state round (cpp/src/evolution/evolve.cpp:331-441 with the owner
+// bodies of cpp/src/evolution/repro/gpu/phase_population_cpu.cpp:314-394) is
+// the kernel sequence
+//   k_record  history row of generation r/per_epoch (best row, fitness vector)
+//   k_vary    tournaments, crossover, mutation, compare, lowering (thread/child)
+//   k_clear + k_index   hash table of the current members
+//   k_reuse   completed-result reuse; pending list of children to score
+//   k_eval    bytecode VM + canonical reduction (wavefront/program)
+//   k_place   mt19937_64 inverse-tournament placement (block/search)
+//   k_commit  placed children overwrite their members (thread/member)
+// over every search of a batch (grid.y = search). Nothing returns to the
+// host between rounds.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <fstream>
+#include <numeric>
+#include <sstream>
+#include <stdexcept>
+
+#include "engine.hpp"
+#include "vm.hpp"
+
+namespace gagp_amd {
+
+#define HIP_OK(expr)                                                                                 \
+  do {                                                                                               \
+    const hipError_t e_ = (expr);                                                                    \
+    if (e_ != hipSuccess) throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e_) + " at " #expr); \
+  } while (0)
+
+namespace {
+
+constexpr int kWavesPerBlock = 4;
+constexpr int kEvalThreads = 64 * kWavesPerBlock;
+
+struct Counters {
+  // evaluated programs and program-cases, runtime errors, timeouts, VM
+  // instructions executed (charged fuel), instructions and constants of the
+  // evaluated programs (their compulsory code bytes).
+  unsigned long long programs, program_cases, errors, timeouts, vm_instructions, code_words, const_words;
+};
+
+// Device state of one search. Every pointer is device memory owned by the
+// host Search object.
+struct DSearch {
+  int active;
+  int P, B, R, code_cap, pool_cap, stack_slots;
+  int rounds, per_epoch;
+  int fuel, lanes;
+  double penalty;
+  Policy policy;
+  int pressure, replacement_size;
+  double p_mut;
+  OpPolicy pol;
+  SlotInfo slot;
+  int fixed_nodes, grammar_sites, has_wrong, expected_type, n_locals, site_depth;
+  std::uint64_t seed;
+  PhaseView g;
+  const std::uint8_t* local_types;
+  CaseData cd;
+  int *cur_origin, *cur_site, *cur_n, *cur_nc;
+  Val* cur_cval;
+  int *x_origin, *x_site, *x_n, *x_nc;
+  Val* x_cval;
+  int *t_origin, *t_site;
+  Val* t_cval;
+  int *s_origin, *s_site;
+  Val* s_cval;
+  double* fitness;
+  unsigned *wrong, *nodes;
+  OpStats* stats;
+  int* reuse;
+  double* c_fit;
+  unsigned *c_wrong, *c_nodes;
+  Ins* code;
+  Val* pool;
+  int *code_len, *code_jumps;
+  std::uint64_t* child_hash;
+  int *pending, *n_pending;
+  int* table;
+  int table_mask;
+  std::uint64_t* mt;
+  int* mt_index;
+  std::uint64_t* round_seed;
+  int *round, *best, *placed;
+  int *hist_origin, *hist_n, *hist_nc;
+  Val* hist_cval;
+  double *hist_fit, *hist_all;
+  unsigned* hist_wrong;
+  Counters* counters;
+  int* error;
+};
+
+__device__ inline CRow member_row(const DSearch& d, int m) {
+  const std::size_t o = std::size_t(m) * d.R;
+  return CRow{d.cur_origin + o, d.cur_site + o, d.cur_cval + o, d.cur_n[m], d.cur_nc[m]};
+}
+__device__ inline CRow child_row(const DSearch& d, int c) {
+  const std::size_t o = std::size_t(c) * d.R;
+  return CRow{d.x_origin + o, d.x_site + o, d.x_cval + o, d.x_n[c], d.x_nc[c]};
+}
+__device__ inline bool search_live(const DSearch& d) { return d.active && *d.round < d.rounds && !*d.error; }
+__device__ inline void raise(const DSearch& d, int code) { atomicCAS(d.error, 0, code); }
+
+// ---- initial population (phase_population_cpu.cpp:54-113) ----
+__global__ void k_init_pop(DSearch* all) {
+  const DSearch& d = all[blockIdx.y];
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (!d.active || m >= d.P) return;
+  const int sd = d.site_depth;
+  const int ramp = sd < 2 ? sd : 2 + m % (sd - 1);
+  const std::size_t o = std::size_t(m) * d.R;
+  GenResult gen{ST_INVALID, 0, 0, 0};
+  const int room = d.slot.max_variable_nodes < kPhaseCapacity ? d.slot.max_variable_nodes : kPhaseCapacity;
+  for (int logical = ramp, attempt = 0; gen.status != ST_OK; ++attempt) {
+    if (room <= 0 || room > d.R) { raise(d, 101); return; }
+    const std::uint64_t s = splitmix_mix(d.seed ^ splitmix_mix((std::uint64_t(m) * 8 + 0) * 64 + std::uint64_t(attempt)));
+    gen = generate_phase(d.g, d.g.root, 0, d.slot.max_depth, room, s, d.cur_origin + o, d.cur_cval + o, d.cur_site + o, logical);
+    if (gen.status != ST_OK && attempt % 4 == 3 && logical < sd) ++logical;
+    if (attempt >= 256) { raise(d, 102); return; }
+  }
+  d.cur_n[m] = gen.nodes;
+  d.cur_nc[m] = gen.constants;
+  d.placed[m] = -1;
+}
+
+// ---- lowering: rows of members (mode 0) or children (mode 1) ----
+__device__ inline void lower_into(const DSearch& d, const CRow& row, int slot) {
+  Ins* code = d.code + std::size_t(slot) * d.code_cap;
+  Val* pool = d.pool + std::size_t(slot) * d.pool_cap;
+  const LowerResult r = lower_phase(d.g, row, d.local_types, d.n_locals, d.expected_type, code, d.code_cap, pool, d.pool_cap);
+  if (r.status != ST_OK || r.stack_bound < 1 || r.stack_bound > d.stack_slots) { raise(d, 200 + r.status); d.code_len[slot] = 0; return; }
+  bool jumps = false;
+  for (int i = 0; i < r.code; ++i) jumps |= code[i].op == OP_JMP || code[i].op == OP_JMP_IF_FALSE || code[i].op == OP_JMP_IF_TRUE;
+  d.code_len[slot] = r.code;
+  d.code_jumps[slot] = jumps;
+}
+__global__ void k_lower_members(DSearch* all) {
+  const DSearch& d = all[blockIdx.y];
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (!d.active || m >= d.P || *d.error) return;
+  lower_into(d, member_row(d, m), m);
+}
+
+// ---- reproduction: one thread per child (phase_operators.cuh:156-265) ----
+__global__ void k_vary(DSearch* all) {
+  const DSearch& d = all[blockIdx.y];
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (!search_live(d) || c >= d.B) return;
+  const std::uint64_t seed = *d.round_seed;
+  const Keys keys{d.fitness, d.wrong, d.nodes, d.has_wrong};
+  const std::size_t o = std::size_t(c) * d.R;
+  Row x{d.x_origin + o, d.x_site + o, d.x_cval + o, 0, 0};
+  OpStats st;
+  crossover_child(d.g, d.slot, c, [&](int m) { return member_row(d, m); }, x, d.x_n + c, d.x_nc + c, keys, d.P, d.pressure, seed,
+                  d.pol, st);
+  x.n = d.x_n[c];
+  x.nc = d.x_nc[c];
+  Row tmp{d.t_origin + o, d.t_site + o, d.t_cval + o, 0, 0};
+  Row spl{d.s_origin + o, d.s_site + o, d.s_cval + o, 0, 0};
+  mutate_child(d.g, d.slot, c, x, d.x_n + c, d.x_nc + c, tmp, spl, d.p_mut, seed, d.pol, st);
+  const CRow child = child_row(d, c);
+  if (!st.error) compare_child(member_row(d, st.parent), child, d.P, st);
+  d.stats[c] = st;
+  if (st.error) { raise(d, 300 + st.error); return; }
+  d.child_hash[c] = row_hash(child);
+  lower_into(d, child, c);
+}
+
+// ---- member hash table (parent_fitness_cache.cuh:11-58) ----
+__global__ void k_clear(DSearch* all) {
+  const DSearch& d = all[blockIdx.y];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (!search_live(d) || i > d.table_mask) return;
+  d.table[i] = -1;
+}
+__global__ void k_index(DSearch* all) {
+  const DSearch& d = all[blockIdx.y];
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (!search_live(d) || m >= d.P) return;
+  std::uint64_t h = row_hash(member_row(d, m));
+  for (int probe = 0; probe <= d.table_mask; ++probe) {
+    const int slot = int((h + std::uint64_t(probe)) & std::uint64_t(d.table_mask));
+    if (atomicCAS(d.table + slot, -1, m) == -1) return;
+  }
+  raise(d, 400);
+}
+__global__ void k_reuse(DSearch* all) {
+  const DSearch& d = all[blockIdx.y];
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (!search_live(d) || c >= d.B) return;
+  int r = d.stats[c].reusable_parent;
+  const CRow child = child_row(d, c);
+  if (r < 0) {
+    const std::uint64_t h = d.child_hash[c];
+    for (int probe = 0; probe <= d.table_mask; ++probe) {
+      const int m = d.table[int((h + std::uint64_t(probe)) & std::uint64_t(d.table_mask))];
+      if (m < 0) break;
+      if (rows_equal(member_row(d, m), child)) { r = m; break; }
+    }
+  }
+  d.reuse[c] = r;
+  if (r >= 0) {
+    // Results are fixed before any placement of the round.
+    d.c_fit[c] = d.fitness[r];
+    d.c_wrong[c] = d.wrong[r];
+    d.c_nodes[c] = d.nodes[r];
+  } else {
+    unsigned n = d.grammar_sites ? 0u : unsigned(d.fixed_nodes);
+    if (d.grammar_sites) { for (int i = 0; i < child.n; ++i) n += child.site[i] >= 0; }
+    else n += unsigned(child.n);
+    d.c_nodes[c] = n;
+    d.pending[atomicAdd(d.n_pending, 1)] = c;
+  }
+}
+
+// ---- evaluation: one wavefront per program ----
+__device__ inline double wave_sum_d(double x) {
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+  return x;
+}
+__device__ inline unsigned wave_sum_u(unsigned x) {
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+  return x;
+}
+struct EvalParams {
+  CaseData cd;
+  int fuel, lanes;
+  double penalty;
+  Policy policy;
+};
+struct EvalOut { double fitness; unsigned wrong, errors, timeouts; unsigned long long fuel; };
+
+__device__ inline double lane_score(const EvalParams& e, const LaneResult& r, int c, bool& exact) {
+  if (r.err != ERR_NONE) { exact = false; return -error_ceiling(e.policy, e.penalty); }
+  Val actual{};
+  actual.i = r.v; actual.tag = r.t; actual.b = r.b;
+  Val expected{};
+  expected.i = e.cd.exp_val[c]; expected.tag = e.cd.exp_tag[c]; expected.b = e.cd.exp_b[c];
+  return case_score(actual, expected, e.penalty, e.policy, exact);
+}
+
+// Scores one program over all cases with the canonical reduction
+// (fitness_cpu.cpp:97-171: L logical lanes over contiguous case chunks,
+// 48-bit rounding after every addition, 32-lane groups then the total).
+// When every case score is an integer of magnitude <= 2^36 (and C <= 4096)
+// every partial sum is an exact integer below 2^48, the rounding is the
+// identity and any summation order gives the canonical total.
+__device__ EvalOut eval_program(const EvalParams& e, const ProgView& pv, WaveStack& st, int lane) {
+  const int C = e.cd.C;
+  double isum = 0;
+  unsigned w = 0, errs = 0, touts = 0;
+  unsigned long long fuel = 0;
+  bool integral = C <= 4096;
+  for (int tile = 0; tile < C; tile += 64) {
+    const int c = tile + lane;
+    const bool active = c < C;
+    const LaneResult r = vm_run(pv, e.cd, active ? c : 0, active, e.fuel, st, lane);
+    if (active) {
+      bool exact = false;
+      const double s = lane_score(e, r, c, exact);
+      w += !exact;
+      errs += r.err != ERR_NONE;
+      touts += r.err == ERR_TIMEOUT;
+      fuel += unsigned(r.fuel_used);
+      integral = integral && s == double(std::int64_t(s)) && s >= -68719476736.0 && s <= 68719476736.0;
+      isum += s;
+    }
+  }
+  EvalOut out;
+  out.wrong = wave_sum_u(w);
+  out.errors = wave_sum_u(errs);
+  out.timeouts = wave_sum_u(touts);
+  for (int off = 32; off > 0; off >>= 1) fuel += __shfl_xor(fuel, off, 64);
+  out.fuel = fuel;
+  if (__ballot(!integral) == 0) {
+    out.fitness = wave_sum_d(isum);
+    return out;
+  }
+  // Canonical order: lane j owns logical lanes j, j+64, ... of L.
+  const int L = e.lanes;
+  double group_total = 0;
+  for (int base = 0; base < L; base += 64) {
+    const int l = base + lane;
+    const bool own = l < L;
+    const int begin = own ? int((std::int64_t(C) * l) / L) : 0;
+    const int end = own ? int((std::int64_t(C) * (l + 1)) / L) : 0;
+    int longest = end - begin;
+    for (int off = 32; off > 0; off >>= 1) { const int y = __shfl_xor(longest, off, 64); longest = y > longest ? y : longest; }
+    double local = 0;
+    for (int s = 0; s < longest; ++s) {
+      const int c = begin + s;
+      const bool active = c < end;
+      const LaneResult r = vm_run(pv, e.cd, active ? c : 0, active, e.fuel, st, lane);
+      if (active) {
+        bool exact = false;
+        local = canon48(local + lane_score(e, r, c, exact));
+      }
+    }
+    // Two 32-lane groups per 64 logical lanes, summed in lane order.
+    for (int half = 0; half < 2; ++half) {
+      if (base + half * 32 >= L) break;
+      double group = 0;
+      for (int k = 0; k < 32; ++k) {
+        const int src = half * 32 + k;
+        const double v = __shfl(local, src, 64);
+        if (base + src < L) group = canon48(group + v);
+      }
+      if (e.policy.metric == METRIC_LEGACY) {
+        // Legacy sums every lane partial in order, with no group stage.
+        for (int k = 0; k < 32; ++k) {
+          const int src = half * 32 + k;
+          const double v = __shfl(local, src, 64);
+          if (base + src < L) group_total = canon48(group_total + v);
+        }
+      } else {
+        group_total = canon48(group_total + group);
+      }
+    }
+  }
+  out.fitness = group_total;
+  return out;
+}
+
+__device__ inline WaveStack stack_of(unsigned char* smem, int slots, int wave) {
+  unsigned char* base = smem + std::size_t(wave) * slots * 64 * 10;
+  return WaveStack{reinterpret_cast<std::int64_t*>(base), base + std::size_t(slots) * 64 * 8,
+                   base + std::size_t(slots) * 64 * 9, slots};
+}
+
+// mode 0: score members 0..P-1 (initial population), mode 1: pending children.
+__global__ __launch_bounds__(kEvalThreads) void k_eval(DSearch* all, int mode) {
+  const DSearch& d = all[blockIdx.y];
+  if (mode == 1 ? !search_live(d) : (!d.active || *d.error)) return;
+  extern __shared__ unsigned char smem[];
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  WaveStack st = stack_of(smem, d.stack_slots, wave);
+  const EvalParams e{d.cd, d.fuel, d.lanes, d.penalty, d.policy};
+  const int count = mode == 1 ? *d.n_pending : d.P;
+  for (int i = blockIdx.x * kWavesPerBlock + wave; i < count; i += gridDim.x * kWavesPerBlock) {
+    const int slot = mode == 1 ? d.pending[i] : i;
+    const ProgView pv{d.code + std::size_t(slot) * d.code_cap, d.pool + std::size_t(slot) * d.pool_cap, d.code_len[slot],
+                      d.n_locals, d.code_jumps[slot]};
+    const EvalOut r = eval_program(e, pv, st, lane);
+    if (lane == 0) {
+      // Ranking rounds every published fitness to a 40-bit mantissa.
+      if (mode == 1) { d.c_fit[slot] = canon40(r.fitness); d.c_wrong[slot] = r.wrong; }
+      else { d.fitness[slot] = canon40(r.fitness); d.wrong[slot] = r.wrong; }
+      atomicAdd(&d.counters->programs, 1ull);
+      atomicAdd(&d.counters->program_cases, (unsigned long long)e.cd.C);
+      atomicAdd(&d.counters->errors, (unsigned long long)r.errors);
+      atomicAdd(&d.counters->timeouts, (unsigned long long)r.timeouts);
+      atomicAdd(&d.counters->vm_instructions, r.fuel);
+      atomicAdd(&d.counters->code_words, (unsigned long long)pv.len);
+    }
+  }
+}
+
+// ---- std::mt19937_64 on the device, generated 312 outputs at a time ----
+__device__ inline std::uint64_t mt_temper(std::uint64_t y) {
+  y ^= (y >> 29) & 0x5555555555555555ULL;
+  y ^= (y << 17) & 0x71D67FFFEDA60000ULL;
+  y ^= (y << 37) & 0xFFF7EEE000000000ULL;
+  return y ^ (y >> 43);
+}
+__device__ inline std::uint64_t mt_next_word(std::uint64_t cur, std::uint64_t nxt, std::uint64_t far) {
+  const std::uint64_t x = (cur & 0xFFFFFFFF80000000ULL) | (nxt & 0x7FFFFFFFULL);
+  return far ^ (x >> 1) ^ ((x & 1ULL) ? 0xB5026F5AA96619E9ULL : 0ULL);
+}
+// In-place twist by the whole block: element i < 156 reads only old words;
+// 156 <= i < 311 reads the new word i-156; 311 reads new words 0 and 155.
+__device__ inline void mt_twist(std::uint64_t* mt) {
+  std::uint64_t v[2];
+  int n = 0;
+  for (int i = threadIdx.x; i < 156; i += blockDim.x) v[n++] = mt_next_word(mt[i], mt[i + 1], mt[i + 156]);
+  __syncthreads();
+  n = 0;
+  for (int i = threadIdx.x; i < 156; i += blockDim.x) mt[i] = v[n++];
+  __syncthreads();
+  n = 0;
+  for (int i = 156 + threadIdx.x; i < 311; i += blockDim.x) v[n++] = mt_next_word(mt[i], mt[i + 1], mt[i - 156]);
+  __syncthreads();
+  n = 0;
+  for (int i = 156 + threadIdx.x; i < 311; i += blockDim.x) mt[i] = v[n++];
+  __syncthreads();
+  if (threadIdx.x == 0) mt[311] = mt_next_word(mt[311], mt[0], mt[155]);
+  __syncthreads();
+}
+
+__global__ void k_seed(DSearch* all) {
+  const DSearch& d = all[blockIdx.x];
+  if (!d.active) return;
+  __shared__ std::uint64_t mt[312];
+  if (threadIdx.x == 0) {
+    mt[0] = d.seed;
+    for (int i = 1; i < 312; ++i) mt[i] = 6364136223846793005ULL * (mt[i - 1] ^ (mt[i - 1] >> 62)) + std::uint64_t(i);
+  }
+  __syncthreads();
+  mt_twist(mt);
+  for (int i = threadIdx.x; i < 312; i += blockDim.x) d.mt[i] = mt[i];
+  if (threadIdx.x == 0) {
+    *d.round_seed = mt_temper(mt[0]);
+    *d.mt_index = 1;
+    *d.round = 0;
+  }
+}
+
+// Node counts, best member, placements reset (after the initial evaluation).
+__global__ void k_init_keys(DSearch* all) {
+  const DSearch& d = all[blockIdx.x];
+  if (!d.active || *d.error) return;
+  for (int m = threadIdx.x; m < d.P; m += blockDim.x) {
+    const CRow r = member_row(d, m);
+    unsigned n = d.grammar_sites ? 0u : unsigned(d.fixed_nodes);
+    if (d.grammar_sites) { for (int i = 0; i < r.n; ++i) n += r.site[i] >= 0; }
+    else n += unsigned(r.n);
+    d.nodes[m] = n;
+  }
+  __syncthreads();
+  __shared__ int best[256];
+  const Keys keys{d.fitness, d.wrong, d.nodes, d.has_wrong};
+  int b = -1;
+  for (int m = threadIdx.x; m < d.P; m += blockDim.x)
+    if (b < 0 || better(keys, m, b)) b = m;
+  best[threadIdx.x] = b;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int x = -1;
+    for (int t = 0; t < int(blockDim.x); ++t)
+      if (best[t] >= 0 && (x < 0 || better(keys, best[t], x))) x = best[t];
+    *d.best = x;
+  }
+}
+
+// ---- history (evolve.cpp:364-372) ----
+__global__ void k_record(DSearch* all) {
+  const DSearch& d = all[blockIdx.x];
+  if (!search_live(d)) return;
+  const int r = *d.round;
+  if (r % d.per_epoch) return;
+  const int gen = r / d.per_epoch;
+  const int b = *d.best;
+  const CRow row = member_row(d, b);
+  const std::size_t o = std::size_t(gen) * d.R;
+  for (int i = threadIdx.x; i < row.n; i += blockDim.x) d.hist_origin[o + i] = row.origin[i];
+  for (int i = threadIdx.x; i < row.nc; i += blockDim.x) d.hist_cval[o + i] = row.cval[i];
+  for (int m = threadIdx.x; m < d.P; m += blockDim.x) d.hist_all[std::size_t(gen) * d.P + m] = d.fitness[m];
+  if (threadIdx.x == 0) {
+    d.hist_n[gen] = row.n;
+    d.hist_nc[gen] = row.nc;
+    d.hist_fit[gen] = d.fitness[b];
+    d.hist_wrong[gen] = d.wrong[b];
+  }
+}
+
+// ---- placement (evolve.cpp:378-422) ----
+__global__ __launch_bounds__(256) void k_place(DSearch* all) {
+  const DSearch& d = all[blockIdx.x];
+  if (!search_live(d)) return;
+  __shared__ std::uint64_t mt[312];
+  __shared__ std::uint64_t raw[640];
+  __shared__ int head, tail, state, b, best;
+  for (int i = threadIdx.x; i < 312; i += blockDim.x) mt[i] = d.mt[i];
+  __syncthreads();
+  const int idx = *d.mt_index;
+  for (int i = idx + threadIdx.x; i < 312; i += blockDim.x) raw[i - idx] = mt_temper(mt[i]);
+  if (threadIdx.x == 0) { head = 0; tail = 312 - idx; state = 0; b = 0; best = *d.best; }
+  __syncthreads();
+  const Keys keys{d.fitness, d.wrong, d.nodes, d.has_wrong};
+  const std::uint64_t P = std::uint64_t(d.P);
+  for (;;) {
+    if (threadIdx.x == 0) {
+      state = 1;  // needs more draws unless finished below
+      int h = head, bb = b, bst = best;
+      const int rs = d.replacement_size;
+      for (;;) {
+        if (bb < d.B) {
+          if (tail - h < rs) break;
+          int worst = int(raw[h] % P);
+          for (int i = 1; i < rs; ++i) {
+            const int cand = int(raw[h + i] % P);
+            if (better(keys, worst, cand)) worst = cand;
+          }
+          h += rs;
+          if (worst == bst) continue;  // the best member is never replaced: redraw
+          d.fitness[worst] = d.c_fit[bb];
+          d.wrong[worst] = d.c_wrong[bb];
+          d.nodes[worst] = d.c_nodes[bb];
+          d.placed[worst] = bb;
+          if (better(keys, worst, bst)) bst = worst;
+          ++bb;
+        } else {
+          if (tail - h < 1) break;
+          *d.round_seed = raw[h++];
+          state = 2;
+          break;
+        }
+      }
+      head = h; b = bb; best = bst;
+    }
+    __syncthreads();
+    if (state == 2) break;
+    // Keep the unconsumed tail, then append the next 312 outputs.
+    const int left = tail - head;
+    std::uint64_t keep = 0;
+    if (int(threadIdx.x) < left) keep = raw[head + threadIdx.x];
+    __syncthreads();
+    if (int(threadIdx.x) < left) raw[threadIdx.x] = keep;
+    mt_twist(mt);
+    for (int i = threadIdx.x; i < 312; i += blockDim.x) raw[left + i] = mt_temper(mt[i]);
+    if (threadIdx.x == 0) { head = 0; tail = left + 312; }
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < 312; i += blockDim.x) d.mt[i] = mt[i];
+  if (threadIdx.x == 0) {
+    *d.mt_index = 312 - (tail - head);
+    *d.best = best;
+    *d.n_pending = 0;
+    atomicAdd(d.round, 1);
+  }
+}
+
+__global__ void k_commit(DSearch* all) {
+  const DSearch& d = all[blockIdx.y];
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (!d.active || m >= d.P) return;
+  const int c = d.placed[m];
+  if (c < 0) return;
+  const CRow src = child_row(d, c);
+  const std::size_t o = std::size_t(m) * d.R;
+  Row dst{d.cur_origin + o, d.cur_site + o, d.cur_cval + o, 0, 0};
+  copy_row(src, dst, d.cur_n + m, d.cur_nc + m);
+  d.placed[m] = -1;
+}
+
+// ---- evaluation session kernel (external bytecode) ----
+struct DProg { const Ins* code; const Val* pool; int len, has_jumps, n_locals; };
+__global__ __launch_bounds__(kEvalThreads) void k_eval_programs(const DProg* progs, int n, EvalParams e, int slots,
+                                                                 double* fitness, unsigned* wrong, unsigned* counts) {
+  extern __shared__ unsigned char smem[];
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  WaveStack st = stack_of(smem, slots, wave);
+  for (int i = blockIdx.x * kWavesPerBlock + wave; i < n; i += gridDim.x * kWavesPerBlock) {
+    const DProg p = progs[i];
+    const ProgView pv{p.code, p.pool, p.len, p.n_locals, p.has_jumps};
+    const EvalOut r = eval_program(e, pv, st, lane);
+    if (lane == 0) {
+      fitness[i] = r.fitness;
+      wrong[i] = e.policy.metric == METRIC_LEGACY ? 0 : r.wrong;
+      counts[3 * i + 0] = unsigned(e.cd.C);
+      counts[3 * i + 1] = r.errors;
+      counts[3 * i + 2] = r.timeouts;
+    }
+  }
+}
+__global__ __launch_bounds__(kEvalThreads) void k_eval_cases(DProg p, CaseData cd, int fuel, int slots, Val* values, int* errors) {
+  extern __shared__ unsigned char smem[];
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  WaveStack st = stack_of(smem, slots, wave);
+  const ProgView pv{p.code, p.pool, p.len, p.n_locals, p.has_jumps};
+  for (int tile = (blockIdx.x * kWavesPerBlock + wave) * 64; tile < cd.C; tile += gridDim.x * kWavesPerBlock * 64) {
+    const int c = tile + lane;
+    const bool active = c < cd.C;
+    const LaneResult r = vm_run(pv, cd, active ? c : 0, active, fuel, st, lane);
+    if (active) {
+      Val v{};
+      v.i = r.v; v.tag = r.t; v.b = r.b;
+      values[c] = v;
+      errors[c] = r.err;
+    }
+  }
+}
+
+// ---- device buffers ----
+struct DeviceArena {
+  std::vector<void*> blocks;
+  std::size_t bytes = 0;
+  template <class T>
+  T* alloc(std::size_t n) {
+    void* p = nullptr;
+    const std::size_t sz = std::max<std::size_t>(n, 1) * sizeof(T);
+    HIP_OK(hipMalloc(&p, sz));
+    blocks.push_back(p);
+    bytes += sz;
+    return static_cast<T*>(p);
+  }
+  template <class T>
+  T* upload(const std::vector<T>& v) {
+    T* p = alloc<T>(v.size());
+    if (!v.empty()) HIP_OK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return p;
+  }
+  ~DeviceArena() { for (void* p : blocks) (void)hipFree(p); }
+};
+
+struct DeviceCases {
+  CaseData cd{};
+  void upload(DeviceArena& a, const Cases& c) {
+    const int C = c.n, K = int(c.input_names.size());
+    std::vector<std::int64_t> v(std::size_t(K) * C);
+    std::vector<std::uint8_t> t(std::size_t(K) * C, TAG_INVALID), b(std::size_t(K) * C, 0);
+    for (int i = 0; i < C; ++i)
+      for (int k = 0; k < K; ++k) {
+        const std::size_t src = std::size_t(i) * K + k, dst = std::size_t(k) * C + i;
+        if (!c.set[src]) continue;
+        v[dst] = c.inputs[src].i;
+        t[dst] = c.inputs[src].tag;
+        b[dst] = c.inputs[src].b;
+      }
+    std::vector<std::int64_t> ev(C);
+    std::vector<std::uint8_t> et(C), eb(C);
+    for (int i = 0; i < C; ++i) { ev[i] = c.expected[i].i; et[i] = c.expected[i].tag; eb[i] = c.expected[i].b; }
+    std::vector<std::int64_t> arena = c.arena;
+    if (arena.empty()) arena.push_back(0);
+    cd.in_val = a.upload(v);
+    cd.in_tag = a.upload(t);
+    cd.in_b = a.upload(b);
+    cd.exp_val = a.upload(ev);
+    cd.exp_tag = a.upload(et);
+    cd.exp_b = a.upload(eb);
+    cd.arena = a.upload(arena);
+    cd.C = C;
+    cd.n_inputs = K;
+  }
+};
+
+int next_pow2(int x) { int p = 1; while (p < x) p <<= 1; return p; }
+
+}  // namespace
+
+
+// ====================== host side ======================
+
+// One admitted search: host documents, tables and the device state it owns.
+struct SearchState {
+  SearchSpec spec;
+  Task task;
+  Grammar grammar;
+  Cases train;
+  SearchConfig cfg;
+  Layout layout;
+  PhaseGrammar phase;
+  int P = 0, B = 0, R = 0, G = 0, rounds = 0, per_epoch = 0;
+  int fuel = 0;
+  double penalty = 0;
+  DeviceArena arena;
+  DSearch d{};
+  double init_ms = 0;
+
+  explicit SearchState(const SearchSpec& s) : spec(s) {
+    task = load_task(s.task_path);
+    grammar = load_grammar_file(task.grammar_path);
+    train = load_cases(task.train_path);
+    cfg = default_search_config();
+    if (!s.search_path.empty()) {
+      std::ifstream in(s.search_path);
+      if (!in) throw std::invalid_argument("search config: cannot read " + s.search_path);
+      std::ostringstream text;
+      text << in.rdbuf();
+      apply_search(cfg, json::parse(text.str()));
+    }
+    if (s.population) cfg.population = *s.population;
+    if (s.generations) cfg.generations = *s.generations;
+    if (s.seed) cfg.seed = *s.seed;
+    if (!cfg.resident) throw Unsupported("native_ast representation (host reproduction)");
+    if (cfg.schedule != Schedule::SteadyState) throw Unsupported("generational schedule");
+    if (cfg.site_depth < 1) throw Unsupported("site_depth 0 (initial population from the native generator)");
+    if (cfg.population < 2 || cfg.population > 16384) throw Unsupported("population outside 2..16384");
+    if (cfg.elitism < 0 || cfg.elitism >= cfg.population) throw std::invalid_argument("elite_count must be >= 0 and < population_size");
+    P = cfg.population;
+    B = cfg.batch ? cfg.batch : P;
+    if (B > P || P % B) throw std::invalid_argument("search config: steady_state batch must divide the population");
+    G = cfg.generations;
+    rounds = int((std::int64_t(P) * G) / B);
+    per_epoch = P / B;
+    penalty = cfg.penalty.value_or(task.case_cap);
+    fuel = cfg.fuel.value_or(int(grammar.fuel));
+    // Inputs must exactly cover the grammar inputs.
+    {
+      std::vector<std::string> want;
+      for (const auto& in : grammar.inputs) want.push_back(in.name);
+      std::sort(want.begin(), want.end());
+      if (want != train.input_names) throw std::invalid_argument("every fitness case must supply every compiled grammar input");
+      for (std::size_t k = 0; k < train.input_names.size(); ++k)
+        for (const auto& in : grammar.inputs)
+          if (in.name == train.input_names[k] && in.type != train.input_types[k])
+            throw std::invalid_argument("case input type disagrees with the grammar");
+    }
+    layout = whole_program_layout(grammar);
+    phase = compile_phase(grammar, grammar.entry, grammar.max_depth, unsigned(cfg.site_depth), train.input_names);
+    R = std::min<int>(kPhaseCapacity, int(layout.max_variable_nodes));
+  }
+
+  void allocate() {
+    auto& a = arena;
+    DSearch x{};
+    x.active = 1;
+    x.P = P; x.B = B; x.R = R;
+    x.code_cap = 4 * R + 8;
+    x.pool_cap = 2 * R + 8;
+    x.stack_slots = std::min(kMaxStack, 2 * int(layout.slot_max_depth) + 2);
+    x.rounds = rounds; x.per_epoch = per_epoch;
+    x.fuel = fuel; x.lanes = 512; x.penalty = penalty; x.policy = task.policy;
+    x.pressure = std::clamp(cfg.tournament, 1, P);
+    x.replacement_size = cfg.replacement_size;
+    x.p_mut = cfg.mutation_probability;
+    x.pol = OpPolicy{cfg.counted, cfg.uniform_phase, cfg.crossover_probability, cfg.crossover_attempts, cfg.mutation_max_nodes};
+    x.slot = SlotInfo{phase.root, 0, int(layout.slot_max_depth), cfg.site_depth, int(layout.max_variable_nodes), R};
+    x.fixed_nodes = int(layout.fixed_nodes);
+    x.grammar_sites = cfg.grammar_sites;
+    x.has_wrong = task.policy.metric != METRIC_LEGACY;
+    x.expected_type = grammar.nts[grammar.entry].type;
+    x.n_locals = int(train.input_names.size());
+    x.site_depth = cfg.site_depth;
+    x.seed = cfg.seed;
+    x.g = PhaseView{a.upload(phase.exprs), int(phase.exprs.size()), a.upload(phase.prods), int(phase.prods.size()),
+                    a.upload(phase.nts), int(phase.nts.size()), a.upload(phase.domains), int(phase.domains.size()),
+                    a.upload(phase.values), int(phase.values.size()), a.upload(phase.minimum), phase.depth_count,
+                    phase.logical_count, phase.site_depth ? 1 : 0, phase.root};
+    x.local_types = a.upload(train.input_types);
+    DeviceCases dc;
+    dc.upload(a, train);
+    x.cd = dc.cd;
+    const std::size_t rows = std::size_t(P) * R, crows = std::size_t(B) * R;
+    x.cur_origin = a.alloc<int>(rows); x.cur_site = a.alloc<int>(rows); x.cur_cval = a.alloc<Val>(rows);
+    x.cur_n = a.alloc<int>(P); x.cur_nc = a.alloc<int>(P);
+    x.x_origin = a.alloc<int>(crows); x.x_site = a.alloc<int>(crows); x.x_cval = a.alloc<Val>(crows);
+    x.x_n = a.alloc<int>(B); x.x_nc = a.alloc<int>(B);
+    x.t_origin = a.alloc<int>(crows); x.t_site = a.alloc<int>(crows); x.t_cval = a.alloc<Val>(crows);
+    x.s_origin = a.alloc<int>(crows); x.s_site = a.alloc<int>(crows); x.s_cval = a.alloc<Val>(crows);
+    x.fitness = a.alloc<double>(P); x.wrong = a.alloc<unsigned>(P); x.nodes = a.alloc<unsigned>(P);
+    x.stats = a.alloc<OpStats>(B); x.reuse = a.alloc<int>(B);
+    x.c_fit = a.alloc<double>(B); x.c_wrong = a.alloc<unsigned>(B); x.c_nodes = a.alloc<unsigned>(B);
+    const int code_rows = std::max(P, B);
+    x.code = a.alloc<Ins>(std::size_t(code_rows) * x.code_cap);
+    x.pool = a.alloc<Val>(std::size_t(code_rows) * x.pool_cap);
+    x.code_len = a.alloc<int>(code_rows); x.code_jumps = a.alloc<int>(code_rows);
+    x.child_hash = a.alloc<std::uint64_t>(B);
+    x.pending = a.alloc<int>(B); x.n_pending = a.alloc<int>(1);
+    const int tsize = next_pow2(2 * P);
+    x.table = a.alloc<int>(tsize); x.table_mask = tsize - 1;
+    x.mt = a.alloc<std::uint64_t>(312); x.mt_index = a.alloc<int>(1); x.round_seed = a.alloc<std::uint64_t>(1);
+    x.round = a.alloc<int>(1); x.best = a.alloc<int>(1); x.placed = a.alloc<int>(P);
+    x.hist_origin = a.alloc<int>(std::size_t(G) * R); x.hist_cval = a.alloc<Val>(std::size_t(G) * R);
+    x.hist_n = a.alloc<int>(G); x.hist_nc = a.alloc<int>(G);
+    x.hist_fit = a.alloc<double>(G); x.hist_wrong = a.alloc<unsigned>(G);
+    x.hist_all = a.alloc<double>(std::size_t(G) * P);
+    x.counters = a.alloc<Counters>(1);
+    x.error = a.alloc<int>(1);
+    HIP_OK(hipMemset(x.n_pending, 0, sizeof(int)));
+    HIP_OK(hipMemset(x.counters, 0, sizeof(Counters)));
+    HIP_OK(hipMemset(x.error, 0, sizeof(int)));
+    HIP_OK(hipMemset(x.round, 0, sizeof(int)));
+    d = x;
+  }
+
+  template <class T>
+  std::vector<T> down(const T* p, std::size_t n) const {
+    std::vector<T> v(n);
+    if (n) HIP_OK(hipMemcpy(v.data(), p, n * sizeof(T), hipMemcpyDeviceToHost));
+    return v;
+  }
+  int device_error() const { return down(d.error, 1)[0]; }
+};
+
+struct KernelTimer {
+  bool on = false;
+  std::map<std::string, double> ms;
+  std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  template <class F>
+  void time(const char* name, hipStream_t s, F&& launch) {
+    if (!on) { launch(); return; }
+    hipEvent_t a, b;
+    HIP_OK(hipEventCreate(&a));
+    HIP_OK(hipEventCreate(&b));
+    HIP_OK(hipEventRecord(a, s));
+    launch();
+    HIP_OK(hipEventRecord(b, s));
+    pending.push_back({name, {a, b}});
+  }
+  void collect() {
+    for (auto& p : pending) {
+      HIP_OK(hipEventSynchronize(p.second.second));
+      float t = 0;
+      HIP_OK(hipEventElapsedTime(&t, p.second.first, p.second.second));
+      ms[p.first] += t;
+      (void)hipEventDestroy(p.second.first);
+      (void)hipEventDestroy(p.second.second);
+    }
+    pending.clear();
+  }
+};
+
+struct Batch::Impl {
+  std::vector<std::unique_ptr<SearchState>> searches;
+  DSearch* d_all = nullptr;
+  hipStream_t stream = nullptr;
+  KernelTimer timer;
+  int max_P = 0, max_B = 0, max_slots = 0, max_table = 0, max_rounds = 0;
+  double run_ms = 0, init_ms = 0;
+  bool graph_ready = false;
+  hipGraphExec_t graph_exec = nullptr;
+  ~Impl() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (d_all) (void)hipFree(d_all);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  std::size_t lds_bytes() const { return std::size_t(kWavesPerBlock) * max_slots * 64 * 10; }
+
+  void enqueue_round() {
+    const int n = int(searches.size());
+    const dim3 child_grid((max_B + 127) / 128, n), member_grid((max_P + 127) / 128, n), table_grid((max_table + 255) / 256, n);
+    const dim3 eval_grid(std::min(4096, (max_B + kWavesPerBlock - 1) / kWavesPerBlock), n);
+    timer.time("k_record", stream, [&] { hipLaunchKernelGGL(k_record, dim3(n), dim3(256), 0, stream, d_all); });
+    timer.time("k_vary", stream, [&] { hipLaunchKernelGGL(k_vary, child_grid, dim3(128), 0, stream, d_all); });
+    timer.time("k_clear", stream, [&] { hipLaunchKernelGGL(k_clear, table_grid, dim3(256), 0, stream, d_all); });
+    timer.time("k_index", stream, [&] { hipLaunchKernelGGL(k_index, member_grid, dim3(128), 0, stream, d_all); });
+    timer.time("k_reuse", stream, [&] { hipLaunchKernelGGL(k_reuse, child_grid, dim3(128), 0, stream, d_all); });
+    timer.time("k_eval", stream, [&] { hipLaunchKernelGGL(k_eval, eval_grid, dim3(kEvalThreads), lds_bytes(), stream, d_all, 1); });
+    timer.time("k_place", stream, [&] { hipLaunchKernelGGL(k_place, dim3(n), dim3(256), 0, stream, d_all); });
+    timer.time("k_commit", stream, [&] { hipLaunchKernelGGL(k_commit, member_grid, dim3(128), 0, stream, d_all); });
+  }
+};
+
+Batch::Batch() : impl_(std::make_unique<Impl>()) {}
+Batch::~Batch() = default;
+std::size_t Batch::size() const { return impl_->searches.size(); }
+void Batch::set_profile(bool on) { impl_->timer.on = on; }
+int Batch::max_rounds() const { return impl_->max_rounds; }
+std::size_t Batch::device_bytes() const {
+  std::size_t b = 0;
+  for (const auto& s : impl_->searches) b += s->arena.bytes;
+  return b;
+}
+std::vector<std::pair<std::string, double>> Batch::kernel_times() const {
+  return {impl_->timer.ms.begin(), impl_->timer.ms.end()};
+}
+
+int Batch::add(const SearchSpec& spec) {
+  auto s = std::make_unique<SearchState>(spec);
+  impl_->searches.push_back(std::move(s));
+  return int(impl_->searches.size() - 1);
+}
+
+void Batch::prepare() {
+  auto& I = *impl_;
+  const auto t0 = std::chrono::steady_clock::now();
+  if (!I.stream) HIP_OK(hipStreamCreateWithFlags(&I.stream, hipStreamNonBlocking));
+  std::vector<DSearch> host;
+  for (auto& s : I.searches) {
+    s->allocate();
+    host.push_back(s->d);
+    I.max_P = std::max(I.max_P, s->P);
+    I.max_B = std::max(I.max_B, s->B);
+    I.max_slots = std::max(I.max_slots, s->d.stack_slots);
+    I.max_table = std::max(I.max_table, s->d.table_mask + 1);
+    I.max_rounds = std::max(I.max_rounds, s->rounds);
+  }
+  const int n = int(host.size());
+  if (!n) return;
+  HIP_OK(hipMalloc(&I.d_all, sizeof(DSearch) * host.size()));
+  HIP_OK(hipMemcpy(I.d_all, host.data(), sizeof(DSearch) * host.size(), hipMemcpyHostToDevice));
+  if (I.lds_bytes() > 65536) {
+    HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_eval), hipFuncAttributeMaxDynamicSharedMemorySize, int(I.lds_bytes())));
+  }
+  // Initial population, its evaluation, keys and the run generator.
+  const dim3 member_grid((I.max_P + 63) / 64, n);
+  I.timer.time("k_init_pop", I.stream, [&] { hipLaunchKernelGGL(k_init_pop, member_grid, dim3(64), 0, I.stream, I.d_all); });
+  I.timer.time("k_lower_members", I.stream, [&] { hipLaunchKernelGGL(k_lower_members, member_grid, dim3(64), 0, I.stream, I.d_all); });
+  const dim3 eval_grid(std::min(4096, (I.max_P + kWavesPerBlock - 1) / kWavesPerBlock), n);
+  I.timer.time("k_eval_init", I.stream, [&] { hipLaunchKernelGGL(k_eval, eval_grid, dim3(kEvalThreads), I.lds_bytes(), I.stream, I.d_all, 0); });
+  I.timer.time("k_init_keys", I.stream, [&] { hipLaunchKernelGGL(k_init_keys, dim3(n), dim3(256), 0, I.stream, I.d_all); });
+  I.timer.time("k_seed", I.stream, [&] { hipLaunchKernelGGL(k_seed, dim3(n), dim3(256), 0, I.stream, I.d_all); });
+  HIP_OK(hipStreamSynchronize(I.stream));
+  I.timer.collect();
+  I.init_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  for (auto& s : I.searches) {
+    if (const int e = s->device_error()) throw std::runtime_error("device initialization error " + std::to_string(e));
+  }
+}
+
+void Batch::step(int rounds, bool use_graph) {
+  auto& I = *impl_;
+  if (I.searches.empty() || rounds <= 0) return;
+  const auto t0 = std::chrono::steady_clock::now();
+  if (use_graph && !I.timer.on) {
+    if (!I.graph_ready) {
+      hipGraph_t graph;
+      HIP_OK(hipStreamBeginCapture(I.stream, hipStreamCaptureModeThreadLocal));
+      I.enqueue_round();
+      HIP_OK(hipStreamEndCapture(I.stream, &graph));
+      HIP_OK(hipGraphInstantiate(&I.graph_exec, graph, nullptr, nullptr, 0));
+      (void)hipGraphDestroy(graph);
+      I.graph_ready = true;
+    }
+    for (int r = 0; r < rounds; ++r) HIP_OK(hipGraphLaunch(I.graph_exec, I.stream));
+    HIP_OK(hipStreamSynchronize(I.stream));
+  } else {
+    for (int r = 0; r < rounds; ++r) {
+      I.enqueue_round();
+      if (I.timer.on && (r % 8 == 7)) I.timer.collect();
+    }
+    HIP_OK(hipStreamSynchronize(I.stream));
+    I.timer.collect();
+  }
+  I.run_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void Batch::finish() {
+  for (auto& s : impl_->searches) {
+    if (const int e = s->device_error()) throw std::runtime_error("device search error " + std::to_string(e));
+  }
+}
+
+void Batch::run(bool use_graph) {
+  prepare();
+  step(impl_->max_rounds, use_graph);
+  finish();
+}
+
+std::vector<std::uint64_t> Batch::counters(int index) const {
+  const SearchState& s = *impl_->searches.at(std::size_t(index));
+  const Counters c = s.down(s.d.counters, 1)[0];
+  return {c.programs, c.program_cases, c.errors, c.timeouts, c.vm_instructions, c.code_words, c.const_words};
+}
+double Batch::init_ms() const { return impl_->init_ms; }
+
+namespace {
+Ast export_row(const SearchState& s, const int* origin, const Val* cval, int n, int nc) {
+  return export_whole_program(s.grammar, s.phase, std::vector<int>(origin, origin + n), std::vector<Val>(cval, cval + nc));
+}
+
+// The run's success criterion on held-out cases (cpp/src/cli/run.cpp:179-239).
+json::Value judge(const SearchState& s, const Ast& ast, const std::vector<int>& origin, const std::vector<Val>& cval) {
+  const Cases test = load_cases(s.task.test_path);
+  // Lower the best phase exactly as the engine does, then score on the device.
+  std::vector<Ins> code(std::size_t(s.d.code_cap));
+  std::vector<Val> pool(std::size_t(s.d.pool_cap));
+  std::vector<int> sites(origin.size(), -1);
+  PhaseView gv{s.phase.exprs.data(), int(s.phase.exprs.size()), s.phase.prods.data(), int(s.phase.prods.size()),
+               s.phase.nts.data(), int(s.phase.nts.size()), s.phase.domains.data(), int(s.phase.domains.size()),
+               s.phase.values.data(), int(s.phase.values.size()), s.phase.minimum.data(), s.phase.depth_count,
+               s.phase.logical_count, s.phase.site_depth ? 1 : 0, s.phase.root};
+  const CRow row{origin.data(), sites.data(), cval.data(), int(origin.size()), int(cval.size())};
+  std::vector<std::uint8_t> local_types = s.train.input_types;
+  if (test.input_names != s.train.input_names) throw std::invalid_argument("test cases must bind the train inputs");
+  const LowerResult lr = lower_phase(gv, row, local_types.data(), int(local_types.size()), s.d.expected_type, code.data(),
+                                     int(code.size()), pool.data(), int(pool.size()));
+  if (lr.status != ST_OK) throw std::runtime_error("lowering of the best program failed");
+  ProgramIn p;
+  for (int i = 0; i < lr.code; ++i) {
+    const Ins& x = code[i];
+    p.op.push_back(x.op);
+    const bool a = x.op == OP_PUSH_CONST || x.op == OP_LOAD || x.op == OP_JMP || x.op == OP_JMP_IF_FALSE || x.op == OP_JMP_IF_TRUE ||
+                   x.op == OP_CALL_BUILTIN;
+    p.has_a.push_back(a);
+    p.has_b.push_back(x.op == OP_CALL_BUILTIN);
+    p.a.push_back(x.a);
+    p.b.push_back(x.argc);
+    p.fuel.push_back(x.fuel);
+  }
+  p.consts.assign(pool.begin(), pool.begin() + lr.constants);
+  p.n_locals = int(local_types.size());
+  Session session(test, s.fuel, 512, s.penalty, s.task.policy);
+  std::vector<Val> values;
+  std::vector<int> errors;
+  session.eval_cases(p, values, errors);
+  std::size_t correct = 0, errs = 0;
+  double loss = 0;
+  const bool absolute = s.task.policy.metric == METRIC_ABSOLUTE;
+  for (int i = 0; i < test.n; ++i) {
+    bool ok = false, error = false;
+    double l = absolute ? s.task.case_cap : 1;
+    const Val& want = test.expected[i];
+    if (errors[i] != ERR_NONE) error = true;
+    else if (want.tag == TAG_INT) {
+      if (values[i].tag != TAG_INT) error = true;
+      else {
+        ok = values[i].i == want.i;
+        if (absolute) l = std::min(s.task.case_cap, integer_gap(values[i].i, want.i));
+      }
+    } else if (want.tag == TAG_INT_LIST) {
+      error = true;  // list results are outside the engine's values
+    } else {
+      bool exact = false;
+      const Policy exact_policy{METRIC_EXACT, PROX_CAPPED, 0, 0.1, 100};
+      (void)case_score(values[i], want, 1, exact_policy, exact);
+      ok = exact;
+    }
+    if (ok) l = 0;
+    correct += ok;
+    errs += error;
+    loss += l;
+  }
+  const double n = double(test.n);
+  json::Value v = json::make_object();
+  v.obj["cases"] = json::make_number(n);
+  v.obj["correct"] = json::make_number(double(correct));
+  v.obj["wrong"] = json::make_number(n - double(correct));
+  v.obj["runtime_errors"] = json::make_number(double(errs));
+  v.obj["mean_loss"] = json::make_number(loss / n);
+  v.obj["solved"] = json::make_bool(correct == std::size_t(test.n));
+  (void)ast;
+  return v;
+}
+}  // namespace
+
+SearchResult Batch::result(int index, bool save_all) {
+  auto& I = *impl_;
+  SearchState& s = *I.searches.at(std::size_t(index));
+  const DSearch& d = s.d;
+  SearchResult out;
+  const int best = s.down(d.best, 1)[0];
+  const auto fitness = s.down(d.fitness, std::size_t(s.P));
+  const auto wrong = s.down(d.wrong, std::size_t(s.P));
+  const auto nodes = s.down(d.nodes, std::size_t(s.P));
+  const auto cur_n = s.down(d.cur_n, std::size_t(s.P));
+  const auto cur_nc = s.down(d.cur_nc, std::size_t(s.P));
+  const Counters counters = s.down(d.counters, 1)[0];
+  out.evaluated_program_cases = counters.program_cases;
+  out.evaluated_programs = counters.programs;
+  // History rows.
+  const auto hn = s.down(d.hist_n, std::size_t(s.G));
+  const auto hnc = s.down(d.hist_nc, std::size_t(s.G));
+  const auto ho = s.down(d.hist_origin, std::size_t(s.G) * s.R);
+  const auto hc = s.down(d.hist_cval, std::size_t(s.G) * s.R);
+  const auto hf = s.down(d.hist_fit, std::size_t(s.G));
+  const auto hw = s.down(d.hist_wrong, std::size_t(s.G));
+  const auto ha = s.down(d.hist_all, std::size_t(s.G) * s.P);
+  const double per_gen_ms = s.G ? I.run_ms / s.G : 0;
+  json::Value history = json::make_array();
+  for (int gen = 0; gen < s.G; ++gen) {
+    const Ast ast = export_row(s, &ho[std::size_t(gen) * s.R], &hc[std::size_t(gen) * s.R], hn[gen], hnc[gen]);
+    long double sum = 0;
+    for (int m = 0; m < s.P; ++m) sum += ha[std::size_t(gen) * s.P + m];
+    json::Value h = json::make_object();
+    h.obj["generation"] = json::make_number(gen);
+    h.obj["best_fitness"] = json::make_number(hf[gen]);
+    h.obj["mean_fitness"] = json::make_number(double(sum / s.P));
+    h.obj["best_wrong"] = json::make_number(hw[gen]);
+    h.obj["best_nodes"] = json::make_number(double(ast.nodes.size()));
+    h.obj["elapsed_ms"] = json::make_number(I.init_ms + per_gen_ms * (gen + 1));
+    h.obj["eval_ms"] = json::make_number(per_gen_ms);
+    h.obj["repro_ms"] = json::make_number(0);
+    history.arr.push_back(h);
+    if (save_all) {
+      json::Value l = json::make_object();
+      l.obj["generation"] = json::make_number(gen);
+      l.obj["fitness"] = json::make_number(hf[gen]);
+      l.obj["program"] = ast_json(ast);
+      out.lineage.push_back(l);
+    }
+  }
+  // Best member (the strict order's front after the last round).
+  const std::vector<int> bo = s.down(d.cur_origin + std::size_t(best) * s.R, std::size_t(cur_n[best]));
+  const std::vector<Val> bc = s.down(d.cur_cval + std::size_t(best) * s.R, std::size_t(cur_nc[best]));
+  const Ast best_ast = export_whole_program(s.grammar, s.phase, bo, bc);
+  json::Value bj = json::make_object();
+  bj.obj["fitness"] = json::make_number(fitness[best]);
+  bj.obj["wrong"] = json::make_number(wrong[best]);
+  bj.obj["nodes"] = json::make_number(double(best_ast.nodes.size()));
+  bj.obj["text"] = json::make_string(program_text(best_ast));
+  bj.obj["program"] = ast_json(best_ast);
+  if (save_all) {
+    const Keys keys{fitness.data(), wrong.data(), nodes.data(), d.has_wrong};
+    std::vector<int> order(std::size_t(s.P));
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return better(keys, a, b); });
+    const auto all_o = s.down(d.cur_origin, std::size_t(s.P) * s.R);
+    const auto all_c = s.down(d.cur_cval, std::size_t(s.P) * s.R);
+    for (int m : order)
+      out.population.push_back(ast_json(export_row(s, &all_o[std::size_t(m) * s.R], &all_c[std::size_t(m) * s.R], cur_n[m], cur_nc[m])));
+  }
+  json::Value r = json::make_object();
+  r.obj["format"] = json::make_string("gagp-run-v1");
+  r.obj["task"] = json::make_string(s.task.name);
+  r.obj["engine"] = json::make_string("gpu");
+  r.obj["population"] = json::make_number(s.P);
+  r.obj["generations"] = json::make_number(s.G);
+  r.obj["seed"] = json::make_string(std::to_string(s.cfg.seed));
+  r.obj["train_cases"] = json::make_number(s.train.n);
+  r.obj["search"] = effective_search_json(s.cfg, s.penalty, s.fuel);
+  r.obj["best"] = bj;
+  r.obj["history"] = history;
+  json::Value tm = json::make_object();
+  tm.obj["total"] = json::make_number(I.init_ms + I.run_ms);
+  tm.obj["search"] = json::make_number(I.init_ms + I.run_ms);
+  tm.obj["initialization"] = json::make_number(I.init_ms);
+  r.obj["time_ms"] = tm;
+  if (!s.task.test_path.empty()) r.obj["test"] = judge(s, best_ast, bo, bc);
+  out.result = r;
+  out.search_ms = I.init_ms + I.run_ms;
+  return out;
+}
+
+// ====================== evaluation session ======================
+namespace {
+// Static stack heights of external bytecode (the verifier's stack-shape
+// pass, cpp/src/runtime/bytecode_verify.cpp): every instruction gets one
+// height; joins must agree; operand/constant/jump domains are checked.
+std::vector<Ins> verify_program(const ProgramIn& p, int& max_stack, bool& jumps) {
+  const int n = int(p.op.size());
+  if (n == 0) throw std::invalid_argument("empty program");
+  if (p.a.size() != std::size_t(n) || p.b.size() != std::size_t(n) || p.has_a.size() != std::size_t(n) ||
+      p.has_b.size() != std::size_t(n) || (!p.fuel.empty() && p.fuel.size() != std::size_t(n)))
+    throw std::invalid_argument("program arrays disagree in length");
+  std::vector<int> height(std::size_t(n) + 1, -1);
+  std::vector<int> work{0};
+  height[0] = 0;
+  max_stack = 0;
+  jumps = false;
+  const auto flow = [&](int to, int h) {
+    if (to < 0 || to > n) throw std::invalid_argument("jump target out of range");
+    if (height[to] < 0) { height[to] = h; work.push_back(to); }
+    else if (height[to] != h) throw std::invalid_argument("stack heights disagree at a join");
+  };
+  while (!work.empty()) {
+    const int ip = work.back();
+    work.pop_back();
+    if (ip == n) continue;
+    const int h = height[ip];
+    const int op = p.op[ip];
+    int pop = 0, push = 0;
+    bool falls = true;
+    switch (op) {
+      case OP_PUSH_CONST:
+        if (!p.has_a[ip] || p.a[ip] < 0 || p.a[ip] >= int(p.consts.size())) throw std::invalid_argument("const index out of range");
+        push = 1; break;
+      case OP_LOAD:
+        if (!p.has_a[ip] || p.a[ip] < 0) throw std::invalid_argument("local index out of range");
+        push = 1; break;
+      case OP_NEG: case OP_NOT: case OP_CHECK_INT: case OP_CHECK_LIST: pop = 1; push = 1; break;
+      case OP_ADD: case OP_SUB: case OP_MUL: case OP_DIV: case OP_MOD:
+      case OP_LT: case OP_LE: case OP_GT: case OP_GE: case OP_EQ: case OP_NE: pop = 2; push = 1; break;
+      case OP_JMP: jumps = true; if (!p.has_a[ip]) throw std::invalid_argument("jump target missing"); flow(p.a[ip], h); falls = false; break;
+      case OP_JMP_IF_FALSE: case OP_JMP_IF_TRUE:
+        jumps = true;
+        if (!p.has_a[ip]) throw std::invalid_argument("jump target missing");
+        if (h < 1) throw std::invalid_argument("stack underflow");
+        flow(p.a[ip], h - 1);
+        pop = 1; break;
+      case OP_CALL_BUILTIN:
+        if (!p.has_b[ip] || p.b[ip] < 0 || p.b[ip] > 8) throw std::invalid_argument("invalid builtin argc");
+        pop = p.b[ip]; push = 1; break;
+      case OP_RETURN: if (h < 1) throw std::invalid_argument("return requires value on stack"); falls = false; break;
+      default: throw Unsupported("opcode outside the device VM: " + std::to_string(op));
+    }
+    if (h < pop) throw std::invalid_argument("stack underflow");
+    const int after = h - pop + push;
+    max_stack = std::max(max_stack, after);
+    if (after > kMaxStack) throw Unsupported("stack deeper than the device VM");
+    if (falls) flow(ip + 1, after);
+  }
+  std::vector<Ins> out(static_cast<std::size_t>(n));
+  for (int i = 0; i < n; ++i) {
+    const std::uint32_t f = p.fuel.empty() ? 1u : p.fuel[i];
+    if (f > 255) throw Unsupported("instruction fuel charges above 255");
+    const int h = height[i] < 0 ? 0 : height[i];  // unreachable instructions never run
+    out[i] = Ins{p.op[i], std::uint8_t(p.b[i]), std::uint8_t(h), std::uint8_t(f), p.a[i]};
+  }
+  return out;
+}
+}  // namespace
+
+struct Session::Impl {
+  DeviceArena arena;
+  DeviceCases cases;
+  std::size_t arena_len = 0;
+  int fuel, lanes;
+  double penalty;
+  Policy policy;
+  float last_ms = 0;
+};
+
+namespace {
+// Container values reach the device only as views inside the case arena.
+void check_view(const Val& v, std::size_t arena_len) {
+  if (v.tag == TAG_INT_LIST) {
+    const std::uint64_t u = std::uint64_t(v.i), off = u >> 16, len = u & 0xffff;
+    if (off > arena_len || len > arena_len - off) throw std::invalid_argument("IntList view outside the list arena");
+  } else if (v.tag == TAG_STRING || v.tag == TAG_FLOAT_LIST || v.tag == TAG_STRING_LIST || v.tag == TAG_CHAR) {
+    throw Unsupported("string, char and non-Int list values are outside the device VM");
+  } else if (v.tag > TAG_INVALID) {
+    throw std::invalid_argument("unknown value tag");
+  }
+}
+}  // namespace
+
+Session::Session(const Cases& c, int fuel, int lanes, double penalty, Policy policy) : impl_(std::make_unique<Impl>()) {
+  for (std::size_t i = 0; i < c.inputs.size(); ++i)
+    if (c.set.empty() || c.set[i]) check_view(c.inputs[i], c.arena.size());
+  for (const auto& v : c.expected) check_view(v, c.arena.size());
+  impl_->arena_len = c.arena.size();
+  impl_->cases.upload(impl_->arena, c);
+  impl_->fuel = fuel;
+  impl_->lanes = lanes;
+  impl_->penalty = penalty;
+  impl_->policy = policy;
+}
+Session::~Session() = default;
+float Session::last_kernel_ms() const { return impl_->last_ms; }
+
+void Session::eval(const std::vector<ProgramIn>& programs, double* fitness, std::uint32_t* wrong, std::uint32_t* counts) {
+  auto& I = *impl_;
+  const int n = int(programs.size());
+  if (!n) return;
+  std::vector<Ins> code;
+  std::vector<Val> pool;
+  std::vector<std::size_t> code_off, pool_off;
+  std::vector<int> lens, jumps_v, locals;
+  int slots = 1;
+  for (const auto& p : programs) {
+    int ms = 0;
+    bool j = false;
+    const auto ins = verify_program(p, ms, j);
+    for (const auto& v : p.consts) check_view(v, I.arena_len);
+    slots = std::max(slots, ms + 1);
+    code_off.push_back(code.size());
+    pool_off.push_back(pool.size());
+    code.insert(code.end(), ins.begin(), ins.end());
+    pool.insert(pool.end(), p.consts.begin(), p.consts.end());
+    lens.push_back(int(ins.size()));
+    jumps_v.push_back(j);
+    locals.push_back(p.n_locals);
+  }
+  if (pool.empty()) pool.push_back(val_invalid());
+  DeviceArena tmp;
+  Ins* dcode = tmp.upload(code);
+  Val* dpool = tmp.upload(pool);
+  std::vector<DProg> progs(static_cast<std::size_t>(n));
+  for (int i = 0; i < n; ++i) progs[i] = DProg{dcode + code_off[i], dpool + pool_off[i], lens[i], jumps_v[i], locals[i]};
+  DProg* dprogs = tmp.upload(progs);
+  double* dfit = tmp.alloc<double>(std::size_t(n));
+  unsigned* dwrong = tmp.alloc<unsigned>(std::size_t(n));
+  unsigned* dcounts = tmp.alloc<unsigned>(std::size_t(n) * 3);
+  const EvalParams e{I.cases.cd, I.fuel, I.lanes, I.penalty, I.policy};
+  const std::size_t lds = std::size_t(kWavesPerBlock) * slots * 64 * 10;
+  if (lds > 65536)
+    HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_eval_programs), hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+  hipEvent_t a, b;
+  HIP_OK(hipEventCreate(&a));
+  HIP_OK(hipEventCreate(&b));
+  const int blocks = std::min(4096, (n + kWavesPerBlock - 1) / kWavesPerBlock);
+  HIP_OK(hipEventRecord(a, nullptr));
+  hipLaunchKernelGGL(k_eval_programs, dim3(blocks), dim3(kEvalThreads), lds, nullptr, dprogs, n, e, slots, dfit, dwrong, dcounts);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipEventRecord(b, nullptr));
+  HIP_OK(hipEventSynchronize(b));
+  HIP_OK(hipEventElapsedTime(&I.last_ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  HIP_OK(hipMemcpy(fitness, dfit, sizeof(double) * n, hipMemcpyDeviceToHost));
+  if (wrong) HIP_OK(hipMemcpy(wrong, dwrong, sizeof(unsigned) * n, hipMemcpyDeviceToHost));
+  if (counts) HIP_OK(hipMemcpy(counts, dcounts, sizeof(unsigned) * 3 * n, hipMemcpyDeviceToHost));
+}
+
+void Session::eval_cases(const ProgramIn& program, std::vector<Val>& values, std::vector<int>& errors) {
+  auto& I = *impl_;
+  int ms = 0;
+  bool j = false;
+  const auto ins = verify_program(program, ms, j);
+  for (const auto& v : program.consts) check_view(v, I.arena_len);
+  const int slots = ms + 1;
+  DeviceArena tmp;
+  std::vector<Val> consts = program.consts;
+  if (consts.empty()) consts.push_back(val_invalid());
+  const DProg p{tmp.upload(ins), tmp.upload(consts), int(ins.size()), j, program.n_locals};
+  const int C = I.cases.cd.C;
+  Val* dv = tmp.alloc<Val>(std::size_t(C));
+  int* de = tmp.alloc<int>(std::size_t(C));
+  const std::size_t lds = std::size_t(kWavesPerBlock) * slots * 64 * 10;
+  if (lds > 65536)
+    HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_eval_cases), hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+  const int blocks = std::max(1, std::min(1024, (C + 255) / 256));
+  hipLaunchKernelGGL(k_eval_cases, dim3(blocks), dim3(kEvalThreads), lds, nullptr, p, I.cases.cd, I.fuel, slots, dv, de);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipDeviceSynchronize());
+  values.resize(std::size_t(C));
+  errors.resize(std::size_t(C));
+  HIP_OK(hipMemcpy(values.data(), dv, sizeof(Val) * C, hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(errors.data(), de, sizeof(int) * C, hipMemcpyDeviceToHost));
+}
+
+}  // namespace gagp_amd

@@ -1,0 +1,101 @@
+// Device-resident GAGP search engine: every search's population, keys,
+// random state and history stay in HBM; one steady-state round is a fixed
+// sequence of kernels over all searches of a batch (no host round trip), so a
+// round can be captured once as a HIP graph and replayed.
+#pragma once
+#include <cstdint>
+#include <memory>
+#include <optional>
+#include <string>
+#include <vector>
+#include "export.hpp"
+#include "grammar.hpp"
+#include "json.hpp"
+#include "phase.hpp"
+#include "task.hpp"
+
+namespace gagp_amd {
+
+// Host description of one search (one `gagp run`).
+struct SearchSpec {
+  std::string task_path;
+  std::string search_path;             // empty: built-in default
+  std::optional<int> population, generations;
+  std::optional<std::uint64_t> seed;
+};
+
+struct HistoryRow {
+  double best_fitness = 0, mean_fitness = 0;
+  int best_wrong = 0, best_nodes = 0;
+  double elapsed_ms = 0, eval_ms = 0, repro_ms = 0;
+};
+
+struct SearchResult {
+  json::Value result;                  // result.json document
+  std::vector<json::Value> population; // ranked final population (ast JSON)
+  std::vector<json::Value> lineage;    // per-generation best (history.json)
+  std::uint64_t evaluated_program_cases = 0;
+  std::uint64_t evaluated_programs = 0;
+  double search_ms = 0;
+};
+
+class Search;  // one admitted search (host + device state)
+
+// A batch of searches stepped together, one kernel sequence per round.
+class Batch {
+ public:
+  Batch();
+  ~Batch();
+  // Admits a search; throws Unsupported (exit 2) for grammars/tasks outside
+  // the engine, std::invalid_argument for invalid documents.
+  int add(const SearchSpec& spec);
+  // Runs every search to completion; `use_graph` captures one round.
+  void run(bool use_graph = true);
+  // Split form of run(): allocate, build and score the initial populations;
+  // then advance every live search by `rounds` steady-state rounds (a search
+  // past its generation budget idles); then check for device errors.
+  void prepare();
+  void step(int rounds, bool use_graph = true);
+  void finish();
+  // Device counters of search `index`: evaluated programs, program-cases,
+  // runtime errors, timeouts.
+  std::vector<std::uint64_t> counters(int index) const;
+  double init_ms() const;
+  SearchResult result(int index, bool save_all);
+  std::size_t size() const;
+  std::size_t device_bytes() const;
+  // Per-kernel device time (ms) accumulated with HIP events, by kernel name.
+  std::vector<std::pair<std::string, double>> kernel_times() const;
+  void set_profile(bool on);
+  // Generation budget of the batch (max over searches).
+  int max_rounds() const;
+
+ private:
+  struct Impl;
+  std::unique_ptr<Impl> impl_;
+};
+
+// Evaluation session: the FitnessSessionGpu boundary (fixed case set,
+// arbitrary verified bytecode programs).
+struct ProgramIn {
+  std::vector<std::uint8_t> op, has_a, has_b;
+  std::vector<std::int32_t> a, b;
+  std::vector<std::uint32_t> fuel;  // empty: unit schedule
+  std::vector<Val> consts;
+  int n_locals = 0;
+};
+class Session {
+ public:
+  Session(const Cases& cases, int fuel, int lanes, double penalty, Policy policy);
+  ~Session();
+  void eval(const std::vector<ProgramIn>& programs, double* fitness, std::uint32_t* wrong, std::uint32_t* counts);
+  // Per-case outcome of one program: value and error code (ERR_NONE = ok).
+  void eval_cases(const ProgramIn& program, std::vector<Val>& values, std::vector<int>& errors);
+  float last_kernel_ms() const;
+
+ private:
+  struct Impl;
+  std::unique_ptr<Impl> impl_;
+};
+
+}  // namespace gagp_amd

@@ -397,7 +397,9 @@ GHD inline void crossover_child(const PhaseView& g, const SlotInfo& s, int child
     const int ae = subtree_end(g, a, dest), be = subtree_end(g, b, src);
     if (ae < 0 || be < 0) { st.error = 3; return; }
     if (total - (ae - dest) + (be - src) > s.max_variable_nodes) { ++st.crossover_rejected; continue; }
-    const GenResult r = splice(g, a, dest, b, src, out, kPhaseCapacity, s.max_depth, s.logical_depth);
+    // The budget check above bounds the splice by the node budget, which the
+    // physical row holds, so the row size never changes an outcome.
+    const GenResult r = splice(g, a, dest, b, src, out, s.row_capacity, s.max_depth, s.logical_depth);
     if (r.status == ST_OK) {
       *out_n = r.nodes;
       *out_nc = r.constants;
@@ -433,12 +435,13 @@ GHD inline void mutate_child(const PhaseView& g, const SlotInfo& s, int child, R
   const int available = whole < phase ? whole : phase, uncapped = available < kPhaseCapacity ? available : kPhaseCapacity;
   const bool capped = pol.mutation_max_nodes > 0 && pol.mutation_max_nodes < uncapped;
   const int bounded = capped ? pol.mutation_max_nodes : uncapped;
+  if (bounded > s.row_capacity) { st.error = 10; return; }  // node budget exceeds the physical row
   const GenResult gen = generate_phase(g, site_entry(site), site_parent_scope(site), s.max_depth - depth + 1, bounded,
                                        random.next(), tmp.origin, tmp.cval, tmp.site, logical);
   if (capped && gen.status == ST_CAPACITY) return;
   if (gen.status != ST_OK) { st.error = 7; return; }
   const CRow sub{tmp.origin, tmp.site, tmp.cval, gen.nodes, gen.constants};
-  const GenResult r = splice(g, a, dest, sub, 0, spliced, kPhaseCapacity, s.max_depth, s.logical_depth);
+  const GenResult r = splice(g, a, dest, sub, 0, spliced, s.row_capacity, s.max_depth, s.logical_depth);
   if (r.status != ST_OK) { st.error = 8; return; }
   const CRow sp{spliced.origin, spliced.site, spliced.cval, r.nodes, r.constants};
   copy_row(sp, x, x_n, x_nc);

@@ -159,9 +159,9 @@ Cases parse_cases(const Value& doc) {
 }
 Cases load_cases(const std::string& path) { return parse_cases(json::parse(read_file(path))); }
 
-Search default_search() { return Search{}; }
+SearchConfig default_search_config() { return SearchConfig{}; }
 
-void apply_search(Search& s, const Value& doc) {
+void apply_search(SearchConfig& s, const Value& doc) {
   reject_unknown("document", doc, {"format", "representation", "population_size", "generations", "seed", "selection",
       "elitism", "crossover", "mutation", "sites", "limits", "ranking", "fitness", "execution", "schedule"});
   if (!doc.has("format") || text(doc.at("format"), "format") != "gagp-search-config-v1")
@@ -286,7 +286,7 @@ void apply_search(Search& s, const Value& doc) {
   }
 }
 
-json::Value effective_search_json(const Search& s, double penalty, int fuel) {
+json::Value effective_search_json(const SearchConfig& s, double penalty, int fuel) {
   using json::make_number;
   using json::make_object;
   using json::make_string;

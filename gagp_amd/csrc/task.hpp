@@ -38,7 +38,7 @@ Cases load_cases(const std::string& path);
 Cases parse_cases(const json::Value& doc);
 
 enum class Schedule { Generational, SteadyState };
-struct Search {
+struct SearchConfig {
   bool resident = true;
   int population = 1024, generations = 64;
   std::uint64_t seed = 0;
@@ -60,8 +60,8 @@ struct Search {
 // The built-in default search (configs/search/default.json) with a search
 // document applied on top; an explicit variation member resets the whole
 // variation policy to the document's values and the policy defaults.
-Search default_search();
-void apply_search(Search& s, const json::Value& doc);
-json::Value effective_search_json(const Search& s, double penalty, int fuel);
+SearchConfig default_search_config();
+void apply_search(SearchConfig& s, const json::Value& doc);
+json::Value effective_search_json(const SearchConfig& s, double penalty, int fuel);
 
 }  // namespace gagp_amd

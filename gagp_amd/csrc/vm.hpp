@@ -1,0 +1,279 @@
+// Device bytecode VM: one wavefront (64 lanes) per program, one case per lane
+// per tile. The instruction stream is wave-uniform: straight-line programs run
+// ip in lockstep; programs with jumps use min-ip scheduling (each step runs the
+// smallest pending ip, for the lanes that are at it), so every fetched
+// instruction and its static stack height are scalar values. The operand
+// stack lives in LDS, slot-major, one 8-byte payload and one tag byte per
+// lane and slot, so every stack access is conflict-free.
+//
+// Instruction semantics restate cpp/src/runtime/cpu/execute_bytecode_cpu.cpp:
+// 194-614 (fuel charged and checked first, the first error ends the case)
+// and the scalar builtins of cpp/src/runtime/cpu/builtins_cpu.cpp:130-217,
+// 340-375,493-500.
+#pragma once
+#include "core.hpp"
+#include "phase.hpp"
+
+namespace gagp_amd {
+
+struct ProgView {
+  const Ins* code;
+  const Val* pool;
+  int len;
+  int n_locals;
+  int has_jumps;
+};
+
+struct CaseData {
+  const std::int64_t* in_val;  // [n_inputs][C]
+  const std::uint8_t* in_tag;  // [n_inputs][C] (TAG_INVALID: unbound)
+  const std::uint8_t* in_b;    // [n_inputs][C]
+  const std::int64_t* exp_val; // [C]
+  const std::uint8_t* exp_tag; // [C]
+  const std::uint8_t* exp_b;   // [C]
+  const std::int64_t* arena;   // IntList elements
+  int C, n_inputs;
+};
+
+// Per-wave LDS stack, slot-major: payload[slot*64 + lane], tag[...], flag[...].
+struct WaveStack {
+  std::int64_t* v;
+  std::uint8_t* t;
+  std::uint8_t* b;
+  int slots;
+};
+
+#if defined(__HIPCC__)
+__device__ inline int wave_min(int x) {
+  for (int off = 32; off > 0; off >>= 1) {
+    const int y = __shfl_xor(x, off, 64);
+    x = y < x ? y : x;
+  }
+  return x;
+}
+#endif
+
+struct LaneResult {
+  std::int64_t v;
+  std::uint8_t t, b;
+  int err;
+  int fuel_used;  // charged fuel (instructions executed under the unit schedule)
+};
+
+#if defined(__HIPCC__)
+// Runs `p` for case `c` on this lane (active = lane has a case). Returns the
+// result or error of the case.
+__device__ inline LaneResult vm_run(const ProgView& p, const CaseData& cd, int c, bool active, int fuel0, WaveStack& st,
+                                    int lane) {
+  LaneResult r{0, TAG_INVALID, 0, active ? ERR_NONE : ERR_VALUE, 0};
+  bool done = !active;
+  int fuel = fuel0;
+  int ip = 0;
+  int uncharged = 0;
+  for (;;) {
+    int at;
+    if (p.has_jumps) {
+      at = wave_min(done ? 0x7fffffff : ip);
+      if (at == 0x7fffffff) break;
+    } else {
+      if (__ballot(!done) == 0) break;  // all lanes finished
+      at = ip;
+      at = __builtin_amdgcn_readfirstlane(at);
+    }
+    const bool mine = !done && ip == at;
+    if (at >= p.len) {  // ran off the end without RETURN
+      if (mine) { r.err = ERR_VALUE; done = true; }
+      continue;
+    }
+    const Ins ins = p.code[at];
+    const int sp = ins.sp;
+    if (mine) {
+      const int cost = ins.fuel;
+      if (fuel < cost) { r.err = ERR_TIMEOUT; done = true; }
+      else {
+        fuel -= cost;
+        if (cost) uncharged = 0;
+        else if (++uncharged > p.len) { r.err = ERR_VALUE; done = true; }
+      }
+    }
+    if (!mine || done) continue;
+    ++ip;
+    switch (ins.op) {
+      case OP_PUSH_CONST: {
+        const Val v = p.pool[ins.a];
+        st.v[(sp) * 64 + lane] = v.i; st.t[(sp) * 64 + lane] = v.tag; st.b[(sp) * 64 + lane] = v.b;
+        break;
+      }
+      case OP_LOAD: {
+        if (ins.a >= cd.n_inputs) { r.err = ERR_NAME; done = true; break; }
+        const std::size_t at_in = std::size_t(ins.a) * cd.C + c;
+        const std::uint8_t tag = cd.in_tag[at_in];
+        if (tag == TAG_INVALID) { r.err = ERR_NAME; done = true; break; }
+        st.v[(sp) * 64 + lane] = cd.in_val[at_in]; st.t[(sp) * 64 + lane] = tag; st.b[(sp) * 64 + lane] = cd.in_b[at_in];
+        break;
+      }
+      case OP_CHECK_INT:
+        if (st.t[(sp - 1) * 64 + lane] != TAG_INT) { r.err = ERR_TYPE; done = true; }
+        break;
+      case OP_CHECK_LIST: {
+        const std::uint8_t t = st.t[(sp - 1) * 64 + lane];
+        if (t != TAG_INT_LIST && t != TAG_FLOAT_LIST && t != TAG_STRING_LIST) { r.err = ERR_TYPE; done = true; }
+        break;
+      }
+      case OP_NEG: {
+        const std::uint8_t t = st.t[(sp - 1) * 64 + lane];
+        if (t == TAG_INT) st.v[(sp - 1) * 64 + lane] = wrap_neg(st.v[(sp - 1) * 64 + lane]);
+        else if (t == TAG_FLOAT) {
+          union { std::int64_t i; double f; } u{st.v[(sp - 1) * 64 + lane]};
+          st.v[(sp - 1) * 64 + lane] = val_float(canon32(-u.f)).i;
+        } else { r.err = ERR_TYPE; done = true; }
+        break;
+      }
+      case OP_NOT:
+        if (st.t[(sp - 1) * 64 + lane] != TAG_BOOL) { r.err = ERR_TYPE; done = true; }
+        else st.b[(sp - 1) * 64 + lane] = !st.b[(sp - 1) * 64 + lane];
+        break;
+      case OP_ADD: case OP_SUB: case OP_MUL: case OP_DIV: case OP_MOD: {
+        const std::uint8_t ta = st.t[(sp - 2) * 64 + lane], tb = st.t[(sp - 1) * 64 + lane];
+        const std::int64_t a = st.v[(sp - 2) * 64 + lane], b = st.v[(sp - 1) * 64 + lane];
+        if (ta == TAG_INT && tb == TAG_INT) {
+          if ((ins.op == OP_DIV || ins.op == OP_MOD) && b == 0) { r.err = ERR_ZERO_DIV; done = true; break; }
+          std::int64_t out;
+          if (ins.op == OP_ADD) out = wrap_add(a, b);
+          else if (ins.op == OP_SUB) out = wrap_sub(a, b);
+          else if (ins.op == OP_MUL) out = wrap_mul(a, b);
+          else if (ins.op == OP_MOD) out = py_mod(a, b);
+          else {
+            st.v[(sp - 2) * 64 + lane] = val_float(canon32(double(a) / double(b))).i;
+            st.t[(sp - 2) * 64 + lane] = TAG_FLOAT;
+            break;
+          }
+          st.v[(sp - 2) * 64 + lane] = out;
+        } else if (ta == TAG_FLOAT && tb == TAG_FLOAT) {
+          union { std::int64_t i; double f; } x{a}, y{b};
+          if ((ins.op == OP_DIV || ins.op == OP_MOD) && y.f == 0.0) { r.err = ERR_ZERO_DIV; done = true; break; }
+          double out;
+          if (ins.op == OP_ADD) out = x.f + y.f;
+          else if (ins.op == OP_SUB) out = x.f - y.f;
+          else if (ins.op == OP_MUL) out = x.f * y.f;
+          else if (ins.op == OP_DIV) out = x.f / y.f;
+          else {
+            double m = fmod(x.f, y.f);
+            if (m == 0.0) m = copysign(0.0, y.f);
+            else if (signbit(m) != signbit(y.f)) m += y.f;
+            out = m;
+          }
+          st.v[(sp - 2) * 64 + lane] = val_float(canon32(out)).i;
+        } else { r.err = ERR_TYPE; done = true; }
+        break;
+      }
+      case OP_LT: case OP_LE: case OP_GT: case OP_GE: case OP_EQ: case OP_NE: {
+        const std::uint8_t ta = st.t[(sp - 2) * 64 + lane], tb = st.t[(sp - 1) * 64 + lane];
+        const std::int64_t a = st.v[(sp - 2) * 64 + lane], b = st.v[(sp - 1) * 64 + lane];
+        bool out = false;
+        if (ins.op == OP_EQ || ins.op == OP_NE) {
+          if (ta != tb) { r.err = ERR_TYPE; done = true; break; }
+          bool eq;
+          if (ta == TAG_BOOL) eq = st.b[(sp - 2) * 64 + lane] == st.b[(sp - 1) * 64 + lane];
+          else if (ta == TAG_FLOAT) { union { std::int64_t i; double f; } x{a}, y{b}; eq = x.f == y.f; }
+          else if (ta == TAG_INT || ta == TAG_CHAR || ta == TAG_STRING || ta == TAG_INT_LIST || ta == TAG_FLOAT_LIST ||
+                   ta == TAG_STRING_LIST) eq = a == b;
+          else { r.err = ERR_TYPE; done = true; break; }
+          out = ins.op == OP_EQ ? eq : !eq;
+        } else if (ta == TAG_INT && tb == TAG_INT) {
+          out = ins.op == OP_LT ? a < b : ins.op == OP_LE ? a <= b : ins.op == OP_GT ? a > b : a >= b;
+        } else if (ta == TAG_FLOAT && tb == TAG_FLOAT) {
+          union { std::int64_t i; double f; } x{a}, y{b};
+          out = ins.op == OP_LT ? x.f < y.f : ins.op == OP_LE ? x.f <= y.f : ins.op == OP_GT ? x.f > y.f : x.f >= y.f;
+        } else { r.err = ERR_TYPE; done = true; break; }
+        st.v[(sp - 2) * 64 + lane] = 0; st.t[(sp - 2) * 64 + lane] = TAG_BOOL; st.b[(sp - 2) * 64 + lane] = out;
+        break;
+      }
+      case OP_JMP:
+        ip = ins.a;
+        break;
+      case OP_JMP_IF_FALSE: case OP_JMP_IF_TRUE: {
+        if (st.t[(sp - 1) * 64 + lane] != TAG_BOOL) { r.err = ERR_TYPE; done = true; break; }
+        const bool cond = st.b[(sp - 1) * 64 + lane] != 0;
+        if ((ins.op == OP_JMP_IF_FALSE && !cond) || (ins.op == OP_JMP_IF_TRUE && cond)) ip = ins.a;
+        break;
+      }
+      case OP_RETURN:
+        r.v = st.v[(sp - 1) * 64 + lane]; r.t = st.t[(sp - 1) * 64 + lane]; r.b = st.b[(sp - 1) * 64 + lane];
+        done = true;
+        break;
+      case OP_CALL_BUILTIN: {
+        const int argc = ins.argc, base = sp - argc;
+        std::int64_t out = 0;
+        std::uint8_t out_t = TAG_INT, out_b = 0;
+        const auto ti = [&](int k) { return st.t[(base + k) * 64 + lane]; };
+        const auto vi = [&](int k) { return st.v[(base + k) * 64 + lane]; };
+        switch (ins.a) {
+          case B_ABS:
+            if (argc != 1 || ti(0) != TAG_INT) { r.err = ERR_TYPE; break; }
+            out = vi(0) < 0 ? wrap_neg(vi(0)) : vi(0);
+            break;
+          case B_MIN: case B_MAX:
+            if (argc != 2 || ti(0) != TAG_INT || ti(1) != TAG_INT) { r.err = ERR_TYPE; break; }
+            out = ins.a == B_MIN ? (vi(0) < vi(1) ? vi(0) : vi(1)) : (vi(0) > vi(1) ? vi(0) : vi(1));
+            break;
+          case B_CLIP: {
+            if (argc != 3 || ti(0) != TAG_INT || ti(1) != TAG_INT || ti(2) != TAG_INT) { r.err = ERR_TYPE; break; }
+            const std::int64_t m = vi(0) > vi(1) ? vi(0) : vi(1);
+            out = m < vi(2) ? m : vi(2);
+            break;
+          }
+          case B_IDIV0: case B_IMOD0: case B_TRUNC_DIV0: case B_TRUNC_MOD0: {
+            if (argc != 2 || ti(0) != TAG_INT || ti(1) != TAG_INT) { r.err = ERR_TYPE; break; }
+            const std::int64_t a = vi(0), b = vi(1);
+            if (b == 0) out = 0;
+            else if (ins.a == B_TRUNC_DIV0) out = trunc_div0(a, b);
+            else if (ins.a == B_TRUNC_MOD0) out = trunc_mod0(a, b);
+            else if (ins.a == B_IDIV0) out = wrap_div(a, b);
+            else out = py_mod(a, b);
+            break;
+          }
+          case B_LEN: {
+            const std::uint8_t t = ti(0);
+            if (argc != 1 || !(t == TAG_INT_LIST || t == TAG_FLOAT_LIST || t == TAG_STRING_LIST)) { r.err = ERR_TYPE; break; }
+            out = std::int64_t(std::uint64_t(vi(0)) & 0xffff);
+            break;
+          }
+          case B_INDEX: case B_INDEX_NONNEG: case B_INDEX_CLAMPED: {
+            if (argc != 2 || ti(0) != TAG_INT_LIST || ti(1) != TAG_INT) { r.err = ERR_TYPE; break; }
+            const std::int64_t n = std::int64_t(std::uint64_t(vi(0)) & 0xffff);
+            const std::uint64_t off = std::uint64_t(vi(0)) >> 16;
+            std::int64_t j = vi(1);
+            if (ins.a == B_INDEX_CLAMPED) {
+              if (n == 0) { out = 0; break; }
+              j = j < 0 ? 0 : (j >= n ? n - 1 : j);
+            } else {
+              if (ins.a == B_INDEX_NONNEG && j < 0) { r.err = ERR_VALUE; break; }
+              if (j < 0) j += n;
+              if (j < 0 || j >= n) { r.err = ERR_VALUE; break; }
+            }
+            out = cd.arena[off + std::uint64_t(j)];
+            break;
+          }
+          case B_IS_INT:
+            if (argc != 1) { r.err = ERR_TYPE; break; }
+            out_t = TAG_BOOL;
+            out_b = ti(0) == TAG_INT;
+            break;
+          default: r.err = ERR_NAME; break;
+        }
+        if (r.err != ERR_NONE) { done = true; break; }
+        st.v[(base) * 64 + lane] = out; st.t[(base) * 64 + lane] = out_t; st.b[(base) * 64 + lane] = out_b;
+        break;
+      }
+      default:
+        r.err = ERR_VALUE; done = true;
+        break;
+    }
+  }
+  r.fuel_used = fuel0 - fuel;
+  return r;
+}
+#endif
+
+}  // namespace gagp_amd

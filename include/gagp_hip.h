@@ -1,0 +1,159 @@
+/*
+ * gagp_hip.h -- C ABI of the MI355X GAGP engine (libgagp_hip.so).
+ *
+ * Plain C types only: pointers, sizes, status codes. Every function returns
+ * GAGP_OK or an error status; gagp_last_error() returns the thread-local
+ * message of the last failure. Handles are opaque; device buffers live inside
+ * them. Reference interfaces each entry point replaces are cited per function
+ * (paths relative to the hschi1106/gagp tree).
+ */
+#ifndef GAGP_HIP_H
+#define GAGP_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  GAGP_OK = 0,
+  GAGP_ERR_INVALID = 1,     /* invalid document, argument or program */
+  GAGP_ERR_UNSUPPORTED = 2, /* outside the engine: the reference would decline to native_ast */
+  GAGP_ERR_DEVICE = 3,      /* HIP runtime or device-side failure */
+};
+
+/* Runtime value, 16 bytes: payload (int64 or float bits), Bool flag, tag.
+ * Tags: 0 Int, 1 Float, 2 Bool, 3 Char, 4 String, 5 IntList, 6 FloatList,
+ * 7 StringList, 8 FallbackToken, 9 Invalid.
+ * IntList values are views into the session's element arena:
+ * payload = offset * 65536 + length.
+ * Reference: Value, cpp/include/gagp/core/value.hpp:14-33. */
+typedef struct gagp_value {
+  int64_t i;
+  uint8_t b;
+  uint8_t tag;
+  uint8_t pad[6];
+} gagp_value;
+
+/* Bytecode instruction. Reference: Instr, cpp/include/gagp/core/bytecode.hpp:15-21. */
+typedef struct gagp_instr {
+  uint8_t op;
+  uint8_t has_a;
+  uint8_t has_b;
+  uint8_t pad;
+  int32_t a;
+  int32_t b;
+} gagp_instr;
+
+/* One bytecode program. Reference: BytecodeProgram,
+ * cpp/include/gagp/core/bytecode.hpp:48-56 (without bounded-region segments). */
+typedef struct gagp_program {
+  const gagp_instr* code;
+  int32_t code_len;
+  const uint32_t* fuel; /* per-instruction charge, or NULL for the unit schedule */
+  const gagp_value* consts;
+  int32_t n_consts;
+  int32_t n_locals;
+} gagp_program;
+
+/* Fitness policy. metric: 0 Legacy, 1 Absolute, 2 Exact; proximity: 0 Capped,
+ * 1 Discounted. Reference: FitnessPolicy, cpp/include/gagp/core/fitness_policy.hpp:6-15. */
+typedef struct gagp_policy {
+  int32_t metric;
+  int32_t proximity;
+  int32_t summary_count;
+  double summary_weight;
+  double summary_cap;
+} gagp_policy;
+
+/* Message of the calling thread's last failure ("" if none). */
+const char* gagp_last_error(void);
+/* Number of visible HIP devices (0 without a GPU). */
+int gagp_device_count(void);
+
+/* ---- evaluation boundary ---- */
+typedef struct gagp_session gagp_session;
+
+/* Uploads a case set once. case_inputs/input_set are [n_cases][n_inputs]
+ * (input_set may be NULL: all bound); inputs are the evaluator locals
+ * 0..n_inputs-1 (sorted input names). lanes is the canonical reduction width
+ * (512 for `gagp run`).
+ * Reference: FitnessSessionGpu::init, cpp/include/gagp/runtime/gpu/fitness_gpu.hpp:91-121,
+ * cpp/src/runtime/gpu/fitness_gpu.cu:540-713. */
+int gagp_session_create(const gagp_value* case_inputs, const uint8_t* input_set, int32_t n_cases, int32_t n_inputs,
+                        const gagp_value* answers, const int64_t* list_arena, int64_t list_arena_len, int32_t fuel,
+                        int32_t lanes, double penalty, gagp_policy policy, gagp_session** out);
+
+/* Scores n programs on every case: fitness (canonical 48-bit total), wrong
+ * counts and per-program counts[3] = {cases, errors, timeouts}.
+ * Reference: FitnessSessionGpu::eval_programs, fitness_gpu.hpp:49-121,
+ * fitness_gpu.cu:715-1790; CPU semantics cpp/src/runtime/cpu/fitness_cpu.cpp:57-176. */
+int gagp_session_eval(gagp_session* s, const gagp_program* programs, int32_t n, double* out_fitness, uint32_t* out_wrong,
+                      uint32_t* out_counts);
+
+/* Per-case outcome of one program: out_values[c] and out_errors[c]
+ * (-1 ok, 0 NameError, 1 TypeError, 2 ZeroDivisionError, 3 ValueError, 4 Timeout).
+ * Reference: CpuExecutionSession::execute as used by the run judge,
+ * cpp/src/cli/run.cpp:208-239. */
+int gagp_session_eval_cases(gagp_session* s, const gagp_program* program, gagp_value* out_values, int32_t* out_errors);
+
+/* Device time (ms) of the last gagp_session_eval kernel. */
+float gagp_session_last_kernel_ms(gagp_session* s);
+void gagp_session_destroy(gagp_session* s);
+
+/* ---- search boundary (`gagp run TASK.json --engine gpu`) ---- */
+typedef struct gagp_batch gagp_batch;
+
+/* A batch of searches stepped together, one device kernel sequence per round.
+ * Reference: evolve_population / run_phase_population_steady,
+ * cpp/src/evolution/evolve.cpp:331-441,492-558; run_task_command,
+ * cpp/src/cli/run.cpp:275-370. */
+int gagp_batch_create(gagp_batch** out);
+
+/* Admits one search. search_path may be NULL (built-in default search);
+ * population/generations <= 0 and seed_is_set == 0 keep the document values.
+ * *out_index receives the search's index in the batch.
+ * Reference: the `gagp run` options, cpp/src/cli/run.cpp:105-141,281-303. */
+int gagp_batch_add(gagp_batch* b, const char* task_path, const char* search_path, int32_t population, int32_t generations,
+                   int32_t seed_is_set, uint64_t seed, int32_t* out_index);
+
+/* Runs every admitted search to completion (use_graph: capture one round as
+ * a HIP graph and replay it). */
+int gagp_batch_run(gagp_batch* b, int32_t use_graph);
+
+/* Split form of gagp_batch_run: gagp_batch_prepare allocates every search
+ * and scores its initial population; gagp_batch_step advances all live
+ * searches by `rounds` steady-state rounds (searches past their generation
+ * budget idle); gagp_batch_finish checks device errors and assembles the
+ * results. */
+int gagp_batch_prepare(gagp_batch* b);
+int gagp_batch_step(gagp_batch* b, int32_t rounds, int32_t use_graph);
+int gagp_batch_finish(gagp_batch* b);
+
+/* Device counters of search `index`: out[6] = {evaluated programs,
+ * evaluated program-cases, runtime errors, timeouts, VM instructions
+ * executed (charged fuel), instructions of the evaluated programs}. */
+int gagp_batch_counters(gagp_batch* b, int32_t index, uint64_t* out);
+
+/* result.json of search `index` (gagp-run-v1, canonical JSON). If buf is
+ * NULL or cap is too small, *needed receives the size including the NUL.
+ * Reference: cpp/src/cli/run.cpp:315-367. */
+int gagp_batch_result_json(gagp_batch* b, int32_t index, char* buf, size_t cap, size_t* needed);
+
+/* Evaluated program-case count of search `index` (non-reused children times
+ * train cases, plus the initial population). Reference: evolve.cpp:586,600. */
+int gagp_batch_evaluated_program_cases(gagp_batch* b, int32_t index, uint64_t* out);
+
+/* Per-kernel device milliseconds when the batch ran with profiling on
+ * (gagp_batch_set_profile before gagp_batch_run): a JSON object
+ * {"k_eval": ms, ...}. */
+int gagp_batch_set_profile(gagp_batch* b, int32_t on);
+int gagp_batch_kernel_times_json(gagp_batch* b, char* buf, size_t cap, size_t* needed);
+void gagp_batch_destroy(gagp_batch* b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GAGP_HIP_H */

@@ -1,0 +1,64 @@
+"""The C-ABI library and command line load and export every declared entry
+point (no device calls), and the host-side documents behave."""
+import ctypes
+import json
+import os
+import re
+import subprocess
+
+import gagp_amd
+from _util import GOLDEN, ROOT
+
+HEADER = os.path.join(ROOT, "include", "gagp_hip.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(gagp_[a-z_]+)\s*\(", text)))
+
+
+def test_header_declares_the_boundary():
+    names = declared_functions()
+    for n in ("gagp_session_create", "gagp_session_eval", "gagp_batch_create", "gagp_batch_add", "gagp_batch_run",
+              "gagp_batch_result_json", "gagp_last_error"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(gagp_amd.LIB_PATH)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_errors_are_reported_without_a_device():
+    L = gagp_amd.lib()
+    # Invalid arguments fail on the host with a message, before any device call.
+    st = L.gagp_batch_add(None, b"x", None, 0, 0, 0, 0, None)
+    assert st == gagp_amd.GAGP_ERR_INVALID and b"invalid" in L.gagp_last_error()
+
+
+def test_batch_rejects_unsupported_grammar_on_the_host():
+    b = gagp_amd.Batch()
+    try:
+        b.add(os.path.join(GOLDEN, "tasks", "list-sum", "task.json"))
+    except gagp_amd.GagpError as e:
+        assert e.status == gagp_amd.GAGP_ERR_UNSUPPORTED
+    else:
+        raise AssertionError("list-sum uses region-binding primitives and must be declined")
+
+
+def test_cli_usage_and_exit_codes():
+    r = subprocess.run([gagp_amd.CLI_PATH, "run"], capture_output=True, text=True)
+    assert r.returncode == 2 and "missing TASK" in r.stderr
+    r = subprocess.run([gagp_amd.CLI_PATH, "run", "--help"], capture_output=True, text=True)
+    assert r.returncode == 0 and "--engine" in r.stdout
+
+
+def test_workload_generators_are_deterministic(tmp_path):
+    from gagp_amd import workloads
+    workloads.poly_c1(str(tmp_path / "c1"))
+    workloads.quintic(str(tmp_path / "c2"))
+    for name, sub in (("poly_c1", "c1"), ("quintic", "c2")):
+        for f in ("task.json", "grammar.json", "train.cases.json", "test.cases.json"):
+            assert json.load(open(tmp_path / sub / f)) == json.load(open(os.path.join(GOLDEN, "tasks", name, f)))

@@ -1,0 +1,84 @@
+"""Whole-search parity on the device: `gagp run --engine gpu` against the
+reference CPU engine's golden result.json (best program, fitness vector
+entries, test verdict, effective search and the per-generation history must
+be identical; only timing fields may differ)."""
+import json
+import os
+import subprocess
+
+import pytest
+
+import gagp_amd
+from _util import GOLDEN, strip_timing
+
+pytestmark = pytest.mark.gpu
+MANIFEST = json.load(open(os.path.join(GOLDEN, "runs", "manifest.json")))
+SMALL = sorted(n for n in MANIFEST if not n.startswith("quintic_p4096"))
+
+
+def golden(name):
+    return json.load(open(os.path.join(GOLDEN, "runs", name, "result.json")))
+
+
+def add(batch, name):
+    m = MANIFEST[name]
+    return batch.add(os.path.join(GOLDEN, m["task"]), None, m["population"], m["generations"], m["seed"])
+
+
+@pytest.mark.parametrize("name", SMALL)
+def test_search_matches_reference(name):
+    print("search", name, flush=True)
+    b = gagp_amd.Batch()
+    add(b, name)
+    b.run()
+    assert strip_timing(b.result(0)) == strip_timing(golden(name))
+
+
+def test_full_quintic_matches_reference():
+    name = "quintic_p4096_g128_s0"
+    print("search", name, flush=True)
+    b = gagp_amd.Batch()
+    add(b, name)
+    b.run()
+    assert strip_timing(b.result(0)) == strip_timing(golden(name))
+
+
+def test_batched_searches_match_alone():
+    # Every search in one batch (uneven populations, case counts and
+    # generations) publishes exactly its stand-alone result.
+    print("batch of", len(SMALL), flush=True)
+    b = gagp_amd.Batch()
+    index = {name: add(b, name) for name in SMALL}
+    b.run()
+    for name, i in index.items():
+        assert strip_timing(b.result(i)) == strip_timing(golden(name)), name
+
+
+def test_graph_and_stream_execution_agree():
+    name = "polynomial_p256_g20_s123"
+    results = []
+    for use_graph in (True, False):
+        b = gagp_amd.Batch()
+        add(b, name)
+        b.run(use_graph=use_graph)
+        results.append(strip_timing(b.result(0)))
+    assert results[0] == results[1] == strip_timing(golden(name))
+
+
+def test_cli_drop_in(tmp_path):
+    name = "polynomial_p1024_g64_s0"
+    m = MANIFEST[name]
+    out = tmp_path / "run"
+    subprocess.run([gagp_amd.CLI_PATH, "run", os.path.join(GOLDEN, m["task"]), "--engine", "gpu", "--seed", "0",
+                    "--out", str(out), "--save-all", "--quiet"], check=True, timeout=300)
+    mine = json.load(open(out / "result.json"))
+    assert strip_timing(mine) == strip_timing(golden(name))
+    ref_pop = json.load(open(os.path.join(GOLDEN, "runs", name, "population.json")))
+    my_pop = json.load(open(out / "population.json"))
+    assert my_pop["programs"] == ref_pop["programs"]
+
+
+def test_cli_rejects_unsupported_with_exit_2(tmp_path):
+    r = subprocess.run([gagp_amd.CLI_PATH, "run", os.path.join(GOLDEN, "tasks", "list-sum", "task.json"), "--quiet"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2 and "gagp:" in r.stderr

@@ -1,0 +1,146 @@
+"""Device VM and fitness reduction (the evaluation boundary) against the
+oracle and the reference's bytecode fixtures."""
+import random
+import struct
+
+import pytest
+
+import gagp_amd
+from _util import E_NAMES, case_inputs, load_fixture_scenarios, oracle_fitness, oracle_run
+
+pytestmark = pytest.mark.gpu
+SCENARIOS, _ = load_fixture_scenarios()
+
+
+@pytest.mark.parametrize("scenario", SCENARIOS, ids=lambda s: s["file"] + ":" + s["intent"][:40])
+def test_device_vm_matches_reference_fixture(scenario):
+    p = scenario["program"]
+    if any(op == 2 for op, _, _ in p["code"]):
+        pytest.skip("STORE: statement locals are not part of the device VM's expression programs")
+    rows, arena = [], []
+    for case in scenario["cases"]:
+        vals, ar = case_inputs(case, p["n_locals"])
+        # re-base list views onto one shared arena
+        vals = [("list", v[1] + len(arena), v[2]) if isinstance(v, tuple) else v for v in vals]
+        arena.extend(ar)
+        rows.append(vals)
+    n_in = p["n_locals"]
+    inputs = [[0 if v is None else v for v in row] for row in rows] if n_in else []
+    input_set = [[v is not None for v in row] for row in rows] if n_in else None
+    s = gagp_amd.Session(inputs, [0] * len(rows), fuel=scenario["fuel"], arena=arena, input_set=input_set)
+    values, errors = s.eval_cases(p)
+    for k, case in enumerate(scenario["cases"]):
+        if "expected_error" in case:
+            assert E_NAMES[errors[k]] == case["expected_error"], case
+            continue
+        assert errors[k] == -1, (case, E_NAMES[errors[k]])
+        tag, i, b = values[k]
+        exp = case["expected"]
+        if exp["type"] == "int":
+            assert (tag, i) == (0, int(exp["value"]))
+        elif exp["type"] == "bool":
+            assert (tag, b) == (2, int(exp["value"]))
+        else:
+            assert tag == 1 and struct.unpack("<d", struct.pack("<q", i))[0] == float(exp["value"])
+
+
+# ---- random programs: postfix expressions with branches, errors, timeouts ----
+ADD, SUB, MUL, MOD, NEG, NOT, LT, LE, GT, GE, EQ, NE = 5, 6, 7, 9, 3, 4, 10, 11, 12, 13, 14, 15
+JMP, JF, JT, CALL, RET, PUSH, LOAD = 16, 17, 18, 19, 20, 0, 1
+
+
+def random_int_expr(rng, depth, code, consts):
+    r = rng.random()
+    if depth == 0 or r < 0.25:
+        if rng.random() < 0.5:
+            code.append((LOAD, rng.randrange(2), None))
+        else:
+            consts.append(rng.choice([0, 1, 2, 3, -1, 7, 1000, 2 ** 62, -(2 ** 63)]))
+            code.append((PUSH, len(consts) - 1, None))
+        return
+    kind = rng.random()
+    if kind < 0.55:
+        random_int_expr(rng, depth - 1, code, consts)
+        random_int_expr(rng, depth - 1, code, consts)
+        code.append((rng.choice([ADD, SUB, MUL, MOD]), None, None))
+    elif kind < 0.65:
+        random_int_expr(rng, depth - 1, code, consts)
+        code.append((NEG, None, None))
+    elif kind < 0.8:
+        random_int_expr(rng, depth - 1, code, consts)
+        random_int_expr(rng, depth - 1, code, consts)
+        code.append((CALL, rng.choice([1, 2, 13, 14, 28, 29]), 2))
+    else:  # if cond then a else b
+        random_bool_expr(rng, depth - 1, code, consts)
+        jf = len(code)
+        code.append([JF, 0, None])
+        random_int_expr(rng, depth - 1, code, consts)
+        j = len(code)
+        code.append([JMP, 0, None])
+        code[jf][1] = len(code)
+        random_int_expr(rng, depth - 1, code, consts)
+        code[j][1] = len(code)
+
+
+def random_bool_expr(rng, depth, code, consts):
+    if depth == 0 or rng.random() < 0.3:
+        random_int_expr(rng, 0, code, consts)
+        random_int_expr(rng, 0, code, consts)
+        code.append((rng.choice([LT, LE, GT, GE, EQ, NE]), None, None))
+        return
+    if rng.random() < 0.5:
+        random_bool_expr(rng, depth - 1, code, consts)
+        code.append((NOT, None, None))
+    else:
+        random_int_expr(rng, depth - 1, code, consts)
+        random_int_expr(rng, depth - 1, code, consts)
+        code.append((rng.choice([LT, EQ, GE]), None, None))
+
+
+def random_program(rng, depth):
+    code, consts = [], []
+    random_int_expr(rng, depth, code, consts)
+    code.append((RET, None, None))
+    return {"code": [tuple(c) for c in code], "consts": consts, "n_locals": 2}
+
+
+@pytest.mark.parametrize("n_cases,lanes,fuel", [(1, 512, 1000), (19, 512, 1000), (64, 512, 1000), (65, 17, 1000),
+                                                (1025, 512, 1000), (300, 33, 12)])
+def test_device_fitness_matches_oracle_on_random_programs(n_cases, lanes, fuel):
+    rng = random.Random(n_cases * 1000 + lanes)
+    programs = [random_program(rng, rng.randrange(1, 6)) for _ in range(200)]
+    inputs = [[rng.randrange(-50, 51), rng.choice([0, 1, -1, rng.randrange(-10 ** 6, 10 ** 6)])] for _ in range(n_cases)]
+    answers = [rng.randrange(-2000, 2000) for _ in range(n_cases)]
+    want = oracle_fitness(programs, inputs, answers, fuel=fuel, penalty=1000.0, lanes=lanes)
+    s = gagp_amd.Session(inputs, answers, fuel=fuel, lanes=lanes, penalty=1000.0)
+    got = s.eval(programs)
+    assert got[0] == want[0]
+    assert got[1] == want[1]
+    assert got[2] == want[2]
+
+
+def test_device_fitness_matches_oracle_on_fractional_scores():
+    # Discounted proximity makes per-case scores fractional, so the canonical
+    # lane/group order of the 48-bit accumulation decides the total.
+    rng = random.Random(5)
+    programs = [random_program(rng, rng.randrange(1, 5)) for _ in range(100)]
+    inputs = [[rng.randrange(-50, 51), rng.randrange(-9, 10)] for _ in range(777)]
+    answers = [rng.randrange(-200, 200) for _ in range(777)]
+    want = oracle_fitness(programs, inputs, answers, penalty=1000.0, lanes=512, proximity=1)
+    s = gagp_amd.Session(inputs, answers, penalty=1000.0, lanes=512, proximity=1)
+    got = s.eval(programs)
+    assert got[0] == want[0] and got[1] == want[1] and got[2] == want[2]
+
+
+def test_device_cases_match_oracle_per_case():
+    rng = random.Random(9)
+    inputs = [[rng.randrange(-50, 51), rng.randrange(-3, 4)] for _ in range(130)]
+    s = gagp_amd.Session(inputs, [0] * len(inputs), fuel=40)
+    for _ in range(40):
+        p = random_program(rng, rng.randrange(1, 6))
+        values, errors = s.eval_cases(p)
+        for k, row in enumerate(inputs):
+            err, out, _ = oracle_run(p, row, fuel=40)
+            assert errors[k] == err
+            if err == -1:
+                assert values[k][0] == out.tag and (values[k][1] == out.i if out.tag != 2 else values[k][2] == out.b)
