@@ -110,6 +110,8 @@ def make_value(v) -> Value:
         x.tag, x.b = TAG_BOOL, int(v)
     elif isinstance(v, int):
         x.tag, x.i = TAG_INT, v
+    elif isinstance(v, float):
+        x.tag, x.i = TAG_FLOAT, ctypes.c_int64.from_buffer_copy(ctypes.c_double(v)).value
     elif isinstance(v, tuple) and v[0] == "list":
         x.tag, x.i = TAG_INT_LIST, v[1] * 65536 + v[2]
     elif v is None:

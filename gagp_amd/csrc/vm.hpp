@@ -53,6 +53,8 @@ __device__ inline int wave_min(int x) {
 }
 #endif
 
+GHD inline double fbits(std::int64_t i) { union { std::int64_t i; double f; } u{i}; return u.f; }
+
 struct LaneResult {
   std::int64_t v;
   std::uint8_t t, b;
@@ -70,15 +72,27 @@ __device__ inline LaneResult vm_run(const ProgView& p, const CaseData& cd, int c
   int fuel = fuel0;
   int ip = 0;
   int uncharged = 0;
+  // Every iteration runs at least one live lane's instruction, which charges
+  // fuel or counts toward the zero-cost cycle limit; this bound is never
+  // reached by a correct schedule and ends the wave with ValueError otherwise.
+  const long long iteration_limit = 64LL * ((long long)fuel0 + 2LL * p.len + 2) + 64;
+  long long iterations = 0;
   for (;;) {
     int at;
     if (p.has_jumps) {
       at = wave_min(done ? 0x7fffffff : ip);
       if (at == 0x7fffffff) break;
     } else {
-      if (__ballot(!done) == 0) break;  // all lanes finished
-      at = ip;
-      at = __builtin_amdgcn_readfirstlane(at);
+      // Straight-line code: every live lane is at the same ip; take it from
+      // the first live lane (finished lanes keep a stale ip).
+      const unsigned long long live = __ballot(!done);
+      if (live == 0) break;
+      const int first = __ffsll((long long)live) - 1;
+      at = __builtin_amdgcn_readlane(ip, first);
+    }
+    if (++iterations > iteration_limit) {
+      if (!done) { r.err = ERR_VALUE; done = true; }
+      break;
     }
     const bool mine = !done && ip == at;
     if (at >= p.len) {  // ran off the end without RETURN
@@ -210,15 +224,30 @@ __device__ inline LaneResult vm_run(const ProgView& p, const CaseData& cd, int c
         const auto vi = [&](int k) { return st.v[(base + k) * 64 + lane]; };
         switch (ins.a) {
           case B_ABS:
-            if (argc != 1 || ti(0) != TAG_INT) { r.err = ERR_TYPE; break; }
-            out = vi(0) < 0 ? wrap_neg(vi(0)) : vi(0);
+            if (argc != 1 || !is_numeric(ti(0))) { r.err = ERR_TYPE; break; }
+            if (ti(0) == TAG_FLOAT) {
+              const double x = fbits(vi(0));
+              out = val_float(canon32(x < 0 ? -x : x)).i;
+              out_t = TAG_FLOAT;
+            } else out = vi(0) < 0 ? wrap_neg(vi(0)) : vi(0);
             break;
           case B_MIN: case B_MAX:
-            if (argc != 2 || ti(0) != TAG_INT || ti(1) != TAG_INT) { r.err = ERR_TYPE; break; }
-            out = ins.a == B_MIN ? (vi(0) < vi(1) ? vi(0) : vi(1)) : (vi(0) > vi(1) ? vi(0) : vi(1));
+            if (argc != 2 || !is_numeric(ti(0)) || ti(0) != ti(1)) { r.err = ERR_TYPE; break; }
+            if (ti(0) == TAG_FLOAT) {
+              const double a = fbits(vi(0)), b = fbits(vi(1));
+              out = val_float(canon32(ins.a == B_MIN ? (a <= b ? a : b) : (a >= b ? a : b))).i;
+              out_t = TAG_FLOAT;
+            } else out = ins.a == B_MIN ? (vi(0) < vi(1) ? vi(0) : vi(1)) : (vi(0) > vi(1) ? vi(0) : vi(1));
             break;
           case B_CLIP: {
-            if (argc != 3 || ti(0) != TAG_INT || ti(1) != TAG_INT || ti(2) != TAG_INT) { r.err = ERR_TYPE; break; }
+            if (argc != 3 || !is_numeric(ti(0)) || ti(0) != ti(1) || ti(0) != ti(2)) { r.err = ERR_TYPE; break; }
+            if (ti(0) == TAG_FLOAT) {
+              const double x = fbits(vi(0)), lo = fbits(vi(1)), hi = fbits(vi(2));
+              const double m = x > lo ? x : lo;  // min(max(x, lo), hi)
+              out = val_float(canon32(m < hi ? m : hi)).i;
+              out_t = TAG_FLOAT;
+              break;
+            }
             const std::int64_t m = vi(0) > vi(1) ? vi(0) : vi(1);
             out = m < vi(2) ? m : vi(2);
             break;
