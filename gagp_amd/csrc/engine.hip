@@ -84,10 +84,14 @@ struct DSearch {
   int *code_len, *code_jumps;
   std::uint64_t* child_hash;
   int *pending, *n_pending;
-  int* table;
+  std::uint64_t* table_key;  // genotype hash | 1, 0 = empty
+  int* table;                // a member with that hash
   int table_mask;
-  std::uint64_t* mt;
-  int* mt_index;
+  std::uint64_t* mt;      // std::mt19937_64 state after its last twist
+  std::uint64_t* mt_left; // tempered words generated and not yet consumed
+  int* n_left;
+  int place_lds;          // placement keys fit in LDS
+  std::uint64_t* member_hash;
   std::uint64_t* round_seed;
   int *round, *best, *placed;
   int *hist_origin, *hist_n, *hist_nc;
@@ -175,20 +179,26 @@ __global__ void k_vary(DSearch* all) {
 }
 
 // ---- member hash table (parent_fitness_cache.cuh:11-58) ----
+// One entry per distinct genotype hash: identical members share a hash, and
+// any one of them carries the result a child identical to them reuses (the
+// rows are compared in full before a result is reused). Keys are hash | 1,
+// 0 marks an empty slot.
 __global__ void k_clear(DSearch* all) {
   const DSearch& d = all[blockIdx.y];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (!search_live(d) || i > d.table_mask) return;
-  d.table[i] = -1;
+  d.table_key[i] = 0;
 }
 __global__ void k_index(DSearch* all) {
   const DSearch& d = all[blockIdx.y];
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (!search_live(d) || m >= d.P) return;
-  std::uint64_t h = row_hash(member_row(d, m));
+  const std::uint64_t h = d.member_hash[m], key = h | 1;
   for (int probe = 0; probe <= d.table_mask; ++probe) {
     const int slot = int((h + std::uint64_t(probe)) & std::uint64_t(d.table_mask));
-    if (atomicCAS(d.table + slot, -1, m) == -1) return;
+    const unsigned long long seen = atomicCAS(reinterpret_cast<unsigned long long*>(d.table_key) + slot, 0ull, key);
+    if (seen == 0) { d.table[slot] = m; return; }
+    if (seen == key) return;  // a member with this hash is indexed
   }
   raise(d, 400);
 }
@@ -200,10 +210,15 @@ __global__ void k_reuse(DSearch* all) {
   const CRow child = child_row(d, c);
   if (r < 0) {
     const std::uint64_t h = d.child_hash[c];
+    const std::uint64_t key = h | 1;
     for (int probe = 0; probe <= d.table_mask; ++probe) {
-      const int m = d.table[int((h + std::uint64_t(probe)) & std::uint64_t(d.table_mask))];
-      if (m < 0) break;
-      if (rows_equal(member_row(d, m), child)) { r = m; break; }
+      const int slot = int((h + std::uint64_t(probe)) & std::uint64_t(d.table_mask));
+      const std::uint64_t k = d.table_key[slot];
+      if (k == 0) break;
+      if (k != key) continue;
+      const int m = d.table[slot];
+      if (d.member_hash[m] == h && rows_equal(member_row(d, m), child)) r = m;
+      break;
     }
   }
   d.reuse[c] = r;
@@ -376,8 +391,9 @@ __device__ inline std::uint64_t mt_next_word(std::uint64_t cur, std::uint64_t nx
 }
 // In-place twist by the whole block: element i < 156 reads only old words;
 // 156 <= i < 311 reads the new word i-156; 311 reads new words 0 and 155.
+// blockDim.x >= 64.
 __device__ inline void mt_twist(std::uint64_t* mt) {
-  std::uint64_t v[2];
+  std::uint64_t v[3];
   int n = 0;
   for (int i = threadIdx.x; i < 156; i += blockDim.x) v[n++] = mt_next_word(mt[i], mt[i + 1], mt[i + 156]);
   __syncthreads();
@@ -405,9 +421,11 @@ __global__ void k_seed(DSearch* all) {
   __syncthreads();
   mt_twist(mt);
   for (int i = threadIdx.x; i < 312; i += blockDim.x) d.mt[i] = mt[i];
+  // Word 0 seeds round 0; words 1..311 wait for the first placement.
+  for (int i = 1 + threadIdx.x; i < 312; i += blockDim.x) d.mt_left[i - 1] = mt_temper(mt[i]);
   if (threadIdx.x == 0) {
     *d.round_seed = mt_temper(mt[0]);
-    *d.mt_index = 1;
+    *d.n_left = 311;
     *d.round = 0;
   }
 }
@@ -422,6 +440,7 @@ __global__ void k_init_keys(DSearch* all) {
     if (d.grammar_sites) { for (int i = 0; i < r.n; ++i) n += r.site[i] >= 0; }
     else n += unsigned(r.n);
     d.nodes[m] = n;
+    d.member_hash[m] = row_hash(r);
   }
   __syncthreads();
   __shared__ int best[256];
@@ -461,66 +480,177 @@ __global__ void k_record(DSearch* all) {
 }
 
 // ---- placement (evolve.cpp:378-422) ----
-__global__ __launch_bounds__(256) void k_place(DSearch* all) {
+// The reference draws victims one group of `replacement_size` words at a
+// time. Here one wavefront examines up to W consecutive groups at once
+// against the keys as they stand at the start of the chunk, then commits the
+// longest prefix of groups whose outcome cannot differ from the sequential
+// order. A chunk ends before the first group that drew a member replaced by
+// an earlier group of the chunk, after the first group whose child becomes
+// the new best (the redraw test and the best tracking change there), and
+// after the group that places the last child. Group 0 is always committed.
+constexpr int kRing = 1024;        // tempered generator words, a power of two
+constexpr int kPlaceWords = 512;   // words examined per chunk at most
+
+template <bool kLds>
+struct PlaceKeys {
+  // kLds: fitness and (wrong << 12 | nodes) staged in LDS; otherwise the
+  // member arrays in HBM, read past the L1 so every lane sees the stores of
+  // the previous chunk.
+  const DSearch* d;
+  double* f;
+  unsigned* wn;
+  __device__ double fit(int m) const {
+    if (kLds) return f[m];
+    const unsigned long long bits = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(d->fitness) + m,
+                                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __longlong_as_double((long long)bits);
+  }
+  __device__ std::uint64_t order(int m) const {
+    if (kLds) return wn[m];
+    const unsigned w = __hip_atomic_load(d->wrong + m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned n = __hip_atomic_load(d->nodes + m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return std::uint64_t(w) << 32 | n;
+  }
+  __device__ std::uint64_t pack(unsigned w, unsigned n) const { return kLds ? std::uint64_t(w << 12 | n) : std::uint64_t(w) << 32 | n; }
+  // `better` (fitness desc, wrong asc, nodes asc, index asc) of a key against member b.
+  __device__ bool key_better(double fa, std::uint64_t oa, int a, int b) const {
+    const double fb = fit(b);
+    if (fa != fb) return fa > fb;
+    if (d->has_wrong) { const std::uint64_t ob = order(b); if (oa != ob) return oa < ob; }
+    return a < b;
+  }
+  __device__ bool better(int a, int b) const { return key_better(fit(a), d->has_wrong ? order(a) : 0, a, b); }
+  __device__ void store(int m, double fv, unsigned w, unsigned n) const {
+    if (kLds) { f[m] = fv; wn[m] = w << 12 | n; }
+    d->fitness[m] = fv;
+    d->wrong[m] = w;
+    d->nodes[m] = n;
+  }
+};
+
+template <bool kLds>
+__global__ __launch_bounds__(64) void k_place(DSearch* all) {
   const DSearch& d = all[blockIdx.x];
-  if (!search_live(d)) return;
+  if (!search_live(d) || d.place_lds != int(kLds)) return;
+  extern __shared__ __attribute__((aligned(16))) unsigned char place_smem[];
+  __shared__ std::uint64_t ring[kRing];
+  __shared__ unsigned ridx[kRing];
   __shared__ std::uint64_t mt[312];
-  __shared__ std::uint64_t raw[640];
-  __shared__ int head, tail, state, b, best;
-  for (int i = threadIdx.x; i < 312; i += blockDim.x) mt[i] = d.mt[i];
-  __syncthreads();
-  const int idx = *d.mt_index;
-  for (int i = idx + threadIdx.x; i < 312; i += blockDim.x) raw[i - idx] = mt_temper(mt[i]);
-  if (threadIdx.x == 0) { head = 0; tail = 312 - idx; state = 0; b = 0; best = *d.best; }
-  __syncthreads();
-  const Keys keys{d.fitness, d.wrong, d.nodes, d.has_wrong};
-  const std::uint64_t P = std::uint64_t(d.P);
-  for (;;) {
-    if (threadIdx.x == 0) {
-      state = 1;  // needs more draws unless finished below
-      int h = head, bb = b, bst = best;
-      const int rs = d.replacement_size;
-      for (;;) {
-        if (bb < d.B) {
-          if (tail - h < rs) break;
-          int worst = int(raw[h] % P);
-          for (int i = 1; i < rs; ++i) {
-            const int cand = int(raw[h + i] % P);
-            if (better(keys, worst, cand)) worst = cand;
-          }
-          h += rs;
-          if (worst == bst) continue;  // the best member is never replaced: redraw
-          d.fitness[worst] = d.c_fit[bb];
-          d.wrong[worst] = d.c_wrong[bb];
-          d.nodes[worst] = d.c_nodes[bb];
-          d.placed[worst] = bb;
-          if (better(keys, worst, bst)) bst = worst;
-          ++bb;
-        } else {
-          if (tail - h < 1) break;
-          *d.round_seed = raw[h++];
-          state = 2;
-          break;
-        }
-      }
-      head = h; b = bb; best = bst;
+  const int lane = threadIdx.x;
+  const int P = d.P, B = d.B, rs = d.replacement_size;
+  const std::uint64_t P64 = std::uint64_t(P);
+  PlaceKeys<kLds> keys{&d, reinterpret_cast<double*>(place_smem),
+                       reinterpret_cast<unsigned*>(place_smem + (kLds ? 8 * std::size_t(P) : 0))};
+  unsigned* mark = reinterpret_cast<unsigned*>(place_smem + (kLds ? 12 * std::size_t(P) : 0));
+  // x mod P by a Barrett step: q = mulhi(x, floor((2^64-1)/P)) is
+  // floor(x/P) or one less, so one conditional subtraction is exact.
+  const std::uint64_t barrett = ~0ull / P64;
+  const auto mod_p = [&](std::uint64_t x) {
+    std::uint64_t r = x - __umul64hi(x, barrett) * P64;
+    return unsigned(r >= P64 ? r - P64 : r);
+  };
+  constexpr int kStage = 16;  // loads in flight per lane while staging keys
+  for (int m0 = 0; m0 < P; m0 += 64 * kStage) {
+    double f[kStage];
+    unsigned w[kStage], n[kStage];
+#pragma unroll
+    for (int k = 0; k < kStage; ++k) {
+      const int m = m0 + k * 64 + lane;
+      if (kLds && m < P) { f[k] = d.fitness[m]; w[k] = d.wrong[m]; n[k] = d.nodes[m]; }
     }
+#pragma unroll
+    for (int k = 0; k < kStage; ++k) {
+      const int m = m0 + k * 64 + lane;
+      if (m < P) {
+        if (kLds) { keys.f[m] = f[k]; keys.wn[m] = d.has_wrong ? w[k] << 12 | n[k] : 0u; }
+        mark[m] = 0;
+      }
+    }
+  }
+  for (int i = lane; i < 312; i += 64) mt[i] = d.mt[i];
+  int head = 0, tail = *d.n_left;
+  for (int i = lane; i < tail; i += 64) {
+    const std::uint64_t t = d.mt_left[i];
+    ring[i] = t;
+    ridx[i] = mod_p(t);
+  }
+  __syncthreads();
+  const auto refill = [&](int need) {
+    while (tail - head < need) {
+      mt_twist(mt);
+      for (int i = lane; i < 312; i += 64) {
+        const std::uint64_t t = mt_temper(mt[i]);
+        ring[(tail + i) & (kRing - 1)] = t;
+        ridx[(tail + i) & (kRing - 1)] = mod_p(t);
+      }
+      tail += 312;
+      __syncthreads();
+    }
+  };
+  const int W = rs < kPlaceWords / 64 ? 64 : kPlaceWords / rs;
+  const std::uint64_t below = (1ull << lane) - 1;
+  int b = 0, best = *d.best;
+  unsigned chunk = 0;
+  const unsigned chunk_limit = 4u * unsigned(B) + 4096u;
+  while (b < B) {
+    if (++chunk > chunk_limit) { if (lane == 0) raise(d, 500); break; }
+    refill(W * rs);
+    // The chunk places children b .. b+63 at most: fetch their keys now.
+    double pf_f = 0;
+    unsigned pf_w = 0, pf_n = 0;
+    if (b + lane < B) { pf_f = d.c_fit[b + lane]; pf_w = d.c_wrong[b + lane]; pf_n = d.c_nodes[b + lane]; }
+    const bool active = lane < W;
+    const int base = head + lane * rs;
+    int worst = -1;
+    if (active) {
+      worst = int(ridx[base & (kRing - 1)]);
+      for (int i = 1; i < rs; ++i) {
+        const int c = int(ridx[(base + i) & (kRing - 1)]);
+        if (keys.better(worst, c)) worst = c;
+      }
+    }
+    const bool place = active && worst != best;
+    const std::uint64_t pmask = __ballot(place);
+    const int rank = __popcll(pmask & below);
+    const int child = b + rank;
+    const bool fills = place && child < B;
+    const double cf = __shfl(pf_f, rank, 64);
+    const unsigned cw = unsigned(__shfl(int(pf_w), rank, 64)), cn = unsigned(__shfl(int(pf_n), rank, 64));
+    if (place) atomicMax(&mark[worst], chunk * 64u + unsigned(63 - lane));
     __syncthreads();
-    if (state == 2) break;
-    // Keep the unconsumed tail, then append the next 312 outputs.
-    const int left = tail - head;
-    std::uint64_t keep = 0;
-    if (int(threadIdx.x) < left) keep = raw[head + threadIdx.x];
-    __syncthreads();
-    if (int(threadIdx.x) < left) raw[threadIdx.x] = keep;
-    mt_twist(mt);
-    for (int i = threadIdx.x; i < 312; i += blockDim.x) raw[left + i] = mt_temper(mt[i]);
-    if (threadIdx.x == 0) { head = 0; tail = left + 312; }
+    bool conflict = false;
+    if (active)
+      for (int i = 0; i < rs; ++i) {
+        const unsigned m = mark[ridx[(base + i) & (kRing - 1)]];
+        conflict = conflict || ((m >> 6) == chunk && int(63u - (m & 63u)) < lane);
+      }
+    const bool raises = fills && keys.key_better(cf, keys.pack(cw, cn), worst, best);
+    int cut = W;
+    const std::uint64_t last = __ballot(fills && child == B - 1);
+    if (last) cut = min(cut, __ffsll((long long)last));
+    const std::uint64_t clash = __ballot(active && conflict);
+    if (clash) cut = min(cut, __ffsll((long long)clash) - 1);
+    const std::uint64_t lifts = __ballot(raises);
+    const int lifter = lifts ? __ffsll((long long)lifts) - 1 : 64;
+    if (lifter < cut) cut = lifter + 1;
+    if (fills && lane < cut) {
+      keys.store(worst, cf, cw, cn);
+      d.placed[worst] = child;
+    }
+    if (lifter < cut) best = __shfl(worst, lifter, 64);
+    b += __popcll(cut >= 64 ? pmask : pmask & ((1ull << cut) - 1));
+    head += cut * rs;
     __syncthreads();
   }
-  for (int i = threadIdx.x; i < 312; i += blockDim.x) d.mt[i] = mt[i];
-  if (threadIdx.x == 0) {
-    *d.mt_index = 312 - (tail - head);
+  refill(1);
+  const std::uint64_t next_seed = ring[head & (kRing - 1)];
+  ++head;
+  const int left = tail - head;
+  for (int i = lane; i < left; i += 64) d.mt_left[i] = ring[(head + i) & (kRing - 1)];
+  for (int i = lane; i < 312; i += 64) d.mt[i] = mt[i];
+  if (lane == 0) {
+    *d.n_left = left;
+    *d.round_seed = next_seed;
     *d.best = best;
     *d.n_pending = 0;
     atomicAdd(d.round, 1);
@@ -537,6 +667,7 @@ __global__ void k_commit(DSearch* all) {
   const std::size_t o = std::size_t(m) * d.R;
   Row dst{d.cur_origin + o, d.cur_site + o, d.cur_cval + o, 0, 0};
   copy_row(src, dst, d.cur_n + m, d.cur_nc + m);
+  d.member_hash[m] = d.child_hash[c];
   d.placed[m] = -1;
 }
 
@@ -745,8 +876,12 @@ struct SearchState {
     x.child_hash = a.alloc<std::uint64_t>(B);
     x.pending = a.alloc<int>(B); x.n_pending = a.alloc<int>(1);
     const int tsize = next_pow2(2 * P);
-    x.table = a.alloc<int>(tsize); x.table_mask = tsize - 1;
-    x.mt = a.alloc<std::uint64_t>(312); x.mt_index = a.alloc<int>(1); x.round_seed = a.alloc<std::uint64_t>(1);
+    x.table_key = a.alloc<std::uint64_t>(tsize); x.table = a.alloc<int>(tsize); x.table_mask = tsize - 1;
+    x.mt = a.alloc<std::uint64_t>(312); x.mt_left = a.alloc<std::uint64_t>(kRing); x.n_left = a.alloc<int>(1);
+    x.round_seed = a.alloc<std::uint64_t>(1);
+    x.member_hash = a.alloc<std::uint64_t>(P);
+    // Placement keys (fitness, wrong << 12 | nodes) and victim marks in LDS.
+    x.place_lds = P <= 8192 && train.n < (1 << 20) && int(layout.fixed_nodes) + R < 4096;
     x.round = a.alloc<int>(1); x.best = a.alloc<int>(1); x.placed = a.alloc<int>(P);
     x.hist_origin = a.alloc<int>(std::size_t(G) * R); x.hist_cval = a.alloc<Val>(std::size_t(G) * R);
     x.hist_n = a.alloc<int>(G); x.hist_nc = a.alloc<int>(G);
@@ -804,6 +939,7 @@ struct Batch::Impl {
   hipStream_t stream = nullptr;
   KernelTimer timer;
   int max_P = 0, max_B = 0, max_slots = 0, max_table = 0, max_rounds = 0;
+  std::size_t place_lds_bytes = 0, place_hbm_bytes = 0;  // dynamic LDS of the two placement variants
   double run_ms = 0, init_ms = 0;
   bool graph_ready = false;
   hipGraphExec_t graph_exec = nullptr;
@@ -824,7 +960,10 @@ struct Batch::Impl {
     timer.time("k_index", stream, [&] { hipLaunchKernelGGL(k_index, member_grid, dim3(128), 0, stream, d_all); });
     timer.time("k_reuse", stream, [&] { hipLaunchKernelGGL(k_reuse, child_grid, dim3(128), 0, stream, d_all); });
     timer.time("k_eval", stream, [&] { hipLaunchKernelGGL(k_eval, eval_grid, dim3(kEvalThreads), lds_bytes(), stream, d_all, 1); });
-    timer.time("k_place", stream, [&] { hipLaunchKernelGGL(k_place, dim3(n), dim3(256), 0, stream, d_all); });
+    if (place_lds_bytes)
+      timer.time("k_place", stream, [&] { hipLaunchKernelGGL(k_place<true>, dim3(n), dim3(64), place_lds_bytes, stream, d_all); });
+    if (place_hbm_bytes)
+      timer.time("k_place_hbm", stream, [&] { hipLaunchKernelGGL(k_place<false>, dim3(n), dim3(64), place_hbm_bytes, stream, d_all); });
     timer.time("k_commit", stream, [&] { hipLaunchKernelGGL(k_commit, member_grid, dim3(128), 0, stream, d_all); });
   }
 };
@@ -862,6 +1001,8 @@ void Batch::prepare() {
     I.max_slots = std::max(I.max_slots, s->d.stack_slots);
     I.max_table = std::max(I.max_table, s->d.table_mask + 1);
     I.max_rounds = std::max(I.max_rounds, s->rounds);
+    if (s->d.place_lds) I.place_lds_bytes = std::max(I.place_lds_bytes, std::size_t(16) * s->P);
+    else I.place_hbm_bytes = std::max(I.place_hbm_bytes, std::size_t(4) * s->P);
   }
   const int n = int(host.size());
   if (!n) return;
@@ -870,6 +1011,12 @@ void Batch::prepare() {
   if (I.lds_bytes() > 65536) {
     HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_eval), hipFuncAttributeMaxDynamicSharedMemorySize, int(I.lds_bytes())));
   }
+  if (I.place_lds_bytes)
+    HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_place<true>), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               int(I.place_lds_bytes)));
+  if (I.place_hbm_bytes)
+    HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_place<false>), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               int(I.place_hbm_bytes)));
   // Initial population, its evaluation, keys and the run generator.
   const dim3 member_grid((I.max_P + 63) / 64, n);
   I.timer.time("k_init_pop", I.stream, [&] { hipLaunchKernelGGL(k_init_pop, member_grid, dim3(64), 0, I.stream, I.d_all); });

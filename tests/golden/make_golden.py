@@ -33,6 +33,9 @@ RUNS = {
     "poly_c1_p1024_g64_s0": ("poly_c1", 1024, 64, 0, True),
     "quintic_p512_g16_s3": ("quintic", 512, 16, 3, False),
     "quintic_p4096_g128_s0": ("quintic", 4096, 128, 0, False),
+    # populations above 8192 keep the placement keys in HBM
+    "quintic_p16384_g3_s11": ("quintic", 16384, 3, 11, False),
+    "polynomial_p12000_g4_s2": ("polynomial", 12000, 4, 2, False),
 }
 
 
