@@ -20,7 +20,19 @@
 #define GHD
 #endif
 
+// Device memory referenced from structs: in the device pass the pointers are
+// typed as global memory, so loads and stores through them are global (not
+// flat) instructions, which do not serialize with LDS traffic.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define GAS __attribute__((address_space(1)))
+#else
+#define GAS
+#endif
+
 namespace gagp_amd {
+
+template <class T>
+GHD inline GAS T* gbl(T* p) { return (GAS T*)p; }
 
 enum Tag : std::uint8_t {
   TAG_INT = 0, TAG_FLOAT = 1, TAG_BOOL = 2, TAG_CHAR = 3, TAG_STRING = 4,

@@ -63,43 +63,43 @@ struct DSearch {
   int fixed_nodes, grammar_sites, has_wrong, expected_type, n_locals, site_depth;
   std::uint64_t seed;
   PhaseView g;
-  const std::uint8_t* local_types;
+  GAS const std::uint8_t* local_types;
   CaseData cd;
-  int *cur_origin, *cur_site, *cur_n, *cur_nc;
-  Val* cur_cval;
-  int *x_origin, *x_site, *x_n, *x_nc;
-  Val* x_cval;
-  int *t_origin, *t_site;
-  Val* t_cval;
-  int *s_origin, *s_site;
-  Val* s_cval;
-  double* fitness;
-  unsigned *wrong, *nodes;
-  OpStats* stats;
-  int* reuse;
-  double* c_fit;
-  unsigned *c_wrong, *c_nodes;
-  Ins* code;
-  Val* pool;
-  int *code_len, *code_jumps;
-  std::uint64_t* child_hash;
-  int *pending, *n_pending;
-  std::uint64_t* table_key;  // genotype hash | 1, 0 = empty
-  int* table;                // a member with that hash
+  GAS int *cur_origin, *cur_site, *cur_n, *cur_nc;
+  GAS Val* cur_cval;
+  GAS int *x_origin, *x_site, *x_n, *x_nc;
+  GAS Val* x_cval;
+  GAS int *t_origin, *t_site;
+  GAS Val* t_cval;
+  GAS int *s_origin, *s_site;
+  GAS Val* s_cval;
+  GAS double* fitness;
+  GAS unsigned *wrong, *nodes;
+  GAS OpStats* stats;
+  GAS int* reuse;
+  GAS double* c_fit;
+  GAS unsigned *c_wrong, *c_nodes;
+  GAS Ins* code;
+  GAS Val* pool;
+  GAS int *code_len, *code_jumps;
+  GAS std::uint64_t* child_hash;
+  GAS int *pending, *n_pending;
+  GAS std::uint64_t* table_key;  // genotype hash | 1, 0 = empty
+  GAS int* table;                // a member with that hash
   int table_mask;
-  std::uint64_t* mt;      // std::mt19937_64 state after its last twist
-  std::uint64_t* mt_left; // tempered words generated and not yet consumed
-  int* n_left;
+  GAS std::uint64_t* mt;      // std::mt19937_64 state after its last twist
+  GAS std::uint64_t* mt_left; // tempered words generated and not yet consumed
+  GAS int* n_left;
   int place_lds;          // placement keys fit in LDS
-  std::uint64_t* member_hash;
-  std::uint64_t* round_seed;
-  int *round, *best, *placed;
-  int *hist_origin, *hist_n, *hist_nc;
-  Val* hist_cval;
-  double *hist_fit, *hist_all;
-  unsigned* hist_wrong;
-  Counters* counters;
-  int* error;
+  GAS std::uint64_t* member_hash;
+  GAS std::uint64_t* round_seed;
+  GAS int *round, *best, *placed;
+  GAS int *hist_origin, *hist_n, *hist_nc;
+  GAS Val* hist_cval;
+  GAS double *hist_fit, *hist_all;
+  GAS unsigned* hist_wrong;
+  GAS Counters* counters;
+  GAS int* error;
 };
 
 __device__ inline CRow member_row(const DSearch& d, int m) {
@@ -672,7 +672,7 @@ __global__ void k_commit(DSearch* all) {
 }
 
 // ---- evaluation session kernel (external bytecode) ----
-struct DProg { const Ins* code; const Val* pool; int len, has_jumps, n_locals; };
+struct DProg { GAS const Ins* code; GAS const Val* pool; int len, has_jumps, n_locals; };
 __global__ __launch_bounds__(kEvalThreads) void k_eval_programs(const DProg* progs, int n, EvalParams e, int slots,
                                                                  double* fitness, unsigned* wrong, unsigned* counts) {
   extern __shared__ unsigned char smem[];
@@ -714,18 +714,18 @@ struct DeviceArena {
   std::vector<void*> blocks;
   std::size_t bytes = 0;
   template <class T>
-  T* alloc(std::size_t n) {
+  GAS T* alloc(std::size_t n) {
     void* p = nullptr;
     const std::size_t sz = std::max<std::size_t>(n, 1) * sizeof(T);
     HIP_OK(hipMalloc(&p, sz));
     blocks.push_back(p);
     bytes += sz;
-    return static_cast<T*>(p);
+    return gbl(static_cast<T*>(p));
   }
   template <class T>
-  T* upload(const std::vector<T>& v) {
-    T* p = alloc<T>(v.size());
-    if (!v.empty()) HIP_OK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  GAS T* upload(const std::vector<T>& v) {
+    GAS T* p = alloc<T>(v.size());
+    if (!v.empty()) HIP_OK(hipMemcpy((void*)p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
     return p;
   }
   ~DeviceArena() { for (void* p : blocks) (void)hipFree(p); }
@@ -897,9 +897,9 @@ struct SearchState {
   }
 
   template <class T>
-  std::vector<T> down(const T* p, std::size_t n) const {
+  std::vector<T> down(GAS const T* p, std::size_t n) const {
     std::vector<T> v(n);
-    if (n) HIP_OK(hipMemcpy(v.data(), p, n * sizeof(T), hipMemcpyDeviceToHost));
+    if (n) HIP_OK(hipMemcpy(v.data(), (const void*)p, n * sizeof(T), hipMemcpyDeviceToHost));
     return v;
   }
   int device_error() const { return down(d.error, 1)[0]; }
@@ -1091,9 +1091,9 @@ json::Value judge(const SearchState& s, const Ast& ast, const std::vector<int>& 
   std::vector<Ins> code(std::size_t(s.d.code_cap));
   std::vector<Val> pool(std::size_t(s.d.pool_cap));
   std::vector<int> sites(origin.size(), -1);
-  PhaseView gv{s.phase.exprs.data(), int(s.phase.exprs.size()), s.phase.prods.data(), int(s.phase.prods.size()),
-               s.phase.nts.data(), int(s.phase.nts.size()), s.phase.domains.data(), int(s.phase.domains.size()),
-               s.phase.values.data(), int(s.phase.values.size()), s.phase.minimum.data(), s.phase.depth_count,
+  PhaseView gv{gbl(s.phase.exprs.data()), int(s.phase.exprs.size()), gbl(s.phase.prods.data()), int(s.phase.prods.size()),
+               gbl(s.phase.nts.data()), int(s.phase.nts.size()), gbl(s.phase.domains.data()), int(s.phase.domains.size()),
+               gbl(s.phase.values.data()), int(s.phase.values.size()), gbl(s.phase.minimum.data()), s.phase.depth_count,
                s.phase.logical_count, s.phase.site_depth ? 1 : 0, s.phase.root};
   const CRow row{origin.data(), sites.data(), cval.data(), int(origin.size()), int(cval.size())};
   std::vector<std::uint8_t> local_types = s.train.input_types;
@@ -1379,14 +1379,14 @@ void Session::eval(const std::vector<ProgramIn>& programs, double* fitness, std:
   }
   if (pool.empty()) pool.push_back(val_invalid());
   DeviceArena tmp;
-  Ins* dcode = tmp.upload(code);
-  Val* dpool = tmp.upload(pool);
+  auto dcode = tmp.upload(code);
+  auto dpool = tmp.upload(pool);
   std::vector<DProg> progs(static_cast<std::size_t>(n));
   for (int i = 0; i < n; ++i) progs[i] = DProg{dcode + code_off[i], dpool + pool_off[i], lens[i], jumps_v[i], locals[i]};
-  DProg* dprogs = tmp.upload(progs);
-  double* dfit = tmp.alloc<double>(std::size_t(n));
-  unsigned* dwrong = tmp.alloc<unsigned>(std::size_t(n));
-  unsigned* dcounts = tmp.alloc<unsigned>(std::size_t(n) * 3);
+  auto dprogs = tmp.upload(progs);
+  auto dfit = tmp.alloc<double>(std::size_t(n));
+  auto dwrong = tmp.alloc<unsigned>(std::size_t(n));
+  auto dcounts = tmp.alloc<unsigned>(std::size_t(n) * 3);
   const EvalParams e{I.cases.cd, I.fuel, I.lanes, I.penalty, I.policy};
   const std::size_t lds = std::size_t(kWavesPerBlock) * slots * 64 * 10;
   if (lds > 65536)
@@ -1420,8 +1420,8 @@ void Session::eval_cases(const ProgramIn& program, std::vector<Val>& values, std
   if (consts.empty()) consts.push_back(val_invalid());
   const DProg p{tmp.upload(ins), tmp.upload(consts), int(ins.size()), j, program.n_locals};
   const int C = I.cases.cd.C;
-  Val* dv = tmp.alloc<Val>(std::size_t(C));
-  int* de = tmp.alloc<int>(std::size_t(C));
+  auto dv = tmp.alloc<Val>(std::size_t(C));
+  auto de = tmp.alloc<int>(std::size_t(C));
   const std::size_t lds = std::size_t(kWavesPerBlock) * slots * 64 * 10;
   if (lds > 65536)
     HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_eval_cases), hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));

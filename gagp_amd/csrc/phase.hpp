@@ -28,12 +28,12 @@ constexpr int kPending = 192;
 enum Status : int { ST_OK = 0, ST_INVALID = 1, ST_UNSUPPORTED = 2, ST_CAPACITY = 3 };
 
 struct PhaseView {
-  const PExpr* expr; int n_expr;
-  const PProd* prod; int n_prod;
-  const PNt* nt; int n_nt;
-  const PDomain* dom; int n_dom;
-  const Val* values; int n_values;
-  const unsigned* minimum; int depth_count, logical_count; int site_depth;
+  GAS const PExpr* expr; int n_expr;
+  GAS const PProd* prod; int n_prod;
+  GAS const PNt* nt; int n_nt;
+  GAS const PDomain* dom; int n_dom;
+  GAS const Val* values; int n_values;
+  GAS const unsigned* minimum; int depth_count, logical_count; int site_depth;
   int root;
 };
 

@@ -17,21 +17,21 @@
 namespace gagp_amd {
 
 struct ProgView {
-  const Ins* code;
-  const Val* pool;
+  GAS const Ins* code;
+  GAS const Val* pool;
   int len;
   int n_locals;
   int has_jumps;
 };
 
 struct CaseData {
-  const std::int64_t* in_val;  // [n_inputs][C]
-  const std::uint8_t* in_tag;  // [n_inputs][C] (TAG_INVALID: unbound)
-  const std::uint8_t* in_b;    // [n_inputs][C]
-  const std::int64_t* exp_val; // [C]
-  const std::uint8_t* exp_tag; // [C]
-  const std::uint8_t* exp_b;   // [C]
-  const std::int64_t* arena;   // IntList elements
+  GAS const std::int64_t* in_val;  // [n_inputs][C]
+  GAS const std::uint8_t* in_tag;  // [n_inputs][C] (TAG_INVALID: unbound)
+  GAS const std::uint8_t* in_b;    // [n_inputs][C]
+  GAS const std::int64_t* exp_val; // [C]
+  GAS const std::uint8_t* exp_tag; // [C]
+  GAS const std::uint8_t* exp_b;   // [C]
+  GAS const std::int64_t* arena;   // IntList elements
   int C, n_inputs;
 };
 
