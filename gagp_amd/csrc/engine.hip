@@ -25,6 +25,7 @@
 #include <stdexcept>
 
 #include "engine.hpp"
+#include "phase_wave.hpp"
 #include "vm.hpp"
 
 namespace gagp_amd {
@@ -135,47 +136,54 @@ __global__ void k_init_pop(DSearch* all) {
   d.placed[m] = -1;
 }
 
-// ---- lowering: rows of members (mode 0) or children (mode 1) ----
-__device__ inline void lower_into(const DSearch& d, const CRow& row, int slot) {
+// ---- lowering: one wavefront per row ----
+__device__ inline void lower_into_w(const DSearch& d, const CRow& row, int slot) {
   Ins* code = d.code + std::size_t(slot) * d.code_cap;
   Val* pool = d.pool + std::size_t(slot) * d.pool_cap;
-  const LowerResult r = lower_phase(d.g, row, d.local_types, d.n_locals, d.expected_type, code, d.code_cap, pool, d.pool_cap);
-  if (r.status != ST_OK || r.stack_bound < 1 || r.stack_bound > d.stack_slots) { raise(d, 200 + r.status); d.code_len[slot] = 0; return; }
-  bool jumps = false;
-  for (int i = 0; i < r.code; ++i) jumps |= code[i].op == OP_JMP || code[i].op == OP_JMP_IF_FALSE || code[i].op == OP_JMP_IF_TRUE;
-  d.code_len[slot] = r.code;
-  d.code_jumps[slot] = jumps;
+  const LowerResult r = lower_phase_w(d.g, row, d.local_types, d.n_locals, d.expected_type, code, d.code_cap, pool, d.pool_cap);
+  const bool ok = r.status == ST_OK && r.stack_bound >= 1 && r.stack_bound <= d.stack_slots;
+  if (wlane() == 0) {
+    if (!ok) raise(d, 200 + r.status);
+    d.code_len[slot] = ok ? r.code : 0;
+    d.code_jumps[slot] = r.jumps;
+  }
 }
-__global__ void k_lower_members(DSearch* all) {
+constexpr int kVaryWaves = 4;  // wavefronts (rows) per block of the row kernels
+__global__ __launch_bounds__(64 * kVaryWaves) void k_lower_members(DSearch* all) {
   const DSearch& d = all[blockIdx.y];
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  const int m = blockIdx.x * kVaryWaves + int(threadIdx.x >> 6);
   if (!d.active || m >= d.P || *d.error) return;
-  lower_into(d, member_row(d, m), m);
+  lower_into_w(d, member_row(d, m), m);
 }
 
-// ---- reproduction: one thread per child (phase_operators.cuh:156-265) ----
-__global__ void k_vary(DSearch* all) {
+// ---- reproduction: one wavefront per child (phase_operators.cuh:156-265) ----
+__global__ __launch_bounds__(64 * kVaryWaves) void k_vary(DSearch* all) {
   const DSearch& d = all[blockIdx.y];
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.x * kVaryWaves + int(threadIdx.x >> 6);
   if (!search_live(d) || c >= d.B) return;
   const std::uint64_t seed = *d.round_seed;
   const Keys keys{d.fitness, d.wrong, d.nodes, d.has_wrong};
   const std::size_t o = std::size_t(c) * d.R;
   Row x{d.x_origin + o, d.x_site + o, d.x_cval + o, 0, 0};
   OpStats st;
-  crossover_child(d.g, d.slot, c, [&](int m) { return member_row(d, m); }, x, d.x_n + c, d.x_nc + c, keys, d.P, d.pressure, seed,
-                  d.pol, st);
-  x.n = d.x_n[c];
-  x.nc = d.x_nc[c];
+  crossover_w(d.g, d.slot, c, [&](int m) { return member_row(d, m); }, x, keys, d.P, d.pressure, seed, d.pol, st);
+  wave_fence();
   Row tmp{d.t_origin + o, d.t_site + o, d.t_cval + o, 0, 0};
   Row spl{d.s_origin + o, d.s_site + o, d.s_cval + o, 0, 0};
-  mutate_child(d.g, d.slot, c, x, d.x_n + c, d.x_nc + c, tmp, spl, d.p_mut, seed, d.pol, st);
-  const CRow child = child_row(d, c);
-  if (!st.error) compare_child(member_row(d, st.parent), child, d.P, st);
-  d.stats[c] = st;
-  if (st.error) { raise(d, 300 + st.error); return; }
-  d.child_hash[c] = row_hash(child);
-  lower_into(d, child, c);
+  mutate_w(d.g, d.slot, c, x, tmp, spl, d.p_mut, seed, d.pol, st);
+  wave_fence();
+  const CRow child = as_const(x);
+  if (!st.error) compare_w(member_row(d, st.parent), child, d.P, st);
+  const std::uint64_t h = st.error ? 0 : row_hash_w(child);
+  if (wlane() == 0) {
+    d.x_n[c] = x.n;
+    d.x_nc[c] = x.nc;
+    d.stats[c] = st;
+    d.child_hash[c] = h;
+    if (st.error) raise(d, 300 + st.error);
+  }
+  if (st.error) return;
+  lower_into_w(d, child, c);
 }
 
 // ---- member hash table (parent_fitness_cache.cuh:11-58) ----
@@ -389,28 +397,32 @@ __device__ inline std::uint64_t mt_next_word(std::uint64_t cur, std::uint64_t nx
   const std::uint64_t x = (cur & 0xFFFFFFFF80000000ULL) | (nxt & 0x7FFFFFFFULL);
   return far ^ (x >> 1) ^ ((x & 1ULL) ? 0xB5026F5AA96619E9ULL : 0ULL);
 }
-// In-place twist by the whole block: element i < 156 reads only old words;
-// 156 <= i < 311 reads the new word i-156; 311 reads new words 0 and 155.
-// blockDim.x >= 64.
+// In-place twist by a block of kThreads >= 64 threads: element i < 156 reads
+// only old words; 156 <= i < 311 reads the new word i-156; 311 reads new
+// words 0 and 155. The thread count is a template constant so the loops do
+// not re-read the dispatch packet.
+template <int kThreads>
 __device__ inline void mt_twist(std::uint64_t* mt) {
+  static_assert(kThreads >= 64, "three words per thread at most");
+  const int t = threadIdx.x;
   std::uint64_t v[3];
   int n = 0;
-  for (int i = threadIdx.x; i < 156; i += blockDim.x) v[n++] = mt_next_word(mt[i], mt[i + 1], mt[i + 156]);
+  for (int i = t; i < 156; i += kThreads) v[n++] = mt_next_word(mt[i], mt[i + 1], mt[i + 156]);
   __syncthreads();
   n = 0;
-  for (int i = threadIdx.x; i < 156; i += blockDim.x) mt[i] = v[n++];
+  for (int i = t; i < 156; i += kThreads) mt[i] = v[n++];
   __syncthreads();
   n = 0;
-  for (int i = 156 + threadIdx.x; i < 311; i += blockDim.x) v[n++] = mt_next_word(mt[i], mt[i + 1], mt[i - 156]);
+  for (int i = 156 + t; i < 311; i += kThreads) v[n++] = mt_next_word(mt[i], mt[i + 1], mt[i - 156]);
   __syncthreads();
   n = 0;
-  for (int i = 156 + threadIdx.x; i < 311; i += blockDim.x) mt[i] = v[n++];
+  for (int i = 156 + t; i < 311; i += kThreads) mt[i] = v[n++];
   __syncthreads();
-  if (threadIdx.x == 0) mt[311] = mt_next_word(mt[311], mt[0], mt[155]);
+  if (t == 0) mt[311] = mt_next_word(mt[311], mt[0], mt[155]);
   __syncthreads();
 }
 
-__global__ void k_seed(DSearch* all) {
+__global__ __launch_bounds__(256) void k_seed(DSearch* all) {
   const DSearch& d = all[blockIdx.x];
   if (!d.active) return;
   __shared__ std::uint64_t mt[312];
@@ -419,10 +431,10 @@ __global__ void k_seed(DSearch* all) {
     for (int i = 1; i < 312; ++i) mt[i] = 6364136223846793005ULL * (mt[i - 1] ^ (mt[i - 1] >> 62)) + std::uint64_t(i);
   }
   __syncthreads();
-  mt_twist(mt);
-  for (int i = threadIdx.x; i < 312; i += blockDim.x) d.mt[i] = mt[i];
+  mt_twist<256>(mt);
+  for (int i = threadIdx.x; i < 312; i += 256) d.mt[i] = mt[i];
   // Word 0 seeds round 0; words 1..311 wait for the first placement.
-  for (int i = 1 + threadIdx.x; i < 312; i += blockDim.x) d.mt_left[i - 1] = mt_temper(mt[i]);
+  for (int i = 1 + threadIdx.x; i < 312; i += 256) d.mt_left[i - 1] = mt_temper(mt[i]);
   if (threadIdx.x == 0) {
     *d.round_seed = mt_temper(mt[0]);
     *d.n_left = 311;
@@ -495,20 +507,24 @@ template <bool kLds>
 struct PlaceKeys {
   // kLds: fitness and (wrong << 12 | nodes) staged in LDS; otherwise the
   // member arrays in HBM, read past the L1 so every lane sees the stores of
-  // the previous chunk.
-  const DSearch* d;
+  // the previous chunk. Fields are copied out of the search state so that
+  // stores to the keys never force it to be re-read.
+  GAS double* fitness;
+  GAS unsigned* wrong;
+  GAS unsigned* nodes;
+  int has_wrong;
   double* f;
   unsigned* wn;
   __device__ double fit(int m) const {
     if (kLds) return f[m];
-    const unsigned long long bits = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(d->fitness) + m,
-                                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long bits = __hip_atomic_load((GAS unsigned long long*)(fitness + m), __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT);
     return __longlong_as_double((long long)bits);
   }
   __device__ std::uint64_t order(int m) const {
     if (kLds) return wn[m];
-    const unsigned w = __hip_atomic_load(d->wrong + m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned n = __hip_atomic_load(d->nodes + m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned w = __hip_atomic_load(wrong + m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned n = __hip_atomic_load(nodes + m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return std::uint64_t(w) << 32 | n;
   }
   __device__ std::uint64_t pack(unsigned w, unsigned n) const { return kLds ? std::uint64_t(w << 12 | n) : std::uint64_t(w) << 32 | n; }
@@ -516,15 +532,15 @@ struct PlaceKeys {
   __device__ bool key_better(double fa, std::uint64_t oa, int a, int b) const {
     const double fb = fit(b);
     if (fa != fb) return fa > fb;
-    if (d->has_wrong) { const std::uint64_t ob = order(b); if (oa != ob) return oa < ob; }
+    if (has_wrong) { const std::uint64_t ob = order(b); if (oa != ob) return oa < ob; }
     return a < b;
   }
-  __device__ bool better(int a, int b) const { return key_better(fit(a), d->has_wrong ? order(a) : 0, a, b); }
+  __device__ bool better(int a, int b) const { return key_better(fit(a), has_wrong ? order(a) : 0, a, b); }
   __device__ void store(int m, double fv, unsigned w, unsigned n) const {
     if (kLds) { f[m] = fv; wn[m] = w << 12 | n; }
-    d->fitness[m] = fv;
-    d->wrong[m] = w;
-    d->nodes[m] = n;
+    fitness[m] = fv;
+    wrong[m] = w;
+    nodes[m] = n;
   }
 };
 
@@ -537,9 +553,13 @@ __global__ __launch_bounds__(64) void k_place(DSearch* all) {
   __shared__ unsigned ridx[kRing];
   __shared__ std::uint64_t mt[312];
   const int lane = threadIdx.x;
-  const int P = d.P, B = d.B, rs = d.replacement_size;
+  const int P = d.P, B = d.B, rs = d.replacement_size, has_wrong = d.has_wrong;
   const std::uint64_t P64 = std::uint64_t(P);
-  PlaceKeys<kLds> keys{&d, reinterpret_cast<double*>(place_smem),
+  GAS const double* const c_fit = d.c_fit;
+  GAS const unsigned* const c_wrong = d.c_wrong;
+  GAS const unsigned* const c_nodes = d.c_nodes;
+  GAS int* const placed = d.placed;
+  PlaceKeys<kLds> keys{d.fitness, d.wrong, d.nodes, has_wrong, reinterpret_cast<double*>(place_smem),
                        reinterpret_cast<unsigned*>(place_smem + (kLds ? 8 * std::size_t(P) : 0))};
   unsigned* mark = reinterpret_cast<unsigned*>(place_smem + (kLds ? 12 * std::size_t(P) : 0));
   // x mod P by a Barrett step: q = mulhi(x, floor((2^64-1)/P)) is
@@ -556,13 +576,13 @@ __global__ __launch_bounds__(64) void k_place(DSearch* all) {
 #pragma unroll
     for (int k = 0; k < kStage; ++k) {
       const int m = m0 + k * 64 + lane;
-      if (kLds && m < P) { f[k] = d.fitness[m]; w[k] = d.wrong[m]; n[k] = d.nodes[m]; }
+      if (kLds && m < P) { f[k] = keys.fitness[m]; w[k] = keys.wrong[m]; n[k] = keys.nodes[m]; }
     }
 #pragma unroll
     for (int k = 0; k < kStage; ++k) {
       const int m = m0 + k * 64 + lane;
       if (m < P) {
-        if (kLds) { keys.f[m] = f[k]; keys.wn[m] = d.has_wrong ? w[k] << 12 | n[k] : 0u; }
+        if (kLds) { keys.f[m] = f[k]; keys.wn[m] = has_wrong ? w[k] << 12 | n[k] : 0u; }
         mark[m] = 0;
       }
     }
@@ -577,7 +597,7 @@ __global__ __launch_bounds__(64) void k_place(DSearch* all) {
   __syncthreads();
   const auto refill = [&](int need) {
     while (tail - head < need) {
-      mt_twist(mt);
+      mt_twist<64>(mt);
       for (int i = lane; i < 312; i += 64) {
         const std::uint64_t t = mt_temper(mt[i]);
         ring[(tail + i) & (kRing - 1)] = t;
@@ -598,7 +618,7 @@ __global__ __launch_bounds__(64) void k_place(DSearch* all) {
     // The chunk places children b .. b+63 at most: fetch their keys now.
     double pf_f = 0;
     unsigned pf_w = 0, pf_n = 0;
-    if (b + lane < B) { pf_f = d.c_fit[b + lane]; pf_w = d.c_wrong[b + lane]; pf_n = d.c_nodes[b + lane]; }
+    if (b + lane < B) { pf_f = c_fit[b + lane]; pf_w = c_wrong[b + lane]; pf_n = c_nodes[b + lane]; }
     const bool active = lane < W;
     const int base = head + lane * rs;
     int worst = -1;
@@ -635,7 +655,7 @@ __global__ __launch_bounds__(64) void k_place(DSearch* all) {
     if (lifter < cut) cut = lifter + 1;
     if (fills && lane < cut) {
       keys.store(worst, cf, cw, cn);
-      d.placed[worst] = child;
+      placed[worst] = child;
     }
     if (lifter < cut) best = __shfl(worst, lifter, 64);
     b += __popcll(cut >= 64 ? pmask : pmask & ((1ull << cut) - 1));
@@ -955,7 +975,8 @@ struct Batch::Impl {
     const dim3 child_grid((max_B + 127) / 128, n), member_grid((max_P + 127) / 128, n), table_grid((max_table + 255) / 256, n);
     const dim3 eval_grid(std::min(4096, (max_B + kWavesPerBlock - 1) / kWavesPerBlock), n);
     timer.time("k_record", stream, [&] { hipLaunchKernelGGL(k_record, dim3(n), dim3(256), 0, stream, d_all); });
-    timer.time("k_vary", stream, [&] { hipLaunchKernelGGL(k_vary, child_grid, dim3(128), 0, stream, d_all); });
+    const dim3 vary_grid((max_B + kVaryWaves - 1) / kVaryWaves, n);
+    timer.time("k_vary", stream, [&] { hipLaunchKernelGGL(k_vary, vary_grid, dim3(64 * kVaryWaves), 0, stream, d_all); });
     timer.time("k_clear", stream, [&] { hipLaunchKernelGGL(k_clear, table_grid, dim3(256), 0, stream, d_all); });
     timer.time("k_index", stream, [&] { hipLaunchKernelGGL(k_index, member_grid, dim3(128), 0, stream, d_all); });
     timer.time("k_reuse", stream, [&] { hipLaunchKernelGGL(k_reuse, child_grid, dim3(128), 0, stream, d_all); });
@@ -1020,7 +1041,8 @@ void Batch::prepare() {
   // Initial population, its evaluation, keys and the run generator.
   const dim3 member_grid((I.max_P + 63) / 64, n);
   I.timer.time("k_init_pop", I.stream, [&] { hipLaunchKernelGGL(k_init_pop, member_grid, dim3(64), 0, I.stream, I.d_all); });
-  I.timer.time("k_lower_members", I.stream, [&] { hipLaunchKernelGGL(k_lower_members, member_grid, dim3(64), 0, I.stream, I.d_all); });
+  const dim3 lower_grid((I.max_P + kVaryWaves - 1) / kVaryWaves, n);
+  I.timer.time("k_lower_members", I.stream, [&] { hipLaunchKernelGGL(k_lower_members, lower_grid, dim3(64 * kVaryWaves), 0, I.stream, I.d_all); });
   const dim3 eval_grid(std::min(4096, (I.max_P + kWavesPerBlock - 1) / kWavesPerBlock), n);
   I.timer.time("k_eval_init", I.stream, [&] { hipLaunchKernelGGL(k_eval, eval_grid, dim3(kEvalThreads), I.lds_bytes(), I.stream, I.d_all, 0); });
   I.timer.time("k_init_keys", I.stream, [&] { hipLaunchKernelGGL(k_init_keys, dim3(n), dim3(256), 0, I.stream, I.d_all); });

@@ -70,19 +70,29 @@ GHD inline unsigned g_minimum(const PhaseView& g, int e, int d, int l) {
 }
 GHD inline int child_scope(const PExpr& x, int i, int scope) { return x.child_scope[i] >= 0 ? x.child_scope[i] : scope; }
 
+// A pending derivation task: expression, remaining depth, scope count,
+// remaining logical (site) depth.
+struct GenTask { int e, d, sc, l; };
+// The pending stack of a derivation on one thread.
+struct LocalTasks {
+  GenTask t[kPending];
+  GHD GenTask get(int k) const { return t[k]; }
+  GHD void set(int k, const GenTask& x) { t[k] = x; }
+};
+
 // Top-down derivation with a minimum-cost reservation and a pending stack.
 // Writes origins (and sites when `sites` is non-null) and constants.
-GHD inline GenResult generate_phase(const PhaseView& g, int root_expression, int scope_count, int depth, int capacity,
-                                    std::uint64_t seed, int* origins, Val* constants, int* sites, int logical_depth) {
+template <class Tasks>
+GHD inline GenResult generate_phase_with(const PhaseView& g, int root_expression, int scope_count, int depth, int capacity,
+                                         std::uint64_t seed, int* origins, Val* constants, int* sites, int logical_depth,
+                                         Tasks& pending) {
   GenResult r{ST_INVALID, 0, 0, 0};
   if (root_expression < 0 || root_expression >= g.n_expr || scope_count < 0 || scope_count > 15 || depth <= 0 ||
       depth >= g.depth_count || depth > 64 || logical_depth < 0 || logical_depth >= g.logical_count || capacity <= 0 ||
       capacity > kPhaseCapacity || g.n_expr > 131071)
     return r;
-  struct Task { int e, d, sc, l; };
-  Task pending[kPending];
   int count = 1;
-  pending[0] = {root_expression, depth, scope_count, logical_depth};
+  pending.set(0, GenTask{root_expression, depth, scope_count, logical_depth});
   unsigned reserved = g_minimum(g, root_expression, depth, logical_depth);
   if (reserved > unsigned(capacity)) { r.status = ST_CAPACITY; return r; }
   int next_site = -1;
@@ -90,7 +100,7 @@ GHD inline GenResult generate_phase(const PhaseView& g, int root_expression, int
   constexpr int unselected = 131071;
   while (count) {
     if (++r.steps > 8192) { r.status = ST_CAPACITY; return r; }
-    const Task task = pending[--count];
+    const GenTask task = pending.get(--count);
     const unsigned required = g_minimum(g, task.e, task.d, task.l);
     if (required > reserved) return r;
     reserved -= required;
@@ -106,7 +116,7 @@ GHD inline GenResult generate_phase(const PhaseView& g, int root_expression, int
           nt.production_offset > g.n_prod || nt.production_count > g.n_prod - nt.production_offset)
         return r;
       if (g.site_depth && task.l == 0) { r.status = ST_CAPACITY; return r; }
-      Task next{unselected, task.d, x.scope_count, task.l - (g.site_depth ? 1 : 0)};
+      GenTask next{unselected, task.d, x.scope_count, task.l - (g.site_depth ? 1 : 0)};
       if (reserved > unsigned(capacity - r.nodes)) { r.status = ST_CAPACITY; return r; }
       const unsigned available = unsigned(capacity - r.nodes) - reserved;
       double maximum = nt.normalized ? 1 : 0, total = 0;
@@ -131,7 +141,7 @@ GHD inline GenResult generate_phase(const PhaseView& g, int root_expression, int
       }
       if (next.e == unselected || count >= kPending) return r;
       reserved += g_minimum(g, next.e, next.d, next.l);
-      pending[count++] = next;
+      pending.set(count++, next);
       continue;
     }
     if (x.kind == PK_ALIAS) return r;  // templates are not admitted by the front end
@@ -165,17 +175,23 @@ GHD inline GenResult generate_phase(const PhaseView& g, int root_expression, int
         r.status = ST_CAPACITY;
         return r;
       }
-      Task next = task;
+      GenTask next = task;
       next.e = child;
       --next.d;
       next.sc = child_scope(x, i, task.sc);
-      pending[count++] = next;
+      pending.set(count++, next);
       reserved += need;
     }
   }
   if (reserved || !r.nodes) return r;
   r.status = ST_OK;
   return r;
+}
+GHD inline GenResult generate_phase(const PhaseView& g, int root_expression, int scope_count, int depth, int capacity,
+                                    std::uint64_t seed, int* origins, Val* constants, int* sites, int logical_depth) {
+  LocalTasks pending;
+  return generate_phase_with(g, root_expression, scope_count, depth, capacity, seed, origins, constants, sites, logical_depth,
+                             pending);
 }
 
 GHD inline int arity_of(const PhaseView& g, int origin) { return g.expr[origin].node.arity; }
@@ -464,14 +480,22 @@ GHD inline void compare_child(const CRow& parent_row, const CRow& child_row, int
   st.reusable_parent = equal && reusable ? st.parent : -1;
 }
 
-// Genotype hash (FNV-1a style mix over origins and constant payloads); only
-// a lookup key, equality is always checked in full.
+// Genotype hash: a wrapping sum of one mixed term per node (origin and
+// position) and per constant (position, tag and the payload that equality
+// compares), so a wavefront can add the terms in any order. Only a lookup
+// key: equality is always checked in full.
+GHD inline std::uint64_t row_hash_node(int i, int origin) {
+  return splitmix_mix(std::uint64_t(std::uint32_t(origin)) << 24 ^ std::uint64_t(i));
+}
+GHD inline std::uint64_t row_hash_constant(int i, const Val& v) {
+  const std::uint64_t payload = v.tag == TAG_BOOL ? std::uint64_t(v.b) : std::uint64_t(v.i);
+  return splitmix_mix(splitmix_mix(payload) ^ (std::uint64_t(i) << 8 | v.tag) ^ 0x5851f42d4c957f2dULL);
+}
+GHD inline std::uint64_t row_hash_counts(int n, int nc) { return splitmix_mix(std::uint64_t(std::uint32_t(n)) << 32 | std::uint32_t(nc)); }
 GHD inline std::uint64_t row_hash(const CRow& r) {
-  std::uint64_t h = 0xcbf29ce484222325ULL;
-  const auto mix = [&](std::uint64_t v) { h ^= v; h *= 0x100000001b3ULL; };
-  mix(std::uint64_t(r.n));
-  for (int i = 0; i < r.n; ++i) mix(std::uint64_t(std::uint32_t(r.origin[i])));
-  for (int i = 0; i < r.nc; ++i) { mix(std::uint64_t(r.cval[i].i)); mix(std::uint64_t(r.cval[i].tag) << 8 | r.cval[i].b); }
+  std::uint64_t h = row_hash_counts(r.n, r.nc);
+  for (int i = 0; i < r.n; ++i) h += row_hash_node(i, r.origin[i]);
+  for (int i = 0; i < r.nc; ++i) h += row_hash_constant(i, r.cval[i]);
   return h;
 }
 // Genotype identity used for completed-result reuse (parent_fitness_cache.cuh:11-58).
@@ -498,17 +522,27 @@ struct Ins {
 static_assert(sizeof(Ins) == 8, "Ins is 8 bytes");
 constexpr int kMaxStack = 64;
 
-struct LowerResult { int status, code, constants, stack_bound, result_type; };
+struct LowerResult { int status, code, constants, stack_bound, result_type, jumps; };
+
+// An open operation of the post-order walk: its node, operands seen so far,
+// pending jump sites, the stack height at its start and its operand types.
+struct Frame { int origin, seen, first_jump, second_jump, base_stack; std::uint8_t args[3]; };
+struct LocalFrames {
+  Frame f[kMaxStack];
+  GHD Frame get(int k) const { return f[k]; }
+  GHD void set(int k, const Frame& x) { f[k] = x; }
+};
 
 // Post-order lowering of an expression phase plus the program terminator,
 // with the reference's jump shapes for if/and/or. Constants are emitted in
-// first-use order (one pool entry per constant node).
-GHD inline LowerResult lower_phase(const PhaseView& g, const CRow& t, const std::uint8_t* local_types, int local_count,
-                                   int expected_type, Ins* code, int code_capacity, Val* pool, int pool_capacity) {
-  LowerResult out{ST_INVALID, 0, 0, 0, TAG_INVALID};
+// first-use order (one pool entry per constant node). `OriginAt(i)` gives the
+// origin of node i in prefix order.
+template <class Frames, class OriginAt>
+GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, OriginAt origin_at, const std::uint8_t* local_types,
+                                        int local_count, int expected_type, Ins* code, int code_capacity, Val* pool,
+                                        int pool_capacity, Frames& frames) {
+  LowerResult out{ST_INVALID, 0, 0, 0, TAG_INVALID, 0};
   if (t.n <= 0 || t.n > 4096) return out;
-  struct Frame { int origin, seen, first_jump, second_jump, base_stack; std::uint8_t args[3]; };
-  Frame frames[64];
   int depth = 0, cursor = 0, stack = 0, cconst = 0;
   int completed = TAG_INVALID;
   bool have_value = false;
@@ -516,13 +550,14 @@ GHD inline LowerResult lower_phase(const PhaseView& g, const CRow& t, const std:
     if (out.code >= code_capacity) { out.status = ST_CAPACITY; return false; }
     if (fuel > 255 || stack < 0 || stack > kMaxStack) { out.status = ST_UNSUPPORTED; return false; }
     code[out.code++] = Ins{std::uint8_t(op), std::uint8_t(argc), std::uint8_t(stack), std::uint8_t(fuel), a};
+    out.jumps |= op == OP_JMP || op == OP_JMP_IF_FALSE || op == OP_JMP_IF_TRUE;
     return true;
   };
   const auto grow = [&](int by) { stack += by; if (stack > out.stack_bound) out.stack_bound = stack; };
   for (;;) {
     if (!have_value) {
       if (cursor >= t.n) return out;
-      const int o = t.origin[cursor++];
+      const int o = origin_at(cursor++);
       const PExpr& x = g.expr[o];
       const PNode& node = x.node;
       if (node.kind == TK_CONSTANT) {
@@ -549,7 +584,7 @@ GHD inline LowerResult lower_phase(const PhaseView& g, const CRow& t, const std:
         else if (node.kind == TK_AND || node.kind == TK_OR) { if (node.arity != 2) return out; }
         else if (node.kind != TK_OPERATION) { out.status = ST_UNSUPPORTED; return out; }
         if (depth == 64) { out.status = ST_CAPACITY; return out; }
-        frames[depth++] = Frame{o, 0, -1, -1, stack, {TAG_INVALID, TAG_INVALID, TAG_INVALID}};
+        frames.set(depth++, Frame{o, 0, -1, -1, stack, {TAG_INVALID, TAG_INVALID, TAG_INVALID}});
         continue;
       }
     }
@@ -562,7 +597,7 @@ GHD inline LowerResult lower_phase(const PhaseView& g, const CRow& t, const std:
       out.result_type = completed;
       return out;
     }
-    Frame& f = frames[depth - 1];
+    Frame f = frames.get(depth - 1);
     const PNode& node = g.expr[f.origin].node;
     if (f.seen < 3) f.args[f.seen] = std::uint8_t(completed);
     ++f.seen;
@@ -578,7 +613,7 @@ GHD inline LowerResult lower_phase(const PhaseView& g, const CRow& t, const std:
       code[f.first_jump].a = out.code;
       stack = f.base_stack;
     }
-    if (f.seen < node.arity) { have_value = false; continue; }
+    if (f.seen < node.arity) { frames.set(depth - 1, f); have_value = false; continue; }
     if (node.kind == TK_IF) {
       if (f.args[1] != f.args[2]) return out;
       code[f.second_jump].a = out.code;
@@ -622,6 +657,12 @@ GHD inline LowerResult lower_phase(const PhaseView& g, const CRow& t, const std:
     }
     --depth;
   }
+}
+GHD inline LowerResult lower_phase(const PhaseView& g, const CRow& t, const std::uint8_t* local_types, int local_count,
+                                   int expected_type, Ins* code, int code_capacity, Val* pool, int pool_capacity) {
+  LocalFrames frames;
+  return lower_phase_with(g, t, [&](int i) { return t.origin[i]; }, local_types, local_count, expected_type, code, code_capacity,
+                          pool, pool_capacity, frames);
 }
 
 }  // namespace gagp_amd
