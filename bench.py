@@ -4,17 +4,21 @@ Workload: the quintic symbolic-regression search (seeded quintic with
 coefficients in [-5,5]; 256 train / 512 test cases, x in [-50,50];
 add/sub/mul + constants 0..9; 64 nodes; population 4096, 128 generations;
 absolute error; the default resident steady-state search). A step is one
-generation (one steady-state round: selection, crossover, mutation,
-lowering, VM scoring of every non-reused child on every case, placement).
+generation (one steady-state round of B = P children: selection, crossover,
+mutation, lowering, VM scoring of every non-reused child on every case,
+placement), with the population resident in HBM.
 
 Prints one JSON line: program-case evaluations per second over K timed steps
-(after W warmup steps) with the population resident in HBM, the roofline
-block of the dominant kernel (the VM), and the reference CPU engine's rate on
-a bounded sample of the same search timed on this host.
+(after W warmup steps), the roofline block of the kernel that takes the most
+time per step (its algorithmic bytes per launch over its average launch
+time, measured with HIP events on the engine's stream), per-kernel times and
+bytes, and the reference CPU engine's rate on the same search timed on this
+host's cores (one thread).
 """
 import argparse
 import json
 import os
+import re
 import subprocess
 import sys
 import tempfile
@@ -25,57 +29,84 @@ sys.path.insert(0, ROOT)
 import gagp_amd  # noqa: E402
 
 TASK = os.path.join(ROOT, "tests", "golden", "tasks", "quintic", "task.json")
-POPULATION, GENERATIONS = 4096, 128
+POPULATION, GENERATIONS, TRAIN_CASES, INPUTS = 4096, 128, 256, 1
 HBM_PEAK_GBS = 8000.0
 REF_GAGP = os.path.join(ROOT, "oracle", "_ref", "gagp")
+TRAFFIC = os.path.join(ROOT, "profiles", "quintic_p4096_hbm_traffic.md")
+ROUND_KERNELS = ("k_record", "k_vary", "k_clear", "k_index", "k_reuse", "k_eval", "k_tally", "k_place", "k_place_hbm",
+                 "k_commit")
 
 
-def program_case_rate(batch, index, steps):
-    before = batch.counters(index)
-    t0 = time.perf_counter()
-    batch.step(steps)
-    t1 = time.perf_counter()
-    after = batch.counters(index)
-    return before, after, t1 - t0
+def algorithmic_bytes(c, steps):
+    """Compulsory HBM bytes per launch of each round kernel, from the device
+    counters of `steps` rounds. Rows: 8 B per node (origin + site), 16 B per
+    constant; code: 8 B per instruction, 16 B per pool entry; keys: 16 B
+    (fitness, wrong, nodes) per member or child."""
+    per = {k: v / steps for k, v in c.items()}
+    case_bytes = TRAIN_CASES * (INPUTS + 1) * 10  # int64 payload + tag + flag per input and per answer
+    row = lambda nodes, consts: 8 * nodes + 16 * consts  # noqa: E731
+    return {
+        "k_eval": 8 * per["code_words"] + 16 * per["pool_words"] + case_bytes + 28 * per["programs"],
+        "k_vary": row(per["parent_nodes"], per["parent_consts"]) + row(per["child_nodes"], per["child_consts"])
+        + 8 * per["child_code_words"] + 16 * per["child_pool_words"] + 60 * per["children"],
+        "k_place": 16 * POPULATION + 16 * per["children"] + 20 * per["children"],
+        "k_reuse": 8 * per["children"] + 16 * per["children"],
+        "k_index": 8 * POPULATION + 12 * POPULATION,
+        "k_commit": row(per["child_nodes"], per["child_consts"]) * 2 + 4 * POPULATION,
+    }
 
 
-def roofline(steps_warm, steps):
-    """Dominant kernel (k_eval) with HIP events around every launch."""
+def profile(warm, steps):
+    """Per-kernel average launch times (HIP events around every launch on the
+    engine's stream) and counters over `steps` rounds of the same search."""
     b = gagp_amd.Batch()
     b.add(TASK, None, POPULATION, GENERATIONS, 0)
     b.set_profile(True)
     b.prepare()
-    b.step(steps_warm, use_graph=False)
-    t_before = b.kernel_times()
-    c0 = b.counters(0)
+    b.step(warm, use_graph=False)
+    t0, c0 = b.kernel_times(), b.counters(0)
     b.step(steps, use_graph=False)
-    c1 = b.counters(0)
-    t_after = b.kernel_times()
-    eval_ms = (t_after["k_eval"] - t_before.get("k_eval", 0.0)) / steps
-    per_kernel = {k: round((t_after[k] - t_before.get(k, 0.0)) / steps, 4) for k in t_after if not k.startswith("k_init")
-                  and k not in ("k_eval_init", "k_lower_members", "k_seed")}
-    # Algorithmic HBM bytes of one k_eval launch: each evaluated program's
-    # code (8 B per instruction) and constants read once, the case set
-    # (int64 + tag + flag per input and answer) read once, results written.
-    programs = (c1["programs"] - c0["programs"]) / steps
-    code_words = (c1["code_words"] - c0["code_words"]) / steps
-    cases = 256
-    case_bytes = cases * (10 + 10)
-    algorithmic = code_words * 8 + programs * 16 * 8 + case_bytes + programs * 12
-    vm_instr = (c1["vm_instructions"] - c0["vm_instructions"]) / steps
-    achieved = algorithmic / (eval_ms * 1e-3) / 1e9
-    return {
-        "bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
-        "kernel": "k_eval", "avg_launch_ms": round(eval_ms, 4), "bytes_per_launch": int(algorithmic),
-        "programs_per_launch": programs, "vm_instructions_per_launch": vm_instr,
-        "vm_instructions_per_s": round(vm_instr / (eval_ms * 1e-3), 1),
-    }, per_kernel
+    t1, c1 = b.kernel_times(), b.counters(0)
+    ms = {k: (t1[k] - t0.get(k, 0.0)) / steps for k in t1 if k in ROUND_KERNELS}
+    counts = {k: c1[k] - c0[k] for k in c1}
+    return ms, counts
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE
+    and WRITE_SIZE summary of this workload (None when absent)."""
+    if not os.path.exists(TRAFFIC):
+        return None
+    for line in open(TRAFFIC):
+        m = re.match(r"\|\s*%s\s*\|\s*([0-9.]+)\s*\|" % re.escape(kernel), line)
+        if m:
+            return float(m.group(1))
+    return None
+
+
+def roofline(ms, counts, steps):
+    nbytes = algorithmic_bytes(counts, steps)
+    dominant = max((k for k in ms if k in nbytes), key=lambda k: ms[k])
+    avg = ms[dominant]
+    achieved = nbytes[dominant] / (avg * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": measured_traffic(dominant),
+            "kernel": dominant, "avg_launch_ms": round(avg, 4), "bytes_per_launch": int(nbytes[dominant])}
+    kernels = {k: {"ms": round(v, 4), "bytes": int(nbytes[k]) if k in nbytes else None,
+                   "gbs": round(nbytes[k] / (v * 1e-3) / 1e9, 2) if k in nbytes and v > 0 else None}
+               for k, v in sorted(ms.items(), key=lambda kv: -kv[1])}
+    vm = {"vm_instructions_per_launch": counts["vm_instructions"] / steps,
+          "vm_instructions_per_s": round(counts["vm_instructions"] / steps / (ms["k_eval"] * 1e-3), 1),
+          "programs_per_launch": counts["programs"] / steps,
+          "program_cases_per_s_in_k_eval": round(counts["program_cases"] / steps / (ms["k_eval"] * 1e-3), 1)}
+    return roof, kernels, vm
 
 
 def cpu_baseline(generations):
-    """The reference CPU engine (built from its sources) on the same search,
-    first `generations` generations, timed on this host (one thread)."""
+    """The reference CPU engine (built from its sources by oracle/ref.mk) on
+    the same search for its first `generations` generations, one thread on
+    this host; the device runs the same search to count the program-cases
+    the reference evaluates and to confirm identical results."""
     if not os.path.exists(REF_GAGP):
         return None
     with tempfile.TemporaryDirectory() as out:
@@ -84,7 +115,6 @@ def cpu_baseline(generations):
                         str(generations), "--seed", "0", "--out", out, "--quiet"], check=True)
         wall = time.perf_counter() - t0
         ref = json.load(open(os.path.join(out, "result.json")))
-    # The same search on the device evaluates exactly the same children.
     b = gagp_amd.Batch()
     b.add(TASK, None, POPULATION, generations, 0)
     b.run()
@@ -94,9 +124,21 @@ def cpu_baseline(generations):
     cases = b.evaluated_program_cases(0)
     search_s = ref["time_ms"]["search"] / 1e3
     return {"value": round(cases / search_s, 1), "unit": "program-case evals/s", "cores": 1, "kind": "reference",
-            "sample": "reference CPU engine (gagp run --engine cpu), quintic P%d seed 0, first %d generations: "
-                      "%d program-cases in %.2f s search time (%.2f s wall); results identical to the device: %s"
-                      % (POPULATION, generations, cases, search_s, wall, same)}
+            "sample": "reference CPU engine (gagp run --engine cpu, built from its sources), quintic P%d seed 0, "
+                      "%d generations: %d program-cases in %.2f s of search time (%.2f s wall); best program and "
+                      "history identical to the device run: %s" % (POPULATION, generations, cases, search_s, wall, same)}
+
+
+def calibrate_copy(mib):
+    """One streaming kernel of known size for the HBM counters: reads and
+    writes `mib` MiB (int32 negation, one read and one write per element)."""
+    import torch
+    a = torch.ones(mib << 18, dtype=torch.int32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    torch.neg(a, out=b)
+    torch.cuda.synchronize()
+    print("copy calibration: %d MiB read, %d MiB written" % (mib, mib), file=sys.stderr)
 
 
 def main():
@@ -106,29 +148,39 @@ def main():
     ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--cpu-generations", type=int, default=GENERATIONS)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0, help="run a streaming kernel of this size first (counter scale)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this engine runs each search on a single device; "
-              "--gpus %d is not supported" % args.gpus, file=sys.stderr)
+        print("bench.py measures one GPU: each search runs on a single device and this build has no multi-GPU "
+              "path; --gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
     if args.warmup + args.steps > GENERATIONS:
         sys.exit("warmup + steps must not exceed the %d generations of the search" % GENERATIONS)
+    if args.copy_mib:
+        calibrate_copy(args.copy_mib)
     b = gagp_amd.Batch()
     b.add(TASK, None, POPULATION, GENERATIONS, 0)
     b.prepare()
     b.step(args.warmup)
-    before, after, seconds = program_case_rate(b, 0, args.steps)
+    before = b.counters(0)
+    t0 = time.perf_counter()
+    b.step(args.steps)  # graph replays, synchronized before returning
+    seconds = time.perf_counter() - t0
+    after = b.counters(0)
     value = (after["program_cases"] - before["program_cases"]) / seconds
-    roof, per_kernel = roofline(args.warmup, min(args.steps, 16))
+    psteps = min(args.steps, 16)
+    ms, counts = profile(args.warmup, psteps)
+    roof, kernels, vm = roofline(ms, counts, psteps)
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_generations)
     line = {
         "metric": "program-case evals/sec", "value": round(value, 1), "unit": "program-case evals/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(seconds * 1e3 / args.steps, 4),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int64", "data": "synthetic",
-        "config": {"workload": "quintic symbolic regression, P4096, 128 generations, 256 train cases, "
-                               "default steady-state search (one step = one generation)",
-                   "population": POPULATION, "generations": GENERATIONS, "train_cases": 256},
-        "roofline": roof, "cpu_baseline": cpu, "kernel_ms_per_step": per_kernel,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int64",
+        "data": "synthetic (seeded quintic task generated in-repo)",
+        "config": {"workload": "quintic symbolic regression, population 4096, 128 generations, 256 train cases, "
+                               "default resident steady-state search; one step = one generation",
+                   "population": POPULATION, "generations": GENERATIONS, "train_cases": TRAIN_CASES},
+        "roofline": roof, "cpu_baseline": cpu, "kernels": kernels, "vm": vm,
     }
     if cpu:
         line["vs_cpu_reference"] = round(value / cpu["value"], 2)

@@ -215,10 +215,12 @@ class Batch:
         _check(lib().gagp_batch_finish(self.h))
 
     def counters(self, index):
-        out = (ctypes.c_uint64 * 6)()
+        out = (ctypes.c_uint64 * 14)()
         _check(lib().gagp_batch_counters(self.h, index, out))
-        return {"programs": out[0], "program_cases": out[1], "errors": out[2], "timeouts": out[3],
-                "vm_instructions": out[4], "code_words": out[5]}
+        names = ("programs", "program_cases", "errors", "timeouts", "vm_instructions", "code_words", "pool_words",
+                 "children", "child_nodes", "child_consts", "parent_nodes", "parent_consts", "child_code_words",
+                 "child_pool_words")
+        return dict(zip(names, out))
 
     def result(self, index):
         return json.loads(_text(lib().gagp_batch_result_json, self.h, index))

@@ -193,7 +193,7 @@ int gagp_batch_counters(gagp_batch* b, int32_t index, uint64_t* out) {
   return guarded([&] {
     if (!b || !b->prepared || index < 0 || std::size_t(index) >= b->b.size() || !out) throw std::invalid_argument("no such search");
     const auto c = b->b.counters(index);
-    for (std::size_t k = 0; k < 6 && k < c.size(); ++k) out[k] = c[k];
+    for (std::size_t k = 0; k < 14 && k < c.size(); ++k) out[k] = c[k];
   });
 }
 

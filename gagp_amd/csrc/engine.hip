@@ -42,10 +42,14 @@ constexpr int kWavesPerBlock = 4;
 constexpr int kEvalThreads = 64 * kWavesPerBlock;
 
 struct Counters {
-  // evaluated programs and program-cases, runtime errors, timeouts, VM
-  // instructions executed (charged fuel), instructions and constants of the
-  // evaluated programs (their compulsory code bytes).
-  unsigned long long programs, program_cases, errors, timeouts, vm_instructions, code_words, const_words;
+  // Evaluated programs and program-cases, runtime errors, timeouts, VM
+  // instructions executed (charged fuel), instructions of the evaluated
+  // programs and their constant-pool entries; when profiling, the children
+  // made, their nodes and constants, the nodes and constants of the parent
+  // and donor rows they read, and the instructions and pool entries lowered
+  // for them.
+  unsigned long long programs, program_cases, errors, timeouts, vm_instructions, code_words, pool_words;
+  unsigned long long children, child_nodes, child_consts, parent_nodes, parent_consts, child_code_words, child_pool_words;
 };
 
 // Device state of one search. Every pointer is device memory owned by the
@@ -82,7 +86,7 @@ struct DSearch {
   GAS unsigned *c_wrong, *c_nodes;
   GAS Ins* code;
   GAS Val* pool;
-  GAS int *code_len, *code_jumps;
+  GAS int *code_len, *pool_len, *code_jumps;
   GAS std::uint64_t* child_hash;
   GAS int *pending, *n_pending;
   GAS std::uint64_t* table_key;  // genotype hash | 1, 0 = empty
@@ -101,6 +105,9 @@ struct DSearch {
   GAS unsigned* hist_wrong;
   GAS Counters* counters;
   GAS int* error;
+  GAS unsigned *e_errors, *e_timeouts;  // per evaluated slot
+  GAS unsigned long long* e_fuel;
+  int profile;  // tally reproduction row sizes too
 };
 
 __device__ inline CRow member_row(const DSearch& d, int m) {
@@ -145,6 +152,7 @@ __device__ inline void lower_into_w(const DSearch& d, const CRow& row, int slot)
   if (wlane() == 0) {
     if (!ok) raise(d, 200 + r.status);
     d.code_len[slot] = ok ? r.code : 0;
+    d.pool_len[slot] = ok ? r.constants : 0;
     d.code_jumps[slot] = r.jumps;
   }
 }
@@ -376,12 +384,64 @@ __global__ __launch_bounds__(kEvalThreads) void k_eval(DSearch* all, int mode) {
       // Ranking rounds every published fitness to a 40-bit mantissa.
       if (mode == 1) { d.c_fit[slot] = canon40(r.fitness); d.c_wrong[slot] = r.wrong; }
       else { d.fitness[slot] = canon40(r.fitness); d.wrong[slot] = r.wrong; }
-      atomicAdd(&d.counters->programs, 1ull);
-      atomicAdd(&d.counters->program_cases, (unsigned long long)e.cd.C);
-      atomicAdd(&d.counters->errors, (unsigned long long)r.errors);
-      atomicAdd(&d.counters->timeouts, (unsigned long long)r.timeouts);
-      atomicAdd(&d.counters->vm_instructions, r.fuel);
-      atomicAdd(&d.counters->code_words, (unsigned long long)pv.len);
+      // Per-program tallies, summed once per launch by k_tally.
+      d.e_errors[slot] = r.errors;
+      d.e_timeouts[slot] = r.timeouts;
+      d.e_fuel[slot] = r.fuel;
+    }
+  }
+}
+
+// ---- counters of one evaluation launch and, when profiling, of the
+// reproduction that fed it (mode as k_eval) ----
+__global__ __launch_bounds__(256) void k_tally(DSearch* all, int mode) {
+  const DSearch& d = all[blockIdx.x];
+  if (mode == 1 ? !search_live(d) : (!d.active || *d.error)) return;
+  const int count = mode == 1 ? *d.n_pending : d.P;
+  unsigned long long v[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int i = threadIdx.x; i < count; i += 256) {
+    const int slot = mode == 1 ? d.pending[i] : i;
+    v[0] += d.e_errors[slot];
+    v[1] += d.e_timeouts[slot];
+    v[2] += d.e_fuel[slot];
+    v[3] += unsigned(d.code_len[slot]);
+    v[9] += unsigned(d.pool_len[slot]);
+  }
+  if (mode == 1 && d.profile)
+    for (int c = threadIdx.x; c < d.B; c += 256) {
+      const OpStats st = d.stats[c];
+      v[4] += unsigned(d.x_n[c]);
+      v[5] += unsigned(d.x_nc[c]);
+      if (st.parent >= 0) { v[6] += unsigned(d.cur_n[st.parent]); v[7] += unsigned(d.cur_nc[st.parent]); }
+      if (st.crossover_attempted && st.donor >= 0) { v[6] += unsigned(d.cur_n[st.donor]); v[7] += unsigned(d.cur_nc[st.donor]); }
+      v[8] += unsigned(d.code_len[c]);
+      v[10] += unsigned(d.pool_len[c]);
+    }
+  __shared__ unsigned long long part[11][256];
+  for (int k = 0; k < 11; ++k) part[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int span = 128; span > 0; span >>= 1) {
+    if (int(threadIdx.x) < span)
+      for (int k = 0; k < 11; ++k) part[k][threadIdx.x] += part[k][threadIdx.x + span];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    Counters& c = *d.counters;
+    c.programs += unsigned(count);
+    c.program_cases += (unsigned long long)count * unsigned(d.cd.C);
+    c.errors += part[0][0];
+    c.timeouts += part[1][0];
+    c.vm_instructions += part[2][0];
+    c.code_words += part[3][0];
+    c.pool_words += part[9][0];
+    if (mode == 1) {
+      c.children += unsigned(d.B);
+      c.child_nodes += part[4][0];
+      c.child_consts += part[5][0];
+      c.parent_nodes += part[6][0];
+      c.parent_consts += part[7][0];
+      c.child_code_words += part[8][0];
+      c.child_pool_words += part[10][0];
     }
   }
 }
@@ -892,7 +952,7 @@ struct SearchState {
     const int code_rows = std::max(P, B);
     x.code = a.alloc<Ins>(std::size_t(code_rows) * x.code_cap);
     x.pool = a.alloc<Val>(std::size_t(code_rows) * x.pool_cap);
-    x.code_len = a.alloc<int>(code_rows); x.code_jumps = a.alloc<int>(code_rows);
+    x.code_len = a.alloc<int>(code_rows); x.pool_len = a.alloc<int>(code_rows); x.code_jumps = a.alloc<int>(code_rows);
     x.child_hash = a.alloc<std::uint64_t>(B);
     x.pending = a.alloc<int>(B); x.n_pending = a.alloc<int>(1);
     const int tsize = next_pow2(2 * P);
@@ -908,6 +968,8 @@ struct SearchState {
     x.hist_fit = a.alloc<double>(G); x.hist_wrong = a.alloc<unsigned>(G);
     x.hist_all = a.alloc<double>(std::size_t(G) * P);
     x.counters = a.alloc<Counters>(1);
+    x.e_errors = a.alloc<unsigned>(code_rows); x.e_timeouts = a.alloc<unsigned>(code_rows);
+    x.e_fuel = a.alloc<unsigned long long>(code_rows);
     x.error = a.alloc<int>(1);
     HIP_OK(hipMemset(x.n_pending, 0, sizeof(int)));
     HIP_OK(hipMemset(x.counters, 0, sizeof(Counters)));
@@ -981,6 +1043,7 @@ struct Batch::Impl {
     timer.time("k_index", stream, [&] { hipLaunchKernelGGL(k_index, member_grid, dim3(128), 0, stream, d_all); });
     timer.time("k_reuse", stream, [&] { hipLaunchKernelGGL(k_reuse, child_grid, dim3(128), 0, stream, d_all); });
     timer.time("k_eval", stream, [&] { hipLaunchKernelGGL(k_eval, eval_grid, dim3(kEvalThreads), lds_bytes(), stream, d_all, 1); });
+    timer.time("k_tally", stream, [&] { hipLaunchKernelGGL(k_tally, dim3(n), dim3(256), 0, stream, d_all, 1); });
     if (place_lds_bytes)
       timer.time("k_place", stream, [&] { hipLaunchKernelGGL(k_place<true>, dim3(n), dim3(64), place_lds_bytes, stream, d_all); });
     if (place_hbm_bytes)
@@ -1016,6 +1079,7 @@ void Batch::prepare() {
   std::vector<DSearch> host;
   for (auto& s : I.searches) {
     s->allocate();
+    s->d.profile = I.timer.on ? 1 : 0;
     host.push_back(s->d);
     I.max_P = std::max(I.max_P, s->P);
     I.max_B = std::max(I.max_B, s->B);
@@ -1045,6 +1109,7 @@ void Batch::prepare() {
   I.timer.time("k_lower_members", I.stream, [&] { hipLaunchKernelGGL(k_lower_members, lower_grid, dim3(64 * kVaryWaves), 0, I.stream, I.d_all); });
   const dim3 eval_grid(std::min(4096, (I.max_P + kWavesPerBlock - 1) / kWavesPerBlock), n);
   I.timer.time("k_eval_init", I.stream, [&] { hipLaunchKernelGGL(k_eval, eval_grid, dim3(kEvalThreads), I.lds_bytes(), I.stream, I.d_all, 0); });
+  I.timer.time("k_tally_init", I.stream, [&] { hipLaunchKernelGGL(k_tally, dim3(n), dim3(256), 0, I.stream, I.d_all, 0); });
   I.timer.time("k_init_keys", I.stream, [&] { hipLaunchKernelGGL(k_init_keys, dim3(n), dim3(256), 0, I.stream, I.d_all); });
   I.timer.time("k_seed", I.stream, [&] { hipLaunchKernelGGL(k_seed, dim3(n), dim3(256), 0, I.stream, I.d_all); });
   HIP_OK(hipStreamSynchronize(I.stream));
@@ -1097,7 +1162,9 @@ void Batch::run(bool use_graph) {
 std::vector<std::uint64_t> Batch::counters(int index) const {
   const SearchState& s = *impl_->searches.at(std::size_t(index));
   const Counters c = s.down(s.d.counters, 1)[0];
-  return {c.programs, c.program_cases, c.errors, c.timeouts, c.vm_instructions, c.code_words, c.const_words};
+  return {c.programs,     c.program_cases, c.errors,       c.timeouts,      c.vm_instructions,  c.code_words,
+          c.pool_words,   c.children,      c.child_nodes,  c.child_consts,  c.parent_nodes,     c.parent_consts,
+          c.child_code_words, c.child_pool_words};
 }
 double Batch::init_ms() const { return impl_->init_ms; }
 

@@ -57,8 +57,7 @@ class Batch {
   void prepare();
   void step(int rounds, bool use_graph = true);
   void finish();
-  // Device counters of search `index`: evaluated programs, program-cases,
-  // runtime errors, timeouts.
+  // Device counters of search `index` (see gagp_batch_counters).
   std::vector<std::uint64_t> counters(int index) const;
   double init_ms() const;
   SearchResult result(int index, bool save_all);

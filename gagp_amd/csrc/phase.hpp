@@ -304,7 +304,7 @@ GHD inline GenResult splice(const PhaseView& g, const CRow& base, int destinatio
   return r;
 }
 
-struct OpStats { int parent, crossover_attempted, crossover, crossover_rejected, mutation, changed, error, reusable_parent; };
+struct OpStats { int parent, crossover_attempted, crossover, crossover_rejected, mutation, changed, error, reusable_parent, donor; };
 struct OpPolicy {
   int counted_sites;
   int uniform_phase;
@@ -390,10 +390,11 @@ template <class MemberRow>
 GHD inline void crossover_child(const PhaseView& g, const SlotInfo& s, int child, MemberRow in, Row out, int* out_n, int* out_nc,
                                 const Keys& keys, int population, int pressure, std::uint64_t seed, const OpPolicy& pol, OpStats& st) {
   SplitMix random(seed ^ (0x617089e168ULL + std::uint64_t(child)));
-  st = OpStats{-1, 0, 0, 0, 0, 0, 0, -1};
+  st = OpStats{-1, 0, 0, 0, 0, 0, 0, -1, -1};
   const int parent = tournament(keys, population, pressure, random);
   const int donor = tournament(keys, population, pressure, random);
   st.parent = parent;
+  st.donor = donor;
   const CRow a = in(parent);
   if (a.n <= 0) { st.error = 1; return; }
   const int total = a.n;

@@ -296,10 +296,11 @@ template <class MemberRow>
 __device__ inline void crossover_w(const PhaseView& g, const SlotInfo& s, int child, MemberRow in, Row& out, const Keys& keys,
                                    int population, int pressure, std::uint64_t seed, const OpPolicy& pol, OpStats& st) {
   SplitMix random(seed ^ (0x617089e168ULL + std::uint64_t(child)));
-  st = OpStats{-1, 0, 0, 0, 0, 0, 0, -1};
+  st = OpStats{-1, 0, 0, 0, 0, 0, 0, -1, -1};
   const int parent = tournament(keys, population, pressure, random);
   const int donor = tournament(keys, population, pressure, random);
   st.parent = parent;
+  st.donor = donor;
   const CRow a = in(parent);
   if (a.n <= 0) { st.error = 1; return; }
   const int total = a.n;

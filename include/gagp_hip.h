@@ -132,9 +132,15 @@ int gagp_batch_prepare(gagp_batch* b);
 int gagp_batch_step(gagp_batch* b, int32_t rounds, int32_t use_graph);
 int gagp_batch_finish(gagp_batch* b);
 
-/* Device counters of search `index`: out[6] = {evaluated programs,
+/* Device counters of search `index`: out[14] = {evaluated programs,
  * evaluated program-cases, runtime errors, timeouts, VM instructions
- * executed (charged fuel), instructions of the evaluated programs}. */
+ * executed (charged fuel), instructions and constant-pool entries of the
+ * evaluated programs, and -- when the batch was prepared with profiling on
+ * (gagp_batch_set_profile) -- children made, their nodes, their constants,
+ * nodes and constants of the parent and donor rows they read, instructions
+ * and pool entries lowered for them}.
+ * Reference counterpart: the per-generation evaluation totals of
+ * cpp/src/evolution/evolve.cpp:347-350,403-409. */
 int gagp_batch_counters(gagp_batch* b, int32_t index, uint64_t* out);
 
 /* result.json of search `index` (gagp-run-v1, canonical JSON). If buf is
