@@ -394,12 +394,14 @@ __global__ __launch_bounds__(kEvalThreads) void k_eval(DSearch* all, int mode) {
 
 // ---- counters of one evaluation launch and, when profiling, of the
 // reproduction that fed it (mode as k_eval) ----
+constexpr int kTallyBlocks = 16;  // blocks per search, one atomic per counter each
 __global__ __launch_bounds__(256) void k_tally(DSearch* all, int mode) {
-  const DSearch& d = all[blockIdx.x];
+  const DSearch& d = all[blockIdx.y];
   if (mode == 1 ? !search_live(d) : (!d.active || *d.error)) return;
   const int count = mode == 1 ? *d.n_pending : d.P;
+  const int first = int(blockIdx.x) * 256 + int(threadIdx.x), stride = kTallyBlocks * 256;
   unsigned long long v[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (int i = threadIdx.x; i < count; i += 256) {
+  for (int i = first; i < count; i += stride) {
     const int slot = mode == 1 ? d.pending[i] : i;
     v[0] += d.e_errors[slot];
     v[1] += d.e_timeouts[slot];
@@ -408,7 +410,7 @@ __global__ __launch_bounds__(256) void k_tally(DSearch* all, int mode) {
     v[9] += unsigned(d.pool_len[slot]);
   }
   if (mode == 1 && d.profile)
-    for (int c = threadIdx.x; c < d.B; c += 256) {
+    for (int c = first; c < d.B; c += stride) {
       const OpStats st = d.stats[c];
       v[4] += unsigned(d.x_n[c]);
       v[5] += unsigned(d.x_nc[c]);
@@ -426,22 +428,24 @@ __global__ __launch_bounds__(256) void k_tally(DSearch* all, int mode) {
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    Counters& c = *d.counters;
-    c.programs += unsigned(count);
-    c.program_cases += (unsigned long long)count * unsigned(d.cd.C);
-    c.errors += part[0][0];
-    c.timeouts += part[1][0];
-    c.vm_instructions += part[2][0];
-    c.code_words += part[3][0];
-    c.pool_words += part[9][0];
-    if (mode == 1) {
-      c.children += unsigned(d.B);
-      c.child_nodes += part[4][0];
-      c.child_consts += part[5][0];
-      c.parent_nodes += part[6][0];
-      c.parent_consts += part[7][0];
-      c.child_code_words += part[8][0];
-      c.child_pool_words += part[10][0];
+    Counters* c = d.counters;
+    if (blockIdx.x == 0) {
+      atomicAdd(&c->programs, (unsigned long long)count);
+      atomicAdd(&c->program_cases, (unsigned long long)count * unsigned(d.cd.C));
+      if (mode == 1) atomicAdd(&c->children, (unsigned long long)d.B);
+    }
+    atomicAdd(&c->errors, part[0][0]);
+    atomicAdd(&c->timeouts, part[1][0]);
+    atomicAdd(&c->vm_instructions, part[2][0]);
+    atomicAdd(&c->code_words, part[3][0]);
+    atomicAdd(&c->pool_words, part[9][0]);
+    if (mode == 1 && d.profile) {
+      atomicAdd(&c->child_nodes, part[4][0]);
+      atomicAdd(&c->child_consts, part[5][0]);
+      atomicAdd(&c->parent_nodes, part[6][0]);
+      atomicAdd(&c->parent_consts, part[7][0]);
+      atomicAdd(&c->child_code_words, part[8][0]);
+      atomicAdd(&c->child_pool_words, part[10][0]);
     }
   }
 }
@@ -1043,7 +1047,7 @@ struct Batch::Impl {
     timer.time("k_index", stream, [&] { hipLaunchKernelGGL(k_index, member_grid, dim3(128), 0, stream, d_all); });
     timer.time("k_reuse", stream, [&] { hipLaunchKernelGGL(k_reuse, child_grid, dim3(128), 0, stream, d_all); });
     timer.time("k_eval", stream, [&] { hipLaunchKernelGGL(k_eval, eval_grid, dim3(kEvalThreads), lds_bytes(), stream, d_all, 1); });
-    timer.time("k_tally", stream, [&] { hipLaunchKernelGGL(k_tally, dim3(n), dim3(256), 0, stream, d_all, 1); });
+    timer.time("k_tally", stream, [&] { hipLaunchKernelGGL(k_tally, dim3(kTallyBlocks, n), dim3(256), 0, stream, d_all, 1); });
     if (place_lds_bytes)
       timer.time("k_place", stream, [&] { hipLaunchKernelGGL(k_place<true>, dim3(n), dim3(64), place_lds_bytes, stream, d_all); });
     if (place_hbm_bytes)
@@ -1109,7 +1113,7 @@ void Batch::prepare() {
   I.timer.time("k_lower_members", I.stream, [&] { hipLaunchKernelGGL(k_lower_members, lower_grid, dim3(64 * kVaryWaves), 0, I.stream, I.d_all); });
   const dim3 eval_grid(std::min(4096, (I.max_P + kWavesPerBlock - 1) / kWavesPerBlock), n);
   I.timer.time("k_eval_init", I.stream, [&] { hipLaunchKernelGGL(k_eval, eval_grid, dim3(kEvalThreads), I.lds_bytes(), I.stream, I.d_all, 0); });
-  I.timer.time("k_tally_init", I.stream, [&] { hipLaunchKernelGGL(k_tally, dim3(n), dim3(256), 0, I.stream, I.d_all, 0); });
+  I.timer.time("k_tally_init", I.stream, [&] { hipLaunchKernelGGL(k_tally, dim3(kTallyBlocks, n), dim3(256), 0, I.stream, I.d_all, 0); });
   I.timer.time("k_init_keys", I.stream, [&] { hipLaunchKernelGGL(k_init_keys, dim3(n), dim3(256), 0, I.stream, I.d_all); });
   I.timer.time("k_seed", I.stream, [&] { hipLaunchKernelGGL(k_seed, dim3(n), dim3(256), 0, I.stream, I.d_all); });
   HIP_OK(hipStreamSynchronize(I.stream));
