@@ -39,6 +39,7 @@ namespace gagp_amd {
 namespace {
 
 constexpr int kWavesPerBlock = 4;
+constexpr int kScoreCases = 4096;  // largest case count with an LDS score buffer per wavefront
 constexpr int kEvalThreads = 64 * kWavesPerBlock;
 
 struct Counters {
@@ -88,6 +89,8 @@ struct DSearch {
   GAS Val* pool;
   GAS int *code_len, *pool_len, *code_jumps;
   GAS std::uint64_t* child_hash;
+  // pending[B]: children to score, general ones from the front and vm_fast
+  // ones from the back; n_pending[2] counts each.
   GAS int *pending, *n_pending;
   GAS std::uint64_t* table_key;  // genotype hash | 1, 0 = empty
   GAS int* table;                // a member with that hash
@@ -108,6 +111,7 @@ struct DSearch {
   GAS unsigned *e_errors, *e_timeouts;  // per evaluated slot
   GAS unsigned long long* e_fuel;
   int profile;  // tally reproduction row sizes too
+  int general_scores;  // the general evaluation kernel has an LDS score buffer for this search
 };
 
 __device__ inline CRow member_row(const DSearch& d, int m) {
@@ -153,13 +157,14 @@ __device__ inline void lower_into_w(const DSearch& d, const CRow& row, int slot)
     if (!ok) raise(d, 200 + r.status);
     d.code_len[slot] = ok ? r.code : 0;
     d.pool_len[slot] = ok ? r.constants : 0;
-    d.code_jumps[slot] = r.jumps;
+    // bit 0: jumps; bit 1: typed straight-line (vm_fast); bit 2: Bool result
+    d.code_jumps[slot] = r.jumps | (r.fast ? 2 : 0) | (r.result_type == TAG_BOOL ? 4 : 0);
   }
 }
 constexpr int kVaryWaves = 4;  // wavefronts (rows) per block of the row kernels
 __global__ __launch_bounds__(64 * kVaryWaves) void k_lower_members(DSearch* all) {
   const DSearch& d = all[blockIdx.y];
-  const int m = blockIdx.x * kVaryWaves + int(threadIdx.x >> 6);
+  const int m = blockIdx.x * kVaryWaves + wave_index();
   if (!d.active || m >= d.P || *d.error) return;
   lower_into_w(d, member_row(d, m), m);
 }
@@ -167,7 +172,7 @@ __global__ __launch_bounds__(64 * kVaryWaves) void k_lower_members(DSearch* all)
 // ---- reproduction: one wavefront per child (phase_operators.cuh:156-265) ----
 __global__ __launch_bounds__(64 * kVaryWaves) void k_vary(DSearch* all) {
   const DSearch& d = all[blockIdx.y];
-  const int c = blockIdx.x * kVaryWaves + int(threadIdx.x >> 6);
+  const int c = blockIdx.x * kVaryWaves + wave_index();
   if (!search_live(d) || c >= d.B) return;
   const std::uint64_t seed = *d.round_seed;
   const Keys keys{d.fitness, d.wrong, d.nodes, d.has_wrong};
@@ -248,7 +253,8 @@ __global__ void k_reuse(DSearch* all) {
     if (d.grammar_sites) { for (int i = 0; i < child.n; ++i) n += child.site[i] >= 0; }
     else n += unsigned(child.n);
     d.c_nodes[c] = n;
-    d.pending[atomicAdd(d.n_pending, 1)] = c;
+    if ((d.code_jumps[c] & 2) && d.cd.C <= kScoreCases) d.pending[d.B - 1 - atomicAdd(d.n_pending + 1, 1)] = c;
+    else d.pending[atomicAdd(d.n_pending, 1)] = c;
   }
 }
 
@@ -284,16 +290,28 @@ __device__ inline double lane_score(const EvalParams& e, const LaneResult& r, in
 // When every case score is an integer of magnitude <= 2^36 (and C <= 4096)
 // every partial sum is an exact integer below 2^48, the rounding is the
 // identity and any summation order gives the canonical total.
-__device__ EvalOut eval_program(const EvalParams& e, const ProgView& pv, WaveStack& st, int lane) {
+// kFast: `pv` is certified for the typed straight-line VM (vm_fast);
+// otherwise the general VM runs it with its stack in LDS.
+// `scores` (this wavefront's LDS buffer of C doubles, or null) keeps every
+// case score of the first pass, so the canonical order is replayed from it
+// instead of by running the program again; the fast kernel always has it.
+template <bool kFast>
+__device__ EvalOut eval_program(const EvalParams& e, const ProgView& pv, WaveStack& st, double* scores, int lane) {
   const int C = e.cd.C;
   double isum = 0;
   unsigned w = 0, errs = 0, touts = 0;
   unsigned long long fuel = 0;
   bool integral = C <= 4096;
+  FastProgram fp{};
+  if (kFast) fp = fast_load(pv, lane);
+  const auto run = [&](int c, bool active) {
+    if (kFast) return vm_fast(fp, fast_inputs(e.cd, pv.n_locals, c, active), active, e.fuel, pv.result_tag);
+    return vm_run(pv, e.cd, active ? c : 0, active, e.fuel, st, lane);
+  };
   for (int tile = 0; tile < C; tile += 64) {
     const int c = tile + lane;
     const bool active = c < C;
-    const LaneResult r = vm_run(pv, e.cd, active ? c : 0, active, e.fuel, st, lane);
+    const LaneResult r = run(c, active);
     if (active) {
       bool exact = false;
       const double s = lane_score(e, r, c, exact);
@@ -303,6 +321,7 @@ __device__ EvalOut eval_program(const EvalParams& e, const ProgView& pv, WaveSta
       fuel += unsigned(r.fuel_used);
       integral = integral && s == double(std::int64_t(s)) && s >= -68719476736.0 && s <= 68719476736.0;
       isum += s;
+      if (scores) scores[c] = s;
     }
   }
   EvalOut out;
@@ -316,6 +335,8 @@ __device__ EvalOut eval_program(const EvalParams& e, const ProgView& pv, WaveSta
     return out;
   }
   // Canonical order: lane j owns logical lanes j, j+64, ... of L.
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");  // scores[] of other lanes
+  __builtin_amdgcn_wave_barrier();
   const int L = e.lanes;
   double group_total = 0;
   for (int base = 0; base < L; base += 64) {
@@ -329,10 +350,16 @@ __device__ EvalOut eval_program(const EvalParams& e, const ProgView& pv, WaveSta
     for (int s = 0; s < longest; ++s) {
       const int c = begin + s;
       const bool active = c < end;
-      const LaneResult r = vm_run(pv, e.cd, active ? c : 0, active, e.fuel, st, lane);
-      if (active) {
-        bool exact = false;
-        local = canon48(local + lane_score(e, r, c, exact));
+      if constexpr (kFast) {
+        if (active) local = canon48(local + scores[c]);
+      } else if (scores) {
+        if (active) local = canon48(local + scores[c]);
+      } else {
+        const LaneResult r = run(active ? c : 0, active);
+        if (active) {
+          bool exact = false;
+          local = canon48(local + lane_score(e, r, c, exact));
+        }
       }
     }
     // Two 32-lane groups per 64 logical lanes, summed in lane order.
@@ -366,20 +393,30 @@ __device__ inline WaveStack stack_of(unsigned char* smem, int slots, int wave) {
                    base + std::size_t(slots) * 64 * 9, slots};
 }
 
-// mode 0: score members 0..P-1 (initial population), mode 1: pending children.
+// mode 0: score members 0..P-1 (initial population), mode 1: pending
+// children. kFast: the programs certified for vm_fast (pending entries from
+// the back of the list); otherwise the rest (from the front).
+template <bool kFast>
 __global__ __launch_bounds__(kEvalThreads) void k_eval(DSearch* all, int mode) {
   const DSearch& d = all[blockIdx.y];
   if (mode == 1 ? !search_live(d) : (!d.active || *d.error)) return;
   extern __shared__ unsigned char smem[];
-  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-  WaveStack st = stack_of(smem, d.stack_slots, wave);
+  const int wave = wave_index(), lane = threadIdx.x % 64;
+  WaveStack st = stack_of(smem, kFast ? 0 : d.stack_slots, wave);
+  // Score buffers follow the stacks of the block's wavefronts.
+  double* scores = (kFast ? d.cd.C <= kScoreCases : d.general_scores != 0)
+                       ? reinterpret_cast<double*>(smem + std::size_t(kWavesPerBlock) * (kFast ? 0 : d.stack_slots) * 64 * 10) +
+                             std::size_t(wave) * d.cd.C
+                       : nullptr;
   const EvalParams e{d.cd, d.fuel, d.lanes, d.penalty, d.policy};
-  const int count = mode == 1 ? *d.n_pending : d.P;
+  const int count = mode == 1 ? d.n_pending[kFast ? 1 : 0] : d.P;
   for (int i = blockIdx.x * kWavesPerBlock + wave; i < count; i += gridDim.x * kWavesPerBlock) {
-    const int slot = mode == 1 ? d.pending[i] : i;
+    const int slot = mode == 1 ? (kFast ? d.pending[d.B - 1 - i] : d.pending[i]) : i;
+    const int pmode = d.code_jumps[slot];
+    if (mode == 0 && ((pmode & 2) != 0 && d.cd.C <= kScoreCases) != kFast) continue;  // the other kernel scores it
     const ProgView pv{d.code + std::size_t(slot) * d.code_cap, d.pool + std::size_t(slot) * d.pool_cap, d.code_len[slot],
-                      d.n_locals, d.code_jumps[slot]};
-    const EvalOut r = eval_program(e, pv, st, lane);
+                      d.n_locals, pmode & 1, (pmode >> 1) & 1, d.pool_len[slot], (pmode & 4) ? TAG_BOOL : TAG_INT};
+    const EvalOut r = eval_program<kFast>(e, pv, st, scores, lane);
     if (lane == 0) {
       // Ranking rounds every published fitness to a 40-bit mantissa.
       if (mode == 1) { d.c_fit[slot] = canon40(r.fitness); d.c_wrong[slot] = r.wrong; }
@@ -398,11 +435,11 @@ constexpr int kTallyBlocks = 16;  // blocks per search, one atomic per counter e
 __global__ __launch_bounds__(256) void k_tally(DSearch* all, int mode) {
   const DSearch& d = all[blockIdx.y];
   if (mode == 1 ? !search_live(d) : (!d.active || *d.error)) return;
-  const int count = mode == 1 ? *d.n_pending : d.P;
+  const int front = mode == 1 ? d.n_pending[0] : d.P, count = mode == 1 ? front + d.n_pending[1] : d.P;
   const int first = int(blockIdx.x) * 256 + int(threadIdx.x), stride = kTallyBlocks * 256;
   unsigned long long v[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int i = first; i < count; i += stride) {
-    const int slot = mode == 1 ? d.pending[i] : i;
+    const int slot = mode == 1 ? (i < front ? d.pending[i] : d.pending[d.B - 1 - (i - front)]) : i;
     v[0] += d.e_errors[slot];
     v[1] += d.e_timeouts[slot];
     v[2] += d.e_fuel[slot];
@@ -736,7 +773,8 @@ __global__ __launch_bounds__(64) void k_place(DSearch* all) {
     *d.n_left = left;
     *d.round_seed = next_seed;
     *d.best = best;
-    *d.n_pending = 0;
+    d.n_pending[0] = 0;
+    d.n_pending[1] = 0;
     atomicAdd(d.round, 1);
   }
 }
@@ -756,16 +794,23 @@ __global__ void k_commit(DSearch* all) {
 }
 
 // ---- evaluation session kernel (external bytecode) ----
-struct DProg { GAS const Ins* code; GAS const Val* pool; int len, has_jumps, n_locals; };
+struct DProg { GAS const Ins* code; GAS const Val* pool; int len, has_jumps, n_locals, fast, n_pool, result_tag; };
+__device__ inline ProgView view_of(const DProg& p) {
+  return ProgView{p.code, p.pool, p.len, p.n_locals, p.has_jumps, p.fast, p.n_pool, p.result_tag};
+}
+// kFast: scores the programs certified for vm_fast (LDS: a score buffer of C
+// doubles per wavefront); otherwise the rest (LDS: the general stacks).
+template <bool kFast>
 __global__ __launch_bounds__(kEvalThreads) void k_eval_programs(const DProg* progs, int n, EvalParams e, int slots,
                                                                  double* fitness, unsigned* wrong, unsigned* counts) {
   extern __shared__ unsigned char smem[];
-  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-  WaveStack st = stack_of(smem, slots, wave);
+  const int wave = wave_index(), lane = threadIdx.x % 64;
+  WaveStack st = stack_of(smem, kFast ? 0 : slots, wave);
+  double* scores = kFast ? reinterpret_cast<double*>(smem) + std::size_t(wave) * e.cd.C : nullptr;
   for (int i = blockIdx.x * kWavesPerBlock + wave; i < n; i += gridDim.x * kWavesPerBlock) {
     const DProg p = progs[i];
-    const ProgView pv{p.code, p.pool, p.len, p.n_locals, p.has_jumps};
-    const EvalOut r = eval_program(e, pv, st, lane);
+    if ((p.fast != 0) != kFast) continue;
+    const EvalOut r = eval_program<kFast>(e, view_of(p), st, scores, lane);
     if (lane == 0) {
       fitness[i] = r.fitness;
       wrong[i] = e.policy.metric == METRIC_LEGACY ? 0 : r.wrong;
@@ -775,15 +820,19 @@ __global__ __launch_bounds__(kEvalThreads) void k_eval_programs(const DProg* pro
     }
   }
 }
+template <bool kFast>
 __global__ __launch_bounds__(kEvalThreads) void k_eval_cases(DProg p, CaseData cd, int fuel, int slots, Val* values, int* errors) {
   extern __shared__ unsigned char smem[];
-  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-  WaveStack st = stack_of(smem, slots, wave);
-  const ProgView pv{p.code, p.pool, p.len, p.n_locals, p.has_jumps};
+  const int wave = wave_index(), lane = threadIdx.x % 64;
+  WaveStack st = stack_of(smem, kFast ? 0 : slots, wave);
+  const ProgView pv = view_of(p);
+  FastProgram fp{};
+  if (kFast) fp = fast_load(pv, lane);
   for (int tile = (blockIdx.x * kWavesPerBlock + wave) * 64; tile < cd.C; tile += gridDim.x * kWavesPerBlock * 64) {
     const int c = tile + lane;
     const bool active = c < cd.C;
-    const LaneResult r = vm_run(pv, cd, active ? c : 0, active, fuel, st, lane);
+    const LaneResult r = kFast ? vm_fast(fp, fast_inputs(cd, pv.n_locals, c, active), active, fuel, pv.result_tag)
+                               : vm_run(pv, cd, active ? c : 0, active, fuel, st, lane);
     if (active) {
       Val v{};
       v.i = r.v; v.tag = r.t; v.b = r.b;
@@ -958,7 +1007,7 @@ struct SearchState {
     x.pool = a.alloc<Val>(std::size_t(code_rows) * x.pool_cap);
     x.code_len = a.alloc<int>(code_rows); x.pool_len = a.alloc<int>(code_rows); x.code_jumps = a.alloc<int>(code_rows);
     x.child_hash = a.alloc<std::uint64_t>(B);
-    x.pending = a.alloc<int>(B); x.n_pending = a.alloc<int>(1);
+    x.pending = a.alloc<int>(B); x.n_pending = a.alloc<int>(2);
     const int tsize = next_pow2(2 * P);
     x.table_key = a.alloc<std::uint64_t>(tsize); x.table = a.alloc<int>(tsize); x.table_mask = tsize - 1;
     x.mt = a.alloc<std::uint64_t>(312); x.mt_left = a.alloc<std::uint64_t>(kRing); x.n_left = a.alloc<int>(1);
@@ -975,7 +1024,7 @@ struct SearchState {
     x.e_errors = a.alloc<unsigned>(code_rows); x.e_timeouts = a.alloc<unsigned>(code_rows);
     x.e_fuel = a.alloc<unsigned long long>(code_rows);
     x.error = a.alloc<int>(1);
-    HIP_OK(hipMemset(x.n_pending, 0, sizeof(int)));
+    HIP_OK(hipMemset((void*)x.n_pending, 0, 2 * sizeof(int)));
     HIP_OK(hipMemset(x.counters, 0, sizeof(Counters)));
     HIP_OK(hipMemset(x.error, 0, sizeof(int)));
     HIP_OK(hipMemset(x.round, 0, sizeof(int)));
@@ -1034,7 +1083,7 @@ struct Batch::Impl {
     if (d_all) (void)hipFree(d_all);
     if (stream) (void)hipStreamDestroy(stream);
   }
-  std::size_t lds_bytes() const { return std::size_t(kWavesPerBlock) * max_slots * 64 * 10; }
+  std::size_t eval_fast_lds = 0, eval_general_lds = 0;  // dynamic LDS of the two evaluation kernels
 
   void enqueue_round() {
     const int n = int(searches.size());
@@ -1046,7 +1095,9 @@ struct Batch::Impl {
     timer.time("k_clear", stream, [&] { hipLaunchKernelGGL(k_clear, table_grid, dim3(256), 0, stream, d_all); });
     timer.time("k_index", stream, [&] { hipLaunchKernelGGL(k_index, member_grid, dim3(128), 0, stream, d_all); });
     timer.time("k_reuse", stream, [&] { hipLaunchKernelGGL(k_reuse, child_grid, dim3(128), 0, stream, d_all); });
-    timer.time("k_eval", stream, [&] { hipLaunchKernelGGL(k_eval, eval_grid, dim3(kEvalThreads), lds_bytes(), stream, d_all, 1); });
+    timer.time("k_eval", stream, [&] { hipLaunchKernelGGL(k_eval<true>, eval_grid, dim3(kEvalThreads), eval_fast_lds, stream, d_all, 1); });
+    timer.time("k_eval_general", stream,
+               [&] { hipLaunchKernelGGL(k_eval<false>, eval_grid, dim3(kEvalThreads), eval_general_lds, stream, d_all, 1); });
     timer.time("k_tally", stream, [&] { hipLaunchKernelGGL(k_tally, dim3(kTallyBlocks, n), dim3(256), 0, stream, d_all, 1); });
     if (place_lds_bytes)
       timer.time("k_place", stream, [&] { hipLaunchKernelGGL(k_place<true>, dim3(n), dim3(64), place_lds_bytes, stream, d_all); });
@@ -1081,9 +1132,17 @@ void Batch::prepare() {
   const auto t0 = std::chrono::steady_clock::now();
   if (!I.stream) HIP_OK(hipStreamCreateWithFlags(&I.stream, hipStreamNonBlocking));
   std::vector<DSearch> host;
+  constexpr std::size_t kLdsLimit = 160 * 1024;
   for (auto& s : I.searches) {
     s->allocate();
     s->d.profile = I.timer.on ? 1 : 0;
+    // Evaluation LDS per block: general kernel = stacks (+ score buffers when
+    // they fit), fast kernel = score buffers.
+    const std::size_t stacks = std::size_t(kWavesPerBlock) * s->d.stack_slots * 64 * 10;
+    const std::size_t score_buf = std::size_t(kWavesPerBlock) * 8 * s->d.cd.C;
+    s->d.general_scores = s->d.cd.C <= kScoreCases && stacks + score_buf <= kLdsLimit;
+    I.eval_general_lds = std::max(I.eval_general_lds, stacks + (s->d.general_scores ? score_buf : 0));
+    if (s->d.cd.C <= kScoreCases) I.eval_fast_lds = std::max(I.eval_fast_lds, score_buf);
     host.push_back(s->d);
     I.max_P = std::max(I.max_P, s->P);
     I.max_B = std::max(I.max_B, s->B);
@@ -1097,8 +1156,13 @@ void Batch::prepare() {
   if (!n) return;
   HIP_OK(hipMalloc(&I.d_all, sizeof(DSearch) * host.size()));
   HIP_OK(hipMemcpy(I.d_all, host.data(), sizeof(DSearch) * host.size(), hipMemcpyHostToDevice));
-  if (I.lds_bytes() > 65536) {
-    HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_eval), hipFuncAttributeMaxDynamicSharedMemorySize, int(I.lds_bytes())));
+  if (I.eval_general_lds > 65536) {
+    HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_eval<false>), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               int(I.eval_general_lds)));
+  }
+  if (I.eval_fast_lds > 65536) {
+    HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_eval<true>), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               int(I.eval_fast_lds)));
   }
   if (I.place_lds_bytes)
     HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_place<true>), hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1112,7 +1176,9 @@ void Batch::prepare() {
   const dim3 lower_grid((I.max_P + kVaryWaves - 1) / kVaryWaves, n);
   I.timer.time("k_lower_members", I.stream, [&] { hipLaunchKernelGGL(k_lower_members, lower_grid, dim3(64 * kVaryWaves), 0, I.stream, I.d_all); });
   const dim3 eval_grid(std::min(4096, (I.max_P + kWavesPerBlock - 1) / kWavesPerBlock), n);
-  I.timer.time("k_eval_init", I.stream, [&] { hipLaunchKernelGGL(k_eval, eval_grid, dim3(kEvalThreads), I.lds_bytes(), I.stream, I.d_all, 0); });
+  I.timer.time("k_eval_init", I.stream, [&] { hipLaunchKernelGGL(k_eval<true>, eval_grid, dim3(kEvalThreads), I.eval_fast_lds, I.stream, I.d_all, 0); });
+  I.timer.time("k_eval_init_general", I.stream,
+               [&] { hipLaunchKernelGGL(k_eval<false>, eval_grid, dim3(kEvalThreads), I.eval_general_lds, I.stream, I.d_all, 0); });
   I.timer.time("k_tally_init", I.stream, [&] { hipLaunchKernelGGL(k_tally, dim3(kTallyBlocks, n), dim3(256), 0, I.stream, I.d_all, 0); });
   I.timer.time("k_init_keys", I.stream, [&] { hipLaunchKernelGGL(k_init_keys, dim3(n), dim3(256), 0, I.stream, I.d_all); });
   I.timer.time("k_seed", I.stream, [&] { hipLaunchKernelGGL(k_seed, dim3(n), dim3(256), 0, I.stream, I.d_all); });
@@ -1409,9 +1475,66 @@ std::vector<Ins> verify_program(const ProgramIn& p, int& max_stack, bool& jumps)
 }
 }  // namespace
 
+// Static tags of straight-line external bytecode: the program is certified
+// for vm_fast when every operand tag is known and Int or Bool and every
+// instruction is one vm_fast restates with checks that cannot fail (the same
+// rules as the lowering's fast_operation). `input_tags[k]` is the one tag
+// input k has wherever it is bound (TAG_INVALID when the cases disagree).
+struct FastCert { bool fast = false; int result_tag = TAG_INT; };
+FastCert certify_fast(const std::vector<Ins>& code, const std::vector<Val>& consts, const std::vector<std::uint8_t>& input_tags,
+                      int n_locals, int max_stack, bool jumps, int C) {
+  FastCert cert;
+  if (jumps || max_stack > kFastStack || code.size() > std::size_t(kFastCode) || consts.size() > std::size_t(kFastPool) ||
+      n_locals > kFastInputs || C > kScoreCases)
+    return cert;
+  std::vector<std::uint8_t> tags;
+  for (const Ins& ins : code) {
+    const int h = int(tags.size());
+    if (ins.sp != h) return cert;
+    const std::uint8_t* args = h >= ins.argc ? tags.data() + (h - ins.argc) : nullptr;
+    switch (ins.op) {
+      case OP_PUSH_CONST: {
+        const int t = consts[std::size_t(ins.a)].tag;
+        if (!fast_type(t)) return cert;
+        tags.push_back(std::uint8_t(t));
+        break;
+      }
+      case OP_LOAD: {
+        if (ins.a >= n_locals || ins.a >= int(input_tags.size()) || !fast_type(input_tags[std::size_t(ins.a)])) return cert;
+        tags.push_back(input_tags[std::size_t(ins.a)]);
+        break;
+      }
+      case OP_RETURN:
+        if (h < 1 || !fast_type(tags.back())) return cert;
+        cert.fast = true;
+        cert.result_tag = tags.back();
+        return cert;
+      case OP_CALL_BUILTIN: {
+        if (!args || !fast_operation(OP_CALL_BUILTIN, ins.a, ins.argc, args)) return cert;
+        tags.resize(std::size_t(h - ins.argc));
+        tags.push_back(TAG_INT);
+        break;
+      }
+      default: {
+        const int arity = ins.op == OP_NEG || ins.op == OP_NOT || ins.op == OP_CHECK_INT ? 1 : 2;
+        if (h < arity) return cert;
+        const std::uint8_t* a = tags.data() + (h - arity);
+        if (!fast_operation(ins.op, 0, arity, a)) return cert;
+        const bool compare = ins.op >= OP_LT && ins.op <= OP_NE;
+        const std::uint8_t rt = compare ? TAG_BOOL : ins.op == OP_NOT ? TAG_BOOL : TAG_INT;
+        tags.resize(std::size_t(h - arity));
+        tags.push_back(rt);
+        break;
+      }
+    }
+  }
+  return cert;  // falls off the end: the general VM reports it
+}
+
 struct Session::Impl {
   DeviceArena arena;
   DeviceCases cases;
+  std::vector<std::uint8_t> input_tags;
   std::size_t arena_len = 0;
   int fuel, lanes;
   double penalty;
@@ -1439,6 +1562,19 @@ Session::Session(const Cases& c, int fuel, int lanes, double penalty, Policy pol
   for (const auto& v : c.expected) check_view(v, c.arena.size());
   impl_->arena_len = c.arena.size();
   impl_->cases.upload(impl_->arena, c);
+  const std::size_t K = c.input_names.size();
+  impl_->input_tags.assign(K, TAG_INVALID);
+  std::vector<bool> seen(K, false), mixed(K, false);
+  for (int i = 0; i < c.n; ++i)
+    for (std::size_t k = 0; k < K; ++k) {
+      const std::size_t at = std::size_t(i) * K + k;
+      if (!c.set.empty() && !c.set[at]) continue;
+      const std::uint8_t t = c.inputs[at].tag;
+      if (!seen[k]) { seen[k] = true; impl_->input_tags[k] = t; }
+      else if (impl_->input_tags[k] != t) mixed[k] = true;
+    }
+  for (std::size_t k = 0; k < K; ++k)
+    if (mixed[k]) impl_->input_tags[k] = TAG_INVALID;
   impl_->fuel = fuel;
   impl_->lanes = lanes;
   impl_->penalty = penalty;
@@ -1454,13 +1590,17 @@ void Session::eval(const std::vector<ProgramIn>& programs, double* fitness, std:
   std::vector<Ins> code;
   std::vector<Val> pool;
   std::vector<std::size_t> code_off, pool_off;
-  std::vector<int> lens, jumps_v, locals;
+  std::vector<int> lens, jumps_v, locals, fast_v, pool_n, result_v;
   int slots = 1;
   for (const auto& p : programs) {
     int ms = 0;
     bool j = false;
     const auto ins = verify_program(p, ms, j);
     for (const auto& v : p.consts) check_view(v, I.arena_len);
+    const FastCert cert = certify_fast(ins, p.consts, I.input_tags, p.n_locals, ms, j, I.cases.cd.C);
+    fast_v.push_back(cert.fast);
+    result_v.push_back(cert.result_tag);
+    pool_n.push_back(int(p.consts.size()));
     slots = std::max(slots, ms + 1);
     code_off.push_back(code.size());
     pool_off.push_back(pool.size());
@@ -1475,21 +1615,33 @@ void Session::eval(const std::vector<ProgramIn>& programs, double* fitness, std:
   auto dcode = tmp.upload(code);
   auto dpool = tmp.upload(pool);
   std::vector<DProg> progs(static_cast<std::size_t>(n));
-  for (int i = 0; i < n; ++i) progs[i] = DProg{dcode + code_off[i], dpool + pool_off[i], lens[i], jumps_v[i], locals[i]};
+  for (int i = 0; i < n; ++i)
+    progs[i] = DProg{dcode + code_off[i], dpool + pool_off[i], lens[i], jumps_v[i], locals[i], fast_v[i], pool_n[i], result_v[i]};
   auto dprogs = tmp.upload(progs);
   auto dfit = tmp.alloc<double>(std::size_t(n));
   auto dwrong = tmp.alloc<unsigned>(std::size_t(n));
   auto dcounts = tmp.alloc<unsigned>(std::size_t(n) * 3);
   const EvalParams e{I.cases.cd, I.fuel, I.lanes, I.penalty, I.policy};
   const std::size_t lds = std::size_t(kWavesPerBlock) * slots * 64 * 10;
+  const std::size_t fast_lds = std::size_t(kWavesPerBlock) * 8 * std::size_t(I.cases.cd.C);
+  const bool any_fast = std::count(fast_v.begin(), fast_v.end(), 1) > 0, any_general = std::count(fast_v.begin(), fast_v.end(), 0) > 0;
   if (lds > 65536)
-    HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_eval_programs), hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+    HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_eval_programs<false>), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               int(lds)));
+  if (any_fast && fast_lds > 65536)
+    HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_eval_programs<true>), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               int(fast_lds)));
   hipEvent_t a, b;
   HIP_OK(hipEventCreate(&a));
   HIP_OK(hipEventCreate(&b));
   const int blocks = std::min(4096, (n + kWavesPerBlock - 1) / kWavesPerBlock);
   HIP_OK(hipEventRecord(a, nullptr));
-  hipLaunchKernelGGL(k_eval_programs, dim3(blocks), dim3(kEvalThreads), lds, nullptr, dprogs, n, e, slots, dfit, dwrong, dcounts);
+  if (any_fast)
+    hipLaunchKernelGGL(k_eval_programs<true>, dim3(blocks), dim3(kEvalThreads), fast_lds, nullptr, dprogs, n, e, slots, dfit, dwrong,
+                       dcounts);
+  if (any_general)
+    hipLaunchKernelGGL(k_eval_programs<false>, dim3(blocks), dim3(kEvalThreads), lds, nullptr, dprogs, n, e, slots, dfit, dwrong,
+                       dcounts);
   HIP_OK(hipGetLastError());
   HIP_OK(hipEventRecord(b, nullptr));
   HIP_OK(hipEventSynchronize(b));
@@ -1511,15 +1663,22 @@ void Session::eval_cases(const ProgramIn& program, std::vector<Val>& values, std
   DeviceArena tmp;
   std::vector<Val> consts = program.consts;
   if (consts.empty()) consts.push_back(val_invalid());
-  const DProg p{tmp.upload(ins), tmp.upload(consts), int(ins.size()), j, program.n_locals};
+  const FastCert cert = certify_fast(ins, program.consts, I.input_tags, program.n_locals, ms, j, I.cases.cd.C);
+  const DProg p{tmp.upload(ins), tmp.upload(consts), int(ins.size()), j, program.n_locals, cert.fast, int(program.consts.size()),
+                cert.result_tag};
   const int C = I.cases.cd.C;
   auto dv = tmp.alloc<Val>(std::size_t(C));
   auto de = tmp.alloc<int>(std::size_t(C));
-  const std::size_t lds = std::size_t(kWavesPerBlock) * slots * 64 * 10;
-  if (lds > 65536)
-    HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_eval_cases), hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
   const int blocks = std::max(1, std::min(1024, (C + 255) / 256));
-  hipLaunchKernelGGL(k_eval_cases, dim3(blocks), dim3(kEvalThreads), lds, nullptr, p, I.cases.cd, I.fuel, slots, dv, de);
+  if (cert.fast) {
+    hipLaunchKernelGGL(k_eval_cases<true>, dim3(blocks), dim3(kEvalThreads), 0, nullptr, p, I.cases.cd, I.fuel, slots, dv, de);
+  } else {
+    const std::size_t lds = std::size_t(kWavesPerBlock) * slots * 64 * 10;
+    if (lds > 65536)
+      HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_eval_cases<false>), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 int(lds)));
+    hipLaunchKernelGGL(k_eval_cases<false>, dim3(blocks), dim3(kEvalThreads), lds, nullptr, p, I.cases.cd, I.fuel, slots, dv, de);
+  }
   HIP_OK(hipGetLastError());
   HIP_OK(hipDeviceSynchronize());
   values.resize(std::size_t(C));

@@ -523,7 +523,33 @@ struct Ins {
 static_assert(sizeof(Ins) == 8, "Ins is 8 bytes");
 constexpr int kMaxStack = 64;
 
-struct LowerResult { int status, code, constants, stack_bound, result_type, jumps; };
+// Limits of the typed straight-line VM path (vm_fast in vm.hpp).
+constexpr int kFastStack = 16, kFastCode = 128, kFastPool = 128, kFastInputs = 4;
+// `fast`: every instruction and operand is certified for the typed
+// straight-line path (no jumps, every value statically Int or Bool, integer
+// builtins only, within the kFast* limits).
+struct LowerResult { int status, code, constants, stack_bound, result_type, jumps, fast; };
+GHD inline bool fast_type(int tag) { return tag == TAG_INT || tag == TAG_BOOL; }
+GHD inline bool fast_operation(int opcode, int builtin, int arity, const std::uint8_t* args) {
+  const auto all_int = [&] {
+    for (int k = 0; k < arity && k < 3; ++k) if (args[k] != TAG_INT) return false;
+    return arity <= 3;
+  };
+  switch (opcode) {
+    case OP_NEG: case OP_CHECK_INT: return args[0] == TAG_INT;
+    case OP_NOT: return args[0] == TAG_BOOL;
+    case OP_ADD: case OP_SUB: case OP_MUL: case OP_MOD: case OP_LT: case OP_LE: case OP_GT: case OP_GE: return all_int();
+    case OP_EQ: case OP_NE: return args[0] == args[1] && fast_type(args[0]);
+    case OP_CALL_BUILTIN:
+      switch (builtin) {
+        case B_ABS: return arity == 1 && all_int();
+        case B_MIN: case B_MAX: case B_IDIV0: case B_IMOD0: case B_TRUNC_DIV0: case B_TRUNC_MOD0: return arity == 2 && all_int();
+        case B_CLIP: return arity == 3 && all_int();
+        default: return false;
+      }
+    default: return false;
+  }
+}
 
 // An open operation of the post-order walk: its node, operands seen so far,
 // pending jump sites, the stack height at its start and its operand types.
@@ -542,8 +568,9 @@ template <class Frames, class OriginAt>
 GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, OriginAt origin_at, const std::uint8_t* local_types,
                                         int local_count, int expected_type, Ins* code, int code_capacity, Val* pool,
                                         int pool_capacity, Frames& frames) {
-  LowerResult out{ST_INVALID, 0, 0, 0, TAG_INVALID, 0};
+  LowerResult out{ST_INVALID, 0, 0, 0, TAG_INVALID, 0, 0};
   if (t.n <= 0 || t.n > 4096) return out;
+  bool fast = local_count <= kFastInputs;
   int depth = 0, cursor = 0, stack = 0, cconst = 0;
   int completed = TAG_INVALID;
   bool have_value = false;
@@ -567,6 +594,7 @@ GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, Origi
         if (v.tag == TAG_INVALID || v.tag == TAG_FALLBACK) { out.status = ST_UNSUPPORTED; return out; }
         if (out.constants >= pool_capacity) { out.status = ST_CAPACITY; return out; }
         pool[out.constants] = v;
+        fast = fast && fast_type(v.tag);
         if (!emit(OP_PUSH_CONST, out.constants, 0, node.fuel)) return out;
         ++out.constants;
         grow(1);
@@ -576,6 +604,7 @@ GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, Origi
         if (node.operand < 0 || node.operand >= local_count) return out;
         completed = local_types[node.operand];
         if (completed == TAG_INVALID || completed == TAG_FALLBACK) { out.status = ST_UNSUPPORTED; return out; }
+        fast = fast && fast_type(completed);
         if (!emit(OP_LOAD, node.operand, 0, node.fuel)) return out;
         grow(1);
         have_value = true;
@@ -596,6 +625,8 @@ GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, Origi
       if (!emit(OP_RETURN, 0, 0, 1)) return out;
       out.status = ST_OK;
       out.result_type = completed;
+      out.fast = fast && !out.jumps && fast_type(completed) && out.stack_bound <= kFastStack && out.code <= kFastCode &&
+                 out.constants <= kFastPool;
       return out;
     }
     Frame f = frames.get(depth - 1);
@@ -650,6 +681,7 @@ GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, Origi
       if (rt == TAG_INVALID) rt = node.type;
       else if (node.type != TAG_INVALID && node.type != rt) return out;
       if (rt == TAG_INVALID || rt == TAG_FALLBACK) { out.status = ST_UNSUPPORTED; return out; }
+      fast = fast && fast_operation(node.opcode, node.operand, node.arity, args) && fast_type(rt);
       // operands are on the stack: the instruction sees the stack with them
       if (node.opcode == OP_CALL_BUILTIN) { if (!emit(OP_CALL_BUILTIN, node.operand, node.arity, node.fuel)) return out; }
       else if (!emit(node.opcode, 0, 0, node.fuel)) return out;

@@ -24,6 +24,10 @@
 namespace gagp_amd {
 
 __device__ inline int wlane() { return int(threadIdx.x & 63u); }
+// This wavefront's index in its block, marked uniform for the compiler (it
+// cannot see that threadIdx.x / 64 is the same on every lane), so values
+// derived from it stay in scalar registers.
+__device__ inline int wave_index() { return __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6)); }
 __device__ inline std::uint64_t lanes_below() { return (1ull << wlane()) - 1ull; }
 __device__ inline int wave_first(std::uint64_t mask) { return __ffsll((long long)mask) - 1; }
 __device__ inline int rdlane(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }

@@ -22,6 +22,9 @@ struct ProgView {
   int len;
   int n_locals;
   int has_jumps;
+  int fast;      // certified for vm_fast (phase.hpp fast_operation)
+  int n_pool;    // constant-pool entries
+  int result_tag;  // static result type of a fast program
 };
 
 struct CaseData {
@@ -301,6 +304,174 @@ __device__ inline LaneResult vm_run(const ProgView& p, const CaseData& cd, int c
     }
   }
   r.fuel_used = fuel0 - fuel;
+  return r;
+}
+
+// ---- typed straight-line path ----
+// A certified program (no jumps, every value statically Int or Bool, integer
+// builtins only) needs no run-time tags: values are int64 (Bool as 0/1), the
+// stack is a register array indexed by the instruction's static height (the
+// index is uniform, so the compiler addresses the array with indexed register
+// moves), and every live lane executes the same instruction. Instruction j is
+// held by lane j % 64 of register group j / 64 and read with readlane; so are
+// the constants. Fuel is charged uniformly: a timeout ends every live lane at
+// the same instruction, as the per-lane charge of vm_run would.
+struct FastProgram {
+  int w0[2], w1[2];  // instruction: op | argc << 8 | sp << 16 | fuel << 24, operand
+  int c0[2], c1[2];  // constant payload, low and high words
+  int len;
+};
+__device__ inline FastProgram fast_load(const ProgView& p, int lane) {
+  FastProgram f;
+  f.len = p.len;
+#pragma unroll
+  for (int g = 0; g < 2; ++g) {
+    const int j = g * 64 + lane;
+    int w0 = 0, w1 = 0;
+    if (j < p.len) {
+      const Ins ins = p.code[j];
+      w0 = int(ins.op) | int(ins.argc) << 8 | int(ins.sp) << 16 | int(unsigned(ins.fuel) << 24);
+      w1 = ins.a;
+    }
+    std::int64_t v = 0;
+    if (j < p.n_pool) {
+      const Val x = p.pool[j];
+      v = x.tag == TAG_BOOL ? std::int64_t(x.b != 0) : x.i;
+    }
+    f.w0[g] = w0;
+    f.w1[g] = w1;
+    f.c0[g] = int(std::uint32_t(std::uint64_t(v)));
+    f.c1[g] = int(std::uint32_t(std::uint64_t(v) >> 32));
+  }
+  return f;
+}
+__device__ inline int fast_pick(const int (&v)[2], int k) {
+  const int lo = __builtin_amdgcn_readlane(v[0], k & 63), hi = __builtin_amdgcn_readlane(v[1], k & 63);
+  return k < 64 ? lo : hi;
+}
+// Inputs of this lane's case: payloads (Bool as 0/1) and the bound mask.
+// (Separate fields and a uniform select keep them in registers; an array
+// indexed by the operand would live in scratch.)
+static_assert(kFastInputs == 4, "four input registers");
+struct FastInputs {
+  std::int64_t v0, v1, v2, v3;
+  unsigned bound;
+  __device__ std::int64_t pick(int k) const { return k == 0 ? v0 : k == 1 ? v1 : k == 2 ? v2 : v3; }
+};
+__device__ inline FastInputs fast_inputs(const CaseData& cd, int n_inputs, int c, bool active) {
+  FastInputs in{0, 0, 0, 0, 0};
+  std::int64_t v[kFastInputs];
+#pragma unroll
+  for (int k = 0; k < kFastInputs; ++k) {
+    v[k] = 0;
+    if (k < n_inputs && k < cd.n_inputs && active) {
+      const std::size_t at = std::size_t(k) * cd.C + c;
+      const std::uint8_t tag = cd.in_tag[at];
+      if (tag != TAG_INVALID) in.bound |= 1u << k;
+      v[k] = tag == TAG_BOOL ? std::int64_t(cd.in_b[at] != 0) : cd.in_val[at];
+    }
+  }
+  in.v0 = v[0];
+  in.v1 = v[1];
+  in.v2 = v[2];
+  in.v3 = v[3];
+  return in;
+}
+__device__ inline LaneResult vm_fast(const FastProgram& f, const FastInputs& in, bool active, int fuel0, int result_tag) {
+  LaneResult r{0, TAG_INVALID, 0, active ? ERR_NONE : ERR_VALUE, 0};
+  bool done = !active;
+  std::int64_t s[kFastStack];
+#pragma unroll
+  for (int k = 0; k < kFastStack; ++k) s[k] = 0;
+  int fuel = fuel0;
+  const auto finish = [&](int err) {
+    if (!done) { r.err = err; r.fuel_used = fuel0 - fuel; done = true; }
+  };
+  for (int ip = 0; ip < f.len; ++ip) {
+    if (__ballot(!done) == 0) break;
+    const int w = fast_pick(f.w0, ip), a = fast_pick(f.w1, ip);
+    const int op = w & 0xff, argc = (w >> 8) & 0xff, sp = (w >> 16) & 0xff, cost = int(unsigned(w) >> 24);
+    if (fuel < cost) { finish(ERR_TIMEOUT); break; }
+    fuel -= cost;
+    // Heights are certified within the stack; the masks keep every index in range.
+    const int top = (sp - 1) & (kFastStack - 1), below = (sp - 2) & (kFastStack - 1);
+    // MOD and the four protected-division builtins share one 64-bit
+    // truncating division: x = y*q + r, with y == -1 giving q = -x wrapped,
+    // r = 0 (never the trapping INT64_MIN / -1).
+    const bool divide = op == OP_MOD || (op == OP_CALL_BUILTIN && (a == B_IDIV0 || a == B_IMOD0 || a == B_TRUNC_DIV0 ||
+                                                                   a == B_TRUNC_MOD0));
+    if (divide) {
+      const int at = op == OP_MOD ? below : (sp - argc) & (kFastStack - 1);
+      const std::int64_t x = s[at], y = s[(at + 1) & (kFastStack - 1)];
+      const std::int64_t safe = (y == 0 || y == -1) ? 1 : y;
+      std::int64_t q = x / safe, rem = x - q * safe;
+      if (y == -1) { q = wrap_neg(x); rem = 0; }
+      const std::int64_t floor_mod = (rem != 0 && ((rem < 0) != (y < 0))) ? rem + y : rem;  // Python %
+      std::int64_t out;
+      if (op == OP_MOD) {
+        if (y == 0) finish(ERR_ZERO_DIV);
+        out = y == 0 ? 0 : floor_mod;
+      } else if (a == B_IDIV0) out = y == 0 ? 0 : q;                                           // wrap_div
+      else if (a == B_IMOD0) out = y == 0 ? 0 : floor_mod;                                     // py_mod
+      else if (a == B_TRUNC_DIV0) out = (y == 0 || (x == INT64_MIN && y == -1)) ? 0 : q;       // trunc_div0
+      else out = (y == 0 || (x == INT64_MIN && y == -1)) ? 0 : rem;                            // trunc_mod0
+      s[at] = out;
+      continue;
+    }
+    switch (op) {
+      case OP_PUSH_CONST: {
+        const std::uint32_t lo = std::uint32_t(fast_pick(f.c0, a)), hi = std::uint32_t(fast_pick(f.c1, a));
+        s[sp & (kFastStack - 1)] = std::int64_t(std::uint64_t(hi) << 32 | lo);
+        break;
+      }
+      case OP_LOAD:
+        if (!((in.bound >> (a & 31)) & 1u)) finish(ERR_NAME);
+        s[sp & (kFastStack - 1)] = in.pick(a);
+        break;
+      case OP_CHECK_INT: break;
+      case OP_NEG: s[top] = wrap_neg(s[top]); break;
+      case OP_NOT: s[top] = s[top] == 0; break;
+      case OP_ADD: s[below] = wrap_add(s[below], s[top]); break;
+      case OP_SUB: s[below] = wrap_sub(s[below], s[top]); break;
+      case OP_MUL: s[below] = wrap_mul(s[below], s[top]); break;
+      case OP_LT: s[below] = s[below] < s[top]; break;
+      case OP_LE: s[below] = s[below] <= s[top]; break;
+      case OP_GT: s[below] = s[below] > s[top]; break;
+      case OP_GE: s[below] = s[below] >= s[top]; break;
+      case OP_EQ: s[below] = s[below] == s[top]; break;
+      case OP_NE: s[below] = s[below] != s[top]; break;
+      case OP_CALL_BUILTIN: {
+        const int base = (sp - argc) & (kFastStack - 1);
+        const std::int64_t x = s[base], y = s[(base + 1) & (kFastStack - 1)];
+        std::int64_t out = 0;
+        switch (a) {
+          case B_ABS: out = x < 0 ? wrap_neg(x) : x; break;
+          case B_MIN: out = x < y ? x : y; break;
+          case B_MAX: out = x > y ? x : y; break;
+          case B_CLIP: {
+            const std::int64_t hi = s[(base + 2) & (kFastStack - 1)], m = x > y ? x : y;
+            out = m < hi ? m : hi;
+            break;
+          }
+          default: finish(ERR_NAME); break;
+        }
+        s[base] = out;
+        break;
+      }
+      case OP_RETURN: {
+        if (!done) {
+          if (result_tag == TAG_BOOL) { r.t = TAG_BOOL; r.b = s[top] != 0; r.v = 0; }
+          else { r.t = TAG_INT; r.v = s[top]; }
+          r.fuel_used = fuel0 - fuel;
+          done = true;
+        }
+        ip = f.len;
+        break;
+      }
+      default: finish(ERR_VALUE); break;
+    }
+  }
+  finish(ERR_VALUE);  // ran off the end without RETURN
   return r;
 }
 #endif
