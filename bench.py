@@ -33,8 +33,8 @@ POPULATION, GENERATIONS, TRAIN_CASES, INPUTS = 4096, 128, 256, 1
 HBM_PEAK_GBS = 8000.0
 REF_GAGP = os.path.join(ROOT, "oracle", "_ref", "gagp")
 TRAFFIC = os.path.join(ROOT, "profiles", "quintic_p4096_hbm_traffic.md")
-ROUND_KERNELS = ("k_record", "k_vary", "k_clear", "k_index", "k_reuse", "k_eval", "k_tally", "k_place", "k_place_hbm",
-                 "k_commit")
+ROUND_KERNELS = ("k_record", "k_vary", "k_vary_global", "k_clear", "k_index", "k_reuse", "k_eval", "k_eval_general",
+                 "k_tally", "k_place", "k_place_hbm", "k_commit")
 
 
 def algorithmic_bytes(c, steps):

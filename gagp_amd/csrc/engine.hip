@@ -68,7 +68,8 @@ struct DSearch {
   SlotInfo slot;
   int fixed_nodes, grammar_sites, has_wrong, expected_type, n_locals, site_depth;
   std::uint64_t seed;
-  PhaseView g;
+  PhaseTables g;
+  int stage_tables;  // the phase tables fit the reproduction kernel's LDS
   GAS const std::uint8_t* local_types;
   CaseData cd;
   GAS int *cur_origin, *cur_site, *cur_n, *cur_nc;
@@ -135,10 +136,11 @@ __global__ void k_init_pop(DSearch* all) {
   const std::size_t o = std::size_t(m) * d.R;
   GenResult gen{ST_INVALID, 0, 0, 0};
   const int room = d.slot.max_variable_nodes < kPhaseCapacity ? d.slot.max_variable_nodes : kPhaseCapacity;
+  const PhaseView g = view_of(d.g);
   for (int logical = ramp, attempt = 0; gen.status != ST_OK; ++attempt) {
     if (room <= 0 || room > d.R) { raise(d, 101); return; }
     const std::uint64_t s = splitmix_mix(d.seed ^ splitmix_mix((std::uint64_t(m) * 8 + 0) * 64 + std::uint64_t(attempt)));
-    gen = generate_phase(d.g, d.g.root, 0, d.slot.max_depth, room, s, d.cur_origin + o, d.cur_cval + o, d.cur_site + o, logical);
+    gen = generate_phase(g, g.root, 0, d.slot.max_depth, room, s, d.cur_origin + o, d.cur_cval + o, d.cur_site + o, logical);
     if (gen.status != ST_OK && attempt % 4 == 3 && logical < sd) ++logical;
     if (attempt >= 256) { raise(d, 102); return; }
   }
@@ -148,10 +150,10 @@ __global__ void k_init_pop(DSearch* all) {
 }
 
 // ---- lowering: one wavefront per row ----
-__device__ inline void lower_into_w(const DSearch& d, const CRow& row, int slot) {
+__device__ inline void lower_into_w(const PhaseView& g, const DSearch& d, const CRow& row, int slot) {
   Ins* code = d.code + std::size_t(slot) * d.code_cap;
   Val* pool = d.pool + std::size_t(slot) * d.pool_cap;
-  const LowerResult r = lower_phase_w(d.g, row, d.local_types, d.n_locals, d.expected_type, code, d.code_cap, pool, d.pool_cap);
+  const LowerResult r = lower_phase_w(g, row, d.local_types, d.n_locals, d.expected_type, code, d.code_cap, pool, d.pool_cap);
   const bool ok = r.status == ST_OK && r.stack_bound >= 1 && r.stack_bound <= d.stack_slots;
   if (wlane() == 0) {
     if (!ok) raise(d, 200 + r.status);
@@ -166,24 +168,30 @@ __global__ __launch_bounds__(64 * kVaryWaves) void k_lower_members(DSearch* all)
   const DSearch& d = all[blockIdx.y];
   const int m = blockIdx.x * kVaryWaves + wave_index();
   if (!d.active || m >= d.P || *d.error) return;
-  lower_into_w(d, member_row(d, m), m);
+  lower_into_w(view_of(d.g), d, member_row(d, m), m);
 }
 
 // ---- reproduction: one wavefront per child (phase_operators.cuh:156-265) ----
+// kStaged: the block first copies the phase tables into LDS (searches whose
+// tables fit); otherwise the operators read them from global memory.
+template <bool kStaged>
 __global__ __launch_bounds__(64 * kVaryWaves) void k_vary(DSearch* all) {
   const DSearch& d = all[blockIdx.y];
+  if (!search_live(d) || (d.stage_tables != 0) != kStaged) return;  // uniform per block
+  extern __shared__ __attribute__((aligned(16))) unsigned char vary_lds[];
+  const PhaseView g = kStaged ? stage_phase_tables<64 * kVaryWaves>(d.g, vary_lds) : view_of(d.g);
   const int c = blockIdx.x * kVaryWaves + wave_index();
-  if (!search_live(d) || c >= d.B) return;
+  if (c >= d.B) return;
   const std::uint64_t seed = *d.round_seed;
   const Keys keys{d.fitness, d.wrong, d.nodes, d.has_wrong};
   const std::size_t o = std::size_t(c) * d.R;
   Row x{d.x_origin + o, d.x_site + o, d.x_cval + o, 0, 0};
   OpStats st;
-  crossover_w(d.g, d.slot, c, [&](int m) { return member_row(d, m); }, x, keys, d.P, d.pressure, seed, d.pol, st);
+  crossover_w(g, d.slot, c, [&](int m) { return member_row(d, m); }, x, keys, d.P, d.pressure, seed, d.pol, st);
   wave_fence();
   Row tmp{d.t_origin + o, d.t_site + o, d.t_cval + o, 0, 0};
   Row spl{d.s_origin + o, d.s_site + o, d.s_cval + o, 0, 0};
-  mutate_w(d.g, d.slot, c, x, tmp, spl, d.p_mut, seed, d.pol, st);
+  mutate_w(g, d.slot, c, x, tmp, spl, d.p_mut, seed, d.pol, st);
   wave_fence();
   const CRow child = as_const(x);
   if (!st.error) compare_w(member_row(d, st.parent), child, d.P, st);
@@ -196,7 +204,7 @@ __global__ __launch_bounds__(64 * kVaryWaves) void k_vary(DSearch* all) {
     if (st.error) raise(d, 300 + st.error);
   }
   if (st.error) return;
-  lower_into_w(d, child, c);
+  lower_into_w(g, d, child, c);
 }
 
 // ---- member hash table (parent_fitness_cache.cuh:11-58) ----
@@ -984,7 +992,7 @@ struct SearchState {
     x.n_locals = int(train.input_names.size());
     x.site_depth = cfg.site_depth;
     x.seed = cfg.seed;
-    x.g = PhaseView{a.upload(phase.exprs), int(phase.exprs.size()), a.upload(phase.prods), int(phase.prods.size()),
+    x.g = PhaseTables{a.upload(phase.exprs), int(phase.exprs.size()), a.upload(phase.prods), int(phase.prods.size()),
                     a.upload(phase.nts), int(phase.nts.size()), a.upload(phase.domains), int(phase.domains.size()),
                     a.upload(phase.values), int(phase.values.size()), a.upload(phase.minimum), phase.depth_count,
                     phase.logical_count, phase.site_depth ? 1 : 0, phase.root};
@@ -1084,6 +1092,8 @@ struct Batch::Impl {
     if (stream) (void)hipStreamDestroy(stream);
   }
   std::size_t eval_fast_lds = 0, eval_general_lds = 0;  // dynamic LDS of the two evaluation kernels
+  std::size_t vary_lds_bytes = 0;  // phase-table copy of the staged reproduction kernel
+  bool vary_global = false;        // some search reproduces from global tables
 
   void enqueue_round() {
     const int n = int(searches.size());
@@ -1091,7 +1101,10 @@ struct Batch::Impl {
     const dim3 eval_grid(std::min(4096, (max_B + kWavesPerBlock - 1) / kWavesPerBlock), n);
     timer.time("k_record", stream, [&] { hipLaunchKernelGGL(k_record, dim3(n), dim3(256), 0, stream, d_all); });
     const dim3 vary_grid((max_B + kVaryWaves - 1) / kVaryWaves, n);
-    timer.time("k_vary", stream, [&] { hipLaunchKernelGGL(k_vary, vary_grid, dim3(64 * kVaryWaves), 0, stream, d_all); });
+    if (vary_lds_bytes)
+      timer.time("k_vary", stream, [&] { hipLaunchKernelGGL(k_vary<true>, vary_grid, dim3(64 * kVaryWaves), vary_lds_bytes, stream, d_all); });
+    if (vary_global)
+      timer.time("k_vary_global", stream, [&] { hipLaunchKernelGGL(k_vary<false>, vary_grid, dim3(64 * kVaryWaves), 0, stream, d_all); });
     timer.time("k_clear", stream, [&] { hipLaunchKernelGGL(k_clear, table_grid, dim3(256), 0, stream, d_all); });
     timer.time("k_index", stream, [&] { hipLaunchKernelGGL(k_index, member_grid, dim3(128), 0, stream, d_all); });
     timer.time("k_reuse", stream, [&] { hipLaunchKernelGGL(k_reuse, child_grid, dim3(128), 0, stream, d_all); });
@@ -1136,6 +1149,11 @@ void Batch::prepare() {
   for (auto& s : I.searches) {
     s->allocate();
     s->d.profile = I.timer.on ? 1 : 0;
+    constexpr std::size_t kStageLimit = 64 * 1024;
+    const std::size_t tables = phase_tables_bytes(s->d.g);
+    s->d.stage_tables = tables <= kStageLimit;
+    if (s->d.stage_tables) I.vary_lds_bytes = std::max(I.vary_lds_bytes, tables);
+    else I.vary_global = true;
     // Evaluation LDS per block: general kernel = stacks (+ score buffers when
     // they fit), fast kernel = score buffers.
     const std::size_t stacks = std::size_t(kWavesPerBlock) * s->d.stack_slots * 64 * 10;
@@ -1156,6 +1174,9 @@ void Batch::prepare() {
   if (!n) return;
   HIP_OK(hipMalloc(&I.d_all, sizeof(DSearch) * host.size()));
   HIP_OK(hipMemcpy(I.d_all, host.data(), sizeof(DSearch) * host.size(), hipMemcpyHostToDevice));
+  if (I.vary_lds_bytes > 65536)
+    HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_vary<true>), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               int(I.vary_lds_bytes)));
   if (I.eval_general_lds > 65536) {
     HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_eval<false>), hipFuncAttributeMaxDynamicSharedMemorySize,
                                int(I.eval_general_lds)));
@@ -1250,9 +1271,9 @@ json::Value judge(const SearchState& s, const Ast& ast, const std::vector<int>& 
   std::vector<Ins> code(std::size_t(s.d.code_cap));
   std::vector<Val> pool(std::size_t(s.d.pool_cap));
   std::vector<int> sites(origin.size(), -1);
-  PhaseView gv{gbl(s.phase.exprs.data()), int(s.phase.exprs.size()), gbl(s.phase.prods.data()), int(s.phase.prods.size()),
-               gbl(s.phase.nts.data()), int(s.phase.nts.size()), gbl(s.phase.domains.data()), int(s.phase.domains.size()),
-               gbl(s.phase.values.data()), int(s.phase.values.size()), gbl(s.phase.minimum.data()), s.phase.depth_count,
+  PhaseView gv{s.phase.exprs.data(), int(s.phase.exprs.size()), s.phase.prods.data(), int(s.phase.prods.size()),
+               s.phase.nts.data(), int(s.phase.nts.size()), s.phase.domains.data(), int(s.phase.domains.size()),
+               s.phase.values.data(), int(s.phase.values.size()), s.phase.minimum.data(), s.phase.depth_count,
                s.phase.logical_count, s.phase.site_depth ? 1 : 0, s.phase.root};
   const CRow row{origin.data(), sites.data(), cval.data(), int(origin.size()), int(cval.size())};
   std::vector<std::uint8_t> local_types = s.train.input_types;
@@ -1510,7 +1531,8 @@ FastCert certify_fast(const std::vector<Ins>& code, const std::vector<Val>& cons
         cert.result_tag = tags.back();
         return cert;
       case OP_CALL_BUILTIN: {
-        if (!args || !fast_operation(OP_CALL_BUILTIN, ins.a, ins.argc, args)) return cert;
+        if (!args || !fast_operation(OP_CALL_BUILTIN, ins.a, ins.argc, args[0], ins.argc > 1 ? args[1] : 0, ins.argc > 2 ? args[2] : 0))
+          return cert;
         tags.resize(std::size_t(h - ins.argc));
         tags.push_back(TAG_INT);
         break;
@@ -1519,7 +1541,7 @@ FastCert certify_fast(const std::vector<Ins>& code, const std::vector<Val>& cons
         const int arity = ins.op == OP_NEG || ins.op == OP_NOT || ins.op == OP_CHECK_INT ? 1 : 2;
         if (h < arity) return cert;
         const std::uint8_t* a = tags.data() + (h - arity);
-        if (!fast_operation(ins.op, 0, arity, a)) return cert;
+        if (!fast_operation(ins.op, 0, arity, a[0], arity > 1 ? a[1] : 0, 0)) return cert;
         const bool compare = ins.op >= OP_LT && ins.op <= OP_NE;
         const std::uint8_t rt = compare ? TAG_BOOL : ins.op == OP_NOT ? TAG_BOOL : TAG_INT;
         tags.resize(std::size_t(h - arity));

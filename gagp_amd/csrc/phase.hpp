@@ -27,7 +27,20 @@ constexpr int kPending = 192;
 
 enum Status : int { ST_OK = 0, ST_INVALID = 1, ST_UNSUPPORTED = 2, ST_CAPACITY = 3 };
 
+// The phase tables as the operators read them. On the device the pointers
+// are built inside the kernel (from PhaseTables, or from an LDS copy), so the
+// compiler sees which memory they address.
 struct PhaseView {
+  const PExpr* expr; int n_expr;
+  const PProd* prod; int n_prod;
+  const PNt* nt; int n_nt;
+  const PDomain* dom; int n_dom;
+  const Val* values; int n_values;
+  const unsigned* minimum; int depth_count, logical_count; int site_depth;
+  int root;
+};
+// Device storage of the phase tables (global memory), kept in the search state.
+struct PhaseTables {
   GAS const PExpr* expr; int n_expr;
   GAS const PProd* prod; int n_prod;
   GAS const PNt* nt; int n_nt;
@@ -36,6 +49,18 @@ struct PhaseView {
   GAS const unsigned* minimum; int depth_count, logical_count; int site_depth;
   int root;
 };
+GHD inline PhaseView view_of(const PhaseTables& t) {
+  return PhaseView{t.expr,   t.n_expr,  t.prod,    t.n_prod,      t.nt,          t.n_nt,         t.dom, t.n_dom,
+                   t.values, t.n_values, t.minimum, t.depth_count, t.logical_count, t.site_depth, t.root};
+}
+// LDS bytes of a table copy: each array padded to 16 bytes.
+GHD inline std::size_t table_bytes(std::size_t bytes) { return (bytes + 15) & ~std::size_t(15); }
+GHD inline std::size_t phase_tables_bytes(const PhaseTables& t) {
+  return table_bytes(sizeof(PExpr) * std::size_t(t.n_expr)) + table_bytes(sizeof(PProd) * std::size_t(t.n_prod)) +
+         table_bytes(sizeof(PNt) * std::size_t(t.n_nt)) + table_bytes(sizeof(PDomain) * std::size_t(t.n_dom)) +
+         table_bytes(sizeof(Val) * std::size_t(t.n_values)) +
+         table_bytes(sizeof(unsigned) * std::size_t(t.n_expr) * std::size_t(t.depth_count) * std::size_t(t.logical_count));
+}
 
 // Site packing: -1 for a non-site node, else entry expression | parent scope << 17.
 GHD inline int site_pack(int entry, int parent_scope) { return entry | (parent_scope << 17); }
@@ -530,16 +555,17 @@ constexpr int kFastStack = 16, kFastCode = 128, kFastPool = 128, kFastInputs = 4
 // builtins only, within the kFast* limits).
 struct LowerResult { int status, code, constants, stack_bound, result_type, jumps, fast; };
 GHD inline bool fast_type(int tag) { return tag == TAG_INT || tag == TAG_BOOL; }
-GHD inline bool fast_operation(int opcode, int builtin, int arity, const std::uint8_t* args) {
+// a0..a2: operand tags (unused ones ignored); passed by value so the callers'
+// frames stay in registers.
+GHD inline bool fast_operation(int opcode, int builtin, int arity, int a0, int a1, int a2) {
   const auto all_int = [&] {
-    for (int k = 0; k < arity && k < 3; ++k) if (args[k] != TAG_INT) return false;
-    return arity <= 3;
+    return arity >= 1 && arity <= 3 && a0 == TAG_INT && (arity < 2 || a1 == TAG_INT) && (arity < 3 || a2 == TAG_INT);
   };
   switch (opcode) {
-    case OP_NEG: case OP_CHECK_INT: return args[0] == TAG_INT;
-    case OP_NOT: return args[0] == TAG_BOOL;
+    case OP_NEG: case OP_CHECK_INT: return a0 == TAG_INT;
+    case OP_NOT: return a0 == TAG_BOOL;
     case OP_ADD: case OP_SUB: case OP_MUL: case OP_MOD: case OP_LT: case OP_LE: case OP_GT: case OP_GE: return all_int();
-    case OP_EQ: case OP_NE: return args[0] == args[1] && fast_type(args[0]);
+    case OP_EQ: case OP_NE: return a0 == a1 && fast_type(a0);
     case OP_CALL_BUILTIN:
       switch (builtin) {
         case B_ABS: return arity == 1 && all_int();
@@ -590,7 +616,11 @@ GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, Origi
       const PNode& node = x.node;
       if (node.kind == TK_CONSTANT) {
         if (cconst >= t.nc) return out;
-        const Val v = t.cval[cconst++];
+        const Val& src = t.cval[cconst++];
+        Val v{};  // fields only: the padding never leaves the row
+        v.i = src.i;
+        v.b = src.b;
+        v.tag = src.tag;
         if (v.tag == TAG_INVALID || v.tag == TAG_FALLBACK) { out.status = ST_UNSUPPORTED; return out; }
         if (out.constants >= pool_capacity) { out.status = ST_CAPACITY; return out; }
         pool[out.constants] = v;
@@ -631,7 +661,10 @@ GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, Origi
     }
     Frame f = frames.get(depth - 1);
     const PNode& node = g.expr[f.origin].node;
-    if (f.seen < 3) f.args[f.seen] = std::uint8_t(completed);
+    // Constant indices keep the frame in registers.
+    if (f.seen == 0) f.args[0] = std::uint8_t(completed);
+    else if (f.seen == 1) f.args[1] = std::uint8_t(completed);
+    else if (f.seen == 2) f.args[2] = std::uint8_t(completed);
     ++f.seen;
     const bool branch = node.kind == TK_IF || node.kind == TK_AND || node.kind == TK_OR;
     if (branch && f.seen == 1) {
@@ -681,7 +714,7 @@ GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, Origi
       if (rt == TAG_INVALID) rt = node.type;
       else if (node.type != TAG_INVALID && node.type != rt) return out;
       if (rt == TAG_INVALID || rt == TAG_FALLBACK) { out.status = ST_UNSUPPORTED; return out; }
-      fast = fast && fast_operation(node.opcode, node.operand, node.arity, args) && fast_type(rt);
+      fast = fast && fast_operation(node.opcode, node.operand, node.arity, args[0], args[1], args[2]) && fast_type(rt);
       // operands are on the stack: the instruction sees the stack with them
       if (node.opcode == OP_CALL_BUILTIN) { if (!emit(OP_CALL_BUILTIN, node.operand, node.arity, node.fuel)) return out; }
       else if (!emit(node.opcode, 0, 0, node.fuel)) return out;

@@ -51,24 +51,59 @@ __device__ inline std::uint64_t wave_sum_u64(std::uint64_t x) {
   return x;
 }
 
+// Copies the phase tables into this block's LDS (all kThreads threads take
+// part; `lds` holds phase_tables_bytes(t) bytes, 16-byte aligned) and returns
+// a view of the copy. The operators then read expression, production,
+// domain and minimum-cost entries at LDS latency instead of one dependent
+// global round trip per lookup.
+template <int kThreads>
+__device__ inline PhaseView stage_phase_tables(const PhaseTables& t, unsigned char* lds) {
+  std::size_t off = 0;
+  const auto copy = [&](GAS const void* src, std::size_t bytes) {
+    unsigned* dst = reinterpret_cast<unsigned*>(lds + off);
+    GAS const unsigned* from = (GAS const unsigned*)src;
+    for (std::size_t k = threadIdx.x; k < bytes / 4; k += kThreads) dst[k] = from[k];
+    off += table_bytes(bytes);
+    return static_cast<const void*>(dst);
+  };
+  PhaseView v;
+  v.expr = static_cast<const PExpr*>(copy(t.expr, sizeof(PExpr) * std::size_t(t.n_expr)));
+  v.prod = static_cast<const PProd*>(copy(t.prod, sizeof(PProd) * std::size_t(t.n_prod)));
+  v.nt = static_cast<const PNt*>(copy(t.nt, sizeof(PNt) * std::size_t(t.n_nt)));
+  v.dom = static_cast<const PDomain*>(copy(t.dom, sizeof(PDomain) * std::size_t(t.n_dom)));
+  v.values = static_cast<const Val*>(copy(t.values, sizeof(Val) * std::size_t(t.n_values)));
+  v.minimum = static_cast<const unsigned*>(
+      copy(t.minimum, sizeof(unsigned) * std::size_t(t.n_expr) * std::size_t(t.depth_count) * std::size_t(t.logical_count)));
+  v.n_expr = t.n_expr;
+  v.n_prod = t.n_prod;
+  v.n_nt = t.n_nt;
+  v.n_dom = t.n_dom;
+  v.n_values = t.n_values;
+  v.depth_count = t.depth_count;
+  v.logical_count = t.logical_count;
+  v.site_depth = t.site_depth;
+  v.root = t.root;
+  __syncthreads();
+  return v;
+}
+
 // ---- lane-register stacks for the shared derivation and lowering bodies ----
-// Pending derivation tasks: entry k in lane k % 64 of group k / 64.
+// Pending derivation tasks: entry k in lane k % 64 of group k / 64. Named
+// registers rather than arrays, so no access can fall back to memory.
 struct WaveTasks {
   static_assert(kPending <= 192, "three groups of 64 lanes");
-  int e[3], d[3], sc[3], l[3];
+  int e0 = 0, e1 = 0, e2 = 0, d0 = 0, d1 = 0, d2 = 0, s0 = 0, s1 = 0, s2 = 0, l0 = 0, l1 = 0, l2 = 0;
   __device__ GenTask get(int k) const {
-    const int grp = k >> 6, ln = k & 63;
-    const int ge = grp == 0 ? e[0] : grp == 1 ? e[1] : e[2];
-    const int gd = grp == 0 ? d[0] : grp == 1 ? d[1] : d[2];
-    const int gs = grp == 0 ? sc[0] : grp == 1 ? sc[1] : sc[2];
-    const int gl = grp == 0 ? l[0] : grp == 1 ? l[1] : l[2];
-    return GenTask{rdlane(ge, ln), rdlane(gd, ln), rdlane(gs, ln), rdlane(gl, ln)};
+    const int ln = k & 63;
+    if (k < 64) return GenTask{rdlane(e0, ln), rdlane(d0, ln), rdlane(s0, ln), rdlane(l0, ln)};
+    if (k < 128) return GenTask{rdlane(e1, ln), rdlane(d1, ln), rdlane(s1, ln), rdlane(l1, ln)};
+    return GenTask{rdlane(e2, ln), rdlane(d2, ln), rdlane(s2, ln), rdlane(l2, ln)};
   }
   __device__ void set(int k, const GenTask& x) {
-    const int grp = k >> 6, ln = k & 63;
-    if (grp == 0) { e[0] = wrlane(x.e, ln, e[0]); d[0] = wrlane(x.d, ln, d[0]); sc[0] = wrlane(x.sc, ln, sc[0]); l[0] = wrlane(x.l, ln, l[0]); }
-    else if (grp == 1) { e[1] = wrlane(x.e, ln, e[1]); d[1] = wrlane(x.d, ln, d[1]); sc[1] = wrlane(x.sc, ln, sc[1]); l[1] = wrlane(x.l, ln, l[1]); }
-    else { e[2] = wrlane(x.e, ln, e[2]); d[2] = wrlane(x.d, ln, d[2]); sc[2] = wrlane(x.sc, ln, sc[2]); l[2] = wrlane(x.l, ln, l[2]); }
+    const int ln = k & 63;
+    if (k < 64) { e0 = wrlane(x.e, ln, e0); d0 = wrlane(x.d, ln, d0); s0 = wrlane(x.sc, ln, s0); l0 = wrlane(x.l, ln, l0); }
+    else if (k < 128) { e1 = wrlane(x.e, ln, e1); d1 = wrlane(x.d, ln, d1); s1 = wrlane(x.sc, ln, s1); l1 = wrlane(x.l, ln, l1); }
+    else { e2 = wrlane(x.e, ln, e2); d2 = wrlane(x.d, ln, d2); s2 = wrlane(x.sc, ln, s2); l2 = wrlane(x.l, ln, l2); }
   }
 };
 // Open operations of the lowering walk: frame k in lane k.
