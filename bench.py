@@ -129,6 +129,54 @@ def cpu_baseline(generations):
                       "history identical to the device run: %s" % (POPULATION, generations, cases, search_s, wall, same)}
 
 
+def percentile(values, q):
+    v = sorted(values)
+    return v[min(len(v) - 1, int(round(q / 100.0 * (len(v) - 1))))]
+
+
+def batching(n, generations):
+    """Searches/s and completion latency of a mixed workload run one search at
+    a time and as one batch: n seeded polynomial searches of degree 2-5
+    (coefficients in [-5,5], x in [-50,50]), population log-uniform in
+    256-4096, train cases log-uniform in 16-1024, `generations` generations,
+    all submitted at t=0. Latency = completion time of each search from t=0."""
+    from gagp_amd import workloads
+
+    def strip(r):
+        return {k: r[k] for k in ("search", "best", "test")} | {"history": [
+            {k: v for k, v in h.items() if k in ("best_fitness", "best_nodes", "best_wrong", "mean_fitness")}
+            for h in r["history"]]}
+
+    with tempfile.TemporaryDirectory() as tmp:
+        tasks = workloads.mixed_searches(tmp, n=n, seed=4, p_range=(256, 4096), c_range=(16, 1024))
+        alone, lat_alone, cases = [], [], 0
+        t0 = time.perf_counter()
+        for i, (task, population) in enumerate(tasks):
+            b = gagp_amd.Batch()
+            b.add(task, None, population, generations, 11 + i)
+            b.run()
+            alone.append(strip(b.result(0)))
+            lat_alone.append(time.perf_counter() - t0)
+        alone_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        b = gagp_amd.Batch()
+        for i, (task, population) in enumerate(tasks):
+            b.add(task, None, population, generations, 11 + i)
+        b.run()
+        together = [strip(b.result(i)) for i in range(n)]
+        batched_s = time.perf_counter() - t0
+        cases = sum(b.evaluated_program_cases(i) for i in range(n))
+    same = sum(x == y for x, y in zip(alone, together))
+    return {"searches": n, "generations": generations,
+            "one_at_a_time": {"searches_per_s": round(n / alone_s, 3), "seconds": round(alone_s, 3),
+                              "p50_latency_s": round(percentile(lat_alone, 50), 3),
+                              "p99_latency_s": round(percentile(lat_alone, 99), 3)},
+            "batched": {"searches_per_s": round(n / batched_s, 3), "seconds": round(batched_s, 3),
+                        "p50_latency_s": round(batched_s, 3), "p99_latency_s": round(batched_s, 3),
+                        "program_case_evals_per_s": round(cases / batched_s, 1)},
+            "speedup": round(alone_s / batched_s, 2), "identical_to_alone": "%d/%d" % (same, n)}
+
+
 def calibrate_copy(mib):
     """One streaming kernel of known size for the HBM counters: reads and
     writes `mib` MiB (int32 negation, one read and one write per element)."""
@@ -149,6 +197,7 @@ def main():
     ap.add_argument("--cpu-generations", type=int, default=GENERATIONS)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=0, help="run a streaming kernel of this size first (counter scale)")
+    ap.add_argument("--batch-searches", type=int, default=64, help="searches of the batching comparison (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: each search runs on a single device and this build has no multi-GPU "
@@ -171,6 +220,7 @@ def main():
     psteps = min(args.steps, 16)
     ms, counts = profile(args.warmup, psteps)
     roof, kernels, vm = roofline(ms, counts, psteps)
+    batch = batching(args.batch_searches, 64) if args.batch_searches else None
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_generations)
     line = {
         "metric": "program-case evals/sec", "value": round(value, 1), "unit": "program-case evals/s", "n_gpus": 1,
@@ -180,7 +230,7 @@ def main():
         "config": {"workload": "quintic symbolic regression, population 4096, 128 generations, 256 train cases, "
                                "default resident steady-state search; one step = one generation",
                    "population": POPULATION, "generations": GENERATIONS, "train_cases": TRAIN_CASES},
-        "roofline": roof, "cpu_baseline": cpu, "kernels": kernels, "vm": vm,
+        "roofline": roof, "cpu_baseline": cpu, "kernels": kernels, "vm": vm, "batching": batch,
     }
     if cpu:
         line["vs_cpu_reference"] = round(value / cpu["value"], 2)
