@@ -162,9 +162,16 @@ def batching(n, generations):
         b = gagp_amd.Batch()
         for i, (task, population) in enumerate(tasks):
             b.add(task, None, population, generations, 11 + i)
-        b.run()
+        t1 = time.perf_counter()
+        b.prepare()
+        t2 = time.perf_counter()
+        b.step(b.max_rounds())
+        b.finish()
+        t3 = time.perf_counter()
         together = [strip(b.result(i)) for i in range(n)]
         batched_s = time.perf_counter() - t0
+        phases = {"load_s": round(t1 - t0, 4), "prepare_s": round(t2 - t1, 4), "rounds_s": round(t3 - t2, 4),
+                  "results_s": round(batched_s - (t3 - t0), 4)}
         cases = sum(b.evaluated_program_cases(i) for i in range(n))
     same = sum(x == y for x, y in zip(alone, together))
     return {"searches": n, "generations": generations,
@@ -173,7 +180,7 @@ def batching(n, generations):
                               "p99_latency_s": round(percentile(lat_alone, 99), 3)},
             "batched": {"searches_per_s": round(n / batched_s, 3), "seconds": round(batched_s, 3),
                         "p50_latency_s": round(batched_s, 3), "p99_latency_s": round(batched_s, 3),
-                        "program_case_evals_per_s": round(cases / batched_s, 1)},
+                        "program_case_evals_per_s": round(cases / batched_s, 1), "phases": phases},
             "speedup": round(alone_s / batched_s, 2), "identical_to_alone": "%d/%d" % (same, n)}
 
 

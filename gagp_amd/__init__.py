@@ -82,6 +82,7 @@ def lib():
     L.gagp_batch_prepare.argtypes = [ctypes.c_void_p]
     L.gagp_batch_step.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]
     L.gagp_batch_finish.argtypes = [ctypes.c_void_p]
+    L.gagp_batch_max_rounds.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]
     L.gagp_batch_counters.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_uint64)]
     L.gagp_batch_evaluated_program_cases.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_uint64)]
     L.gagp_batch_set_profile.argtypes = [ctypes.c_void_p, ctypes.c_int32]
@@ -210,6 +211,11 @@ class Batch:
 
     def step(self, rounds, use_graph=True):
         _check(lib().gagp_batch_step(self.h, rounds, int(use_graph)))
+
+    def max_rounds(self):
+        out = ctypes.c_int32()
+        _check(lib().gagp_batch_max_rounds(self.h, ctypes.byref(out)))
+        return out.value
 
     def finish(self):
         _check(lib().gagp_batch_finish(self.h))

@@ -179,6 +179,13 @@ int gagp_batch_step(gagp_batch* b, int32_t rounds, int32_t use_graph) {
   });
 }
 
+int gagp_batch_max_rounds(gagp_batch* b, int32_t* out) {
+  return guarded([&] {
+    if (!b || !b->prepared || !out) throw std::invalid_argument("batch not prepared");
+    *out = b->b.max_rounds();
+  });
+}
+
 int gagp_batch_finish(gagp_batch* b) {
   return guarded([&] {
     if (!b || !b->prepared || b->ran) throw std::invalid_argument("batch not prepared");

@@ -132,6 +132,10 @@ int gagp_batch_prepare(gagp_batch* b);
 int gagp_batch_step(gagp_batch* b, int32_t rounds, int32_t use_graph);
 int gagp_batch_finish(gagp_batch* b);
 
+/* Rounds the longest search of a prepared batch needs: P*G/B
+ * (cpp/src/evolution/evolve.cpp:385). */
+int gagp_batch_max_rounds(gagp_batch* b, int32_t* out);
+
 /* Device counters of search `index`: out[14] = {evaluated programs,
  * evaluated program-cases, runtime errors, timeouts, VM instructions
  * executed (charged fuel), instructions and constant-pool entries of the
