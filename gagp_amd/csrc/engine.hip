@@ -126,10 +126,17 @@ __device__ inline CRow child_row(const DSearch& d, int c) {
 __device__ inline bool search_live(const DSearch& d) { return d.active && *d.round < d.rounds && !*d.error; }
 __device__ inline void raise(const DSearch& d, int code) { atomicCAS(d.error, 0, code); }
 
+// One entry per launched block of a batch kernel: its search, its block
+// index within that search's share and the share's size. Batches launch a
+// flat grid of exactly the blocks their searches need (no padding to the
+// largest search).
+struct BlockMap { int search, block, blocks; };
+
 // ---- initial population (phase_population_cpu.cpp:54-113) ----
-__global__ void k_init_pop(DSearch* all) {
-  const DSearch& d = all[blockIdx.y];
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_init_pop(DSearch* all, const BlockMap* map) {
+  const BlockMap bm = map[blockIdx.x];
+  const DSearch& d = all[bm.search];
+  const int m = bm.block * blockDim.x + threadIdx.x;
   if (!d.active || m >= d.P) return;
   const int sd = d.site_depth;
   const int ramp = sd < 2 ? sd : 2 + m % (sd - 1);
@@ -164,9 +171,10 @@ __device__ inline void lower_into_w(const PhaseView& g, const DSearch& d, const 
   }
 }
 constexpr int kVaryWaves = 4;  // wavefronts (rows) per block of the row kernels
-__global__ __launch_bounds__(64 * kVaryWaves) void k_lower_members(DSearch* all) {
-  const DSearch& d = all[blockIdx.y];
-  const int m = blockIdx.x * kVaryWaves + wave_index();
+__global__ __launch_bounds__(64 * kVaryWaves) void k_lower_members(DSearch* all, const BlockMap* map) {
+  const BlockMap bm = map[blockIdx.x];
+  const DSearch& d = all[bm.search];
+  const int m = bm.block * kVaryWaves + wave_index();
   if (!d.active || m >= d.P || *d.error) return;
   lower_into_w(view_of(d.g), d, member_row(d, m), m);
 }
@@ -175,12 +183,13 @@ __global__ __launch_bounds__(64 * kVaryWaves) void k_lower_members(DSearch* all)
 // kStaged: the block first copies the phase tables into LDS (searches whose
 // tables fit); otherwise the operators read them from global memory.
 template <bool kStaged>
-__global__ __launch_bounds__(64 * kVaryWaves) void k_vary(DSearch* all) {
-  const DSearch& d = all[blockIdx.y];
+__global__ __launch_bounds__(64 * kVaryWaves) void k_vary(DSearch* all, const BlockMap* map) {
+  const BlockMap bm = map[blockIdx.x];
+  const DSearch& d = all[bm.search];
   if (!search_live(d) || (d.stage_tables != 0) != kStaged) return;  // uniform per block
   extern __shared__ __attribute__((aligned(16))) unsigned char vary_lds[];
   const PhaseView g = kStaged ? stage_phase_tables<64 * kVaryWaves>(d.g, vary_lds) : view_of(d.g);
-  const int c = blockIdx.x * kVaryWaves + wave_index();
+  const int c = bm.block * kVaryWaves + wave_index();
   if (c >= d.B) return;
   const std::uint64_t seed = *d.round_seed;
   const Keys keys{d.fitness, d.wrong, d.nodes, d.has_wrong};
@@ -212,15 +221,17 @@ __global__ __launch_bounds__(64 * kVaryWaves) void k_vary(DSearch* all) {
 // any one of them carries the result a child identical to them reuses (the
 // rows are compared in full before a result is reused). Keys are hash | 1,
 // 0 marks an empty slot.
-__global__ void k_clear(DSearch* all) {
-  const DSearch& d = all[blockIdx.y];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_clear(DSearch* all, const BlockMap* map) {
+  const BlockMap bm = map[blockIdx.x];
+  const DSearch& d = all[bm.search];
+  const int i = bm.block * blockDim.x + threadIdx.x;
   if (!search_live(d) || i > d.table_mask) return;
   d.table_key[i] = 0;
 }
-__global__ void k_index(DSearch* all) {
-  const DSearch& d = all[blockIdx.y];
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_index(DSearch* all, const BlockMap* map) {
+  const BlockMap bm = map[blockIdx.x];
+  const DSearch& d = all[bm.search];
+  const int m = bm.block * blockDim.x + threadIdx.x;
   if (!search_live(d) || m >= d.P) return;
   const std::uint64_t h = d.member_hash[m], key = h | 1;
   for (int probe = 0; probe <= d.table_mask; ++probe) {
@@ -231,9 +242,10 @@ __global__ void k_index(DSearch* all) {
   }
   raise(d, 400);
 }
-__global__ void k_reuse(DSearch* all) {
-  const DSearch& d = all[blockIdx.y];
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_reuse(DSearch* all, const BlockMap* map) {
+  const BlockMap bm = map[blockIdx.x];
+  const DSearch& d = all[bm.search];
+  const int c = bm.block * blockDim.x + threadIdx.x;
   if (!search_live(d) || c >= d.B) return;
   int r = d.stats[c].reusable_parent;
   const CRow child = child_row(d, c);
@@ -405,8 +417,9 @@ __device__ inline WaveStack stack_of(unsigned char* smem, int slots, int wave) {
 // children. kFast: the programs certified for vm_fast (pending entries from
 // the back of the list); otherwise the rest (from the front).
 template <bool kFast>
-__global__ __launch_bounds__(kEvalThreads) void k_eval(DSearch* all, int mode) {
-  const DSearch& d = all[blockIdx.y];
+__global__ __launch_bounds__(kEvalThreads) void k_eval(DSearch* all, const BlockMap* map, int mode) {
+  const BlockMap bm = map[blockIdx.x];
+  const DSearch& d = all[bm.search];
   if (mode == 1 ? !search_live(d) : (!d.active || *d.error)) return;
   extern __shared__ unsigned char smem[];
   const int wave = wave_index(), lane = threadIdx.x % 64;
@@ -418,7 +431,7 @@ __global__ __launch_bounds__(kEvalThreads) void k_eval(DSearch* all, int mode) {
                        : nullptr;
   const EvalParams e{d.cd, d.fuel, d.lanes, d.penalty, d.policy};
   const int count = mode == 1 ? d.n_pending[kFast ? 1 : 0] : d.P;
-  for (int i = blockIdx.x * kWavesPerBlock + wave; i < count; i += gridDim.x * kWavesPerBlock) {
+  for (int i = bm.block * kWavesPerBlock + wave; i < count; i += bm.blocks * kWavesPerBlock) {
     const int slot = mode == 1 ? (kFast ? d.pending[d.B - 1 - i] : d.pending[i]) : i;
     const int pmode = d.code_jumps[slot];
     if (mode == 0 && ((pmode & 2) != 0 && d.cd.C <= kScoreCases) != kFast) continue;  // the other kernel scores it
@@ -787,9 +800,10 @@ __global__ __launch_bounds__(64) void k_place(DSearch* all) {
   }
 }
 
-__global__ void k_commit(DSearch* all) {
-  const DSearch& d = all[blockIdx.y];
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_commit(DSearch* all, const BlockMap* map) {
+  const BlockMap bm = map[blockIdx.x];
+  const DSearch& d = all[bm.search];
+  const int m = bm.block * blockDim.x + threadIdx.x;
   if (!d.active || m >= d.P) return;
   const int c = d.placed[m];
   if (c < 0) return;
@@ -851,15 +865,29 @@ __global__ __launch_bounds__(kEvalThreads) void k_eval_cases(DProg p, CaseData c
 }
 
 // ---- device buffers ----
+// Bump allocator over device chunks that double in size (1 MiB up to
+// 256 MiB): a search's arrays take a handful of hipMalloc calls in all.
+// Arrays are 256-byte aligned.
 struct DeviceArena {
   std::vector<void*> blocks;
   std::size_t bytes = 0;
+  char* cursor = nullptr;
+  std::size_t left = 0, next_chunk = std::size_t(1) << 20;
   template <class T>
   GAS T* alloc(std::size_t n) {
-    void* p = nullptr;
-    const std::size_t sz = std::max<std::size_t>(n, 1) * sizeof(T);
-    HIP_OK(hipMalloc(&p, sz));
-    blocks.push_back(p);
+    const std::size_t sz = (std::max<std::size_t>(n, 1) * sizeof(T) + 255) & ~std::size_t(255);
+    if (sz > left) {
+      const std::size_t chunk = std::max(sz, next_chunk);
+      void* p = nullptr;
+      HIP_OK(hipMalloc(&p, chunk));
+      blocks.push_back(p);
+      cursor = static_cast<char*>(p);
+      left = chunk;
+      next_chunk = std::min(next_chunk * 2, std::size_t(256) << 20);
+    }
+    void* p = cursor;
+    cursor += sz;
+    left -= sz;
     bytes += sz;
     return gbl(static_cast<T*>(p));
   }
@@ -1095,28 +1123,54 @@ struct Batch::Impl {
   std::size_t vary_lds_bytes = 0;  // phase-table copy of the staged reproduction kernel
   bool vary_global = false;        // some search reproduces from global tables
 
+  // Flat grids: every search contributes exactly its own blocks.
+  struct Grid { GAS BlockMap* map = nullptr; int blocks = 0; };
+  DeviceArena map_arena;
+  Grid g_init, g_lower, g_eval_init, g_vary_staged, g_vary_global, g_table, g_member, g_child, g_eval;
+  template <class Blocks>
+  Grid build_grid(Blocks blocks_of) {
+    std::vector<BlockMap> v;
+    for (std::size_t i = 0; i < searches.size(); ++i) {
+      const int nb = blocks_of(*searches[i]);
+      for (int j = 0; j < nb; ++j) v.push_back(BlockMap{int(i), j, nb});
+    }
+    Grid g;
+    g.blocks = int(v.size());
+    if (!v.empty()) g.map = map_arena.upload(v);
+    return g;
+  }
+  void build_grids() {
+    const auto up = [](int x, int k) { return (x + k - 1) / k; };
+    g_init = build_grid([&](const SearchState& s) { return up(s.P, 64); });
+    g_lower = build_grid([&](const SearchState& s) { return up(s.P, kVaryWaves); });
+    g_eval_init = build_grid([&](const SearchState& s) { return std::min(4096, up(s.P, kWavesPerBlock)); });
+    g_vary_staged = build_grid([&](const SearchState& s) { return s.d.stage_tables ? up(s.B, kVaryWaves) : 0; });
+    g_vary_global = build_grid([&](const SearchState& s) { return s.d.stage_tables ? 0 : up(s.B, kVaryWaves); });
+    g_table = build_grid([&](const SearchState& s) { return up(s.d.table_mask + 1, 256); });
+    g_member = build_grid([&](const SearchState& s) { return up(s.P, 128); });
+    g_child = build_grid([&](const SearchState& s) { return up(s.B, 128); });
+    g_eval = build_grid([&](const SearchState& s) { return std::min(4096, up(s.B, kWavesPerBlock)); });
+  }
+
   void enqueue_round() {
     const int n = int(searches.size());
-    const dim3 child_grid((max_B + 127) / 128, n), member_grid((max_P + 127) / 128, n), table_grid((max_table + 255) / 256, n);
-    const dim3 eval_grid(std::min(4096, (max_B + kWavesPerBlock - 1) / kWavesPerBlock), n);
+    const auto launch = [&](const char* name, const Grid& g, auto kernel, int threads, std::size_t lds, auto... args) {
+      if (g.blocks) timer.time(name, stream, [&] { hipLaunchKernelGGL(kernel, dim3(g.blocks), dim3(threads), lds, stream, d_all, g.map, args...); });
+    };
     timer.time("k_record", stream, [&] { hipLaunchKernelGGL(k_record, dim3(n), dim3(256), 0, stream, d_all); });
-    const dim3 vary_grid((max_B + kVaryWaves - 1) / kVaryWaves, n);
-    if (vary_lds_bytes)
-      timer.time("k_vary", stream, [&] { hipLaunchKernelGGL(k_vary<true>, vary_grid, dim3(64 * kVaryWaves), vary_lds_bytes, stream, d_all); });
-    if (vary_global)
-      timer.time("k_vary_global", stream, [&] { hipLaunchKernelGGL(k_vary<false>, vary_grid, dim3(64 * kVaryWaves), 0, stream, d_all); });
-    timer.time("k_clear", stream, [&] { hipLaunchKernelGGL(k_clear, table_grid, dim3(256), 0, stream, d_all); });
-    timer.time("k_index", stream, [&] { hipLaunchKernelGGL(k_index, member_grid, dim3(128), 0, stream, d_all); });
-    timer.time("k_reuse", stream, [&] { hipLaunchKernelGGL(k_reuse, child_grid, dim3(128), 0, stream, d_all); });
-    timer.time("k_eval", stream, [&] { hipLaunchKernelGGL(k_eval<true>, eval_grid, dim3(kEvalThreads), eval_fast_lds, stream, d_all, 1); });
-    timer.time("k_eval_general", stream,
-               [&] { hipLaunchKernelGGL(k_eval<false>, eval_grid, dim3(kEvalThreads), eval_general_lds, stream, d_all, 1); });
+    launch("k_vary", g_vary_staged, k_vary<true>, 64 * kVaryWaves, vary_lds_bytes);
+    launch("k_vary_global", g_vary_global, k_vary<false>, 64 * kVaryWaves, 0);
+    launch("k_clear", g_table, k_clear, 256, 0);
+    launch("k_index", g_member, k_index, 128, 0);
+    launch("k_reuse", g_child, k_reuse, 128, 0);
+    launch("k_eval", g_eval, k_eval<true>, kEvalThreads, eval_fast_lds, 1);
+    launch("k_eval_general", g_eval, k_eval<false>, kEvalThreads, eval_general_lds, 1);
     timer.time("k_tally", stream, [&] { hipLaunchKernelGGL(k_tally, dim3(kTallyBlocks, n), dim3(256), 0, stream, d_all, 1); });
     if (place_lds_bytes)
       timer.time("k_place", stream, [&] { hipLaunchKernelGGL(k_place<true>, dim3(n), dim3(64), place_lds_bytes, stream, d_all); });
     if (place_hbm_bytes)
       timer.time("k_place_hbm", stream, [&] { hipLaunchKernelGGL(k_place<false>, dim3(n), dim3(64), place_hbm_bytes, stream, d_all); });
-    timer.time("k_commit", stream, [&] { hipLaunchKernelGGL(k_commit, member_grid, dim3(128), 0, stream, d_all); });
+    launch("k_commit", g_member, k_commit, 128, 0);
   }
 };
 
@@ -1192,14 +1246,20 @@ void Batch::prepare() {
     HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_place<false>), hipFuncAttributeMaxDynamicSharedMemorySize,
                                int(I.place_hbm_bytes)));
   // Initial population, its evaluation, keys and the run generator.
-  const dim3 member_grid((I.max_P + 63) / 64, n);
-  I.timer.time("k_init_pop", I.stream, [&] { hipLaunchKernelGGL(k_init_pop, member_grid, dim3(64), 0, I.stream, I.d_all); });
-  const dim3 lower_grid((I.max_P + kVaryWaves - 1) / kVaryWaves, n);
-  I.timer.time("k_lower_members", I.stream, [&] { hipLaunchKernelGGL(k_lower_members, lower_grid, dim3(64 * kVaryWaves), 0, I.stream, I.d_all); });
-  const dim3 eval_grid(std::min(4096, (I.max_P + kWavesPerBlock - 1) / kWavesPerBlock), n);
-  I.timer.time("k_eval_init", I.stream, [&] { hipLaunchKernelGGL(k_eval<true>, eval_grid, dim3(kEvalThreads), I.eval_fast_lds, I.stream, I.d_all, 0); });
-  I.timer.time("k_eval_init_general", I.stream,
-               [&] { hipLaunchKernelGGL(k_eval<false>, eval_grid, dim3(kEvalThreads), I.eval_general_lds, I.stream, I.d_all, 0); });
+  I.build_grids();
+  I.timer.time("k_init_pop", I.stream,
+               [&] { hipLaunchKernelGGL(k_init_pop, dim3(I.g_init.blocks), dim3(64), 0, I.stream, I.d_all, I.g_init.map); });
+  I.timer.time("k_lower_members", I.stream, [&] {
+    hipLaunchKernelGGL(k_lower_members, dim3(I.g_lower.blocks), dim3(64 * kVaryWaves), 0, I.stream, I.d_all, I.g_lower.map);
+  });
+  I.timer.time("k_eval_init", I.stream, [&] {
+    hipLaunchKernelGGL(k_eval<true>, dim3(I.g_eval_init.blocks), dim3(kEvalThreads), I.eval_fast_lds, I.stream, I.d_all,
+                       I.g_eval_init.map, 0);
+  });
+  I.timer.time("k_eval_init_general", I.stream, [&] {
+    hipLaunchKernelGGL(k_eval<false>, dim3(I.g_eval_init.blocks), dim3(kEvalThreads), I.eval_general_lds, I.stream, I.d_all,
+                       I.g_eval_init.map, 0);
+  });
   I.timer.time("k_tally_init", I.stream, [&] { hipLaunchKernelGGL(k_tally, dim3(kTallyBlocks, n), dim3(256), 0, I.stream, I.d_all, 0); });
   I.timer.time("k_init_keys", I.stream, [&] { hipLaunchKernelGGL(k_init_keys, dim3(n), dim3(256), 0, I.stream, I.d_all); });
   I.timer.time("k_seed", I.stream, [&] { hipLaunchKernelGGL(k_seed, dim3(n), dim3(256), 0, I.stream, I.d_all); });
