@@ -112,6 +112,8 @@ struct DSearch {
   GAS unsigned *e_errors, *e_timeouts;  // per evaluated slot
   GAS unsigned long long* e_fuel;
   int profile;  // tally reproduction row sizes too
+  int generational, elites;  // generational schedule (B = P, one round per generation) and its elite count
+  GAS int* elite;            // elite members of the current generation, best first
   int general_scores;  // the general evaluation kernel has an LDS score buffer for this search
 };
 
@@ -196,14 +198,24 @@ __global__ __launch_bounds__(64 * kVaryWaves) void k_vary(DSearch* all, const Bl
   const std::size_t o = std::size_t(c) * d.R;
   Row x{d.x_origin + o, d.x_site + o, d.x_cval + o, 0, 0};
   OpStats st;
-  crossover_w(g, d.slot, c, [&](int m) { return member_row(d, m); }, x, keys, d.P, d.pressure, seed, d.pol, st);
-  wave_fence();
-  Row tmp{d.t_origin + o, d.t_site + o, d.t_cval + o, 0, 0};
-  Row spl{d.s_origin + o, d.s_site + o, d.s_cval + o, 0, 0};
-  mutate_w(g, d.slot, c, x, tmp, spl, d.p_mut, seed, d.pol, st);
-  wave_fence();
+  if (d.generational && c < d.elites) {
+    // Elite rows replace the first children after variation; a child's
+    // draws depend only on (seed, child), so the elite children skip theirs.
+    // An elite child is its source member and reuses its result.
+    const int src = d.elite[c];
+    copy_row_w(member_row(d, src), x);
+    st = OpStats{src, 0, 0, 0, 0, 0, 0, src, -1};
+    wave_fence();
+  } else {
+    crossover_w(g, d.slot, c, [&](int m) { return member_row(d, m); }, x, keys, d.P, d.pressure, seed, d.pol, st);
+    wave_fence();
+    Row tmp{d.t_origin + o, d.t_site + o, d.t_cval + o, 0, 0};
+    Row spl{d.s_origin + o, d.s_site + o, d.s_cval + o, 0, 0};
+    mutate_w(g, d.slot, c, x, tmp, spl, d.p_mut, seed, d.pol, st);
+    wave_fence();
+    if (!st.error) compare_w(member_row(d, st.parent), as_const(x), d.P, st);
+  }
   const CRow child = as_const(x);
-  if (!st.error) compare_w(member_row(d, st.parent), child, d.P, st);
   const std::uint64_t h = st.error ? 0 : row_hash_w(child);
   if (wlane() == 0) {
     d.x_n[c] = x.n;
@@ -669,7 +681,7 @@ struct PlaceKeys {
 template <bool kLds>
 __global__ __launch_bounds__(64) void k_place(DSearch* all) {
   const DSearch& d = all[blockIdx.x];
-  if (!search_live(d) || d.place_lds != int(kLds)) return;
+  if (!search_live(d) || d.generational || d.place_lds != int(kLds)) return;
   extern __shared__ __attribute__((aligned(16))) unsigned char place_smem[];
   __shared__ std::uint64_t ring[kRing];
   __shared__ unsigned ridx[kRing];
@@ -798,6 +810,86 @@ __global__ __launch_bounds__(64) void k_place(DSearch* all) {
     d.n_pending[1] = 0;
     atomicAdd(d.round, 1);
   }
+}
+
+// ---- generational schedule (evolve.cpp:253-322; phase_population_cpu.cpp:269-312) ----
+constexpr int kMaxElites = 32;
+// Elites (the top `elites` members by the ranking order, best first; the
+// best alone when elitism is 0) and the generation's seed, one generator
+// word drawn after the history row as in the reference's loop.
+__global__ __launch_bounds__(256) void k_gen_begin(DSearch* all) {
+  const DSearch& d = all[blockIdx.x];
+  if (!d.generational || !search_live(d)) return;
+  const int t = threadIdx.x, P = d.P, want = d.elites > 0 ? d.elites : 1;
+  const Keys keys{d.fitness, d.wrong, d.nodes, d.has_wrong};
+  __shared__ unsigned taken[16384 / 32];
+  __shared__ int cand[256];
+  __shared__ std::uint64_t mt[312];
+  for (int i = t; i < (P + 31) / 32; i += 256) taken[i] = 0;
+  __syncthreads();
+  for (int k = 0; k < want; ++k) {
+    int b = -1;
+    for (int m = t; m < P; m += 256)
+      if (!((taken[m >> 5] >> (m & 31)) & 1u) && (b < 0 || better(keys, m, b))) b = m;
+    cand[t] = b;
+    __syncthreads();
+    for (int span = 128; span > 0; span >>= 1) {
+      if (t < span && cand[t + span] >= 0 && (cand[t] < 0 || better(keys, cand[t + span], cand[t]))) cand[t] = cand[t + span];
+      __syncthreads();
+    }
+    const int pick = cand[0];
+    if (t == 0) {
+      if (k < d.elites) d.elite[k] = pick;
+      if (k == 0) *d.best = pick;
+      taken[pick >> 5] |= 1u << (pick & 31);
+    }
+    __syncthreads();
+  }
+  // One word of the run generator: the oldest unconsumed word, refilled by a
+  // twist when none is left. Word 0 seeds generation 0 (stored by k_seed).
+  if (*d.round == 0) return;
+  int left = *d.n_left;
+  if (left == 0) {
+    for (int i = t; i < 312; i += 256) mt[i] = d.mt[i];
+    __syncthreads();
+    mt_twist<256>(mt);
+    for (int i = t; i < 312; i += 256) { d.mt[i] = mt[i]; d.mt_left[i] = mt_temper(mt[i]); }
+    left = 312;
+    __syncthreads();
+  }
+  const std::uint64_t word = d.mt_left[0];
+  std::uint64_t keep[4];
+  for (int j = 0; j < 4; ++j) {
+    const int i = t + 256 * j + 1;
+    keep[j] = i < left ? d.mt_left[i] : 0;
+  }
+  __syncthreads();
+  for (int j = 0; j < 4; ++j) {
+    const int i = t + 256 * j + 1;
+    if (i < left) d.mt_left[i - 1] = keep[j];
+  }
+  if (t == 0) {
+    *d.round_seed = word;
+    *d.n_left = left - 1;
+  }
+}
+// Every child becomes the member of its index (keys now; rows in k_commit).
+__global__ void k_gen_place(DSearch* all, const BlockMap* map) {
+  const BlockMap bm = map[blockIdx.x];
+  const DSearch& d = all[bm.search];
+  const int m = bm.block * blockDim.x + threadIdx.x;
+  if (!d.generational || !search_live(d) || m >= d.P) return;
+  d.fitness[m] = d.c_fit[m];
+  d.wrong[m] = d.c_wrong[m];
+  d.nodes[m] = d.c_nodes[m];
+  d.placed[m] = m;
+}
+__global__ void k_gen_end(DSearch* all) {
+  const DSearch& d = all[blockIdx.x];
+  if (!d.generational || !search_live(d) || threadIdx.x) return;
+  d.n_pending[0] = 0;
+  d.n_pending[1] = 0;
+  atomicAdd(d.round, 1);
 }
 
 __global__ void k_commit(DSearch* all, const BlockMap* map) {
@@ -970,12 +1062,13 @@ struct SearchState {
     if (s.generations) cfg.generations = *s.generations;
     if (s.seed) cfg.seed = *s.seed;
     if (!cfg.resident) throw Unsupported("native_ast representation (host reproduction)");
-    if (cfg.schedule != Schedule::SteadyState) throw Unsupported("generational schedule");
+    if (cfg.schedule == Schedule::Generational && cfg.elitism > kMaxElites)
+      throw Unsupported("generational elitism above " + std::to_string(kMaxElites));
     if (cfg.site_depth < 1) throw Unsupported("site_depth 0 (initial population from the native generator)");
     if (cfg.population < 2 || cfg.population > 16384) throw Unsupported("population outside 2..16384");
     if (cfg.elitism < 0 || cfg.elitism >= cfg.population) throw std::invalid_argument("elite_count must be >= 0 and < population_size");
     P = cfg.population;
-    B = cfg.batch ? cfg.batch : P;
+    B = cfg.schedule == Schedule::Generational ? P : (cfg.batch ? cfg.batch : P);
     if (B > P || P % B) throw std::invalid_argument("search config: steady_state batch must divide the population");
     G = cfg.generations;
     rounds = int((std::int64_t(P) * G) / B);
@@ -1057,6 +1150,9 @@ struct SearchState {
     x.hist_fit = a.alloc<double>(G); x.hist_wrong = a.alloc<unsigned>(G);
     x.hist_all = a.alloc<double>(std::size_t(G) * P);
     x.counters = a.alloc<Counters>(1);
+    x.generational = cfg.schedule == Schedule::Generational;
+    x.elites = x.generational ? cfg.elitism : 0;
+    x.elite = a.alloc<int>(kMaxElites);
     x.e_errors = a.alloc<unsigned>(code_rows); x.e_timeouts = a.alloc<unsigned>(code_rows);
     x.e_fuel = a.alloc<unsigned long long>(code_rows);
     x.error = a.alloc<int>(1);
@@ -1122,6 +1218,7 @@ struct Batch::Impl {
   std::size_t eval_fast_lds = 0, eval_general_lds = 0;  // dynamic LDS of the two evaluation kernels
   std::size_t vary_lds_bytes = 0;  // phase-table copy of the staged reproduction kernel
   bool vary_global = false;        // some search reproduces from global tables
+  bool any_generational = false;
 
   // Flat grids: every search contributes exactly its own blocks.
   struct Grid { GAS BlockMap* map = nullptr; int blocks = 0; };
@@ -1157,6 +1254,7 @@ struct Batch::Impl {
     const auto launch = [&](const char* name, const Grid& g, auto kernel, int threads, std::size_t lds, auto... args) {
       if (g.blocks) timer.time(name, stream, [&] { hipLaunchKernelGGL(kernel, dim3(g.blocks), dim3(threads), lds, stream, d_all, g.map, args...); });
     };
+    if (any_generational) timer.time("k_gen_begin", stream, [&] { hipLaunchKernelGGL(k_gen_begin, dim3(n), dim3(256), 0, stream, d_all); });
     timer.time("k_record", stream, [&] { hipLaunchKernelGGL(k_record, dim3(n), dim3(256), 0, stream, d_all); });
     launch("k_vary", g_vary_staged, k_vary<true>, 64 * kVaryWaves, vary_lds_bytes);
     launch("k_vary_global", g_vary_global, k_vary<false>, 64 * kVaryWaves, 0);
@@ -1170,7 +1268,9 @@ struct Batch::Impl {
       timer.time("k_place", stream, [&] { hipLaunchKernelGGL(k_place<true>, dim3(n), dim3(64), place_lds_bytes, stream, d_all); });
     if (place_hbm_bytes)
       timer.time("k_place_hbm", stream, [&] { hipLaunchKernelGGL(k_place<false>, dim3(n), dim3(64), place_hbm_bytes, stream, d_all); });
+    if (any_generational) launch("k_gen_place", g_member, k_gen_place, 128, 0);
     launch("k_commit", g_member, k_commit, 128, 0);
+    if (any_generational) timer.time("k_gen_end", stream, [&] { hipLaunchKernelGGL(k_gen_end, dim3(n), dim3(64), 0, stream, d_all); });
   }
 };
 
@@ -1203,6 +1303,7 @@ void Batch::prepare() {
   for (auto& s : I.searches) {
     s->allocate();
     s->d.profile = I.timer.on ? 1 : 0;
+    I.any_generational = I.any_generational || s->d.generational;
     constexpr std::size_t kStageLimit = 64 * 1024;
     const std::size_t tables = phase_tables_bytes(s->d.g);
     s->d.stage_tables = tables <= kStageLimit;

@@ -24,7 +24,7 @@ REFERENCE_FIXTURES = "/root/reference/cpp/tests/fixtures/runtime"
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 TASKS = os.path.join(GOLDEN, "tasks")
 
-# name: (task dir, population, generations, seed, save_all)
+# name: (task dir, population, generations, seed, save_all[, search config under tests/golden/search])
 RUNS = {
     "polynomial_p1024_g64_s0": ("polynomial", 1024, 64, 0, True),
     "polynomial_p512_g32_s7": ("polynomial", 512, 32, 7, False),
@@ -36,12 +36,19 @@ RUNS = {
     # populations above 8192 keep the placement keys in HBM
     "quintic_p16384_g3_s11": ("quintic", 16384, 3, 11, False),
     "polynomial_p12000_g4_s2": ("polynomial", 12000, 4, 2, False),
+    # generational schedule (elites, reservoir sites, pressure 3) and a steady batch below P with 3-member inverse tournaments
+    "polynomial_gen_e1_p256_g12_s3": ("polynomial", 256, 12, 3, False, "generational_e1.json"),
+    "poly_c1_gen_e3r_p512_g24_s9": ("poly_c1", 512, 24, 9, True, "generational_e3_reservoir.json"),
+    "quintic_gen_e1_p1024_g16_s1": ("quintic", 1024, 16, 1, False, "generational_e1.json"),
+    "polynomial_steady_b128_rs3_p512_g16_s4": ("polynomial", 512, 16, 4, False, "steady_batch_rs3.json"),
 }
 
 
-def run_reference(task, population, generations, seed, out, save_all):
+def run_reference(task, population, generations, seed, out, save_all, search=None):
     cmd = [REF, "run", task, "--engine", "cpu", "--population", str(population), "--generations", str(generations),
            "--seed", str(seed), "--out", out, "--quiet"]
+    if search:
+        cmd += ["--search", search]
     if save_all:
         cmd.append("--save-all")
     subprocess.run(cmd, check=True)
@@ -60,13 +67,17 @@ def main():
     for i, (task, population) in enumerate(mixed):
         runs["mixed_%d" % i] = (os.path.relpath(os.path.dirname(task), TASKS), population, 8, 11 + i, False)
     manifest = {}
-    for name, (task_dir, population, generations, seed, save_all) in runs.items():
+    for name, spec in runs.items():
+        task_dir, population, generations, seed, save_all = spec[:5]
+        search = os.path.join(GOLDEN, "search", spec[5]) if len(spec) > 5 else None
         out = os.path.join(GOLDEN, "runs", name)
         os.makedirs(out, exist_ok=True)
         task = os.path.join(TASKS, task_dir, "task.json")
-        run_reference(task, population, generations, seed, out, save_all)
+        run_reference(task, population, generations, seed, out, save_all, search)
         manifest[name] = {"task": os.path.relpath(task, GOLDEN), "population": population, "generations": generations,
                           "seed": seed, "save_all": save_all}
+        if search:
+            manifest[name]["search"] = os.path.relpath(search, GOLDEN)
         print("golden", name)
     with open(os.path.join(GOLDEN, "runs", "manifest.json"), "w") as f:
         json.dump(manifest, f, indent=1, sort_keys=True)

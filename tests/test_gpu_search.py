@@ -22,7 +22,8 @@ def golden(name):
 
 def add(batch, name):
     m = MANIFEST[name]
-    return batch.add(os.path.join(GOLDEN, m["task"]), None, m["population"], m["generations"], m["seed"])
+    search = os.path.join(GOLDEN, m["search"]) if "search" in m else None
+    return batch.add(os.path.join(GOLDEN, m["task"]), search, m["population"], m["generations"], m["seed"])
 
 
 @pytest.mark.parametrize("name", SMALL)
