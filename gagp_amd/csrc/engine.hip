@@ -1505,10 +1505,15 @@ SearchResult Batch::result(int index, bool save_all) {
   SearchState& s = *I.searches.at(std::size_t(index));
   const DSearch& d = s.d;
   SearchResult out;
-  const int best = s.down(d.best, 1)[0];
   const auto fitness = s.down(d.fitness, std::size_t(s.P));
   const auto wrong = s.down(d.wrong, std::size_t(s.P));
   const auto nodes = s.down(d.nodes, std::size_t(s.P));
+  // The front of the strict order over the final keys (for the generational
+  // schedule the device's best index predates the last replacement).
+  const Keys final_keys{fitness.data(), wrong.data(), nodes.data(), d.has_wrong};
+  int best = 0;
+  for (int m = 1; m < s.P; ++m)
+    if (better(final_keys, m, best)) best = m;
   const auto cur_n = s.down(d.cur_n, std::size_t(s.P));
   const auto cur_nc = s.down(d.cur_nc, std::size_t(s.P));
   const Counters counters = s.down(d.counters, 1)[0];
