@@ -678,15 +678,22 @@ struct PlaceKeys {
   }
 };
 
+// The block's four wavefronts stage the keys and refill the generator ring
+// together; wavefront 0 alone runs the chunks (their LDS ordering needs only
+// workgroup fences, not barriers). Children's keys are prefetched into two
+// 64-child register windows one window ahead of use.
+constexpr int kPlaceThreads = 256;
 template <bool kLds>
-__global__ __launch_bounds__(64) void k_place(DSearch* all) {
+__global__ __launch_bounds__(kPlaceThreads) void k_place(DSearch* all) {
   const DSearch& d = all[blockIdx.x];
   if (!search_live(d) || d.generational || d.place_lds != int(kLds)) return;
   extern __shared__ __attribute__((aligned(16))) unsigned char place_smem[];
   __shared__ std::uint64_t ring[kRing];
   __shared__ unsigned ridx[kRing];
   __shared__ std::uint64_t mt[312];
-  const int lane = threadIdx.x;
+  __shared__ int s_head, s_b, s_best, s_stop;
+  const int t = threadIdx.x, lane = t & 63;
+  const bool leader = wave_index() == 0;
   const int P = d.P, B = d.B, rs = d.replacement_size, has_wrong = d.has_wrong;
   const std::uint64_t P64 = std::uint64_t(P);
   GAS const double* const c_fit = d.c_fit;
@@ -703,39 +710,39 @@ __global__ __launch_bounds__(64) void k_place(DSearch* all) {
     std::uint64_t r = x - __umul64hi(x, barrett) * P64;
     return unsigned(r >= P64 ? r - P64 : r);
   };
-  constexpr int kStage = 16;  // loads in flight per lane while staging keys
-  for (int m0 = 0; m0 < P; m0 += 64 * kStage) {
+  constexpr int kStage = 8;  // loads in flight per thread while staging keys
+  for (int m0 = 0; m0 < P; m0 += kPlaceThreads * kStage) {
     double f[kStage];
     unsigned w[kStage], n[kStage];
 #pragma unroll
     for (int k = 0; k < kStage; ++k) {
-      const int m = m0 + k * 64 + lane;
+      const int m = m0 + k * kPlaceThreads + t;
       if (kLds && m < P) { f[k] = keys.fitness[m]; w[k] = keys.wrong[m]; n[k] = keys.nodes[m]; }
     }
 #pragma unroll
     for (int k = 0; k < kStage; ++k) {
-      const int m = m0 + k * 64 + lane;
+      const int m = m0 + k * kPlaceThreads + t;
       if (m < P) {
         if (kLds) { keys.f[m] = f[k]; keys.wn[m] = has_wrong ? w[k] << 12 | n[k] : 0u; }
         mark[m] = 0;
       }
     }
   }
-  for (int i = lane; i < 312; i += 64) mt[i] = d.mt[i];
+  for (int i = t; i < 312; i += kPlaceThreads) mt[i] = d.mt[i];
   int head = 0, tail = *d.n_left;
-  for (int i = lane; i < tail; i += 64) {
-    const std::uint64_t t = d.mt_left[i];
-    ring[i] = t;
-    ridx[i] = mod_p(t);
+  for (int i = t; i < tail; i += kPlaceThreads) {
+    const std::uint64_t x = d.mt_left[i];
+    ring[i] = x;
+    ridx[i] = mod_p(x);
   }
   __syncthreads();
-  const auto refill = [&](int need) {
+  const auto refill = [&](int need) {  // every thread, uniformly
     while (tail - head < need) {
-      mt_twist<64>(mt);
-      for (int i = lane; i < 312; i += 64) {
-        const std::uint64_t t = mt_temper(mt[i]);
-        ring[(tail + i) & (kRing - 1)] = t;
-        ridx[(tail + i) & (kRing - 1)] = mod_p(t);
+      mt_twist<kPlaceThreads>(mt);
+      for (int i = t; i < 312; i += kPlaceThreads) {
+        const std::uint64_t x = mt_temper(mt[i]);
+        ring[(tail + i) & (kRing - 1)] = x;
+        ridx[(tail + i) & (kRing - 1)] = mod_p(x);
       }
       tail += 312;
       __syncthreads();
@@ -746,63 +753,89 @@ __global__ __launch_bounds__(64) void k_place(DSearch* all) {
   int b = 0, best = *d.best;
   unsigned chunk = 0;
   const unsigned chunk_limit = 4u * unsigned(B) + 4096u;
+  // Child-key windows [win, win+64) and [win+64, win+128), lane l holding child win+l.
+  int win = 0;
+  double f0 = 0, f1 = 0;
+  unsigned w0 = 0, n0 = 0, w1 = 0, n1 = 0;
+  if (leader) {
+    if (lane < B) { f0 = c_fit[lane]; w0 = c_wrong[lane]; n0 = c_nodes[lane]; }
+    if (64 + lane < B) { f1 = c_fit[64 + lane]; w1 = c_wrong[64 + lane]; n1 = c_nodes[64 + lane]; }
+  }
   while (b < B) {
-    if (++chunk > chunk_limit) { if (lane == 0) raise(d, 500); break; }
     refill(W * rs);
-    // The chunk places children b .. b+63 at most: fetch their keys now.
-    double pf_f = 0;
-    unsigned pf_w = 0, pf_n = 0;
-    if (b + lane < B) { pf_f = c_fit[b + lane]; pf_w = c_wrong[b + lane]; pf_n = c_nodes[b + lane]; }
-    const bool active = lane < W;
-    const int base = head + lane * rs;
-    int worst = -1;
-    if (active) {
-      worst = int(ridx[base & (kRing - 1)]);
-      for (int i = 1; i < rs; ++i) {
-        const int c = int(ridx[(base + i) & (kRing - 1)]);
-        if (keys.better(worst, c)) worst = c;
+    if (leader) {
+      bool stop = ++chunk > chunk_limit;
+      if (stop && lane == 0) raise(d, 500);
+      while (!stop && b >= win + 64) {  // slide the windows
+        f0 = f1; w0 = w1; n0 = n1;
+        win += 64;
+        const int next = win + 64 + lane;
+        f1 = 0; w1 = 0; n1 = 0;
+        if (next < B) { f1 = c_fit[next]; w1 = c_wrong[next]; n1 = c_nodes[next]; }
       }
+      if (!stop) {
+        const bool active = lane < W;
+        const int base = head + lane * rs;
+        int worst = -1;
+        if (active) {
+          worst = int(ridx[base & (kRing - 1)]);
+          for (int i = 1; i < rs; ++i) {
+            const int c = int(ridx[(base + i) & (kRing - 1)]);
+            if (keys.better(worst, c)) worst = c;
+          }
+        }
+        const bool place = active && worst != best;
+        const std::uint64_t pmask = __ballot(place);
+        const int rank = __popcll(pmask & below);
+        const int child = b + rank;
+        const bool fills = place && child < B;
+        const int k = child - win;  // 0 <= k < 128
+        const double cf = k < 64 ? __shfl(f0, k & 63, 64) : __shfl(f1, k & 63, 64);
+        const unsigned cw = unsigned(k < 64 ? __shfl(int(w0), k & 63, 64) : __shfl(int(w1), k & 63, 64));
+        const unsigned cn = unsigned(k < 64 ? __shfl(int(n0), k & 63, 64) : __shfl(int(n1), k & 63, 64));
+        if (place) atomicMax(&mark[worst], chunk * 64u + unsigned(63 - lane));
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        bool conflict = false;
+        if (active)
+          for (int i = 0; i < rs; ++i) {
+            const unsigned m = mark[ridx[(base + i) & (kRing - 1)]];
+            conflict = conflict || ((m >> 6) == chunk && int(63u - (m & 63u)) < lane);
+          }
+        const bool raises = fills && keys.key_better(cf, keys.pack(cw, cn), worst, best);
+        int cut = W;
+        const std::uint64_t last = __ballot(fills && child == B - 1);
+        if (last) cut = min(cut, __ffsll((long long)last));
+        const std::uint64_t clash = __ballot(active && conflict);
+        if (clash) cut = min(cut, __ffsll((long long)clash) - 1);
+        const std::uint64_t lifts = __ballot(raises);
+        const int lifter = lifts ? __ffsll((long long)lifts) - 1 : 64;
+        if (lifter < cut) cut = lifter + 1;
+        if (fills && lane < cut) {
+          keys.store(worst, cf, cw, cn);
+          placed[worst] = child;
+        }
+        if (lifter < cut) best = __shfl(worst, lifter, 64);
+        b += __popcll(cut >= 64 ? pmask : pmask & ((1ull << cut) - 1));
+        head += cut * rs;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+      }
+      if (lane == 0) { s_head = head; s_b = stop ? B : b; s_best = best; s_stop = stop; }
     }
-    const bool place = active && worst != best;
-    const std::uint64_t pmask = __ballot(place);
-    const int rank = __popcll(pmask & below);
-    const int child = b + rank;
-    const bool fills = place && child < B;
-    const double cf = __shfl(pf_f, rank, 64);
-    const unsigned cw = unsigned(__shfl(int(pf_w), rank, 64)), cn = unsigned(__shfl(int(pf_n), rank, 64));
-    if (place) atomicMax(&mark[worst], chunk * 64u + unsigned(63 - lane));
     __syncthreads();
-    bool conflict = false;
-    if (active)
-      for (int i = 0; i < rs; ++i) {
-        const unsigned m = mark[ridx[(base + i) & (kRing - 1)]];
-        conflict = conflict || ((m >> 6) == chunk && int(63u - (m & 63u)) < lane);
-      }
-    const bool raises = fills && keys.key_better(cf, keys.pack(cw, cn), worst, best);
-    int cut = W;
-    const std::uint64_t last = __ballot(fills && child == B - 1);
-    if (last) cut = min(cut, __ffsll((long long)last));
-    const std::uint64_t clash = __ballot(active && conflict);
-    if (clash) cut = min(cut, __ffsll((long long)clash) - 1);
-    const std::uint64_t lifts = __ballot(raises);
-    const int lifter = lifts ? __ffsll((long long)lifts) - 1 : 64;
-    if (lifter < cut) cut = lifter + 1;
-    if (fills && lane < cut) {
-      keys.store(worst, cf, cw, cn);
-      placed[worst] = child;
-    }
-    if (lifter < cut) best = __shfl(worst, lifter, 64);
-    b += __popcll(cut >= 64 ? pmask : pmask & ((1ull << cut) - 1));
-    head += cut * rs;
+    head = s_head;
+    b = s_b;
+    best = s_best;
     __syncthreads();
   }
   refill(1);
   const std::uint64_t next_seed = ring[head & (kRing - 1)];
   ++head;
   const int left = tail - head;
-  for (int i = lane; i < left; i += 64) d.mt_left[i] = ring[(head + i) & (kRing - 1)];
-  for (int i = lane; i < 312; i += 64) d.mt[i] = mt[i];
-  if (lane == 0) {
+  for (int i = t; i < left; i += kPlaceThreads) d.mt_left[i] = ring[(head + i) & (kRing - 1)];
+  for (int i = t; i < 312; i += kPlaceThreads) d.mt[i] = mt[i];
+  if (t == 0) {
     *d.n_left = left;
     *d.round_seed = next_seed;
     *d.best = best;
@@ -1265,9 +1298,9 @@ struct Batch::Impl {
     launch("k_eval_general", g_eval, k_eval<false>, kEvalThreads, eval_general_lds, 1);
     timer.time("k_tally", stream, [&] { hipLaunchKernelGGL(k_tally, dim3(kTallyBlocks, n), dim3(256), 0, stream, d_all, 1); });
     if (place_lds_bytes)
-      timer.time("k_place", stream, [&] { hipLaunchKernelGGL(k_place<true>, dim3(n), dim3(64), place_lds_bytes, stream, d_all); });
+      timer.time("k_place", stream, [&] { hipLaunchKernelGGL(k_place<true>, dim3(n), dim3(kPlaceThreads), place_lds_bytes, stream, d_all); });
     if (place_hbm_bytes)
-      timer.time("k_place_hbm", stream, [&] { hipLaunchKernelGGL(k_place<false>, dim3(n), dim3(64), place_hbm_bytes, stream, d_all); });
+      timer.time("k_place_hbm", stream, [&] { hipLaunchKernelGGL(k_place<false>, dim3(n), dim3(kPlaceThreads), place_hbm_bytes, stream, d_all); });
     if (any_generational) launch("k_gen_place", g_member, k_gen_place, 128, 0);
     launch("k_commit", g_member, k_commit, 128, 0);
     if (any_generational) timer.time("k_gen_end", stream, [&] { hipLaunchKernelGGL(k_gen_end, dim3(n), dim3(64), 0, stream, d_all); });
