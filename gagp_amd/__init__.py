@@ -83,6 +83,17 @@ def lib():
     L.gagp_batch_step.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]
     L.gagp_batch_finish.argtypes = [ctypes.c_void_p]
     L.gagp_batch_max_rounds.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]
+    L.gagp_server_create.argtypes = [ctypes.c_uint64, ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]
+    L.gagp_server_submit.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32,
+                                     ctypes.c_int32, ctypes.c_uint64, ctypes.c_int32, ctypes.c_double,
+                                     ctypes.POINTER(ctypes.c_int32)]
+    L.gagp_server_run.argtypes = [ctypes.c_void_p]
+    L.gagp_server_outcome.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
+                                      ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
+    L.gagp_server_result_json.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_char_p, ctypes.c_size_t,
+                                          ctypes.POINTER(ctypes.c_size_t)]
+    L.gagp_server_destroy.argtypes = [ctypes.c_void_p]
+    L.gagp_server_destroy.restype = None
     L.gagp_batch_counters.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_uint64)]
     L.gagp_batch_evaluated_program_cases.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_uint64)]
     L.gagp_batch_set_profile.argtypes = [ctypes.c_void_p, ctypes.c_int32]
@@ -243,3 +254,37 @@ class Batch:
         if getattr(self, "h", None):
             lib().gagp_batch_destroy(self.h)
             self.h = None
+
+
+class Server:
+    """Request loop with continuous batching (gagp_server_*): submit requests
+    with arrival offsets, run() serves them in real time."""
+
+    def __init__(self, budget_bytes=64 << 30, poll_rounds=2):
+        self.h = ctypes.c_void_p()
+        _check(lib().gagp_server_create(budget_bytes, poll_rounds, ctypes.byref(self.h)))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().gagp_server_destroy(self.h)
+            self.h = None
+
+    def submit(self, task, search=None, population=0, generations=0, seed=None, early_exit=False, arrival=0.0):
+        t = ctypes.c_int32()
+        _check(lib().gagp_server_submit(self.h, task.encode(), search.encode() if search else None, population or 0,
+                                        generations or 0, int(seed is not None), ctypes.c_uint64(seed or 0),
+                                        int(early_exit), float(arrival), ctypes.byref(t)))
+        return t.value
+
+    def run(self):
+        _check(lib().gagp_server_run(self.h))
+
+    def outcome(self, ticket):
+        times = (ctypes.c_double * 3)()
+        gens, status = ctypes.c_int32(), ctypes.c_int32()
+        _check(lib().gagp_server_outcome(self.h, ticket, times, ctypes.byref(gens), ctypes.byref(status)))
+        return {"arrival": times[0], "admit": times[1], "done": times[2], "generations": gens.value,
+                "status": status.value}
+
+    def result(self, ticket):
+        return json.loads(_text(lib().gagp_server_result_json, self.h, ticket))

@@ -8,11 +8,13 @@
 #include <string>
 
 #include "engine.hpp"
+#include "serve.hpp"
 
 using namespace gagp_amd;
 
 struct gagp_session { std::unique_ptr<Session> s; };
 struct gagp_batch { Batch b; std::vector<SearchResult> results; bool prepared = false, ran = false; };
+struct gagp_server { Server s; bool ran = false; };
 
 namespace {
 thread_local std::string g_last_error;
@@ -241,5 +243,60 @@ int gagp_batch_kernel_times_json(gagp_batch* b, char* buf, size_t cap, size_t* n
 }
 
 void gagp_batch_destroy(gagp_batch* b) { delete b; }
+
+
+int gagp_server_create(uint64_t budget_bytes, int32_t poll_rounds, gagp_server** out) {
+  return guarded([&] {
+    if (!out) throw std::invalid_argument("invalid output pointer");
+    *out = new gagp_server{Server(std::size_t(budget_bytes), poll_rounds)};
+  });
+}
+
+int gagp_server_submit(gagp_server* s, const char* task_path, const char* search_path, int32_t population, int32_t generations,
+                       int32_t seed_is_set, uint64_t seed, int32_t early_exit, double arrival_s, int32_t* out_ticket) {
+  return guarded([&] {
+    if (!s || !task_path || !out_ticket || s->ran) throw std::invalid_argument("invalid server request");
+    SearchSpec spec;
+    spec.task_path = task_path;
+    if (search_path) spec.search_path = search_path;
+    if (population > 0) spec.population = population;
+    if (generations > 0) spec.generations = generations;
+    if (seed_is_set) spec.seed = seed;
+    spec.early_exit = early_exit != 0;
+    *out_ticket = s->s.submit(spec, arrival_s);
+  });
+}
+
+int gagp_server_run(gagp_server* s) {
+  return guarded([&] {
+    if (!s || s->ran) throw std::invalid_argument("server already ran");
+    s->ran = true;
+    s->s.run();
+  });
+}
+
+int gagp_server_outcome(gagp_server* s, int32_t ticket, double* times, int32_t* generations, int32_t* status) {
+  return guarded([&] {
+    if (!s || !s->ran || ticket < 0 || std::size_t(ticket) >= s->s.size()) throw std::invalid_argument("no such request");
+    const ServeOutcome& o = s->s.outcome(ticket);
+    if (times) { times[0] = o.arrival_s; times[1] = o.admit_s; times[2] = o.done_s; }
+    if (generations) *generations = o.generations;
+    if (status) *status = o.status;
+  });
+}
+
+int gagp_server_result_json(gagp_server* s, int32_t ticket, char* buf, size_t cap, size_t* needed) {
+  std::string text;
+  const int st = guarded([&] {
+    if (!s || !s->ran || ticket < 0 || std::size_t(ticket) >= s->s.size()) throw std::invalid_argument("no such request");
+    const ServeOutcome& o = s->s.outcome(ticket);
+    if (o.status != GAGP_OK) throw std::invalid_argument("request failed: " + o.message);
+    text = o.result_json;
+  });
+  if (st != GAGP_OK) return st;
+  return copy_out(text, buf, cap, needed);
+}
+
+void gagp_server_destroy(gagp_server* s) { delete s; }
 
 }  // extern "C"

@@ -113,6 +113,8 @@ struct DSearch {
   GAS unsigned long long* e_fuel;
   int profile;  // tally reproduction row sizes too
   int generational, elites;  // generational schedule (B = P, one round per generation) and its elite count
+  int early_exit;            // stop at the first generation boundary whose best member has no wrong case
+  GAS int* stop;             // generations completed when it stopped early, else 0
   GAS int* elite;            // elite members of the current generation, best first
   int general_scores;  // the general evaluation kernel has an LDS score buffer for this search
 };
@@ -125,7 +127,7 @@ __device__ inline CRow child_row(const DSearch& d, int c) {
   const std::size_t o = std::size_t(c) * d.R;
   return CRow{d.x_origin + o, d.x_site + o, d.x_cval + o, d.x_n[c], d.x_nc[c]};
 }
-__device__ inline bool search_live(const DSearch& d) { return d.active && *d.round < d.rounds && !*d.error; }
+__device__ inline bool search_live(const DSearch& d) { return d.active && *d.round < d.rounds && !*d.error && !*d.stop; }
 __device__ inline void raise(const DSearch& d, int code) { atomicCAS(d.error, 0, code); }
 
 // One entry per launched block of a batch kernel: its search, its block
@@ -612,6 +614,12 @@ __global__ void k_record(DSearch* all) {
   if (r % d.per_epoch) return;
   const int gen = r / d.per_epoch;
   const int b = *d.best;
+  // Early exit: the state after `gen` generations is exactly that of a run
+  // with `gen` generations, whose history has no row `gen`.
+  if (d.early_exit && gen > 0 && d.has_wrong && d.wrong[b] == 0) {
+    if (threadIdx.x == 0) *d.stop = gen;
+    return;
+  }
   const CRow row = member_row(d, b);
   const std::size_t o = std::size_t(gen) * d.R;
   for (int i = threadIdx.x; i < row.n; i += blockDim.x) d.hist_origin[o + i] = row.origin[i];
@@ -623,6 +631,16 @@ __global__ void k_record(DSearch* all) {
     d.hist_fit[gen] = d.fitness[b];
     d.hist_wrong[gen] = d.wrong[b];
   }
+}
+
+// Round, early-stop generation and error of every search of the table.
+__global__ void k_status(DSearch* all, int n, int* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const DSearch& d = all[i];
+  out[3 * i + 0] = *d.round;
+  out[3 * i + 1] = *d.stop;
+  out[3 * i + 2] = *d.error;
 }
 
 // ---- placement (evolve.cpp:378-422) ----
@@ -1078,6 +1096,7 @@ struct SearchState {
   DeviceArena arena;
   DSearch d{};
   double init_ms = 0;
+  bool configured = false, retired = false;
 
   explicit SearchState(const SearchSpec& s) : spec(s) {
     task = load_task(s.task_path);
@@ -1186,6 +1205,8 @@ struct SearchState {
     x.generational = cfg.schedule == Schedule::Generational;
     x.elites = x.generational ? cfg.elitism : 0;
     x.elite = a.alloc<int>(kMaxElites);
+    x.early_exit = spec.early_exit ? 1 : 0;
+    x.stop = a.alloc<int>(1);
     x.e_errors = a.alloc<unsigned>(code_rows); x.e_timeouts = a.alloc<unsigned>(code_rows);
     x.e_fuel = a.alloc<unsigned long long>(code_rows);
     x.error = a.alloc<int>(1);
@@ -1193,6 +1214,7 @@ struct SearchState {
     HIP_OK(hipMemset(x.counters, 0, sizeof(Counters)));
     HIP_OK(hipMemset(x.error, 0, sizeof(int)));
     HIP_OK(hipMemset(x.round, 0, sizeof(int)));
+    HIP_OK(hipMemset((void*)x.stop, 0, sizeof(int)));
     d = x;
   }
 
@@ -1234,7 +1256,11 @@ struct KernelTimer {
 };
 
 struct Batch::Impl {
-  std::vector<std::unique_ptr<SearchState>> searches;
+  std::vector<std::unique_ptr<SearchState>> searches;  // every search ever added (stable indices)
+  std::vector<int> live;                                // searches of the round table, in table order
+  bool dirty = true;                                    // table, grids and graph need rebuilding
+  int* d_status = nullptr;                              // k_status output, 3 ints per table entry
+  std::size_t status_cap = 0;
   DSearch* d_all = nullptr;
   hipStream_t stream = nullptr;
   KernelTimer timer;
@@ -1246,6 +1272,7 @@ struct Batch::Impl {
   ~Impl() {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (d_all) (void)hipFree(d_all);
+    if (d_status) (void)hipFree(d_status);
     if (stream) (void)hipStreamDestroy(stream);
   }
   std::size_t eval_fast_lds = 0, eval_general_lds = 0;  // dynamic LDS of the two evaluation kernels
@@ -1258,32 +1285,127 @@ struct Batch::Impl {
   DeviceArena map_arena;
   Grid g_init, g_lower, g_eval_init, g_vary_staged, g_vary_global, g_table, g_member, g_child, g_eval;
   template <class Blocks>
-  Grid build_grid(Blocks blocks_of) {
+  Grid build_grid(DeviceArena& arena, const std::vector<int>& table, Blocks blocks_of) {
     std::vector<BlockMap> v;
-    for (std::size_t i = 0; i < searches.size(); ++i) {
-      const int nb = blocks_of(*searches[i]);
+    for (std::size_t i = 0; i < table.size(); ++i) {
+      const int nb = blocks_of(*searches[std::size_t(table[i])]);
       for (int j = 0; j < nb; ++j) v.push_back(BlockMap{int(i), j, nb});
     }
     Grid g;
     g.blocks = int(v.size());
-    if (!v.empty()) g.map = map_arena.upload(v);
+    if (!v.empty()) g.map = arena.upload(v);
     return g;
   }
-  void build_grids() {
-    const auto up = [](int x, int k) { return (x + k - 1) / k; };
-    g_init = build_grid([&](const SearchState& s) { return up(s.P, 64); });
-    g_lower = build_grid([&](const SearchState& s) { return up(s.P, kVaryWaves); });
-    g_eval_init = build_grid([&](const SearchState& s) { return std::min(4096, up(s.P, kWavesPerBlock)); });
-    g_vary_staged = build_grid([&](const SearchState& s) { return s.d.stage_tables ? up(s.B, kVaryWaves) : 0; });
-    g_vary_global = build_grid([&](const SearchState& s) { return s.d.stage_tables ? 0 : up(s.B, kVaryWaves); });
-    g_table = build_grid([&](const SearchState& s) { return up(s.d.table_mask + 1, 256); });
-    g_member = build_grid([&](const SearchState& s) { return up(s.P, 128); });
-    g_child = build_grid([&](const SearchState& s) { return up(s.B, 128); });
-    g_eval = build_grid([&](const SearchState& s) { return std::min(4096, up(s.B, kWavesPerBlock)); });
+  static int up(int x, int k) { return (x + k - 1) / k; }
+  void build_round_grids() {
+    map_arena = DeviceArena{};
+    g_vary_staged = build_grid(map_arena, live, [&](const SearchState& s) { return s.d.stage_tables ? up(s.B, kVaryWaves) : 0; });
+    g_vary_global = build_grid(map_arena, live, [&](const SearchState& s) { return s.d.stage_tables ? 0 : up(s.B, kVaryWaves); });
+    g_table = build_grid(map_arena, live, [&](const SearchState& s) { return up(s.d.table_mask + 1, 256); });
+    g_member = build_grid(map_arena, live, [&](const SearchState& s) { return up(s.P, 128); });
+    g_child = build_grid(map_arena, live, [&](const SearchState& s) { return up(s.B, 128); });
+    g_eval = build_grid(map_arena, live, [&](const SearchState& s) { return std::min(4096, up(s.B, kWavesPerBlock)); });
+  }
+
+  // Allocates one search and fixes its kernel variants and LDS needs.
+  void configure(SearchState& s) {
+    constexpr std::size_t kLdsLimit = 160 * 1024, kStageLimit = 64 * 1024;
+    s.configured = true;
+    s.allocate();
+    s.d.profile = timer.on ? 1 : 0;
+    s.d.stage_tables = phase_tables_bytes(s.d.g) <= kStageLimit;
+    const std::size_t stacks = std::size_t(kWavesPerBlock) * s.d.stack_slots * 64 * 10;
+    const std::size_t score_buf = std::size_t(kWavesPerBlock) * 8 * s.d.cd.C;
+    s.d.general_scores = s.d.cd.C <= kScoreCases && stacks + score_buf <= kLdsLimit;
+  }
+  // LDS sizes and kernel attributes for a table of searches.
+  void size_kernels(const std::vector<int>& table) {
+    eval_fast_lds = eval_general_lds = vary_lds_bytes = place_lds_bytes = place_hbm_bytes = 0;
+    vary_global = any_generational = false;
+    max_P = max_B = max_slots = max_table = 0;
+    for (int id : table) {
+      const SearchState& x = *searches[std::size_t(id)];
+      const std::size_t stacks = std::size_t(kWavesPerBlock) * x.d.stack_slots * 64 * 10;
+      const std::size_t score_buf = std::size_t(kWavesPerBlock) * 8 * x.d.cd.C;
+      eval_general_lds = std::max(eval_general_lds, stacks + (x.d.general_scores ? score_buf : 0));
+      if (x.d.cd.C <= kScoreCases) eval_fast_lds = std::max(eval_fast_lds, score_buf);
+      if (x.d.stage_tables) vary_lds_bytes = std::max(vary_lds_bytes, phase_tables_bytes(x.d.g));
+      else vary_global = true;
+      if (x.d.place_lds) place_lds_bytes = std::max(place_lds_bytes, std::size_t(16) * x.P);
+      else place_hbm_bytes = std::max(place_hbm_bytes, std::size_t(4) * x.P);
+      any_generational = any_generational || x.d.generational;
+      max_P = std::max(max_P, x.P);
+      max_B = std::max(max_B, x.B);
+      max_slots = std::max(max_slots, x.d.stack_slots);
+      max_table = std::max(max_table, x.d.table_mask + 1);
+    }
+    const auto attr = [](const void* k, std::size_t bytes) {
+      if (bytes > 65536) HIP_OK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)));
+    };
+    attr(reinterpret_cast<const void*>(&k_vary<true>), vary_lds_bytes);
+    attr(reinterpret_cast<const void*>(&k_eval<false>), eval_general_lds);
+    attr(reinterpret_cast<const void*>(&k_eval<true>), eval_fast_lds);
+    attr(reinterpret_cast<const void*>(&k_place<true>), place_lds_bytes);
+    attr(reinterpret_cast<const void*>(&k_place<false>), place_hbm_bytes);
+  }
+  // Builds and scores the initial populations of `ids` (not yet in the
+  // round table) on the stream, through a table of their own.
+  void init_searches(const std::vector<int>& ids) {
+    if (ids.empty()) return;
+    std::vector<DSearch> host;
+    for (int id : ids) host.push_back(searches[std::size_t(id)]->d);
+    const int n = int(ids.size());
+    DeviceArena tmp;
+    GAS DSearch* table = tmp.upload(host);
+    DSearch* t = (DSearch*)table;
+    size_kernels(ids);
+    const Grid gi = build_grid(tmp, ids, [&](const SearchState& x) { return up(x.P, 64); });
+    const Grid gl = build_grid(tmp, ids, [&](const SearchState& x) { return up(x.P, kVaryWaves); });
+    const Grid ge = build_grid(tmp, ids, [&](const SearchState& x) { return std::min(4096, up(x.P, kWavesPerBlock)); });
+    timer.time("k_init_pop", stream, [&] { hipLaunchKernelGGL(k_init_pop, dim3(gi.blocks), dim3(64), 0, stream, t, gi.map); });
+    timer.time("k_lower_members", stream,
+               [&] { hipLaunchKernelGGL(k_lower_members, dim3(gl.blocks), dim3(64 * kVaryWaves), 0, stream, t, gl.map); });
+    timer.time("k_eval_init", stream,
+               [&] { hipLaunchKernelGGL(k_eval<true>, dim3(ge.blocks), dim3(kEvalThreads), eval_fast_lds, stream, t, ge.map, 0); });
+    timer.time("k_eval_init_general", stream, [&] {
+      hipLaunchKernelGGL(k_eval<false>, dim3(ge.blocks), dim3(kEvalThreads), eval_general_lds, stream, t, ge.map, 0);
+    });
+    timer.time("k_tally_init", stream, [&] { hipLaunchKernelGGL(k_tally, dim3(kTallyBlocks, n), dim3(256), 0, stream, t, 0); });
+    timer.time("k_init_keys", stream, [&] { hipLaunchKernelGGL(k_init_keys, dim3(n), dim3(256), 0, stream, t); });
+    timer.time("k_seed", stream, [&] { hipLaunchKernelGGL(k_seed, dim3(n), dim3(256), 0, stream, t); });
+    HIP_OK(hipStreamSynchronize(stream));
+    timer.collect();
+    for (int id : ids)
+      if (const int e = searches[std::size_t(id)]->device_error())
+        throw std::runtime_error("device initialization error " + std::to_string(e));
+  }
+  // Round table, grids, kernel sizes and graph for the current live set.
+  void rebuild() {
+    if (graph_exec) { (void)hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
+    graph_ready = false;
+    if (d_all) { (void)hipFree(d_all); d_all = nullptr; }
+    max_rounds = 0;
+    if (!live.empty()) {
+      std::vector<DSearch> host;
+      for (int id : live) {
+        host.push_back(searches[std::size_t(id)]->d);
+        max_rounds = std::max(max_rounds, searches[std::size_t(id)]->rounds);
+      }
+      HIP_OK(hipMalloc(&d_all, sizeof(DSearch) * host.size()));
+      HIP_OK(hipMemcpy(d_all, host.data(), sizeof(DSearch) * host.size(), hipMemcpyHostToDevice));
+      size_kernels(live);
+      build_round_grids();
+      if (status_cap < live.size()) {
+        if (d_status) (void)hipFree(d_status);
+        status_cap = std::max<std::size_t>(64, 2 * live.size());
+        HIP_OK(hipMalloc(&d_status, sizeof(int) * 3 * status_cap));
+      }
+    }
+    dirty = false;
   }
 
   void enqueue_round() {
-    const int n = int(searches.size());
+    const int n = int(live.size());
     const auto launch = [&](const char* name, const Grid& g, auto kernel, int threads, std::size_t lds, auto... args) {
       if (g.blocks) timer.time(name, stream, [&] { hipLaunchKernelGGL(kernel, dim3(g.blocks), dim3(threads), lds, stream, d_all, g.map, args...); });
     };
@@ -1331,83 +1453,67 @@ void Batch::prepare() {
   auto& I = *impl_;
   const auto t0 = std::chrono::steady_clock::now();
   if (!I.stream) HIP_OK(hipStreamCreateWithFlags(&I.stream, hipStreamNonBlocking));
-  std::vector<DSearch> host;
-  constexpr std::size_t kLdsLimit = 160 * 1024;
-  for (auto& s : I.searches) {
-    s->allocate();
-    s->d.profile = I.timer.on ? 1 : 0;
-    I.any_generational = I.any_generational || s->d.generational;
-    constexpr std::size_t kStageLimit = 64 * 1024;
-    const std::size_t tables = phase_tables_bytes(s->d.g);
-    s->d.stage_tables = tables <= kStageLimit;
-    if (s->d.stage_tables) I.vary_lds_bytes = std::max(I.vary_lds_bytes, tables);
-    else I.vary_global = true;
-    // Evaluation LDS per block: general kernel = stacks (+ score buffers when
-    // they fit), fast kernel = score buffers.
-    const std::size_t stacks = std::size_t(kWavesPerBlock) * s->d.stack_slots * 64 * 10;
-    const std::size_t score_buf = std::size_t(kWavesPerBlock) * 8 * s->d.cd.C;
-    s->d.general_scores = s->d.cd.C <= kScoreCases && stacks + score_buf <= kLdsLimit;
-    I.eval_general_lds = std::max(I.eval_general_lds, stacks + (s->d.general_scores ? score_buf : 0));
-    if (s->d.cd.C <= kScoreCases) I.eval_fast_lds = std::max(I.eval_fast_lds, score_buf);
-    host.push_back(s->d);
-    I.max_P = std::max(I.max_P, s->P);
-    I.max_B = std::max(I.max_B, s->B);
-    I.max_slots = std::max(I.max_slots, s->d.stack_slots);
-    I.max_table = std::max(I.max_table, s->d.table_mask + 1);
-    I.max_rounds = std::max(I.max_rounds, s->rounds);
-    if (s->d.place_lds) I.place_lds_bytes = std::max(I.place_lds_bytes, std::size_t(16) * s->P);
-    else I.place_hbm_bytes = std::max(I.place_hbm_bytes, std::size_t(4) * s->P);
+  std::vector<int> fresh;
+  for (std::size_t i = 0; i < I.searches.size(); ++i) {
+    if (I.searches[i]->configured) continue;
+    I.configure(*I.searches[i]);
+    fresh.push_back(int(i));
   }
-  const int n = int(host.size());
-  if (!n) return;
-  HIP_OK(hipMalloc(&I.d_all, sizeof(DSearch) * host.size()));
-  HIP_OK(hipMemcpy(I.d_all, host.data(), sizeof(DSearch) * host.size(), hipMemcpyHostToDevice));
-  if (I.vary_lds_bytes > 65536)
-    HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_vary<true>), hipFuncAttributeMaxDynamicSharedMemorySize,
-                               int(I.vary_lds_bytes)));
-  if (I.eval_general_lds > 65536) {
-    HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_eval<false>), hipFuncAttributeMaxDynamicSharedMemorySize,
-                               int(I.eval_general_lds)));
-  }
-  if (I.eval_fast_lds > 65536) {
-    HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_eval<true>), hipFuncAttributeMaxDynamicSharedMemorySize,
-                               int(I.eval_fast_lds)));
-  }
-  if (I.place_lds_bytes)
-    HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_place<true>), hipFuncAttributeMaxDynamicSharedMemorySize,
-                               int(I.place_lds_bytes)));
-  if (I.place_hbm_bytes)
-    HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_place<false>), hipFuncAttributeMaxDynamicSharedMemorySize,
-                               int(I.place_hbm_bytes)));
-  // Initial population, its evaluation, keys and the run generator.
-  I.build_grids();
-  I.timer.time("k_init_pop", I.stream,
-               [&] { hipLaunchKernelGGL(k_init_pop, dim3(I.g_init.blocks), dim3(64), 0, I.stream, I.d_all, I.g_init.map); });
-  I.timer.time("k_lower_members", I.stream, [&] {
-    hipLaunchKernelGGL(k_lower_members, dim3(I.g_lower.blocks), dim3(64 * kVaryWaves), 0, I.stream, I.d_all, I.g_lower.map);
-  });
-  I.timer.time("k_eval_init", I.stream, [&] {
-    hipLaunchKernelGGL(k_eval<true>, dim3(I.g_eval_init.blocks), dim3(kEvalThreads), I.eval_fast_lds, I.stream, I.d_all,
-                       I.g_eval_init.map, 0);
-  });
-  I.timer.time("k_eval_init_general", I.stream, [&] {
-    hipLaunchKernelGGL(k_eval<false>, dim3(I.g_eval_init.blocks), dim3(kEvalThreads), I.eval_general_lds, I.stream, I.d_all,
-                       I.g_eval_init.map, 0);
-  });
-  I.timer.time("k_tally_init", I.stream, [&] { hipLaunchKernelGGL(k_tally, dim3(kTallyBlocks, n), dim3(256), 0, I.stream, I.d_all, 0); });
-  I.timer.time("k_init_keys", I.stream, [&] { hipLaunchKernelGGL(k_init_keys, dim3(n), dim3(256), 0, I.stream, I.d_all); });
-  I.timer.time("k_seed", I.stream, [&] { hipLaunchKernelGGL(k_seed, dim3(n), dim3(256), 0, I.stream, I.d_all); });
-  HIP_OK(hipStreamSynchronize(I.stream));
-  I.timer.collect();
+  I.init_searches(fresh);
+  I.live.insert(I.live.end(), fresh.begin(), fresh.end());
+  I.rebuild();
   I.init_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-  for (auto& s : I.searches) {
-    if (const int e = s->device_error()) throw std::runtime_error("device initialization error " + std::to_string(e));
-  }
 }
+
+int Batch::admit(const SearchSpec& spec) {
+  auto& I = *impl_;
+  if (!I.stream) HIP_OK(hipStreamCreateWithFlags(&I.stream, hipStreamNonBlocking));
+  const int id = add(spec);
+  I.configure(*I.searches[std::size_t(id)]);
+  I.init_searches({id});
+  I.live.push_back(id);
+  I.dirty = true;
+  return id;
+}
+
+void Batch::retire(int index) {
+  auto& I = *impl_;
+  const auto it = std::find(I.live.begin(), I.live.end(), index);
+  if (it == I.live.end()) throw std::invalid_argument("search is not in the batch");
+  I.live.erase(it);
+  I.dirty = true;
+  SearchState& s = *I.searches.at(std::size_t(index));
+  s.arena = DeviceArena{};  // the stream is idle between steps: the memory is free to return
+  s.d.active = 0;
+  s.retired = true;
+}
+
+std::vector<Batch::Status> Batch::status() {
+  auto& I = *impl_;
+  if (I.dirty) I.rebuild();
+  std::vector<Status> out;
+  if (I.live.empty()) return out;
+  const int n = int(I.live.size());
+  hipLaunchKernelGGL(k_status, dim3((n + 63) / 64), dim3(64), 0, I.stream, I.d_all, n, I.d_status);
+  HIP_OK(hipGetLastError());
+  std::vector<int> v(std::size_t(3 * n));
+  HIP_OK(hipMemcpyAsync(v.data(), I.d_status, sizeof(int) * v.size(), hipMemcpyDeviceToHost, I.stream));
+  HIP_OK(hipStreamSynchronize(I.stream));
+  for (int i = 0; i < n; ++i) {
+    const SearchState& x = *I.searches[std::size_t(I.live[std::size_t(i)])];
+    Status st{I.live[std::size_t(i)], v[std::size_t(3 * i)], v[std::size_t(3 * i + 1)], v[std::size_t(3 * i + 2)], false};
+    st.done = st.error != 0 || st.stopped_at != 0 || st.round >= x.rounds;
+    out.push_back(st);
+  }
+  return out;
+}
+
+std::size_t Batch::footprint(int index) const { return impl_->searches.at(std::size_t(index))->arena.bytes; }
 
 void Batch::step(int rounds, bool use_graph) {
   auto& I = *impl_;
-  if (I.searches.empty() || rounds <= 0) return;
+  if (I.dirty) I.rebuild();
+  if (I.live.empty() || rounds <= 0) return;
   const auto t0 = std::chrono::steady_clock::now();
   if (use_graph && !I.timer.on) {
     if (!I.graph_ready) {
@@ -1433,8 +1539,8 @@ void Batch::step(int rounds, bool use_graph) {
 }
 
 void Batch::finish() {
-  for (auto& s : impl_->searches) {
-    if (const int e = s->device_error()) throw std::runtime_error("device search error " + std::to_string(e));
+  for (int id : impl_->live) {
+    if (const int e = impl_->searches[std::size_t(id)]->device_error()) throw std::runtime_error("device search error " + std::to_string(e));
   }
 }
 
@@ -1538,6 +1644,12 @@ SearchResult Batch::result(int index, bool save_all) {
   SearchState& s = *I.searches.at(std::size_t(index));
   const DSearch& d = s.d;
   SearchResult out;
+  // A search that stopped early after G generations publishes what a run
+  // with G generations publishes.
+  const int stopped = s.down(d.stop, 1)[0];
+  const int G = stopped ? stopped : s.G;
+  SearchConfig published = s.cfg;
+  published.generations = G;
   const auto fitness = s.down(d.fitness, std::size_t(s.P));
   const auto wrong = s.down(d.wrong, std::size_t(s.P));
   const auto nodes = s.down(d.nodes, std::size_t(s.P));
@@ -1553,16 +1665,16 @@ SearchResult Batch::result(int index, bool save_all) {
   out.evaluated_program_cases = counters.program_cases;
   out.evaluated_programs = counters.programs;
   // History rows.
-  const auto hn = s.down(d.hist_n, std::size_t(s.G));
-  const auto hnc = s.down(d.hist_nc, std::size_t(s.G));
-  const auto ho = s.down(d.hist_origin, std::size_t(s.G) * s.R);
-  const auto hc = s.down(d.hist_cval, std::size_t(s.G) * s.R);
-  const auto hf = s.down(d.hist_fit, std::size_t(s.G));
-  const auto hw = s.down(d.hist_wrong, std::size_t(s.G));
-  const auto ha = s.down(d.hist_all, std::size_t(s.G) * s.P);
-  const double per_gen_ms = s.G ? I.run_ms / s.G : 0;
+  const auto hn = s.down(d.hist_n, std::size_t(G));
+  const auto hnc = s.down(d.hist_nc, std::size_t(G));
+  const auto ho = s.down(d.hist_origin, std::size_t(G) * s.R);
+  const auto hc = s.down(d.hist_cval, std::size_t(G) * s.R);
+  const auto hf = s.down(d.hist_fit, std::size_t(G));
+  const auto hw = s.down(d.hist_wrong, std::size_t(G));
+  const auto ha = s.down(d.hist_all, std::size_t(G) * s.P);
+  const double per_gen_ms = G ? I.run_ms / G : 0;
   json::Value history = json::make_array();
-  for (int gen = 0; gen < s.G; ++gen) {
+  for (int gen = 0; gen < G; ++gen) {
     const Ast ast = export_row(s, &ho[std::size_t(gen) * s.R], &hc[std::size_t(gen) * s.R], hn[gen], hnc[gen]);
     long double sum = 0;
     for (int m = 0; m < s.P; ++m) sum += ha[std::size_t(gen) * s.P + m];
@@ -1609,10 +1721,10 @@ SearchResult Batch::result(int index, bool save_all) {
   r.obj["task"] = json::make_string(s.task.name);
   r.obj["engine"] = json::make_string("gpu");
   r.obj["population"] = json::make_number(s.P);
-  r.obj["generations"] = json::make_number(s.G);
+  r.obj["generations"] = json::make_number(G);
   r.obj["seed"] = json::make_string(std::to_string(s.cfg.seed));
   r.obj["train_cases"] = json::make_number(s.train.n);
-  r.obj["search"] = effective_search_json(s.cfg, s.penalty, s.fuel);
+  r.obj["search"] = effective_search_json(published, s.penalty, s.fuel);
   r.obj["best"] = bj;
   r.obj["history"] = history;
   json::Value tm = json::make_object();

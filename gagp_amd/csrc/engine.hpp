@@ -22,6 +22,9 @@ struct SearchSpec {
   std::string search_path;             // empty: built-in default
   std::optional<int> population, generations;
   std::optional<std::uint64_t> seed;
+  // Stop at the first generation boundary whose best member has no wrong
+  // train case; the result is that of a run with that many generations.
+  bool early_exit = false;
 };
 
 struct HistoryRow {
@@ -68,6 +71,16 @@ class Batch {
   void set_profile(bool on);
   // Generation budget of the batch (max over searches).
   int max_rounds() const;
+
+  // Continuous batching. admit() adds, allocates and initializes one search
+  // between rounds (it joins the round table at the next step); status()
+  // reports every search of the table; retire() drops a finished search and
+  // returns its device memory (assemble its result first).
+  struct Status { int index, round, stopped_at, error; bool done; };
+  int admit(const SearchSpec& spec);
+  std::vector<Status> status();
+  void retire(int index);
+  std::size_t footprint(int index) const;  // device bytes of one search
 
  private:
   struct Impl;

@@ -1,0 +1,97 @@
+// Request loop of serve.hpp.
+#include "serve.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <map>
+#include <numeric>
+#include <stdexcept>
+#include <thread>
+
+#include "../../include/gagp_hip.h"
+
+namespace gagp_amd {
+
+Server::Server(std::size_t budget_bytes, int poll_rounds) : budget_(budget_bytes), poll_rounds_(std::max(1, poll_rounds)) {}
+
+int Server::submit(const SearchSpec& spec, double arrival_s) {
+  requests_.push_back(Request{spec, arrival_s});
+  ServeOutcome o;
+  o.arrival_s = arrival_s;
+  outcomes_.push_back(o);
+  return int(requests_.size() - 1);
+}
+
+const ServeOutcome& Server::outcome(int ticket) const { return outcomes_.at(std::size_t(ticket)); }
+
+void Server::run() {
+  using Clock = std::chrono::steady_clock;
+  const std::size_t n = requests_.size();
+  std::vector<int> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return requests_[a].arrival_s < requests_[b].arrival_s; });
+  Batch batch;
+  std::map<int, int> ticket_of;  // batch index -> ticket
+  std::map<int, std::size_t> bytes_of;
+  std::size_t used = 0, largest = 0, next = 0, finished = 0;
+  const auto t0 = Clock::now();
+  const auto now = [&] { return std::chrono::duration<double>(Clock::now() - t0).count(); };
+  const auto fail = [&](ServeOutcome& o, int status, const char* what) {
+    o.status = status;
+    o.message = what;
+    o.done_s = now();
+    ++finished;
+  };
+  while (finished < n) {
+    // Admission in arrival order while the next search fits (head-of-line:
+    // a search that does not fit waits for memory, later ones wait behind it).
+    while (next < n && requests_[std::size_t(order[next])].arrival_s <= now()) {
+      if (!ticket_of.empty() && used + largest > budget_) break;
+      const int ticket = order[next];
+      ServeOutcome& o = outcomes_[std::size_t(ticket)];
+      try {
+        const int index = batch.admit(requests_[std::size_t(ticket)].spec);
+        o.admit_s = now();
+        ticket_of[index] = ticket;
+        bytes_of[index] = batch.footprint(index);
+        used += bytes_of[index];
+        largest = std::max(largest, bytes_of[index]);
+        peak_bytes_ = std::max(peak_bytes_, used);
+        peak_live_ = std::max(peak_live_, int(ticket_of.size()));
+      } catch (const Unsupported& e) {
+        fail(o, GAGP_ERR_UNSUPPORTED, e.what());
+      } catch (const std::invalid_argument& e) {
+        fail(o, GAGP_ERR_INVALID, e.what());
+      }
+      ++next;
+    }
+    if (ticket_of.empty()) {
+      if (next < n) {
+        const double wait = requests_[std::size_t(order[next])].arrival_s - now();
+        if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.001)));
+      }
+      continue;
+    }
+    batch.step(poll_rounds_);
+    for (const Batch::Status& st : batch.status()) {
+      if (!st.done) continue;
+      ServeOutcome& o = outcomes_[std::size_t(ticket_of.at(st.index))];
+      if (st.error) {
+        o.status = GAGP_ERR_DEVICE;
+        o.message = "device search error " + std::to_string(st.error);
+      } else {
+        const SearchResult r = batch.result(st.index, false);
+        o.result_json = json::canonical(r.result);
+        o.generations = int(r.result.at("generations").num);
+      }
+      o.done_s = now();
+      used -= bytes_of.at(st.index);
+      batch.retire(st.index);
+      ticket_of.erase(st.index);
+      bytes_of.erase(st.index);
+      ++finished;
+    }
+  }
+}
+
+}  // namespace gagp_amd

@@ -163,6 +163,36 @@ int gagp_batch_set_profile(gagp_batch* b, int32_t on);
 int gagp_batch_kernel_times_json(gagp_batch* b, char* buf, size_t cap, size_t* needed);
 void gagp_batch_destroy(gagp_batch* b);
 
+/* ---- serving: a request loop with continuous batching ----
+ * No counterpart in the reference, which runs one search per process; every
+ * published result is the reference's `gagp run` result for that request
+ * (cpp/src/cli/run.cpp:275-370), or, with early_exit, the result of the run
+ * with as many generations as were completed when the search first had a
+ * best member with no wrong train case. */
+typedef struct gagp_server gagp_server;
+
+/* budget_bytes: device memory the live searches may hold together;
+ * poll_rounds: rounds stepped between completion checks. */
+int gagp_server_create(uint64_t budget_bytes, int32_t poll_rounds, gagp_server** out);
+
+/* Queues one request arriving `arrival_s` seconds after gagp_server_run
+ * starts; arguments as gagp_batch_add. *out_ticket identifies it. */
+int gagp_server_submit(gagp_server* s, const char* task_path, const char* search_path, int32_t population, int32_t generations,
+                       int32_t seed_is_set, uint64_t seed, int32_t early_exit, double arrival_s, int32_t* out_ticket);
+
+/* Serves every submitted request to completion in real time: a request is
+ * admitted once it has arrived and fits the budget, stepped with all live
+ * searches in one kernel sequence per round, and retired when done. */
+int gagp_server_run(gagp_server* s);
+
+/* times[3] = {arrival, admission, completion} in seconds from the start of
+ * gagp_server_run; *generations of the published result; *status is the
+ * request's own status (GAGP_OK or an error, see gagp_last_error after
+ * gagp_server_result_json). */
+int gagp_server_outcome(gagp_server* s, int32_t ticket, double* times, int32_t* generations, int32_t* status);
+int gagp_server_result_json(gagp_server* s, int32_t ticket, char* buf, size_t cap, size_t* needed);
+void gagp_server_destroy(gagp_server* s);
+
 #ifdef __cplusplus
 }
 #endif

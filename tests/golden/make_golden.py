@@ -41,6 +41,10 @@ RUNS = {
     "poly_c1_gen_e3r_p512_g24_s9": ("poly_c1", 512, 24, 9, True, "generational_e3_reservoir.json"),
     "quintic_gen_e1_p1024_g16_s1": ("quintic", 1024, 16, 1, False, "generational_e1.json"),
     "polynomial_steady_b128_rs3_p512_g16_s4": ("polynomial", 512, 16, 4, False, "steady_batch_rs3.json"),
+    # the runs an early-exit request stops at: the first generation >= 1 whose
+    # recorded best has no wrong train case (18 and 10 in the two histories)
+    "polynomial_p1024_g18_s0": ("polynomial", 1024, 18, 0, False),
+    "poly_c1_p1024_g10_s0": ("poly_c1", 1024, 10, 0, False),
 }
 
 

@@ -1,0 +1,73 @@
+"""Serving (continuous batching): requests arriving over time share the
+device and each publishes exactly the reference engine's result.json for its
+request; with early exit, the result of the reference run with as many
+generations as the search completed when it first had a best member with no
+wrong train case."""
+import json
+import os
+
+import pytest
+
+import gagp_amd
+from _util import GOLDEN, strip_timing
+
+pytestmark = pytest.mark.gpu
+MANIFEST = json.load(open(os.path.join(GOLDEN, "runs", "manifest.json")))
+
+
+def golden(name):
+    return strip_timing(json.load(open(os.path.join(GOLDEN, "runs", name, "result.json"))))
+
+
+def submit(server, name, arrival, early_exit=False, generations=None):
+    m = MANIFEST[name]
+    search = os.path.join(GOLDEN, m["search"]) if "search" in m else None
+    return server.submit(os.path.join(GOLDEN, m["task"]), search, m["population"], generations or m["generations"],
+                         m["seed"], early_exit, arrival)
+
+
+def test_staggered_requests_match_reference():
+    print("serve: staggered requests", flush=True)
+    server = gagp_amd.Server(budget_bytes=8 << 30, poll_rounds=2)
+    plain = ["polynomial_p512_g32_s7", "quintic_p512_g16_s3", "mixed_0", "mixed_3", "polynomial_gen_e1_p256_g12_s3",
+             "polynomial_steady_b128_rs3_p512_g16_s4", "polynomial_p100_g10_s5"]
+    tickets = {name: submit(server, name, 0.004 * i) for i, name in enumerate(plain)}
+    # Early exit: the 64-generation requests stop where the reference's
+    # shorter runs end.
+    early = {"polynomial_p1024_g18_s0": submit(server, "polynomial_p1024_g64_s0", 0.002, early_exit=True),
+             "poly_c1_p1024_g10_s0": submit(server, "poly_c1_p1024_g64_s0", 0.011, early_exit=True)}
+    server.run()
+    for name, t in list(tickets.items()) + list(early.items()):
+        o = server.outcome(t)
+        assert o["status"] == 0, (name, o)
+        assert o["arrival"] <= o["admit"] <= o["done"], (name, o)
+        assert strip_timing(server.result(t)) == golden(name), name
+    assert server.outcome(early["polynomial_p1024_g18_s0"])["generations"] == 18
+    assert server.outcome(early["poly_c1_p1024_g10_s0"])["generations"] == 10
+
+
+def test_budget_admits_one_search_at_a_time():
+    # A budget smaller than any two searches serializes admission; results
+    # are unchanged.
+    print("serve: one-at-a-time budget", flush=True)
+    server = gagp_amd.Server(budget_bytes=1, poll_rounds=3)
+    names = ["mixed_1", "mixed_2", "polynomial_p100_g10_s5"]
+    tickets = [submit(server, n, 0.0) for n in names]
+    server.run()
+    outcomes = [server.outcome(t) for t in tickets]
+    for name, t, o in zip(names, tickets, outcomes):
+        assert o["status"] == 0
+        assert strip_timing(server.result(t)) == golden(name), name
+    # admission waits for the previous search to retire
+    for before, after in zip(outcomes, outcomes[1:]):
+        assert after["admit"] >= before["done"]
+
+
+def test_unsupported_request_fails_alone():
+    print("serve: unsupported request", flush=True)
+    server = gagp_amd.Server(poll_rounds=2)
+    bad = server.submit(os.path.join(GOLDEN, "tasks", "list-sum", "task.json"), arrival=0.0)
+    good = submit(server, "polynomial_p100_g10_s5", 0.0)
+    server.run()
+    assert server.outcome(bad)["status"] == gagp_amd.GAGP_ERR_UNSUPPORTED
+    assert strip_timing(server.result(good)) == golden("polynomial_p100_g10_s5")
