@@ -184,6 +184,44 @@ def batching(n, generations):
             "speedup": round(alone_s / batched_s, 2), "identical_to_alone": "%d/%d" % (same, n)}
 
 
+def serving(n, rate):
+    """The same Poisson request stream served with continuous batching and by
+    a server that admits one search at a time (a budget of one search):
+    completed searches/s over the makespan, p50/p99 latency from arrival to
+    published result, and whether every result is identical in both."""
+    from gagp_amd import workloads
+
+    def serve(requests, budget):
+        server = gagp_amd.Server(budget_bytes=budget, poll_rounds=2)
+        tickets = [server.submit(task, None, population, generations, seed, True, arrival)
+                   for task, population, generations, seed, arrival in requests]
+        t0 = time.perf_counter()
+        server.run()
+        makespan = time.perf_counter() - t0
+        outcomes = [server.outcome(t) for t in tickets]
+        results = [json.dumps(strip_timing(server.result(t)), sort_keys=True) for t in tickets]
+        lat = [o["done"] - o["arrival"] for o in outcomes]
+        return {"searches_per_s": round(n / makespan, 3), "makespan_s": round(makespan, 3),
+                "p50_latency_s": round(percentile(lat, 50), 4), "p99_latency_s": round(percentile(lat, 99), 4),
+                "generations_run": sum(o["generations"] for o in outcomes)}, results
+
+    with tempfile.TemporaryDirectory() as tmp:
+        requests = workloads.served_requests(tmp, n=n, seed=5, rate=rate)
+        batched, rb = serve(requests, 64 << 30)
+        serial, rs = serve(requests, 1)
+    return {"requests": n, "rate_per_s": rate, "early_exit": True,
+            "populations": "log-uniform 256-16384", "train_cases": "log-uniform 16-2048",
+            "generation_caps": "uniform 32-256", "continuous_batching": batched, "one_at_a_time": serial,
+            "throughput_ratio": round(batched["searches_per_s"] / serial["searches_per_s"], 2),
+            "identical_results": "%d/%d" % (sum(a == b for a, b in zip(rb, rs)), n)}
+
+
+def strip_timing(r):
+    out = {k: v for k, v in r.items() if k != "time_ms"}
+    out["history"] = [{k: v for k, v in h.items() if not k.endswith("_ms")} for h in r["history"]]
+    return out
+
+
 def calibrate_copy(mib):
     """One streaming kernel of known size for the HBM counters: reads and
     writes `mib` MiB (int32 negation, one read and one write per element)."""
@@ -205,6 +243,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=0, help="run a streaming kernel of this size first (counter scale)")
     ap.add_argument("--batch-searches", type=int, default=64, help="searches of the batching comparison (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=60, help="requests of the serving comparison (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: each search runs on a single device and this build has no multi-GPU "
@@ -228,6 +267,7 @@ def main():
     ms, counts = profile(args.warmup, psteps)
     roof, kernels, vm = roofline(ms, counts, psteps)
     batch = batching(args.batch_searches, 64) if args.batch_searches else None
+    serve = serving(args.serve_requests, 20.0) if args.serve_requests else None
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_generations)
     line = {
         "metric": "program-case evals/sec", "value": round(value, 1), "unit": "program-case evals/s", "n_gpus": 1,
@@ -237,7 +277,7 @@ def main():
         "config": {"workload": "quintic symbolic regression, population 4096, 128 generations, 256 train cases, "
                                "default resident steady-state search; one step = one generation",
                    "population": POPULATION, "generations": GENERATIONS, "train_cases": TRAIN_CASES},
-        "roofline": roof, "cpu_baseline": cpu, "kernels": kernels, "vm": vm, "batching": batch,
+        "roofline": roof, "cpu_baseline": cpu, "kernels": kernels, "vm": vm, "batching": batch, "serving": serve,
     }
     if cpu:
         line["vs_cpu_reference"] = round(value / cpu["value"], 2)

@@ -147,3 +147,19 @@ def mixed_searches(directory: str, n: int = 64, seed: int = 4, p_range=(256, 409
                           polynomial_grammar(list(range(10)), 64, 12), train, test)
         out.append((task, population))
     return out
+
+
+def served_requests(directory: str, n: int = 60, seed: int = 5, rate: float = 20.0, p_range=(256, 16384),
+                    c_range=(16, 2048), g_range=(32, 256)):
+    """A Poisson request stream over mixed polynomial searches: arrival
+    offsets with exponential gaps of mean 1/rate seconds, generation caps
+    uniform in g_range. Returns [(task_path, population, generations,
+    seed, arrival_s)]."""
+    tasks = mixed_searches(directory, n=n, seed=seed, p_range=p_range, c_range=c_range)
+    rng = SplitMix64(seed * 7919 + 1)
+    t, out = 0.0, []
+    for i, (task, population) in enumerate(tasks):
+        out.append((task, population, rng.uniform_int(*g_range), 100 + i, t))
+        t += -math.log(1.0 - rng.unit()) / rate
+    return out
+
