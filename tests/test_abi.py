@@ -62,3 +62,14 @@ def test_workload_generators_are_deterministic(tmp_path):
     for name, sub in (("poly_c1", "c1"), ("quintic", "c2")):
         for f in ("task.json", "grammar.json", "train.cases.json", "test.cases.json"):
             assert json.load(open(tmp_path / sub / f)) == json.load(open(os.path.join(GOLDEN, "tasks", name, f)))
+
+
+def test_server_rejects_requests_before_and_after_run_on_the_host():
+    s = gagp_amd.Server(poll_rounds=1)
+    L = gagp_amd.lib()
+    # No outcome before the server has run; bad handles fail with a message.
+    times = (ctypes.c_double * 3)()
+    st = L.gagp_server_outcome(s.h, 0, times, None, None)
+    assert st == gagp_amd.GAGP_ERR_INVALID and b"no such request" in L.gagp_last_error()
+    assert L.gagp_server_submit(None, b"x", None, 0, 0, 0, 0, 0, 0.0, None) == gagp_amd.GAGP_ERR_INVALID
+    assert L.gagp_server_run(None) == gagp_amd.GAGP_ERR_INVALID
