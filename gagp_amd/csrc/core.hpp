@@ -25,8 +25,10 @@
 // flat) instructions, which do not serialize with LDS traffic.
 #if defined(__HIP_DEVICE_COMPILE__)
 #define GAS __attribute__((address_space(1)))
+#define LAS __attribute__((address_space(3)))
 #else
 #define GAS
+#define LAS
 #endif
 
 namespace gagp_amd {

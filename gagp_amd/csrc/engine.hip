@@ -117,6 +117,7 @@ struct DSearch {
   GAS int* stop;             // generations completed when it stopped early, else 0
   GAS int* elite;            // elite members of the current generation, best first
   int general_scores;  // the general evaluation kernel has an LDS score buffer for this search
+  int stage_cases;     // the typed evaluation kernel stages the case set in LDS
 };
 
 __device__ inline CRow member_row(const DSearch& d, int m) {
@@ -309,13 +310,75 @@ struct EvalParams {
 };
 struct EvalOut { double fitness; unsigned wrong, errors, timeouts; unsigned long long fuel; };
 
-__device__ inline double lane_score(const EvalParams& e, const LaneResult& r, int c, bool& exact) {
+__device__ inline Val expected_of(const CaseData& cd, int c) {
+  Val expected{};
+  expected.i = cd.exp_val[c]; expected.tag = cd.exp_tag[c]; expected.b = cd.exp_b[c];
+  return expected;
+}
+__device__ inline double lane_score(const EvalParams& e, const LaneResult& r, const Val& expected, bool& exact) {
   if (r.err != ERR_NONE) { exact = false; return -error_ceiling(e.policy, e.penalty); }
   Val actual{};
   actual.i = r.v; actual.tag = r.t; actual.b = r.b;
-  Val expected{};
-  expected.i = e.cd.exp_val[c]; expected.tag = e.cd.exp_tag[c]; expected.b = e.cd.exp_b[c];
   return case_score(actual, expected, e.penalty, e.policy, exact);
+}
+__device__ inline double lane_score(const EvalParams& e, const LaneResult& r, int c, bool& exact) {
+  return lane_score(e, r, expected_of(e.cd, c), exact);
+}
+
+// The case set of one search copied into a block's LDS for the typed path:
+// per case, each input's payload (Bool as 0/1) and a bound mask, and the
+// expected value. Every wavefront of a block scores programs of the same
+// search, so the block stages it once.
+struct StagedCases {
+  LAS std::int64_t* value;      // [n_inputs][C]
+  LAS std::uint8_t* bound;      // [C]: bit k = input k bound
+  LAS std::int64_t* expected;   // [C] payload
+  LAS std::uint8_t* exp_tag;    // [C]
+  LAS std::uint8_t* exp_b;      // [C]
+  bool on;                      // staged for this block
+};
+// Input rows the typed path reads: those bound by the program's locals and
+// present in the case data, at most kFastInputs.
+GHD inline int staged_inputs_of(int n_locals, int n_inputs) {
+  return std::min(std::min(n_locals, n_inputs), kFastInputs);
+}
+GHD inline std::size_t staged_case_bytes(int C, int n_rows) { return std::size_t(C) * (8 * std::size_t(n_rows) + 8 + 3); }
+template <int kThreads>
+__device__ inline StagedCases stage_cases(const CaseData& cd, int n_locals, unsigned char* lds) {
+  const int C = cd.C, n_inputs = staged_inputs_of(n_locals, cd.n_inputs);
+  StagedCases sc;
+  sc.on = true;
+  sc.value = (LAS std::int64_t*)lds;
+  sc.expected = sc.value + std::size_t(n_inputs) * C;
+  sc.bound = (LAS std::uint8_t*)(sc.expected + C);
+  sc.exp_tag = sc.bound + C;
+  sc.exp_b = sc.exp_tag + C;
+  for (int c = threadIdx.x; c < C; c += kThreads) {
+    unsigned mask = 0;
+    for (int k = 0; k < n_inputs; ++k) {
+      const std::size_t at = std::size_t(k) * C + c;
+      const std::uint8_t tag = cd.in_tag[at];
+      if (tag != TAG_INVALID) mask |= 1u << k;
+      sc.value[at] = tag == TAG_BOOL ? std::int64_t(cd.in_b[at] != 0) : cd.in_val[at];
+    }
+    sc.bound[c] = std::uint8_t(mask);
+    sc.expected[c] = cd.exp_val[c];
+    sc.exp_tag[c] = cd.exp_tag[c];
+    sc.exp_b[c] = cd.exp_b[c];
+  }
+  __syncthreads();
+  return sc;
+}
+__device__ inline FastInputs staged_inputs(const StagedCases& sc, int C, int n_inputs, int c, bool active) {
+  FastInputs in{0, 0, 0, 0, 0};
+  if (!active) return in;
+  // Unstaged rows beyond n_inputs are unbound, as in fast_inputs.
+  in.bound = sc.bound[c];
+  if (n_inputs > 0) in.v0 = sc.value[c];
+  if (n_inputs > 1) in.v1 = sc.value[std::size_t(C) + c];
+  if (n_inputs > 2) in.v2 = sc.value[2 * std::size_t(C) + c];
+  if (n_inputs > 3) in.v3 = sc.value[3 * std::size_t(C) + c];
+  return in;
 }
 
 // Scores one program over all cases with the canonical reduction
@@ -330,7 +393,8 @@ __device__ inline double lane_score(const EvalParams& e, const LaneResult& r, in
 // case score of the first pass, so the canonical order is replayed from it
 // instead of by running the program again; the fast kernel always has it.
 template <bool kFast>
-__device__ EvalOut eval_program(const EvalParams& e, const ProgView& pv, WaveStack& st, double* scores, int lane) {
+__device__ EvalOut eval_program(const EvalParams& e, const ProgView& pv, WaveStack& st, double* scores, int lane,
+                                StagedCases staged = StagedCases{}) {
   const int C = e.cd.C;
   double isum = 0;
   unsigned w = 0, errs = 0, touts = 0;
@@ -345,10 +409,19 @@ __device__ EvalOut eval_program(const EvalParams& e, const ProgView& pv, WaveSta
   for (int tile = 0; tile < C; tile += 64) {
     const int c = tile + lane;
     const bool active = c < C;
-    const LaneResult r = run(c, active);
+    LaneResult r;
+    Val expected{};
+    if (kFast && staged.on) {
+      r = vm_fast(fp, staged_inputs(staged, C, staged_inputs_of(pv.n_locals, e.cd.n_inputs), c, active), active, e.fuel,
+                  pv.result_tag);
+      if (active) { expected.i = staged.expected[c]; expected.tag = staged.exp_tag[c]; expected.b = staged.exp_b[c]; }
+    } else {
+      r = run(c, active);
+      if (active) expected = expected_of(e.cd, c);
+    }
     if (active) {
       bool exact = false;
-      const double s = lane_score(e, r, c, exact);
+      const double s = lane_score(e, r, expected, exact);
       w += !exact;
       errs += r.err != ERR_NONE;
       touts += r.err == ERR_TIMEOUT;
@@ -444,6 +517,12 @@ __global__ __launch_bounds__(kEvalThreads) void k_eval(DSearch* all, const Block
                              std::size_t(wave) * d.cd.C
                        : nullptr;
   const EvalParams e{d.cd, d.fuel, d.lanes, d.penalty, d.policy};
+  // The typed kernel stages the case set after the score buffers.
+  StagedCases staged{};
+  const bool stage = kFast && d.stage_cases;
+  if (stage)
+    staged = stage_cases<kEvalThreads>(d.cd, d.n_locals,
+                                       smem + std::size_t(kWavesPerBlock) * 8 * std::size_t(d.cd.C));
   const int count = mode == 1 ? d.n_pending[kFast ? 1 : 0] : d.P;
   for (int i = bm.block * kWavesPerBlock + wave; i < count; i += bm.blocks * kWavesPerBlock) {
     const int slot = mode == 1 ? (kFast ? d.pending[d.B - 1 - i] : d.pending[i]) : i;
@@ -451,7 +530,7 @@ __global__ __launch_bounds__(kEvalThreads) void k_eval(DSearch* all, const Block
     if (mode == 0 && ((pmode & 2) != 0 && d.cd.C <= kScoreCases) != kFast) continue;  // the other kernel scores it
     const ProgView pv{d.code + std::size_t(slot) * d.code_cap, d.pool + std::size_t(slot) * d.pool_cap, d.code_len[slot],
                       d.n_locals, pmode & 1, (pmode >> 1) & 1, d.pool_len[slot], (pmode & 4) ? TAG_BOOL : TAG_INT};
-    const EvalOut r = eval_program<kFast>(e, pv, st, scores, lane);
+    const EvalOut r = eval_program<kFast>(e, pv, st, scores, lane, staged);
     if (lane == 0) {
       // Ranking rounds every published fitness to a 40-bit mantissa.
       if (mode == 1) { d.c_fit[slot] = canon40(r.fitness); d.c_wrong[slot] = r.wrong; }
@@ -808,9 +887,14 @@ __global__ __launch_bounds__(kPlaceThreads) void k_place(DSearch* all) {
         const int child = b + rank;
         const bool fills = place && child < B;
         const int k = child - win;  // 0 <= k < 128
-        const double cf = k < 64 ? __shfl(f0, k & 63, 64) : __shfl(f1, k & 63, 64);
-        const unsigned cw = unsigned(k < 64 ? __shfl(int(w0), k & 63, 64) : __shfl(int(w1), k & 63, 64));
-        const unsigned cn = unsigned(k < 64 ? __shfl(int(n0), k & 63, 64) : __shfl(int(n1), k & 63, 64));
+        // Both windows are read by every lane before the select: a shuffle
+        // under a divergent branch reads 0 from the lanes that branch masks off.
+        const double a0 = __shfl(f0, k & 63, 64), a1 = __shfl(f1, k & 63, 64);
+        const int b0 = __shfl(int(w0), k & 63, 64), b1 = __shfl(int(w1), k & 63, 64);
+        const int c0 = __shfl(int(n0), k & 63, 64), c1 = __shfl(int(n1), k & 63, 64);
+        const double cf = k < 64 ? a0 : a1;
+        const unsigned cw = unsigned(k < 64 ? b0 : b1);
+        const unsigned cn = unsigned(k < 64 ? c0 : c1);
         if (place) atomicMax(&mark[worst], chunk * 64u + unsigned(63 - lane));
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
         __builtin_amdgcn_wave_barrier();
@@ -1317,6 +1401,8 @@ struct Batch::Impl {
     const std::size_t stacks = std::size_t(kWavesPerBlock) * s.d.stack_slots * 64 * 10;
     const std::size_t score_buf = std::size_t(kWavesPerBlock) * 8 * s.d.cd.C;
     s.d.general_scores = s.d.cd.C <= kScoreCases && stacks + score_buf <= kLdsLimit;
+    const std::size_t cases = staged_case_bytes(s.d.cd.C, staged_inputs_of(s.d.n_locals, s.d.cd.n_inputs));
+    s.d.stage_cases = s.d.cd.C <= kScoreCases && cases <= 48 * 1024 && score_buf + cases + 16 <= kLdsLimit;
   }
   // LDS sizes and kernel attributes for a table of searches.
   void size_kernels(const std::vector<int>& table) {
@@ -1328,7 +1414,8 @@ struct Batch::Impl {
       const std::size_t stacks = std::size_t(kWavesPerBlock) * x.d.stack_slots * 64 * 10;
       const std::size_t score_buf = std::size_t(kWavesPerBlock) * 8 * x.d.cd.C;
       eval_general_lds = std::max(eval_general_lds, stacks + (x.d.general_scores ? score_buf : 0));
-      if (x.d.cd.C <= kScoreCases) eval_fast_lds = std::max(eval_fast_lds, score_buf);
+      const std::size_t cases = x.d.stage_cases ? staged_case_bytes(x.d.cd.C, staged_inputs_of(x.d.n_locals, x.d.cd.n_inputs)) : 0;
+      if (x.d.cd.C <= kScoreCases) eval_fast_lds = std::max(eval_fast_lds, score_buf + cases);
       if (x.d.stage_tables) vary_lds_bytes = std::max(vary_lds_bytes, phase_tables_bytes(x.d.g));
       else vary_global = true;
       if (x.d.place_lds) place_lds_bytes = std::max(place_lds_bytes, std::size_t(16) * x.P);
