@@ -116,6 +116,8 @@ struct DSearch {
   int early_exit;            // stop at the first generation boundary whose best member has no wrong case
   GAS int* stop;             // generations completed when it stopped early, else 0
   GAS int* elite;            // elite members of the current generation, best first
+  int temp_slots;      // hidden-local slots per lane of the general VM (traversal temporaries)
+  int walks;           // the grammar has traversals: lowering runs the kWalks instantiations
   int general_scores;  // the general evaluation kernel has an LDS score buffer for this search
   int stage_cases;     // the typed evaluation kernel stages the case set in LDS
 };
@@ -162,11 +164,15 @@ __global__ void k_init_pop(DSearch* all, const BlockMap* map) {
 }
 
 // ---- lowering: one wavefront per row ----
+// kWalks: the lowering instantiation with traversals (grammars that have them).
+template <bool kWalks>
 __device__ inline void lower_into_w(const PhaseView& g, const DSearch& d, const CRow& row, int slot) {
   Ins* code = d.code + std::size_t(slot) * d.code_cap;
   Val* pool = d.pool + std::size_t(slot) * d.pool_cap;
-  const LowerResult r = lower_phase_w(g, row, d.local_types, d.n_locals, d.expected_type, code, d.code_cap, pool, d.pool_cap);
-  const bool ok = r.status == ST_OK && r.stack_bound >= 1 && r.stack_bound <= d.stack_slots;
+  const LowerResult r =
+      lower_phase_w<kWalks>(g, row, d.local_types, d.n_locals, d.expected_type, code, d.code_cap, pool, d.pool_cap);
+  const bool ok = r.status == ST_OK && r.stack_bound >= 1 && r.stack_bound <= d.stack_slots &&
+                  r.locals - d.n_locals <= d.temp_slots;
   if (wlane() == 0) {
     if (!ok) raise(d, 200 + r.status);
     d.code_len[slot] = ok ? r.code : 0;
@@ -176,22 +182,23 @@ __device__ inline void lower_into_w(const PhaseView& g, const DSearch& d, const 
   }
 }
 constexpr int kVaryWaves = 4;  // wavefronts (rows) per block of the row kernels
+template <bool kWalks>
 __global__ __launch_bounds__(64 * kVaryWaves) void k_lower_members(DSearch* all, const BlockMap* map) {
   const BlockMap bm = map[blockIdx.x];
   const DSearch& d = all[bm.search];
   const int m = bm.block * kVaryWaves + wave_index();
-  if (!d.active || m >= d.P || *d.error) return;
-  lower_into_w(view_of(d.g), d, member_row(d, m), m);
+  if (!d.active || (d.walks != 0) != kWalks || m >= d.P || *d.error) return;
+  lower_into_w<kWalks>(view_of(d.g), d, member_row(d, m), m);
 }
 
 // ---- reproduction: one wavefront per child (phase_operators.cuh:156-265) ----
 // kStaged: the block first copies the phase tables into LDS (searches whose
 // tables fit); otherwise the operators read them from global memory.
-template <bool kStaged>
+template <bool kStaged, bool kWalks>
 __global__ __launch_bounds__(64 * kVaryWaves) void k_vary(DSearch* all, const BlockMap* map) {
   const BlockMap bm = map[blockIdx.x];
   const DSearch& d = all[bm.search];
-  if (!search_live(d) || (d.stage_tables != 0) != kStaged) return;  // uniform per block
+  if (!search_live(d) || (d.stage_tables != 0) != kStaged || (d.walks != 0) != kWalks) return;  // uniform per block
   extern __shared__ __attribute__((aligned(16))) unsigned char vary_lds[];
   const PhaseView g = kStaged ? stage_phase_tables<64 * kVaryWaves>(d.g, vary_lds) : view_of(d.g);
   const int c = bm.block * kVaryWaves + wave_index();
@@ -228,7 +235,7 @@ __global__ __launch_bounds__(64 * kVaryWaves) void k_vary(DSearch* all, const Bl
     if (st.error) raise(d, 300 + st.error);
   }
   if (st.error) return;
-  lower_into_w(g, d, child, c);
+  lower_into_w<kWalks>(g, d, child, c);
 }
 
 // ---- member hash table (parent_fitness_cache.cuh:11-58) ----
@@ -494,10 +501,13 @@ __device__ EvalOut eval_program(const EvalParams& e, const ProgView& pv, WaveSta
   return out;
 }
 
-__device__ inline WaveStack stack_of(unsigned char* smem, int slots, int wave) {
-  unsigned char* base = smem + std::size_t(wave) * slots * 64 * 10;
-  return WaveStack{reinterpret_cast<std::int64_t*>(base), base + std::size_t(slots) * 64 * 8,
-                   base + std::size_t(slots) * 64 * 9, slots};
+// A wavefront's LDS stack of `slots` operand slots followed by `temps`
+// hidden-local slots, 10 bytes per lane and slot.
+__device__ inline WaveStack stack_of(unsigned char* smem, int slots, int temps, int wave) {
+  const int all = slots + temps;
+  unsigned char* base = smem + std::size_t(wave) * all * 64 * 10;
+  return WaveStack{reinterpret_cast<std::int64_t*>(base), base + std::size_t(all) * 64 * 8,
+                   base + std::size_t(all) * 64 * 9, slots, temps};
 }
 
 // mode 0: score members 0..P-1 (initial population), mode 1: pending
@@ -510,12 +520,13 @@ __global__ __launch_bounds__(kEvalThreads) void k_eval(DSearch* all, const Block
   if (mode == 1 ? !search_live(d) : (!d.active || *d.error)) return;
   extern __shared__ unsigned char smem[];
   const int wave = wave_index(), lane = threadIdx.x % 64;
-  WaveStack st = stack_of(smem, kFast ? 0 : d.stack_slots, wave);
+  WaveStack st = stack_of(smem, kFast ? 0 : d.stack_slots, kFast ? 0 : d.temp_slots, wave);
   // Score buffers follow the stacks of the block's wavefronts.
-  double* scores = (kFast ? d.cd.C <= kScoreCases : d.general_scores != 0)
-                       ? reinterpret_cast<double*>(smem + std::size_t(kWavesPerBlock) * (kFast ? 0 : d.stack_slots) * 64 * 10) +
-                             std::size_t(wave) * d.cd.C
-                       : nullptr;
+  double* scores =
+      (kFast ? d.cd.C <= kScoreCases : d.general_scores != 0)
+          ? reinterpret_cast<double*>(smem + std::size_t(kWavesPerBlock) * (kFast ? 0 : d.stack_slots + d.temp_slots) * 64 * 10) +
+                std::size_t(wave) * d.cd.C
+          : nullptr;
   const EvalParams e{d.cd, d.fuel, d.lanes, d.penalty, d.policy};
   // The typed kernel stages the case set after the score buffers.
   StagedCases staged{};
@@ -788,7 +799,7 @@ __global__ __launch_bounds__(kPlaceThreads) void k_place(DSearch* all) {
   __shared__ std::uint64_t ring[kRing];
   __shared__ unsigned ridx[kRing];
   __shared__ std::uint64_t mt[312];
-  __shared__ int s_head, s_b, s_best, s_stop;
+  __shared__ int s_head, s_b, s_best;
   const int t = threadIdx.x, lane = t & 63;
   const bool leader = wave_index() == 0;
   const int P = d.P, B = d.B, rs = d.replacement_size, has_wrong = d.has_wrong;
@@ -923,7 +934,7 @@ __global__ __launch_bounds__(kPlaceThreads) void k_place(DSearch* all) {
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
         __builtin_amdgcn_wave_barrier();
       }
-      if (lane == 0) { s_head = head; s_b = stop ? B : b; s_best = best; s_stop = stop; }
+      if (lane == 0) { s_head = head; s_b = stop ? B : b; s_best = best; }
     }
     __syncthreads();
     head = s_head;
@@ -1050,11 +1061,11 @@ __device__ inline ProgView view_of(const DProg& p) {
 // kFast: scores the programs certified for vm_fast (LDS: a score buffer of C
 // doubles per wavefront); otherwise the rest (LDS: the general stacks).
 template <bool kFast>
-__global__ __launch_bounds__(kEvalThreads) void k_eval_programs(const DProg* progs, int n, EvalParams e, int slots,
+__global__ __launch_bounds__(kEvalThreads) void k_eval_programs(const DProg* progs, int n, EvalParams e, int slots, int temps,
                                                                  double* fitness, unsigned* wrong, unsigned* counts) {
   extern __shared__ unsigned char smem[];
   const int wave = wave_index(), lane = threadIdx.x % 64;
-  WaveStack st = stack_of(smem, kFast ? 0 : slots, wave);
+  WaveStack st = stack_of(smem, kFast ? 0 : slots, kFast ? 0 : temps, wave);
   double* scores = kFast ? reinterpret_cast<double*>(smem) + std::size_t(wave) * e.cd.C : nullptr;
   for (int i = blockIdx.x * kWavesPerBlock + wave; i < n; i += gridDim.x * kWavesPerBlock) {
     const DProg p = progs[i];
@@ -1070,10 +1081,11 @@ __global__ __launch_bounds__(kEvalThreads) void k_eval_programs(const DProg* pro
   }
 }
 template <bool kFast>
-__global__ __launch_bounds__(kEvalThreads) void k_eval_cases(DProg p, CaseData cd, int fuel, int slots, Val* values, int* errors) {
+__global__ __launch_bounds__(kEvalThreads) void k_eval_cases(DProg p, CaseData cd, int fuel, int slots, int temps, Val* values,
+                                                              int* errors) {
   extern __shared__ unsigned char smem[];
   const int wave = wave_index(), lane = threadIdx.x % 64;
-  WaveStack st = stack_of(smem, kFast ? 0 : slots, wave);
+  WaveStack st = stack_of(smem, kFast ? 0 : slots, kFast ? 0 : temps, wave);
   const ProgView pv = view_of(p);
   FastProgram fp{};
   if (kFast) fp = fast_load(pv, lane);
@@ -1232,9 +1244,14 @@ struct SearchState {
     DSearch x{};
     x.active = 1;
     x.P = P; x.B = B; x.R = R;
-    x.code_cap = 4 * R + 8;
-    x.pool_cap = 2 * R + 8;
+    // A traversal lowers to at most 64 instructions and 8 constants of its
+    // own besides those of its nodes, and keeps 9 temporaries live.
+    const int walks = phase.traversals;
+    x.code_cap = 4 * R + 8 + 64 * walks;
+    x.pool_cap = 2 * R + 8 + 8 * walks;
     x.stack_slots = std::min(kMaxStack, 2 * int(layout.slot_max_depth) + 2);
+    x.walks = walks > 0;
+    x.temp_slots = walks ? std::max(0, std::min(64 - int(train.input_names.size()), 9 * std::min(walks, 8))) : 0;
     x.rounds = rounds; x.per_epoch = per_epoch;
     x.fuel = fuel; x.lanes = 512; x.penalty = penalty; x.policy = task.policy;
     x.pressure = std::clamp(cfg.tournament, 1, P);
@@ -1367,7 +1384,13 @@ struct Batch::Impl {
   // Flat grids: every search contributes exactly its own blocks.
   struct Grid { GAS BlockMap* map = nullptr; int blocks = 0; };
   DeviceArena map_arena;
-  Grid g_init, g_lower, g_eval_init, g_vary_staged, g_vary_global, g_table, g_member, g_child, g_eval;
+  Grid g_init, g_lower, g_eval_init, g_vary_staged, g_vary_global, g_vary_staged_w, g_vary_global_w, g_table, g_member, g_child,
+      g_eval;
+  bool any_walks(const std::vector<int>& ids) const {
+    for (int id : ids)
+      if (searches[std::size_t(id)]->d.walks) return true;
+    return false;
+  }
   template <class Blocks>
   Grid build_grid(DeviceArena& arena, const std::vector<int>& table, Blocks blocks_of) {
     std::vector<BlockMap> v;
@@ -1383,8 +1406,15 @@ struct Batch::Impl {
   static int up(int x, int k) { return (x + k - 1) / k; }
   void build_round_grids() {
     map_arena = DeviceArena{};
-    g_vary_staged = build_grid(map_arena, live, [&](const SearchState& s) { return s.d.stage_tables ? up(s.B, kVaryWaves) : 0; });
-    g_vary_global = build_grid(map_arena, live, [&](const SearchState& s) { return s.d.stage_tables ? 0 : up(s.B, kVaryWaves); });
+    const auto vary = [&](bool staged, bool walks) {
+      return build_grid(map_arena, live, [&](const SearchState& s) {
+        return (s.d.stage_tables != 0) == staged && (s.d.walks != 0) == walks ? up(s.B, kVaryWaves) : 0;
+      });
+    };
+    g_vary_staged = vary(true, false);
+    g_vary_global = vary(false, false);
+    g_vary_staged_w = vary(true, true);
+    g_vary_global_w = vary(false, true);
     g_table = build_grid(map_arena, live, [&](const SearchState& s) { return up(s.d.table_mask + 1, 256); });
     g_member = build_grid(map_arena, live, [&](const SearchState& s) { return up(s.P, 128); });
     g_child = build_grid(map_arena, live, [&](const SearchState& s) { return up(s.B, 128); });
@@ -1398,7 +1428,7 @@ struct Batch::Impl {
     s.allocate();
     s.d.profile = timer.on ? 1 : 0;
     s.d.stage_tables = phase_tables_bytes(s.d.g) <= kStageLimit;
-    const std::size_t stacks = std::size_t(kWavesPerBlock) * s.d.stack_slots * 64 * 10;
+    const std::size_t stacks = std::size_t(kWavesPerBlock) * (s.d.stack_slots + s.d.temp_slots) * 64 * 10;
     const std::size_t score_buf = std::size_t(kWavesPerBlock) * 8 * s.d.cd.C;
     s.d.general_scores = s.d.cd.C <= kScoreCases && stacks + score_buf <= kLdsLimit;
     const std::size_t cases = staged_case_bytes(s.d.cd.C, staged_inputs_of(s.d.n_locals, s.d.cd.n_inputs));
@@ -1411,7 +1441,7 @@ struct Batch::Impl {
     max_P = max_B = max_slots = max_table = 0;
     for (int id : table) {
       const SearchState& x = *searches[std::size_t(id)];
-      const std::size_t stacks = std::size_t(kWavesPerBlock) * x.d.stack_slots * 64 * 10;
+      const std::size_t stacks = std::size_t(kWavesPerBlock) * (x.d.stack_slots + x.d.temp_slots) * 64 * 10;
       const std::size_t score_buf = std::size_t(kWavesPerBlock) * 8 * x.d.cd.C;
       eval_general_lds = std::max(eval_general_lds, stacks + (x.d.general_scores ? score_buf : 0));
       const std::size_t cases = x.d.stage_cases ? staged_case_bytes(x.d.cd.C, staged_inputs_of(x.d.n_locals, x.d.cd.n_inputs)) : 0;
@@ -1429,7 +1459,8 @@ struct Batch::Impl {
     const auto attr = [](const void* k, std::size_t bytes) {
       if (bytes > 65536) HIP_OK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)));
     };
-    attr(reinterpret_cast<const void*>(&k_vary<true>), vary_lds_bytes);
+    attr(reinterpret_cast<const void*>(&k_vary<true, false>), vary_lds_bytes);
+    attr(reinterpret_cast<const void*>(&k_vary<true, true>), vary_lds_bytes);
     attr(reinterpret_cast<const void*>(&k_eval<false>), eval_general_lds);
     attr(reinterpret_cast<const void*>(&k_eval<true>), eval_fast_lds);
     attr(reinterpret_cast<const void*>(&k_place<true>), place_lds_bytes);
@@ -1451,7 +1482,10 @@ struct Batch::Impl {
     const Grid ge = build_grid(tmp, ids, [&](const SearchState& x) { return std::min(4096, up(x.P, kWavesPerBlock)); });
     timer.time("k_init_pop", stream, [&] { hipLaunchKernelGGL(k_init_pop, dim3(gi.blocks), dim3(64), 0, stream, t, gi.map); });
     timer.time("k_lower_members", stream,
-               [&] { hipLaunchKernelGGL(k_lower_members, dim3(gl.blocks), dim3(64 * kVaryWaves), 0, stream, t, gl.map); });
+               [&] { hipLaunchKernelGGL(k_lower_members<false>, dim3(gl.blocks), dim3(64 * kVaryWaves), 0, stream, t, gl.map); });
+    if (any_walks(ids))
+      timer.time("k_lower_members_walks", stream,
+                 [&] { hipLaunchKernelGGL(k_lower_members<true>, dim3(gl.blocks), dim3(64 * kVaryWaves), 0, stream, t, gl.map); });
     timer.time("k_eval_init", stream,
                [&] { hipLaunchKernelGGL(k_eval<true>, dim3(ge.blocks), dim3(kEvalThreads), eval_fast_lds, stream, t, ge.map, 0); });
     timer.time("k_eval_init_general", stream, [&] {
@@ -1498,8 +1532,10 @@ struct Batch::Impl {
     };
     if (any_generational) timer.time("k_gen_begin", stream, [&] { hipLaunchKernelGGL(k_gen_begin, dim3(n), dim3(256), 0, stream, d_all); });
     timer.time("k_record", stream, [&] { hipLaunchKernelGGL(k_record, dim3(n), dim3(256), 0, stream, d_all); });
-    launch("k_vary", g_vary_staged, k_vary<true>, 64 * kVaryWaves, vary_lds_bytes);
-    launch("k_vary_global", g_vary_global, k_vary<false>, 64 * kVaryWaves, 0);
+    launch("k_vary", g_vary_staged, k_vary<true, false>, 64 * kVaryWaves, vary_lds_bytes);
+    launch("k_vary_global", g_vary_global, k_vary<false, false>, 64 * kVaryWaves, 0);
+    launch("k_vary_walks", g_vary_staged_w, k_vary<true, true>, 64 * kVaryWaves, vary_lds_bytes);
+    launch("k_vary_global_walks", g_vary_global_w, k_vary<false, true>, 64 * kVaryWaves, 0);
     launch("k_clear", g_table, k_clear, 256, 0);
     launch("k_index", g_member, k_index, 128, 0);
     launch("k_reuse", g_child, k_reuse, 128, 0);
@@ -1665,15 +1701,17 @@ json::Value judge(const SearchState& s, const Ast& ast, const std::vector<int>& 
   const CRow row{origin.data(), sites.data(), cval.data(), int(origin.size()), int(cval.size())};
   std::vector<std::uint8_t> local_types = s.train.input_types;
   if (test.input_names != s.train.input_names) throw std::invalid_argument("test cases must bind the train inputs");
-  const LowerResult lr = lower_phase(gv, row, local_types.data(), int(local_types.size()), s.d.expected_type, code.data(),
-                                     int(code.size()), pool.data(), int(pool.size()));
+  const LowerResult lr = s.d.walks ? lower_phase<true>(gv, row, local_types.data(), int(local_types.size()), s.d.expected_type,
+                                                       code.data(), int(code.size()), pool.data(), int(pool.size()))
+                                    : lower_phase<false>(gv, row, local_types.data(), int(local_types.size()), s.d.expected_type,
+                                                         code.data(), int(code.size()), pool.data(), int(pool.size()));
   if (lr.status != ST_OK) throw std::runtime_error("lowering of the best program failed");
   ProgramIn p;
   for (int i = 0; i < lr.code; ++i) {
     const Ins& x = code[i];
     p.op.push_back(x.op);
-    const bool a = x.op == OP_PUSH_CONST || x.op == OP_LOAD || x.op == OP_JMP || x.op == OP_JMP_IF_FALSE || x.op == OP_JMP_IF_TRUE ||
-                   x.op == OP_CALL_BUILTIN;
+    const bool a = x.op == OP_PUSH_CONST || x.op == OP_LOAD || x.op == OP_STORE || x.op == OP_JMP || x.op == OP_JMP_IF_FALSE ||
+                   x.op == OP_JMP_IF_TRUE || x.op == OP_CALL_BUILTIN;
     p.has_a.push_back(a);
     p.has_b.push_back(x.op == OP_CALL_BUILTIN);
     p.a.push_back(x.a);
@@ -1681,7 +1719,7 @@ json::Value judge(const SearchState& s, const Ast& ast, const std::vector<int>& 
     p.fuel.push_back(x.fuel);
   }
   p.consts.assign(pool.begin(), pool.begin() + lr.constants);
-  p.n_locals = int(local_types.size());
+  p.n_locals = lr.locals;
   Session session(test, s.fuel, 512, s.penalty, s.task.policy);
   std::vector<Val> values;
   std::vector<int> errors;
@@ -1830,7 +1868,7 @@ namespace {
 // Static stack heights of external bytecode (the verifier's stack-shape
 // pass, cpp/src/runtime/bytecode_verify.cpp): every instruction gets one
 // height; joins must agree; operand/constant/jump domains are checked.
-std::vector<Ins> verify_program(const ProgramIn& p, int& max_stack, bool& jumps) {
+std::vector<Ins> verify_program(const ProgramIn& p, int n_inputs, int& max_stack, bool& jumps) {
   const int n = int(p.op.size());
   if (n == 0) throw std::invalid_argument("empty program");
   if (p.a.size() != std::size_t(n) || p.b.size() != std::size_t(n) || p.has_a.size() != std::size_t(n) ||
@@ -1859,8 +1897,12 @@ std::vector<Ins> verify_program(const ProgramIn& p, int& max_stack, bool& jumps)
         if (!p.has_a[ip] || p.a[ip] < 0 || p.a[ip] >= int(p.consts.size())) throw std::invalid_argument("const index out of range");
         push = 1; break;
       case OP_LOAD:
-        if (!p.has_a[ip] || p.a[ip] < 0) throw std::invalid_argument("local index out of range");
+        if (!p.has_a[ip] || p.a[ip] < 0 || p.a[ip] >= 64) throw std::invalid_argument("local index out of range");
         push = 1; break;
+      case OP_STORE:
+        if (!p.has_a[ip] || p.a[ip] < 0 || p.a[ip] >= 64) throw std::invalid_argument("local index out of range");
+        if (p.a[ip] < n_inputs) throw Unsupported("stores into evaluator inputs");
+        pop = 1; break;
       case OP_NEG: case OP_NOT: case OP_CHECK_INT: case OP_CHECK_LIST: pop = 1; push = 1; break;
       case OP_ADD: case OP_SUB: case OP_MUL: case OP_DIV: case OP_MOD:
       case OP_LT: case OP_LE: case OP_GT: case OP_GE: case OP_EQ: case OP_NE: pop = 2; push = 1; break;
@@ -2011,11 +2053,12 @@ void Session::eval(const std::vector<ProgramIn>& programs, double* fitness, std:
   std::vector<Val> pool;
   std::vector<std::size_t> code_off, pool_off;
   std::vector<int> lens, jumps_v, locals, fast_v, pool_n, result_v;
-  int slots = 1;
+  int slots = 1, temps = 0;
   for (const auto& p : programs) {
+    temps = std::max(temps, std::min(64, p.n_locals) - I.cases.cd.n_inputs);
     int ms = 0;
     bool j = false;
-    const auto ins = verify_program(p, ms, j);
+    const auto ins = verify_program(p, I.cases.cd.n_inputs, ms, j);
     for (const auto& v : p.consts) check_view(v, I.arena_len);
     const FastCert cert = certify_fast(ins, p.consts, I.input_tags, p.n_locals, ms, j, I.cases.cd.C);
     fast_v.push_back(cert.fast);
@@ -2042,7 +2085,7 @@ void Session::eval(const std::vector<ProgramIn>& programs, double* fitness, std:
   auto dwrong = tmp.alloc<unsigned>(std::size_t(n));
   auto dcounts = tmp.alloc<unsigned>(std::size_t(n) * 3);
   const EvalParams e{I.cases.cd, I.fuel, I.lanes, I.penalty, I.policy};
-  const std::size_t lds = std::size_t(kWavesPerBlock) * slots * 64 * 10;
+  const std::size_t lds = std::size_t(kWavesPerBlock) * (slots + temps) * 64 * 10;
   const std::size_t fast_lds = std::size_t(kWavesPerBlock) * 8 * std::size_t(I.cases.cd.C);
   const bool any_fast = std::count(fast_v.begin(), fast_v.end(), 1) > 0, any_general = std::count(fast_v.begin(), fast_v.end(), 0) > 0;
   if (lds > 65536)
@@ -2057,11 +2100,11 @@ void Session::eval(const std::vector<ProgramIn>& programs, double* fitness, std:
   const int blocks = std::min(4096, (n + kWavesPerBlock - 1) / kWavesPerBlock);
   HIP_OK(hipEventRecord(a, nullptr));
   if (any_fast)
-    hipLaunchKernelGGL(k_eval_programs<true>, dim3(blocks), dim3(kEvalThreads), fast_lds, nullptr, dprogs, n, e, slots, dfit, dwrong,
-                       dcounts);
+    hipLaunchKernelGGL(k_eval_programs<true>, dim3(blocks), dim3(kEvalThreads), fast_lds, nullptr, dprogs, n, e, slots, temps, dfit,
+                       dwrong, dcounts);
   if (any_general)
-    hipLaunchKernelGGL(k_eval_programs<false>, dim3(blocks), dim3(kEvalThreads), lds, nullptr, dprogs, n, e, slots, dfit, dwrong,
-                       dcounts);
+    hipLaunchKernelGGL(k_eval_programs<false>, dim3(blocks), dim3(kEvalThreads), lds, nullptr, dprogs, n, e, slots, temps, dfit,
+                       dwrong, dcounts);
   HIP_OK(hipGetLastError());
   HIP_OK(hipEventRecord(b, nullptr));
   HIP_OK(hipEventSynchronize(b));
@@ -2077,9 +2120,9 @@ void Session::eval_cases(const ProgramIn& program, std::vector<Val>& values, std
   auto& I = *impl_;
   int ms = 0;
   bool j = false;
-  const auto ins = verify_program(program, ms, j);
+  const auto ins = verify_program(program, I.cases.cd.n_inputs, ms, j);
   for (const auto& v : program.consts) check_view(v, I.arena_len);
-  const int slots = ms + 1;
+  const int slots = ms + 1, temps = std::max(0, std::min(64, program.n_locals) - I.cases.cd.n_inputs);
   DeviceArena tmp;
   std::vector<Val> consts = program.consts;
   if (consts.empty()) consts.push_back(val_invalid());
@@ -2091,13 +2134,14 @@ void Session::eval_cases(const ProgramIn& program, std::vector<Val>& values, std
   auto de = tmp.alloc<int>(std::size_t(C));
   const int blocks = std::max(1, std::min(1024, (C + 255) / 256));
   if (cert.fast) {
-    hipLaunchKernelGGL(k_eval_cases<true>, dim3(blocks), dim3(kEvalThreads), 0, nullptr, p, I.cases.cd, I.fuel, slots, dv, de);
+    hipLaunchKernelGGL(k_eval_cases<true>, dim3(blocks), dim3(kEvalThreads), 0, nullptr, p, I.cases.cd, I.fuel, slots, temps, dv, de);
   } else {
-    const std::size_t lds = std::size_t(kWavesPerBlock) * slots * 64 * 10;
+    const std::size_t lds = std::size_t(kWavesPerBlock) * (slots + temps) * 64 * 10;
     if (lds > 65536)
       HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_eval_cases<false>), hipFuncAttributeMaxDynamicSharedMemorySize,
                                  int(lds)));
-    hipLaunchKernelGGL(k_eval_cases<false>, dim3(blocks), dim3(kEvalThreads), lds, nullptr, p, I.cases.cd, I.fuel, slots, dv, de);
+    hipLaunchKernelGGL(k_eval_cases<false>, dim3(blocks), dim3(kEvalThreads), lds, nullptr, p, I.cases.cd, I.fuel, slots, temps, dv,
+                       de);
   }
   HIP_OK(hipGetLastError());
   HIP_OK(hipDeviceSynchronize());

@@ -1,6 +1,7 @@
 // Genotype export (see export.hpp).
 #include "export.hpp"
 
+#include <functional>
 #include <stdexcept>
 
 namespace gagp_amd {
@@ -17,13 +18,18 @@ Ast export_whole_program(const Grammar& g, const PhaseGrammar& phase, const std:
   for (const auto& lo : g.locals) table.push_back(lo.name);
   std::vector<int> used(table.size(), 0);
   std::vector<int> name_of_node(origins.size(), -1);
-  std::size_t next_const = 0;
-  for (std::size_t j = 0; j < origins.size(); ++j) {
-    const int o = origins[j];
+  std::size_t next_const = 0, j = 0;
+  int next_binder = 6 * phase.traversals;
+  // Prefix-order emission; `env` maps binder numbers to public ids.
+  std::function<void(const std::vector<int>&)> emit = [&](const std::vector<int>& env) {
+    if (j >= origins.size()) throw std::runtime_error("invalid gene shape");
+    const std::size_t at = j++;
+    const int o = origins[at];
     if (o < 0 || o >= int(g.exprs.size())) throw std::runtime_error("unknown expression origin");
     const CExpr& e = g.exprs[o];
     const PExpr& m = phase.exprs[o];
     AstNode n{m.native_kind, 0, 0};
+    int walk = -1;
     if (e.kind == EKind::Constant) {
       if (next_const >= consts.size()) throw std::runtime_error("constant index");
       n.i0 = int(ast.consts.size());
@@ -31,10 +37,32 @@ Ast export_whole_program(const Grammar& g, const PhaseGrammar& phase, const std:
     } else if (e.kind == EKind::Input) {
       const int idx = int(e.target);  // grammar declaration order = prototype table order
       used[idx] = 1;
-      name_of_node[j] = idx;
+      name_of_node[at] = idx;
+    } else if (e.kind == EKind::Bound) {
+      const int b = m.node.operand;
+      if (b < 0 || b >= 16 || env[std::size_t(b)] < 0) throw std::runtime_error("lexical index");
+      n.i0 = env[std::size_t(b)];
+    } else if (m.node.kind == TK_TRAVERSE) {
+      // Recorded before its subtrees.
+      walk = next_binder;
+      LexicalRegion region{int(ast.nodes.size()), e.region_argument, {}};
+      for (int k = 0; k < 3; ++k) region.bindings.push_back({next_binder++, e.region_bindings[std::size_t(k)].type});
+      ast.lexical_regions.push_back(region);
+      ast.traversal_specs.push_back({int(ast.nodes.size()), e.direction == 2});
     }
     ast.nodes.push_back(n);
-  }
+    for (int c = 0; c < m.node.arity; ++c) {
+      if (walk >= 0 && c == m.node.arity - 1) {
+        std::vector<int> inner = env;
+        for (int k = 0; k < 3; ++k) inner[std::size_t(m.node.operand + k)] = walk + k;
+        emit(inner);
+      } else {
+        emit(env);
+      }
+    }
+  };
+  emit(std::vector<int>(16, -1));
+  if (j != origins.size()) throw std::runtime_error("invalid gene shape");
   ast.nodes.push_back({NK_BLOCK_NIL, 0, 0});
   std::vector<int> remap(table.size(), -1);
   for (std::size_t i = 0; i < table.size(); ++i)
@@ -72,6 +100,34 @@ json::Value ast_json(const Ast& ast) {
   json::Value consts = json::make_array();
   for (const auto& c : ast.consts) consts.arr.push_back(typed_value_json(c));
   o.obj["consts"] = consts;
+  if (!ast.lexical_regions.empty()) {
+    json::Value regions = json::make_array();
+    for (const auto& r : ast.lexical_regions) {
+      json::Value x = json::make_object();
+      x.obj["node_index"] = json::make_number(r.node_index);
+      x.obj["body_argument"] = json::make_number(r.body_argument);
+      json::Value bs = json::make_array();
+      for (const auto& b : r.bindings) {
+        json::Value y = json::make_object();
+        y.obj["id"] = json::make_number(b.id);
+        y.obj["type"] = json::make_string(type_name(b.type));
+        bs.arr.push_back(y);
+      }
+      x.obj["bindings"] = bs;
+      regions.arr.push_back(x);
+    }
+    o.obj["lexical_regions"] = regions;
+  }
+  if (!ast.traversal_specs.empty()) {
+    json::Value specs = json::make_array();
+    for (const auto& t : ast.traversal_specs) {
+      json::Value x = json::make_object();
+      x.obj["node_index"] = json::make_number(t.node_index);
+      x.obj["direction"] = json::make_string(t.reverse ? "reverse" : "forward");
+      specs.arr.push_back(x);
+    }
+    o.obj["traversal_specs"] = specs;
+  }
   return o;
 }
 
@@ -81,6 +137,7 @@ struct Printer {
   std::size_t pos = 0;
   void node(std::string& out) {
     if (pos >= ast.nodes.size()) throw std::runtime_error("truncated AST program");
+    const std::size_t index = pos;
     const AstNode n = ast.nodes[pos++];
     switch (n.kind) {
       case NK_CONST: {
@@ -111,9 +168,18 @@ struct Printer {
     out += '(';
     out += name;
     if (node_i0_is_name(n.kind)) out += " " + ast.names.at(n.i0);
+    // A binder argument shows its bound variables first: `[v6 v7 v8] body`.
+    const LexicalRegion* binder = nullptr;
+    for (const auto& r : ast.lexical_regions)
+      if (r.node_index == int(index)) binder = &r;
     const int arity = node_prefix_arity(n.kind);
     for (int i = 0; i < arity; ++i) {
       out += ' ';
+      if (binder && i == binder->body_argument) {
+        out += '[';
+        for (std::size_t b = 0; b < binder->bindings.size(); ++b) out += (b ? " v" : "v") + std::to_string(binder->bindings[b].id);
+        out += "] ";
+      }
       node(out);
     }
     out += ')';

@@ -7,6 +7,13 @@
 // in prefix order; names are the grammar inputs (declaration order) that the
 // program uses. Writers: write_ast_json (cpp/src/cli/program_json.cpp:211-232)
 // and ProgramPrinter (program_json.cpp:876-946).
+//
+// Traversal binders take fresh public ids in prefix order, above every id of
+// the layout prototype (phase_layout.hpp:330-340,388-397). A search's layout
+// is its first initial member, itself exported against a natively generated
+// prototype that numbers its binders from 0; with K traversals in every
+// program those hold ids 0..3K-1 and 3K..6K-1, so exported programs number
+// theirs from 6K.
 #pragma once
 #include <string>
 #include <vector>
@@ -16,10 +23,16 @@
 namespace gagp_amd {
 
 struct AstNode { int kind, i0, i1; };
+// A traversal's bindings (lexical_regions) and direction (traversal_specs).
+struct LexicalBinding { int id; std::uint8_t type; };
+struct LexicalRegion { int node_index, body_argument; std::vector<LexicalBinding> bindings; };
+struct TraversalSpec { int node_index; bool reverse; };
 struct Ast {
   std::vector<AstNode> nodes;
   std::vector<std::string> names;
   std::vector<Val> consts;
+  std::vector<LexicalRegion> lexical_regions;
+  std::vector<TraversalSpec> traversal_specs;
 };
 
 // `origins`/`consts` are one phase row; `phase` is its expression table.

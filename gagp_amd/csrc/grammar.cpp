@@ -140,6 +140,40 @@ void resolve_primitive(const Sig& s, CExpr& e) {
   throw Unsupported("primitive outside the engine's admitted catalog: " + s.op);
 }
 
+// A lexical binding name: an identifier not already visible
+// (cpp/src/evolution/grammar/compiled.cpp:118-125).
+void add_binding(std::vector<Binding>& scope, const std::string& name, std::uint8_t type) {
+  const auto letter = [](char c) { return (c >= 'a' && c <= 'z') || (c >= 'A' && c <= 'Z') || c == '_'; };
+  if (name.empty() || !letter(name.front()) ||
+      !std::all_of(name.begin(), name.end(), [&](char c) { return letter(c) || (c >= '0' && c <= '9'); }))
+    throw std::invalid_argument("invalid lexical binding name");
+  for (const auto& b : scope)
+    if (b.name == name) throw std::invalid_argument("duplicate/shadowed lexical binding: " + name);
+  scope.push_back({name, type});
+}
+
+// Traversals of the catalog (catalog.cpp:134-150) over an IntList source with
+// an Int or Bool accumulator: traverse[_reverse](xs, start, init, body) and
+// traverse_range[_reverse](xs, start, begin, end, init, body). The body
+// argument binds element (Int), index (Int) and accumulator.
+bool resolve_traversal(const Sig& s, CExpr& e) {
+  const bool ranged = s.op == "traverse_range" || s.op == "traverse_range_reverse";
+  if (!ranged && s.op != "traverse" && s.op != "traverse_reverse") return false;
+  const auto& a = s.args;
+  const std::size_t arity = ranged ? 6 : 4;
+  if (a.size() != arity || a[arity - 1] != s.result || a[arity - 2] != s.result)
+    throw std::invalid_argument("unknown exact primitive overload: " + s.op);
+  for (std::size_t i = 1; i + 2 < arity; ++i)
+    if (a[i] != TAG_INT) throw std::invalid_argument("unknown exact primitive overload: " + s.op);
+  if (a[0] != TAG_INT_LIST) throw Unsupported("traversal over a non-IntList sequence");
+  if (s.result != TAG_INT && s.result != TAG_BOOL) throw Unsupported("traversal accumulator outside Int/Bool");
+  e.node_kind = ranged ? NK_TRAVERSE_RANGE : NK_TRAVERSE;
+  e.direction = s.op == "traverse" || s.op == "traverse_range" ? 1 : 2;
+  e.region_argument = int(arity - 1);
+  e.region_bindings = {{"element", TAG_INT}, {"index", TAG_INT}, {"accumulator", s.result}};
+  return true;
+}
+
 class Compiler {
  public:
   Grammar compile(const Value& doc) {
@@ -288,18 +322,38 @@ class Compiler {
       e.target = binding(scope, v.at(input ? "input" : "bound").as_string("binding name"));
       e.type = scope[e.target].type;
       e.node_kind = input ? NK_VAR : NK_REGION_VAR;
-      if (!input) throw Unsupported("lexical bound references (let/traverse bodies)");
     } else if (v.has("signature")) {
-      keys(v, {"signature", "args", "bind"}, "primitive expression");
-      if (v.has("bind")) throw Unsupported("region-binding primitives (let/traverse)");
+      keys(v, {"signature", "args", "bind", "fuel_events", "resource_charge"}, "primitive expression");
+      if (v.has("fuel_events")) throw Unsupported("source fuel profiles (fuel_events)");
+      if (v.has("resource_charge")) throw Unsupported("resource charges");
       const Sig s = parse_signature(v.at("signature").as_string("signature"));
       e.kind = EKind::Primitive;
-      resolve_primitive(s, e);
+      if (!resolve_traversal(s, e)) resolve_primitive(s, e);
       e.type = s.result;
       const auto& args = v.at("args").as_array("args");
       if (args.size() != s.args.size()) throw std::invalid_argument("primitive argument count mismatch");
+      // Region arguments see the enclosing scope plus the names bound there
+      // (compiled.cpp:664-690: no shadowing, one name per region binding).
+      std::vector<Binding> body = env;
+      if (v.has("bind")) {
+        const auto& bind = v.at("bind");
+        if (bind.kind != Value::Object) throw std::invalid_argument("bind must be an object keyed by region argument");
+        for (const auto& item : bind.obj) {
+          if (e.region_argument < 0 || std::to_string(e.region_argument) != item.first)
+            throw std::invalid_argument("binding names target a non-region argument");
+          const auto& names = item.second.as_array("region binding names");
+          if (names.size() != e.region_bindings.size()) throw std::invalid_argument("region binding arity mismatch");
+          for (std::size_t k = 0; k < names.size(); ++k) {
+            const std::string name = names[k].as_string("region binding name");
+            add_binding(body, name, e.region_bindings[k].type);
+            e.region_bindings[k].name = name;
+          }
+        }
+      }
+      if (e.region_argument >= 0 && body.size() == env.size())
+        throw std::invalid_argument("primitive requires explicit region binding names");
       for (std::size_t i = 0; i < args.size(); ++i) {
-        const auto child = expression(args[i], env, depth + 1);
+        const auto child = expression(args[i], int(i) == e.region_argument ? body : env, depth + 1);
         if (g_.exprs[child].type != s.args[i] || g_.exprs[child].cat != Cat::Expression)
           throw std::invalid_argument("primitive argument type mismatch");
         e.children.push_back(child);
@@ -421,6 +475,7 @@ PhaseGrammar compile_phase(const Grammar& g, unsigned root, unsigned max_depth, 
     expr_env[id] = env;
     const auto& e = g.exprs[id];
     auto& x = out.exprs[id];
+    if (env.size() > 15) fail("lexical environment capacity");
     x.node.type = e.type;
     if (e.kind == EKind::Reference) {
       Env inner;
@@ -431,8 +486,10 @@ PhaseGrammar compile_phase(const Grammar& g, unsigned root, unsigned max_depth, 
       visit_nt(e.target, inner);
       return;
     }
-    if (e.children.size() > 3) fail("arity capacity");
+    const bool traversal = e.kind == EKind::Primitive && (e.node_kind == NK_TRAVERSE || e.node_kind == NK_TRAVERSE_RANGE);
+    if (e.children.size() > (traversal ? std::size_t(kMaxArity) : 3)) fail("arity capacity");
     x.node.arity = int(e.children.size());
+    std::vector<Env> child_env(e.children.size(), env);
     for (std::size_t i = 0; i < e.children.size(); ++i) x.children[i] = int(e.children[i]);
     if (e.kind == EKind::Constant) {
       x.kind = PK_CONSTANT;
@@ -463,6 +520,28 @@ PhaseGrammar compile_phase(const Grammar& g, unsigned root, unsigned max_depth, 
       x.native_kind = NK_VAR;
       x.node.kind = TK_LOCAL;
       x.node.operand = position;
+    } else if (e.kind == EKind::Bound) {
+      // A lexical read: the binder number the environment gives the name.
+      if (e.target >= env.size()) fail("bound reference outside its environment");
+      x.kind = PK_BOUND;
+      x.target = int(e.target);
+      x.native_kind = NK_REGION_VAR;
+      x.node.kind = TK_BOUND;
+      x.node.operand = env[e.target];
+    } else if (traversal) {
+      // Element, index and accumulator bind the three binder numbers above
+      // every visible one for the body (the last argument); the node keeps
+      // the first of them and the direction.
+      const int binder = env.empty() ? 0 : *std::max_element(env.begin(), env.end()) + 1;
+      if (binder + 2 >= 16 || e.region_argument != int(e.children.size()) - 1 || e.region_bindings.size() != 3)
+        fail("traversal capability");
+      x.kind = PK_OPERATION;
+      x.native_kind = e.node_kind;
+      x.node.kind = TK_TRAVERSE;
+      x.node.operand = binder;
+      x.node.opcode = e.direction;
+      for (int k = 0; k < 3; ++k) child_env.back().push_back(binder + k);
+      out.lets = true;
     } else if (e.kind == EKind::Primitive) {
       x.kind = PK_OPERATION;
       x.native_kind = e.node_kind;
@@ -498,12 +577,71 @@ PhaseGrammar compile_phase(const Grammar& g, unsigned root, unsigned max_depth, 
       fail("expression kind requires native fallback");
     }
     for (std::size_t i = 0; i < e.children.size(); ++i) {
-      x.child_scope[i] = static_cast<signed char>(env.size());
-      visit_expr(e.children[i], env);
+      if (child_env[i].size() > 15) fail("lexical environment capacity");
+      x.child_scope[i] = static_cast<signed char>(child_env[i].size());
+      visit_expr(e.children[i], child_env[i]);
     }
   };
-  visit_nt(root, Env(g.nts[root].scope.size()));
-  {  // identity scope for the root environment (binder numbers 0..n-1)
+  {  // the root environment numbers its scope 0..n-1
+    Env root_env;
+    for (std::size_t i = 0; i < g.nts[root].scope.size(); ++i) root_env.push_back(int(i));
+    visit_nt(root, root_env);
+  }
+  // Exported programs number their traversal binders after those of the
+  // first member and of the prototype it was exported against, so the
+  // engine admits traversals only where every program has the same number
+  // of them.
+  {
+    const auto is_traversal = [&](const CExpr& e) {
+      return e.kind == EKind::Primitive && (e.node_kind == NK_TRAVERSE || e.node_kind == NK_TRAVERSE_RANGE);
+    };
+    // Nonterminals that can derive a traversal (fixpoint over references).
+    std::vector<char> reach(g.nts.size(), 0);
+    std::function<bool(unsigned)> expr_reach = [&](unsigned id) -> bool {
+      const auto& e = g.exprs[id];
+      if (e.kind == EKind::Reference) return reach[e.target] != 0;
+      if (is_traversal(e)) return true;
+      for (auto c : e.children) if (expr_reach(c)) return true;
+      return false;
+    };
+    for (bool changed = true; changed;) {
+      changed = false;
+      for (std::size_t n = 0; n < g.nts.size(); ++n) {
+        if (reach[n]) continue;
+        for (auto p : g.nts[n].productions)
+          if (expr_reach(g.prods[p].expression)) { reach[n] = 1; changed = true; break; }
+      }
+    }
+    std::vector<int> nt_count(g.nts.size(), -2);  // -2 unvisited, -3 in progress
+    std::function<int(unsigned)> count_expr, count_nt;
+    count_expr = [&](unsigned id) -> int {
+      const auto& e = g.exprs[id];
+      if (e.kind == EKind::Reference) return count_nt(e.target);
+      int n = is_traversal(e) ? 1 : 0;
+      for (auto c : e.children) {
+        const int k = count_expr(c);
+        if (k < 0) return -1;
+        n += k;
+      }
+      return n;
+    };
+    count_nt = [&](unsigned id) -> int {
+      if (!reach[id]) return 0;
+      if (nt_count[id] == -3) return -1;  // a traversal under recursion: no fixed count
+      if (nt_count[id] != -2) return nt_count[id];
+      nt_count[id] = -3;
+      int n = -2;
+      for (auto p : g.nts[id].productions) {
+        const int k = count_expr(g.prods[p].expression);
+        if (k < 0 || (n != -2 && k != n)) { n = -1; break; }
+        n = k;
+      }
+      nt_count[id] = n < 0 ? -1 : n;
+      return nt_count[id];
+    };
+    const int k = count_nt(root);
+    if (k < 0) fail("traversal count that varies between programs");
+    out.traversals = k;
   }
   // Each production root must resolve to a distinct node (site annotation
   // recovers a production from its first node's origin).

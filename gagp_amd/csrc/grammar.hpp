@@ -58,6 +58,11 @@ struct CExpr {
   std::vector<std::uint32_t> scope_mapping;
   int node_kind = -1;        // lowering node of a primitive
   int builtin = -1;          // builtin id of a builtin primitive
+  // A traversal primitive: 1 forward, 2 reverse; its body argument and the
+  // names and types it binds there (element, index, accumulator).
+  int direction = 0;
+  int region_argument = -1;
+  std::vector<Binding> region_bindings;
 };
 struct CNt {
   std::string id;
@@ -134,6 +139,9 @@ struct PhaseGrammar {
   int root = -1, depth_count = 0, logical_count = 1;
   bool site_depth = false;
   bool statements = false, lets = false;
+  // Traversal nodes in every program of the entry nonterminal (the grammar
+  // admits traversals only where that count is the same for all programs).
+  int traversals = 0;
 };
 
 // compile_phase_grammar + variation flags + static normalization.

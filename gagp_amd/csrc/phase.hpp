@@ -553,7 +553,8 @@ constexpr int kFastStack = 16, kFastCode = 128, kFastPool = 128, kFastInputs = 4
 // `fast`: every instruction and operand is certified for the typed
 // straight-line path (no jumps, every value statically Int or Bool, integer
 // builtins only, within the kFast* limits).
-struct LowerResult { int status, code, constants, stack_bound, result_type, jumps, fast; };
+// `locals`: evaluator locals plus the hidden temporaries of traversals.
+struct LowerResult { int status, code, constants, stack_bound, result_type, jumps, fast, locals; };
 GHD inline bool fast_type(int tag) { return tag == TAG_INT || tag == TAG_BOOL; }
 // a0..a2: operand tags (unused ones ignored); passed by value so the callers'
 // frames stay in registers.
@@ -586,16 +587,40 @@ struct LocalFrames {
   GHD void set(int k, const Frame& x) { f[k] = x; }
 };
 
+// Lowering state of traversals (compile_phase_tree's temporaries, binder
+// environment and open traversals, phase_compile.cuh:241-276): hidden
+// temporaries are allocated after the evaluator locals and recycled LIFO
+// when their traversal completes; binder numbers map to the temporaries
+// holding element, index and accumulator inside a body.
+struct Walk {
+  int live_begin, xs, start, begin, limit, length, counter, element, index, state, loop_head, done_jump;
+  int saved_slot[3], saved_type[3];
+};
+template <bool kWalks> struct WalkStore {};
+template <> struct WalkStore<true> {
+  int live_count = 0, free_count = 0, walk_depth = 0;
+  signed char live[64], freed[64];
+  signed char env_slot[16];
+  unsigned char env_type[16];
+  Walk walks[8];
+};
+
 // Post-order lowering of an expression phase plus the program terminator,
 // with the reference's jump shapes for if/and/or. Constants are emitted in
 // first-use order (one pool entry per constant node). `OriginAt(i)` gives the
-// origin of node i in prefix order.
-template <class Frames, class OriginAt>
+// origin of node i in prefix order. kWalks adds lexical reads and the
+// traversal lowering of phase_compile.cuh:444-542,596-637,730-740 without
+// source fuel profiles: each emitted instruction costs one, no operand is
+// aliased and begin is a stored temporary.
+template <class Frames, class OriginAt, bool kWalks = false>
 GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, OriginAt origin_at, const std::uint8_t* local_types,
                                         int local_count, int expected_type, Ins* code, int code_capacity, Val* pool,
                                         int pool_capacity, Frames& frames) {
-  LowerResult out{ST_INVALID, 0, 0, 0, TAG_INVALID, 0, 0};
+  LowerResult out{ST_INVALID, 0, 0, 0, TAG_INVALID, 0, 0, local_count};
   if (t.n <= 0 || t.n > 4096) return out;
+  [[maybe_unused]] WalkStore<kWalks> ws;
+  if constexpr (kWalks)
+    for (int i = 0; i < 16; ++i) { ws.env_slot[i] = -1; ws.env_type[i] = TAG_INVALID; }
   bool fast = local_count <= kFastInputs;
   int depth = 0, cursor = 0, stack = 0, cconst = 0;
   int completed = TAG_INVALID;
@@ -608,6 +633,109 @@ GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, Origi
     return true;
   };
   const auto grow = [&](int by) { stack += by; if (stack > out.stack_bound) out.stack_bound = stack; };
+  // Traversal emission helpers: one unit of fuel per instruction.
+  const auto put = [&](int op, int a, int argc, int delta) -> bool {
+    if (!emit(op, a, argc, 1)) return false;
+    if (delta > 0) grow(delta);
+    else stack += delta;
+    return true;
+  };
+  const auto put_load = [&](int slot) { return put(OP_LOAD, slot, 0, 1); };
+  const auto put_store = [&](int slot) { return put(OP_STORE, slot, 0, -1); };
+  const auto put_int = [&](std::int64_t v) -> bool {
+    if (out.constants >= pool_capacity) { out.status = ST_CAPACITY; return false; }
+    pool[out.constants] = val_int(v);
+    if (!put(OP_PUSH_CONST, out.constants, 0, 1)) return false;
+    ++out.constants;
+    return true;
+  };
+  const auto put_jump = [&](int op, int delta) -> int {
+    const int at = out.code;
+    return put(op, 0, 0, delta) ? at : -1;
+  };
+  const auto temporary = [&]() -> int {
+    int slot = -1;
+    if constexpr (kWalks) {
+      if (ws.free_count) slot = ws.freed[--ws.free_count];
+      else if (out.locals >= 64) out.status = ST_CAPACITY;
+      else slot = out.locals++;
+      if (slot >= 0) ws.live[ws.live_count++] = static_cast<signed char>(slot);
+    }
+    return slot;
+  };
+  // Code after child `c` of traversal `w` (source, start, [begin, end,]
+  // initial accumulator, body) completed (walk_after without aliases).
+  const auto walk_after = [&](Walk& w, const PNode& node, int c) -> bool {
+    if constexpr (kWalks) {
+      const bool ranged = node.arity == 6, reverse = node.opcode == 2;
+      if (c == 0 && !put_store(w.xs)) return false;
+      if (c == 1 && !put_store(w.start)) return false;
+      if (ranged && c == 2 && !put_store(w.begin)) return false;
+      if (ranged && c == 3 && !put_store(w.limit)) return false;
+      if (c == (ranged ? 3 : 1)) {
+        const int checked[3] = {w.start, w.begin, w.limit};
+        for (int role = 0; role < (ranged ? 3 : 1); ++role)
+          if (!put_load(checked[role]) || !put(OP_CHECK_INT, 0, 0, 0) || !put_store(checked[role])) return false;
+        if (!put_load(w.xs) || !put(OP_CALL_BUILTIN, B_LEN, 1, 0) || !put_store(w.length)) return false;
+        if (ranged) {
+          for (int k = 0; k < 2; ++k) {  // clamp begin, then end, to [0, length]
+            const int slot = k ? w.limit : w.begin;
+            if (!put_load(slot) || !put_int(0) || !put(OP_LT, 0, 0, -1)) return false;
+            const int nonnegative = put_jump(OP_JMP_IF_FALSE, -1);
+            if (nonnegative < 0 || !put_int(0) || !put_store(slot)) return false;
+            const int clamped = put_jump(OP_JMP, 0);
+            if (clamped < 0) return false;
+            code[nonnegative].a = out.code;
+            if (!put_load(slot) || !put_load(w.length) || !put(OP_GT, 0, 0, -1)) return false;
+            const int above = put_jump(OP_JMP_IF_FALSE, -1);
+            if (above < 0 || !put_load(w.length) || !put_store(slot)) return false;
+            code[clamped].a = out.code;
+            code[above].a = out.code;
+          }
+        } else {
+          if (!put_int(0) || !put_store(w.begin)) return false;
+          if (!put_load(w.length) || !put_store(w.limit)) return false;
+        }
+      }
+      if (c == node.arity - 2) {
+        if (!put_store(w.state)) return false;
+        if (!put_load(reverse ? w.limit : w.begin)) return false;
+        if (reverse && (!put_int(1) || !put(OP_SUB, 0, 0, -1))) return false;
+        if (!put_store(w.counter)) return false;
+        w.loop_head = out.code;
+        if (!put_load(w.counter) || !put_load(reverse ? w.begin : w.limit) || !put(reverse ? OP_GE : OP_LT, 0, 0, -1))
+          return false;
+        w.done_jump = put_jump(OP_JMP_IF_FALSE, -1);
+        if (w.done_jump < 0) return false;
+        if (!put_load(w.xs) || !put_load(w.counter) || !put(OP_CALL_BUILTIN, B_INDEX, 2, -1)) return false;
+        if (!put_store(w.element)) return false;
+        if (!put_load(w.start) || !put_load(w.counter) || !put(OP_ADD, 0, 0, -1) || !put_store(w.index)) return false;
+        const int slots[3] = {w.element, w.index, w.state};
+        const int types[3] = {TAG_INT, TAG_INT, node.type};
+        for (int k = 0; k < 3; ++k) {
+          const int number = node.operand + k;
+          w.saved_slot[k] = ws.env_slot[number];
+          w.saved_type[k] = ws.env_type[number];
+          ws.env_slot[number] = static_cast<signed char>(slots[k]);
+          ws.env_type[number] = static_cast<unsigned char>(types[k]);
+        }
+      }
+      if (c == node.arity - 1) {
+        for (int k = 2; k >= 0; --k) {
+          ws.env_slot[node.operand + k] = static_cast<signed char>(w.saved_slot[k]);
+          ws.env_type[node.operand + k] = static_cast<unsigned char>(w.saved_type[k]);
+        }
+        if (!put_store(w.state)) return false;
+        if (!put_load(w.counter) || !put_int(1) || !put(reverse ? OP_SUB : OP_ADD, 0, 0, -1) || !put_store(w.counter))
+          return false;
+        if (!emit(OP_JMP, w.loop_head, 0, 1)) return false;
+        code[w.done_jump].a = out.code;
+        if (!put_load(w.state)) return false;
+      }
+      return true;
+    }
+    return false;
+  };
   for (;;) {
     if (!have_value) {
       if (cursor >= t.n) return out;
@@ -630,6 +758,41 @@ GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, Origi
         grow(1);
         completed = v.tag;
         have_value = true;
+      } else if (kWalks && node.kind == TK_BOUND) {
+        if constexpr (kWalks) {
+          const int b = node.operand;
+          if (node.arity != 0 || b < 0 || b >= 16 || ws.env_slot[b] < 0) return out;
+          completed = ws.env_type[b];
+          if (node.type != TAG_INVALID && node.type != completed) return out;
+          if (!emit(OP_LOAD, ws.env_slot[b], 0, node.fuel)) return out;
+          grow(1);
+          fast = false;
+          have_value = true;
+        }
+      } else if (kWalks && node.kind == TK_TRAVERSE) {
+        if constexpr (kWalks) {
+          if (node.arity != 4 && node.arity != 6) return out;
+          if (node.operand < 0 || node.operand + 2 >= 16) return out;
+          if (depth == 64 || ws.walk_depth == 8) { out.status = ST_CAPACITY; return out; }
+          Walk& w = ws.walks[ws.walk_depth++];
+          w.live_begin = ws.live_count;
+          // Temporaries in the native order; all stored, none aliased.
+          w.xs = temporary();
+          w.start = temporary();
+          w.begin = temporary();
+          w.limit = temporary();
+          w.length = temporary();
+          w.counter = temporary();
+          w.element = temporary();
+          w.index = temporary();
+          w.state = temporary();
+          if (w.xs < 0 || w.start < 0 || w.begin < 0 || w.limit < 0 || w.length < 0 || w.counter < 0 || w.element < 0 ||
+              w.index < 0 || w.state < 0)
+            return out;
+          fast = false;
+          frames.set(depth++, Frame{o, 0, -1, -1, stack, {TAG_INVALID, TAG_INVALID, TAG_INVALID}});
+          continue;
+        }
       } else if (node.kind == TK_LOCAL) {
         if (node.operand < 0 || node.operand >= local_count) return out;
         completed = local_types[node.operand];
@@ -666,6 +829,20 @@ GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, Origi
     else if (f.seen == 1) f.args[1] = std::uint8_t(completed);
     else if (f.seen == 2) f.args[2] = std::uint8_t(completed);
     ++f.seen;
+    if constexpr (kWalks) {
+      if (node.kind == TK_TRAVERSE) {
+        Walk& w = ws.walks[ws.walk_depth - 1];
+        const int c = f.seen - 1;
+        const int want = c == 0 ? TAG_INT_LIST : c >= node.arity - 2 ? node.type : TAG_INT;
+        if (completed != want || !walk_after(w, node, c)) return out;
+        if (f.seen < node.arity) { frames.set(depth - 1, f); have_value = false; continue; }
+        while (ws.live_count > w.live_begin) ws.freed[ws.free_count++] = ws.live[--ws.live_count];
+        --ws.walk_depth;
+        completed = node.type;
+        --depth;
+        continue;
+      }
+    }
     const bool branch = node.kind == TK_IF || node.kind == TK_AND || node.kind == TK_OR;
     if (branch && f.seen == 1) {
       if (completed != TAG_BOOL) return out;
@@ -724,11 +901,13 @@ GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, Origi
     --depth;
   }
 }
+template <bool kWalks = false>
 GHD inline LowerResult lower_phase(const PhaseView& g, const CRow& t, const std::uint8_t* local_types, int local_count,
                                    int expected_type, Ins* code, int code_capacity, Val* pool, int pool_capacity) {
   LocalFrames frames;
-  return lower_phase_with(g, t, [&](int i) { return t.origin[i]; }, local_types, local_count, expected_type, code, code_capacity,
-                          pool, pool_capacity, frames);
+  const auto origin_at = [&](int i) { return t.origin[i]; };
+  return lower_phase_with<LocalFrames, decltype(origin_at), kWalks>(g, t, origin_at, local_types, local_count, expected_type, code,
+                                                                    code_capacity, pool, pool_capacity, frames);
 }
 
 }  // namespace gagp_amd

@@ -435,6 +435,7 @@ __device__ inline std::uint64_t row_hash_w(const CRow& r) {
 
 // Lowering of one row by the whole wavefront (uniform), origins read from a
 // per-tile register by lane.
+template <bool kWalks = false>
 __device__ inline LowerResult lower_phase_w(const PhaseView& g, const CRow& t, const std::uint8_t* local_types, int local_count,
                                             int expected_type, Ins* code, int code_capacity, Val* pool, int pool_capacity) {
   WaveFrames frames;
@@ -447,8 +448,8 @@ __device__ inline LowerResult lower_phase_w(const PhaseView& g, const CRow& t, c
     }
     return rdlane(tile_origin, i - tile);
   };
-  return lower_phase_with(g, t, origin_at, local_types, local_count, expected_type, code, code_capacity, pool, pool_capacity,
-                          frames);
+  return lower_phase_with<WaveFrames, decltype(origin_at), kWalks>(g, t, origin_at, local_types, local_count, expected_type, code,
+                                                                   code_capacity, pool, pool_capacity, frames);
 }
 
 }  // namespace gagp_amd

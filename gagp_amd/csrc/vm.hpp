@@ -39,11 +39,14 @@ struct CaseData {
 };
 
 // Per-wave LDS stack, slot-major: payload[slot*64 + lane], tag[...], flag[...].
+// Slots [slots, slots + temps) hold the hidden locals after the evaluator
+// inputs (local index n_inputs + k in slot slots + k).
 struct WaveStack {
   std::int64_t* v;
   std::uint8_t* t;
   std::uint8_t* b;
   int slots;
+  int temps;
 };
 
 #if defined(__HIPCC__)
@@ -80,6 +83,7 @@ __device__ inline LaneResult vm_run(const ProgView& p, const CaseData& cd, int c
   // reached by a correct schedule and ends the wave with ValueError otherwise.
   const long long iteration_limit = 64LL * ((long long)fuel0 + 2LL * p.len + 2) + 64;
   long long iterations = 0;
+  for (int k = 0; k < st.temps; ++k) st.t[(st.slots + k) * 64 + lane] = TAG_INVALID;  // unassigned
   for (;;) {
     int at;
     if (p.has_jumps) {
@@ -122,11 +126,24 @@ __device__ inline LaneResult vm_run(const ProgView& p, const CaseData& cd, int c
         break;
       }
       case OP_LOAD: {
-        if (ins.a >= cd.n_inputs) { r.err = ERR_NAME; done = true; break; }
+        if (ins.a >= cd.n_inputs) {  // a hidden local
+          const int k = ins.a - cd.n_inputs;
+          const int at_t = (st.slots + k) * 64 + lane;
+          if (k >= st.temps || st.t[at_t] == TAG_INVALID) { r.err = ERR_NAME; done = true; break; }
+          st.v[(sp) * 64 + lane] = st.v[at_t]; st.t[(sp) * 64 + lane] = st.t[at_t]; st.b[(sp) * 64 + lane] = st.b[at_t];
+          break;
+        }
         const std::size_t at_in = std::size_t(ins.a) * cd.C + c;
         const std::uint8_t tag = cd.in_tag[at_in];
         if (tag == TAG_INVALID) { r.err = ERR_NAME; done = true; break; }
         st.v[(sp) * 64 + lane] = cd.in_val[at_in]; st.t[(sp) * 64 + lane] = tag; st.b[(sp) * 64 + lane] = cd.in_b[at_in];
+        break;
+      }
+      case OP_STORE: {  // lowering stores only hidden locals
+        const int k = ins.a - cd.n_inputs;
+        if (ins.a < cd.n_inputs || k >= st.temps) { r.err = ERR_VALUE; done = true; break; }
+        const int at_t = (st.slots + k) * 64 + lane;
+        st.v[at_t] = st.v[(sp - 1) * 64 + lane]; st.t[at_t] = st.t[(sp - 1) * 64 + lane]; st.b[at_t] = st.b[(sp - 1) * 64 + lane];
         break;
       }
       case OP_CHECK_INT:

@@ -12,7 +12,13 @@ Workloads:
   quintic   seeded quintic, coefficients ~ U{-5..5}; 256 train / 512 test,
             x ~ U{-50..50}; add/sub/mul, constants 0..9; 64 nodes.
   mixed     ``mixed_searches(n)``: polynomials of degree 2..5 with
-            log-uniform population and case counts (the batched workload).
+            log-uniform population and case counts (the batched workload);
+            with ``lists=True`` every second search is a list reduction.
+  lists     ``list_task(kind)``: a reduction of an Int list (sum, max, count
+            of elements > k, sum of the even elements), lengths 0..64 and
+            values ~ U{-1000..1000}; grammar: one traversal of the list whose
+            body nonterminal combines element, index and accumulator with
+            add/sub/max/mod, comparisons and if.
 """
 from __future__ import annotations
 
@@ -124,17 +130,136 @@ def quintic(directory: str, n_train: int = 256, n_test: int = 512, seed: int = 2
     return write_task(directory, "quintic", polynomial_grammar(list(range(10)), 64, 12), train, test)
 
 
+def _const(values) -> dict:
+    return {"constant": {"type": "Int", "values": [str(v) for v in values]}}
+
+
+def _op(signature: str, *args) -> dict:
+    return {"signature": signature, "args": list(args)}
+
+
+def list_grammar(constants, max_nodes: int = 40, max_depth: int = 11, fuel: int = 4096, with_mod: bool = True,
+                 fold: str = "traverse", nested: bool = False) -> dict:
+    """One traversal of the list ``xs`` (``fold``: traverse,
+    traverse_reverse, traverse_range or traverse_range_reverse; the ranged
+    ones take begin ~ {0, 1, -1} and end ~ {64, 3, -2}); its body
+    ``List.Step`` sees the element, its index and the accumulator. With
+    ``nested`` the initial accumulator is itself a forward traversal
+    (``List.Inner``), so every program has two."""
+    step, cond = {"ref": "List.Step"}, {"ref": "List.Cond"}
+    alternatives = [
+        ("add", _op("add(Int,Int)->Int", step, step)),
+        ("sub", _op("sub(Int,Int)->Int", step, step)),
+        ("max", _op("max(Int,Int)->Int", step, step)),
+        ("if", _op("if(Bool,Int,Int)->Int", cond, step, step)),
+        ("element", {"bound": "element"}),
+        ("index", {"bound": "index"}),
+        ("acc", {"bound": "acc"}),
+        ("const", _const(constants)),
+    ]
+    if with_mod:
+        alternatives.append(("mod", _op("mod(Int,Int)->Int", step, step)))
+    scope = [{"name": "element", "type": "Int"}, {"name": "index", "type": "Int"}, {"name": "acc", "type": "Int"}]
+    ranged = fold.startswith("traverse_range")
+    bounds = [_const([0, 1, -1]), _const([64, 3, -2])] if ranged else []
+    body = "5" if ranged else "3"
+    init = ([{"id": "inner", "weight": 1, "expression": {
+                "signature": "traverse(IntList,Int,Int,Int)->Int",
+                "args": [{"input": "xs"}, _const([0]), _const([0]), {"ref": "List.Inner"}],
+                "bind": {"3": ["e2", "i2", "a2"]}}}] if nested else
+            [{"id": "const", "weight": 1, "expression": _const(constants)}])
+    extra = []
+    if nested:
+        inner = [{"name": "e2", "type": "Int"}, {"name": "i2", "type": "Int"}, {"name": "a2", "type": "Int"}]
+        extra = [{"id": "List.Inner", "type": "Int", "scope": inner, "alternatives": [
+            {"id": "add", "weight": 1, "expression": _op("add(Int,Int)->Int", {"bound": "a2"}, {"bound": "e2"})},
+            {"id": "max", "weight": 1, "expression": _op("max(Int,Int)->Int", {"bound": "a2"}, {"bound": "e2"})},
+            {"id": "index", "weight": 1, "expression": {"bound": "i2"}}]}]
+    return {
+        "format_version": "grammar-definition-v2",
+        "entry": {"nonterminal": "List.Reduce", "type": "Int"},
+        "inputs": [{"name": "xs", "type": "IntList"}],
+        "search_limits": {"max_nodes": max_nodes, "max_depth": max_depth},
+        "execution_limits": {"fuel": fuel},
+        "nonterminals": [
+            {"id": "List.Reduce", "type": "Int", "scope": [], "alternatives": [
+                {"id": "fold", "weight": 1, "expression": {
+                    "signature": "%s(IntList,Int,%sInt,Int)->Int" % (fold, "Int,Int," if ranged else ""),
+                    "args": [{"input": "xs"}, _const([0])] + bounds + [{"ref": "List.Init"}, step],
+                    "bind": {body: ["element", "index", "acc"]}}}]},
+            {"id": "List.Init", "type": "Int", "scope": [], "alternatives": init},
+            {"id": "List.Step", "type": "Int", "scope": scope, "alternatives": [
+                {"id": name, "weight": 1, "expression": e} for name, e in alternatives]},
+            {"id": "List.Cond", "type": "Bool", "scope": scope, "alternatives": [
+                {"id": "gt", "weight": 1, "expression": _op("gt(Int,Int)->Bool", step, step)},
+                {"id": "eq", "weight": 1, "expression": _op("eq(Int,Int)->Bool", step, step)}]},
+        ] + extra,
+    }
+
+
+LIST_KINDS = ("sum", "max", "count_gt", "sum_even")
+
+
+def _list_cases(rng: SplitMix64, kind: str, k: int, n: int, max_len: int):
+    cases = []
+    for _ in range(n):
+        xs = [rng.uniform_int(-1000, 1000) for _ in range(rng.uniform_int(0, max_len))]
+        if kind == "sum":
+            y = sum(xs)
+        elif kind == "max":
+            y = max(xs) if xs else -1001
+        elif kind == "count_gt":
+            y = sum(1 for x in xs if x > k)
+        else:
+            y = sum(x for x in xs if x % 2 == 0)
+        cases.append({"inputs": {"xs": {"type": "int_list", "value": xs}}, "expected": {"type": "int", "value": y}})
+    return cases
+
+
+def list_task(directory: str, kind: str, seed: int, n_train: int = 200, n_test: int = 400, max_len: int = 64,
+              fold: str = "traverse", nested: bool = False, fuel: int = 4096) -> str:
+    """A list reduction (``LIST_KINDS``); count_gt draws k ~ U{-100..100},
+    the max of an empty list is -1001."""
+    rng = SplitMix64(seed)
+    k = rng.uniform_int(-100, 100)
+    train = _list_cases(rng, kind, k, n_train, max_len)
+    test = _list_cases(rng, kind, k, n_test, max_len)
+    constants = {"sum": [0, 1, 2], "max": [0, 1, -1001], "count_gt": [0, 1, k], "sum_even": [0, 1, 2]}[kind]
+    return write_task(directory, "list_%s" % kind, list_grammar(constants, fuel=fuel, fold=fold, nested=nested), train, test)
+
+
+def sum_even(directory: str, seed: int = 3) -> str:
+    """The loop-heavy search: sum of the even elements of an Int list,
+    200 train / 400 test cases, lengths 0..50, values ~ U{-1000..1000}."""
+    return list_task(directory, "sum_even", seed, 200, 400, 50)
+
+
 def log_uniform_int(rng: SplitMix64, lo: int, hi: int) -> int:
     v = math.exp(math.log(lo) + rng.unit() * (math.log(hi) - math.log(lo)))
     return max(lo, min(hi, int(round(v))))
 
 
-def mixed_searches(directory: str, n: int = 64, seed: int = 4, p_range=(256, 4096), c_range=(16, 1024)):
+def mixed_searches(directory: str, n: int = 64, seed: int = 4, p_range=(256, 4096), c_range=(16, 1024), lists: bool = False):
     """Independent polynomial searches of degree 2..5 with log-uniform
-    population and train-case counts. Returns [(task_path, population)]."""
+    population and train-case counts; with ``lists`` every odd-numbered one
+    is a list reduction (sum, max or count > k; lengths 0..64) instead.
+    Returns [(task_path, population)]."""
     rng = SplitMix64(seed)
     out = []
     for i in range(n):
+        if lists and i % 2 == 1:
+            kind = LIST_KINDS[rng.bounded(3)]
+            population = log_uniform_int(rng, *p_range)
+            cases = log_uniform_int(rng, *c_range)
+            sub = SplitMix64(rng.next())
+            k = sub.uniform_int(-100, 100)
+            train = _list_cases(sub, kind, k, cases, 64)
+            test = _list_cases(sub, kind, k, 64, 64)
+            constants = {"sum": [0, 1, 2], "max": [0, 1, -1001], "count_gt": [0, 1, k]}[kind]
+            task = write_task(os.path.join(directory, "s%03d" % i), "mixed_%03d_%s" % (i, kind), list_grammar(constants),
+                              train, test)
+            out.append((task, population))
+            continue
         degree = 2 + rng.bounded(4)
         coeffs = [rng.uniform_int(-5, 5) for _ in range(degree + 1)]
         if coeffs[-1] == 0:
@@ -150,12 +275,12 @@ def mixed_searches(directory: str, n: int = 64, seed: int = 4, p_range=(256, 409
 
 
 def served_requests(directory: str, n: int = 60, seed: int = 5, rate: float = 20.0, p_range=(256, 16384),
-                    c_range=(16, 2048), g_range=(32, 256)):
+                    c_range=(16, 2048), g_range=(32, 256), lists: bool = False):
     """A Poisson request stream over mixed polynomial searches: arrival
     offsets with exponential gaps of mean 1/rate seconds, generation caps
     uniform in g_range. Returns [(task_path, population, generations,
     seed, arrival_s)]."""
-    tasks = mixed_searches(directory, n=n, seed=seed, p_range=p_range, c_range=c_range)
+    tasks = mixed_searches(directory, n=n, seed=seed, p_range=p_range, c_range=c_range, lists=lists)
     rng = SplitMix64(seed * 7919 + 1)
     t, out = 0.0, []
     for i, (task, population) in enumerate(tasks):

@@ -7,13 +7,23 @@ under tests/golden/runs/<name>/. It also copies the reference's bytecode
 fixture files (cpp/tests/fixtures/runtime/*.json: programs, cases and
 expected values or error names) into tests/golden/bytecode/.
 
+It also writes tests/golden/traversal/<task>.json: the reference's
+`gagp eval --split train` of the programs tests/host_lowering.cpp exports for
+the list tasks (seeded random traversal programs), which
+tests/test_host_lowering.py compares with the engine's lowering run on the
+oracle VM.
+
     python tests/golden/make_golden.py            (from the repository root)
+    python tests/golden/make_golden.py --only 'list_'   (runs whose name matches)
 """
+import argparse
 import json
 import os
+import re
 import shutil
 import subprocess
 import sys
+import tempfile
 
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 sys.path.insert(0, ROOT)
@@ -45,7 +55,60 @@ RUNS = {
     # recorded best has no wrong train case (18 and 10 in the two histories)
     "polynomial_p1024_g18_s0": ("polynomial", 1024, 18, 0, False),
     "poly_c1_p1024_g10_s0": ("poly_c1", 1024, 10, 0, False),
+    # traversal grammars: the reference's list-sum example, the loop-heavy
+    # sum of even elements at its full size (P 8192, G 64, 200 cases) and
+    # smaller runs over the reverse, ranged, nested and low-fuel variants
+    "list_sum_example_p256_g8_s1": ("list-sum", 256, 8, 1, False),
+    "list_sum_even_p8192_g64_s0": ("list_sum_even", 8192, 64, 0, False),
+    "list_sum_even_p512_g10_s5": ("list_sum_even", 512, 10, 5, True),
+    "list_count_gt_rev_p300_g8_s6": ("list_count_gt_rev", 300, 8, 6, False),
+    "list_range_nested_p256_g8_s7": ("list_range_nested", 256, 8, 7, False),
+    "list_lowfuel_p400_g8_s8": ("list_lowfuel", 400, 8, 8, False),
+    "list_max_range_p200_g6_s9": ("list_max_range", 200, 6, 9, False),
+    "list_count_gt_rev_gen_e1_p256_g8_s2": ("list_count_gt_rev", 256, 8, 2, False, "generational_e1.json"),
 }
+# Tasks whose random traversal programs are checked against `gagp eval`.
+TRAVERSAL_TASKS = ["list-sum", "list_count_gt_rev", "list_range_nested", "list_lowfuel", "list_max_range", "list_sum_even"]
+
+
+def list_tasks():
+    workloads.sum_even(os.path.join(TASKS, "list_sum_even"), 3)
+    workloads.list_task(os.path.join(TASKS, "list_count_gt_rev"), "count_gt", 8, 64, 32, 40, fold="traverse_reverse")
+    workloads.list_task(os.path.join(TASKS, "list_range_nested"), "count_gt", 13, 64, 32, 40, fold="traverse_range_reverse",
+                        nested=True, fuel=420)
+    workloads.list_task(os.path.join(TASKS, "list_lowfuel"), "sum_even", 12, 64, 32, 40, fuel=300)
+    workloads.list_task(os.path.join(TASKS, "list_max_range"), "max", 9, 48, 32, 40, fold="traverse_range")
+
+
+def traversal_goldens(n=40):
+    """Builds tests/host_lowering.cpp, exports n programs per task and keeps
+    the reference's evaluation of each."""
+    out_dir = os.path.join(GOLDEN, "traversal")
+    os.makedirs(out_dir, exist_ok=True)
+    with tempfile.TemporaryDirectory() as tmp:
+        driver = os.path.join(tmp, "host_lowering")
+        build = os.path.join(ROOT, "gagp_amd", "_build")
+        subprocess.run(["g++", "-O1", "-std=c++17", "-I" + os.path.join(ROOT, "gagp_amd", "csrc"),
+                        os.path.join(ROOT, "tests", "host_lowering.cpp"), os.path.join(build, "grammar.o"),
+                        os.path.join(build, "task.o"), os.path.join(build, "export.o"),
+                        os.path.join(ROOT, "oracle", "liboracle.so"), "-Wl,-rpath," + os.path.join(ROOT, "oracle"),
+                        "-o", driver], check=True)
+        for task_dir in TRAVERSAL_TASKS:
+            task = os.path.join(TASKS, task_dir, "task.json")
+            lines = subprocess.run([driver, task, str(n)], check=True, capture_output=True, text=True).stdout.splitlines()
+            programs = []
+            for line in lines:
+                row = json.loads(line)
+                path = os.path.join(tmp, "p.json")
+                with open(path, "w") as f:
+                    json.dump({"best": {"program": row["program"]}}, f)
+                r = subprocess.run([REF, "eval", task, path, "--split", "train"], check=True, capture_output=True, text=True)
+                e = json.loads(r.stdout)
+                programs.append({"i": row["i"], "text": row["text"], "cases": e["cases"], "mean_loss": e["mean_loss"],
+                                 "wrong": e["wrong"], "runtime_errors": e["runtime_errors"]})
+            with open(os.path.join(out_dir, task_dir + ".json"), "w") as f:
+                json.dump({"task": os.path.relpath(task, GOLDEN), "n": n, "programs": programs}, f, indent=1)
+            print("traversal golden", task_dir, len(programs))
 
 
 def run_reference(task, population, generations, seed, out, save_all, search=None):
@@ -62,16 +125,27 @@ def run_reference(task, population, generations, seed, out, save_all, search=Non
 
 
 def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--only", default=None, help="regenerate only the runs whose name matches this regex")
+    args = ap.parse_args()
     if not os.path.exists(REF):
         sys.exit("build the reference first: make -f oracle/ref.mk -j8")
     workloads.poly_c1(os.path.join(TASKS, "poly_c1"))
     workloads.quintic(os.path.join(TASKS, "quintic"))
+    list_tasks()
     mixed = workloads.mixed_searches(os.path.join(TASKS, "mixed"), n=8, seed=4, p_range=(64, 512), c_range=(16, 256))
+    mixedl = workloads.mixed_searches(os.path.join(TASKS, "mixedl"), n=6, seed=6, p_range=(64, 512), c_range=(16, 256),
+                                      lists=True)
     runs = dict(RUNS)
     for i, (task, population) in enumerate(mixed):
         runs["mixed_%d" % i] = (os.path.relpath(os.path.dirname(task), TASKS), population, 8, 11 + i, False)
-    manifest = {}
+    for i, (task, population) in enumerate(mixedl):
+        runs["mixedl_%d" % i] = (os.path.relpath(os.path.dirname(task), TASKS), population, 8, 31 + i, False)
+    manifest_path = os.path.join(GOLDEN, "runs", "manifest.json")
+    manifest = json.load(open(manifest_path)) if args.only and os.path.exists(manifest_path) else {}
     for name, spec in runs.items():
+        if args.only and not re.search(args.only, name):
+            continue
         task_dir, population, generations, seed, save_all = spec[:5]
         search = os.path.join(GOLDEN, "search", spec[5]) if len(spec) > 5 else None
         out = os.path.join(GOLDEN, "runs", name)
@@ -83,8 +157,10 @@ def main():
         if search:
             manifest[name]["search"] = os.path.relpath(search, GOLDEN)
         print("golden", name)
-    with open(os.path.join(GOLDEN, "runs", "manifest.json"), "w") as f:
+    with open(manifest_path, "w") as f:
         json.dump(manifest, f, indent=1, sort_keys=True)
+    if not args.only or re.search(args.only, "traversal"):
+        traversal_goldens()
     if os.path.isdir(REFERENCE_FIXTURES):
         dst = os.path.join(GOLDEN, "bytecode")
         os.makedirs(dst, exist_ok=True)

@@ -1,0 +1,93 @@
+// Test driver (CPU only): lowers seeded random programs of a task's grammar
+// with the engine's host lowering (the kWalks instantiation, which the
+// device kernels share), scores the bytecode on the task's train cases with
+// the C oracle VM, and prints per program one JSON line with the score and
+// the exported AST. tests/test_host_lowering.py compares those lines with the
+// reference's own evaluation (`gagp eval --split train`) of the same ASTs.
+//
+//   host_lowering TASK.json N
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+#include "export.hpp"
+#include "grammar.hpp"
+#include "phase.hpp"
+#include "task.hpp"
+
+extern "C" {
+// oracle/vm_oracle.c
+typedef struct { std::uint8_t op, has_a, has_b, pad; std::int32_t a, b; } OIns;
+typedef struct {
+  const OIns* code; std::int32_t code_len;
+  const std::uint32_t* fuel;
+  const gagp_amd::Val* consts; std::int32_t n_consts;
+  std::int32_t n_locals;
+} OProg;
+void orc_eval_fitness(const OProg* programs, int n_programs, const gagp_amd::Val* inputs, const std::uint8_t* set, int n_cases,
+                      int n_inputs, const gagp_amd::Val* answers, const std::int64_t* arena, int fuel, double penalty, int lanes,
+                      int metric, int proximity, double* fitness, std::uint32_t* wrong, std::uint32_t* counts);
+}
+
+using namespace gagp_amd;
+
+int main(int argc, char** argv) {
+  if (argc != 3) { std::fprintf(stderr, "usage: host_lowering TASK.json N\n"); return 2; }
+  const Task task = load_task(argv[1]);
+  const int n = std::atoi(argv[2]);
+  const Grammar g = load_grammar_file(task.grammar_path);
+  const Cases cases = load_cases(task.train_path);
+  const Layout layout = whole_program_layout(g);
+  const PhaseGrammar ph = compile_phase(g, g.entry, g.max_depth, 7, cases.input_names);
+  const PhaseView gv{ph.exprs.data(), int(ph.exprs.size()), ph.prods.data(),  int(ph.prods.size()),  ph.nts.data(),
+                     int(ph.nts.size()), ph.domains.data(), int(ph.domains.size()), ph.values.data(), int(ph.values.size()),
+                     ph.minimum.data(), ph.depth_count,     ph.logical_count,     ph.site_depth ? 1 : 0, ph.root};
+  const int R = std::min<int>(kPhaseCapacity, int(layout.max_variable_nodes));
+  const std::size_t cap = std::size_t(R);
+  std::vector<int> origin(cap), site(cap);
+  std::vector<Val> cval(cap);
+  // The engine's per-program capacities.
+  std::vector<Ins> code(std::size_t(4 * R + 8 + 64 * ph.traversals));
+  std::vector<Val> pool(std::size_t(2 * R + 8 + 8 * ph.traversals));
+  for (int i = 0; i < n; ++i) {
+    GenResult gen{ST_INVALID, 0, 0, 0};
+    for (int attempt = 0; gen.status != ST_OK && attempt < 64; ++attempt)
+      gen = generate_phase(gv, gv.root, 0, int(layout.slot_max_depth), R, 0x9e3779b97f4a7c15ull * std::uint64_t(i + 1) + attempt,
+                           origin.data(), cval.data(), site.data(), 2 + i % 6);
+    if (gen.status != ST_OK) continue;
+    const CRow row{origin.data(), site.data(), cval.data(), gen.nodes, gen.constants};
+    const LowerResult lr = lower_phase<true>(gv, row, cases.input_types.data(), int(cases.input_types.size()),
+                                             g.nts[g.entry].type, code.data(), int(code.size()), pool.data(), int(pool.size()));
+    if (lr.status != ST_OK) { std::printf("{\"i\":%d,\"lowering_status\":%d}\n", i, lr.status); continue; }
+    std::vector<OIns> oc;
+    std::vector<std::uint32_t> fuel;
+    for (int k = 0; k < lr.code; ++k) {
+      const Ins& x = code[std::size_t(k)];
+      const bool has_a = x.op == OP_PUSH_CONST || x.op == OP_LOAD || x.op == OP_STORE || x.op == OP_JMP ||
+                         x.op == OP_JMP_IF_FALSE || x.op == OP_JMP_IF_TRUE || x.op == OP_CALL_BUILTIN;
+      oc.push_back(OIns{x.op, std::uint8_t(has_a), std::uint8_t(x.op == OP_CALL_BUILTIN), 0, x.a, x.argc});
+      fuel.push_back(x.fuel);
+    }
+    const OProg p{oc.data(), std::int32_t(oc.size()), fuel.data(), pool.data(), lr.constants, lr.locals};
+    double fit = 0;
+    std::uint32_t wrong = 0, counts[3] = {0, 0, 0};
+    orc_eval_fitness(&p, 1, cases.inputs.data(), cases.set.data(), cases.n, int(cases.input_names.size()), cases.expected.data(),
+                     cases.arena.data(), int(g.fuel), task.case_cap, 512, task.policy.metric, task.policy.proximity, &fit, &wrong,
+                     counts);
+    const Ast ast = export_whole_program(g, ph, std::vector<int>(origin.begin(), origin.begin() + gen.nodes),
+                                         std::vector<Val>(cval.begin(), cval.begin() + gen.constants));
+    json::Value line = json::make_object();
+    line.obj["i"] = json::make_number(i);
+    line.obj["fitness"] = json::make_number(fit);
+    line.obj["wrong"] = json::make_number(wrong);
+    line.obj["runtime_errors"] = json::make_number(counts[1]);
+    line.obj["timeouts"] = json::make_number(counts[2]);
+    line.obj["code"] = json::make_number(lr.code);
+    line.obj["locals"] = json::make_number(lr.locals);
+    line.obj["text"] = json::make_string(program_text(ast));
+    line.obj["program"] = ast_json(ast);
+    std::printf("%s\n", json::canonical(line).c_str());
+  }
+  return 0;
+}

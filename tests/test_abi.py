@@ -41,11 +41,11 @@ def test_errors_are_reported_without_a_device():
 def test_batch_rejects_unsupported_grammar_on_the_host():
     b = gagp_amd.Batch()
     try:
-        b.add(os.path.join(GOLDEN, "tasks", "list-sum", "task.json"))
+        b.add(os.path.join(GOLDEN, "tasks", "let-sum", "task.json"))
     except gagp_amd.GagpError as e:
         assert e.status == gagp_amd.GAGP_ERR_UNSUPPORTED
     else:
-        raise AssertionError("list-sum uses region-binding primitives and must be declined")
+        raise AssertionError("let-sum binds a let region, which the engine declines")
 
 
 def test_cli_usage_and_exit_codes():

@@ -13,7 +13,8 @@ from _util import GOLDEN, strip_timing
 
 pytestmark = pytest.mark.gpu
 MANIFEST = json.load(open(os.path.join(GOLDEN, "runs", "manifest.json")))
-SMALL = sorted(n for n in MANIFEST if not n.startswith("quintic_p4096"))
+FULL = ("quintic_p4096_g128_s0", "list_sum_even_p8192_g64_s0")  # the full-size single searches
+SMALL = sorted(n for n in MANIFEST if n not in FULL)
 
 
 def golden(name):
@@ -35,8 +36,8 @@ def test_search_matches_reference(name):
     assert strip_timing(b.result(0)) == strip_timing(golden(name))
 
 
-def test_full_quintic_matches_reference():
-    name = "quintic_p4096_g128_s0"
+@pytest.mark.parametrize("name", FULL)
+def test_full_size_search_matches_reference(name):
     print("search", name, flush=True)
     b = gagp_amd.Batch()
     add(b, name)
@@ -80,6 +81,6 @@ def test_cli_drop_in(tmp_path):
 
 
 def test_cli_rejects_unsupported_with_exit_2(tmp_path):
-    r = subprocess.run([gagp_amd.CLI_PATH, "run", os.path.join(GOLDEN, "tasks", "list-sum", "task.json"), "--quiet"],
+    r = subprocess.run([gagp_amd.CLI_PATH, "run", os.path.join(GOLDEN, "tasks", "let-sum", "task.json"), "--quiet"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2 and "gagp:" in r.stderr

@@ -30,7 +30,8 @@ def test_staggered_requests_match_reference():
     print("serve: staggered requests", flush=True)
     server = gagp_amd.Server(budget_bytes=8 << 30, poll_rounds=2)
     plain = ["polynomial_p512_g32_s7", "quintic_p512_g16_s3", "mixed_0", "mixed_3", "polynomial_gen_e1_p256_g12_s3",
-             "polynomial_steady_b128_rs3_p512_g16_s4", "polynomial_p100_g10_s5"]
+             "polynomial_steady_b128_rs3_p512_g16_s4", "polynomial_p100_g10_s5", "list_count_gt_rev_p300_g8_s6",
+             "list_range_nested_p256_g8_s7", "mixedl_1"]
     tickets = {name: submit(server, name, 0.004 * i) for i, name in enumerate(plain)}
     # Early exit: the 64-generation requests stop where the reference's
     # shorter runs end.
@@ -66,7 +67,7 @@ def test_budget_admits_one_search_at_a_time():
 def test_unsupported_request_fails_alone():
     print("serve: unsupported request", flush=True)
     server = gagp_amd.Server(poll_rounds=2)
-    bad = server.submit(os.path.join(GOLDEN, "tasks", "list-sum", "task.json"), arrival=0.0)
+    bad = server.submit(os.path.join(GOLDEN, "tasks", "let-sum", "task.json"), arrival=0.0)
     good = submit(server, "polynomial_p100_g10_s5", 0.0)
     server.run()
     assert server.outcome(bad)["status"] == gagp_amd.GAGP_ERR_UNSUPPORTED
