@@ -129,6 +129,48 @@ def cpu_baseline(generations):
                       "history identical to the device run: %s" % (POPULATION, generations, cases, search_s, wall, same)}
 
 
+LOOP_TASK = os.path.join(ROOT, "tests", "golden", "tasks", "list_sum_even", "task.json")
+LOOP_GOLDEN = os.path.join(ROOT, "tests", "golden", "runs", "list_sum_even_p8192_g64_s0", "result.json")
+
+
+def loop_heavy(cpu_generations):
+    """The loop-heavy search: the sum of the even elements of an Int list
+    (200 train / 400 test cases, lengths 0-50, values in [-1000,1000]; one
+    traversal per program with mod/eq/if/add/sub/max in its body), population
+    8192, 64 generations, seed 0, on the device, checked against the
+    reference's golden result; and the reference CPU engine on the first
+    `cpu_generations` generations of the same search (one thread), or None."""
+    b = gagp_amd.Batch()
+    b.add(LOOP_TASK, None, 8192, 64, 0)
+    t0 = time.perf_counter()
+    b.prepare()
+    t1 = time.perf_counter()
+    b.step(b.max_rounds())
+    b.finish()
+    t2 = time.perf_counter()
+    cases = b.evaluated_program_cases(0)
+    same = strip_timing(b.result(0)) == strip_timing(json.load(open(LOOP_GOLDEN)))
+    out = {"workload": "sum of the even elements of an Int list, population 8192, 64 generations, 200 train cases",
+           "value": round(cases / (t2 - t0), 1), "unit": "program-case evals/s", "seconds": round(t2 - t0, 3),
+           "prepare_s": round(t1 - t0, 3), "ms_per_generation": round((t2 - t1) * 1e3 / 64, 3),
+           "identical_to_reference": same, "cpu_reference": None}
+    if cpu_generations and os.path.exists(REF_GAGP):
+        with tempfile.TemporaryDirectory() as tmp:
+            subprocess.run([REF_GAGP, "run", LOOP_TASK, "--engine", "cpu", "--population", "8192", "--generations",
+                            str(cpu_generations), "--seed", "0", "--out", tmp, "--quiet"], check=True)
+            ref = json.load(open(os.path.join(tmp, "result.json")))
+        c = gagp_amd.Batch()
+        c.add(LOOP_TASK, None, 8192, cpu_generations, 0)
+        c.run()
+        ref_cases = c.evaluated_program_cases(0)
+        search_s = ref["time_ms"]["search"] / 1e3
+        out["cpu_reference"] = {
+            "value": round(ref_cases / search_s, 1), "unit": "program-case evals/s", "cores": 1, "kind": "reference",
+            "sample": "first %d generations: %d program-cases in %.2f s of search time" % (cpu_generations, ref_cases, search_s)}
+        out["vs_cpu_reference"] = round(out["value"] / out["cpu_reference"]["value"], 1)
+    return out
+
+
 def percentile(values, q):
     v = sorted(values)
     return v[min(len(v) - 1, int(round(q / 100.0 * (len(v) - 1))))]
@@ -229,7 +271,9 @@ def serving(n, rate):
 
 
 def strip_timing(r):
-    out = {k: v for k, v in r.items() if k != "time_ms"}
+    """The result.json members that must equal the reference engine's (its
+    engine name, task path and timings are not compared)."""
+    out = {k: r[k] for k in ("search", "best", "test", "population", "generations", "seed", "train_cases", "format") if k in r}
     out["history"] = [{k: v for k, v in h.items() if not k.endswith("_ms")} for h in r["history"]]
     return out
 
@@ -256,6 +300,7 @@ def main():
     ap.add_argument("--copy-mib", type=int, default=0, help="run a streaming kernel of this size first (counter scale)")
     ap.add_argument("--batch-searches", type=int, default=64, help="searches of the batching comparison (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=60, help="requests of the serving comparison (0: skip)")
+    ap.add_argument("--no-loop-heavy", dest="loop_heavy", action="store_false", help="skip the loop-heavy search")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: each search runs on a single device and this build has no multi-GPU "
@@ -278,6 +323,7 @@ def main():
     psteps = min(args.steps, 16)
     ms, counts = profile(args.warmup, psteps)
     roof, kernels, vm = roofline(ms, counts, psteps)
+    loop = loop_heavy(0 if args.no_cpu else 2) if args.loop_heavy else None
     batch = batching(args.batch_searches, 64) if args.batch_searches else None
     serve = serving(args.serve_requests, 20.0) if args.serve_requests else None
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_generations)
@@ -289,7 +335,8 @@ def main():
         "config": {"workload": "quintic symbolic regression, population 4096, 128 generations, 256 train cases, "
                                "default resident steady-state search; one step = one generation",
                    "population": POPULATION, "generations": GENERATIONS, "train_cases": TRAIN_CASES},
-        "roofline": roof, "cpu_baseline": cpu, "kernels": kernels, "vm": vm, "batching": batch, "serving": serve,
+        "roofline": roof, "cpu_baseline": cpu, "kernels": kernels, "vm": vm, "loop_heavy": loop, "batching": batch,
+        "serving": serve,
     }
     if cpu:
         line["vs_cpu_reference"] = round(value / cpu["value"], 2)

@@ -90,8 +90,8 @@ struct DSearch {
   GAS Val* pool;
   GAS int *code_len, *pool_len, *code_jumps;
   GAS std::uint64_t* child_hash;
-  // pending[B]: children to score, general ones from the front and vm_fast
-  // ones from the back; n_pending[2] counts each.
+  // pending[2B]: children to score, general ones from the front of [0, B),
+  // vm_fast ones from its back, vm_loop ones in [B, 2B); n_pending[3] counts each.
   GAS int *pending, *n_pending;
   GAS std::uint64_t* table_key;  // genotype hash | 1, 0 = empty
   GAS int* table;                // a member with that hash
@@ -177,8 +177,8 @@ __device__ inline void lower_into_w(const PhaseView& g, const DSearch& d, const 
     if (!ok) raise(d, 200 + r.status);
     d.code_len[slot] = ok ? r.code : 0;
     d.pool_len[slot] = ok ? r.constants : 0;
-    // bit 0: jumps; bit 1: typed straight-line (vm_fast); bit 2: Bool result
-    d.code_jumps[slot] = r.jumps | (r.fast ? 2 : 0) | (r.result_type == TAG_BOOL ? 4 : 0);
+    // bit 0: jumps; bit 1: typed straight-line (vm_fast); bit 2: Bool result; bit 3: typed looping (vm_loop)
+    d.code_jumps[slot] = r.jumps | (r.fast ? 2 : 0) | (r.result_type == TAG_BOOL ? 4 : 0) | (r.loop ? 8 : 0);
   }
 }
 constexpr int kVaryWaves = 4;  // wavefronts (rows) per block of the row kernels
@@ -264,6 +264,16 @@ __global__ void k_index(DSearch* all, const BlockMap* map) {
   }
   raise(d, 400);
 }
+// The evaluation kernel that scores a program: 1 the typed straight-line
+// path, 2 the typed looping path (both with an LDS score buffer, so only
+// where the cases fit it), 0 the general VM.
+__device__ inline int eval_class(int code_flags, int C) {
+  if (C > kScoreCases) return 0;
+  return (code_flags & 2) ? 1 : (code_flags & 8) ? 2 : 0;
+}
+// Pending list of a round: general children from the front of [0, B), the
+// straight-line ones from its back, the looping ones in [B, 2B).
+__device__ inline int pending_at(int B, int kind, int i) { return kind == 0 ? i : kind == 1 ? B - 1 - i : B + i; }
 __global__ void k_reuse(DSearch* all, const BlockMap* map) {
   const BlockMap bm = map[blockIdx.x];
   const DSearch& d = all[bm.search];
@@ -295,8 +305,8 @@ __global__ void k_reuse(DSearch* all, const BlockMap* map) {
     if (d.grammar_sites) { for (int i = 0; i < child.n; ++i) n += child.site[i] >= 0; }
     else n += unsigned(child.n);
     d.c_nodes[c] = n;
-    if ((d.code_jumps[c] & 2) && d.cd.C <= kScoreCases) d.pending[d.B - 1 - atomicAdd(d.n_pending + 1, 1)] = c;
-    else d.pending[atomicAdd(d.n_pending, 1)] = c;
+    const int k = eval_class(d.code_jumps[c], d.cd.C);
+    d.pending[pending_at(d.B, k, atomicAdd(d.n_pending + k, 1))] = c;
   }
 }
 
@@ -399,18 +409,26 @@ __device__ inline FastInputs staged_inputs(const StagedCases& sc, int C, int n_i
 // `scores` (this wavefront's LDS buffer of C doubles, or null) keeps every
 // case score of the first pass, so the canonical order is replayed from it
 // instead of by running the program again; the fast kernel always has it.
-template <bool kFast>
+template <int kKind>
 __device__ EvalOut eval_program(const EvalParams& e, const ProgView& pv, WaveStack& st, double* scores, int lane,
-                                StagedCases staged = StagedCases{}) {
+                                StagedCases staged = StagedCases{}, LAS std::int64_t* loop_slots = nullptr) {
+  constexpr bool kFast = kKind != 0;  // a typed path: scores in LDS, inputs untagged
   const int C = e.cd.C;
   double isum = 0;
   unsigned w = 0, errs = 0, touts = 0;
   unsigned long long fuel = 0;
   bool integral = C <= 4096;
   FastProgram fp{};
-  if (kFast) fp = fast_load(pv, lane);
+  LoopProgram lp{};
+  if (kKind == 1) fp = fast_load(pv, lane);
+  if (kKind == 2) lp = loop_load(pv, lane);
+  const int n_in = staged_inputs_of(pv.n_locals, e.cd.n_inputs);
+  const auto typed = [&](const FastInputs& in, bool active) {
+    if (kKind == 2) return vm_loop(lp, in, n_in, e.cd.arena, loop_slots, lane, active, e.fuel, pv.result_tag);
+    return vm_fast(fp, in, active, e.fuel, pv.result_tag);
+  };
   const auto run = [&](int c, bool active) {
-    if (kFast) return vm_fast(fp, fast_inputs(e.cd, pv.n_locals, c, active), active, e.fuel, pv.result_tag);
+    if (kFast) return typed(fast_inputs(e.cd, pv.n_locals, c, active), active);
     return vm_run(pv, e.cd, active ? c : 0, active, e.fuel, st, lane);
   };
   for (int tile = 0; tile < C; tile += 64) {
@@ -418,10 +436,15 @@ __device__ EvalOut eval_program(const EvalParams& e, const ProgView& pv, WaveSta
     const bool active = c < C;
     LaneResult r;
     Val expected{};
-    if (kFast && staged.on) {
-      r = vm_fast(fp, staged_inputs(staged, C, staged_inputs_of(pv.n_locals, e.cd.n_inputs), c, active), active, e.fuel,
-                  pv.result_tag);
-      if (active) { expected.i = staged.expected[c]; expected.tag = staged.exp_tag[c]; expected.b = staged.exp_b[c]; }
+    if constexpr (kFast) {
+      // One call site of the typed VM for both input sources.
+      const FastInputs in =
+          staged.on ? staged_inputs(staged, C, n_in, c, active) : fast_inputs(e.cd, pv.n_locals, c, active);
+      r = typed(in, active);
+      if (active) {
+        if (staged.on) { expected.i = staged.expected[c]; expected.tag = staged.exp_tag[c]; expected.b = staged.exp_b[c]; }
+        else expected = expected_of(e.cd, c);
+      }
     } else {
       r = run(c, active);
       if (active) expected = expected_of(e.cd, c);
@@ -513,18 +536,23 @@ __device__ inline WaveStack stack_of(unsigned char* smem, int slots, int temps, 
 // mode 0: score members 0..P-1 (initial population), mode 1: pending
 // children. kFast: the programs certified for vm_fast (pending entries from
 // the back of the list); otherwise the rest (from the front).
-template <bool kFast>
+template <int kKind>
 __global__ __launch_bounds__(kEvalThreads) void k_eval(DSearch* all, const BlockMap* map, int mode) {
+  constexpr bool kFast = kKind != 0;
   const BlockMap bm = map[blockIdx.x];
   const DSearch& d = all[bm.search];
   if (mode == 1 ? !search_live(d) : (!d.active || *d.error)) return;
   extern __shared__ unsigned char smem[];
   const int wave = wave_index(), lane = threadIdx.x % 64;
   WaveStack st = stack_of(smem, kFast ? 0 : d.stack_slots, kFast ? 0 : d.temp_slots, wave);
+  // The looping kernel's slot areas (one per wavefront) come first.
+  constexpr std::size_t kLead = kKind == 2 ? std::size_t(kWavesPerBlock) * kLoopSlots * 64 * 8 : 0;
+  LAS std::int64_t* const loop_slots = (LAS std::int64_t*)smem + std::size_t(wave) * kLoopSlots * 64;
   // Score buffers follow the stacks of the block's wavefronts.
   double* scores =
       (kFast ? d.cd.C <= kScoreCases : d.general_scores != 0)
-          ? reinterpret_cast<double*>(smem + std::size_t(kWavesPerBlock) * (kFast ? 0 : d.stack_slots + d.temp_slots) * 64 * 10) +
+          ? reinterpret_cast<double*>(smem + kLead +
+                                      std::size_t(kWavesPerBlock) * (kFast ? 0 : d.stack_slots + d.temp_slots) * 64 * 10) +
                 std::size_t(wave) * d.cd.C
           : nullptr;
   const EvalParams e{d.cd, d.fuel, d.lanes, d.penalty, d.policy};
@@ -533,15 +561,15 @@ __global__ __launch_bounds__(kEvalThreads) void k_eval(DSearch* all, const Block
   const bool stage = kFast && d.stage_cases;
   if (stage)
     staged = stage_cases<kEvalThreads>(d.cd, d.n_locals,
-                                       smem + std::size_t(kWavesPerBlock) * 8 * std::size_t(d.cd.C));
-  const int count = mode == 1 ? d.n_pending[kFast ? 1 : 0] : d.P;
+                                       smem + kLead + std::size_t(kWavesPerBlock) * 8 * std::size_t(d.cd.C));
+  const int count = mode == 1 ? d.n_pending[kKind] : d.P;
   for (int i = bm.block * kWavesPerBlock + wave; i < count; i += bm.blocks * kWavesPerBlock) {
-    const int slot = mode == 1 ? (kFast ? d.pending[d.B - 1 - i] : d.pending[i]) : i;
+    const int slot = mode == 1 ? d.pending[pending_at(d.B, kKind, i)] : i;
     const int pmode = d.code_jumps[slot];
-    if (mode == 0 && ((pmode & 2) != 0 && d.cd.C <= kScoreCases) != kFast) continue;  // the other kernel scores it
+    if (mode == 0 && eval_class(pmode, d.cd.C) != kKind) continue;  // another kernel scores it
     const ProgView pv{d.code + std::size_t(slot) * d.code_cap, d.pool + std::size_t(slot) * d.pool_cap, d.code_len[slot],
                       d.n_locals, pmode & 1, (pmode >> 1) & 1, d.pool_len[slot], (pmode & 4) ? TAG_BOOL : TAG_INT};
-    const EvalOut r = eval_program<kFast>(e, pv, st, scores, lane, staged);
+    const EvalOut r = eval_program<kKind>(e, pv, st, scores, lane, staged, loop_slots);
     if (lane == 0) {
       // Ranking rounds every published fitness to a 40-bit mantissa.
       if (mode == 1) { d.c_fit[slot] = canon40(r.fitness); d.c_wrong[slot] = r.wrong; }
@@ -560,11 +588,15 @@ constexpr int kTallyBlocks = 16;  // blocks per search, one atomic per counter e
 __global__ __launch_bounds__(256) void k_tally(DSearch* all, int mode) {
   const DSearch& d = all[blockIdx.y];
   if (mode == 1 ? !search_live(d) : (!d.active || *d.error)) return;
-  const int front = mode == 1 ? d.n_pending[0] : d.P, count = mode == 1 ? front + d.n_pending[1] : d.P;
+  const int front = mode == 1 ? d.n_pending[0] : d.P, back = mode == 1 ? d.n_pending[1] : 0;
+  const int count = mode == 1 ? front + back + d.n_pending[2] : d.P;
   const int first = int(blockIdx.x) * 256 + int(threadIdx.x), stride = kTallyBlocks * 256;
   unsigned long long v[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int i = first; i < count; i += stride) {
-    const int slot = mode == 1 ? (i < front ? d.pending[i] : d.pending[d.B - 1 - (i - front)]) : i;
+    const int slot = mode != 1        ? i
+                     : i < front        ? d.pending[i]
+                     : i < front + back ? d.pending[d.B - 1 - (i - front)]
+                                        : d.pending[d.B + (i - front - back)];
     v[0] += d.e_errors[slot];
     v[1] += d.e_timeouts[slot];
     v[2] += d.e_fuel[slot];
@@ -954,6 +986,7 @@ __global__ __launch_bounds__(kPlaceThreads) void k_place(DSearch* all) {
     *d.best = best;
     d.n_pending[0] = 0;
     d.n_pending[1] = 0;
+    d.n_pending[2] = 0;
     atomicAdd(d.round, 1);
   }
 }
@@ -1035,6 +1068,7 @@ __global__ void k_gen_end(DSearch* all) {
   if (!d.generational || !search_live(d) || threadIdx.x) return;
   d.n_pending[0] = 0;
   d.n_pending[1] = 0;
+  d.n_pending[2] = 0;
   atomicAdd(d.round, 1);
 }
 
@@ -1070,7 +1104,7 @@ __global__ __launch_bounds__(kEvalThreads) void k_eval_programs(const DProg* pro
   for (int i = blockIdx.x * kWavesPerBlock + wave; i < n; i += gridDim.x * kWavesPerBlock) {
     const DProg p = progs[i];
     if ((p.fast != 0) != kFast) continue;
-    const EvalOut r = eval_program<kFast>(e, view_of(p), st, scores, lane);
+    const EvalOut r = eval_program<kFast ? 1 : 0>(e, view_of(p), st, scores, lane);
     if (lane == 0) {
       fitness[i] = r.fitness;
       wrong[i] = e.policy.metric == METRIC_LEGACY ? 0 : r.wrong;
@@ -1189,6 +1223,7 @@ struct SearchState {
   int P = 0, B = 0, R = 0, G = 0, rounds = 0, per_epoch = 0;
   int fuel = 0;
   double penalty = 0;
+  bool may_jump = false;  // the grammar has if/and/or or traversals
   DeviceArena arena;
   DSearch d{};
   double init_ms = 0;
@@ -1237,6 +1272,8 @@ struct SearchState {
     layout = whole_program_layout(grammar);
     phase = compile_phase(grammar, grammar.entry, grammar.max_depth, unsigned(cfg.site_depth), train.input_names);
     R = std::min<int>(kPhaseCapacity, int(layout.max_variable_nodes));
+    for (const auto& x : phase.exprs)
+      may_jump = may_jump || x.node.kind == TK_IF || x.node.kind == TK_AND || x.node.kind == TK_OR || x.node.kind == TK_TRAVERSE;
   }
 
   void allocate() {
@@ -1248,6 +1285,8 @@ struct SearchState {
     // own besides those of its nodes, and keeps 9 temporaries live.
     const int walks = phase.traversals;
     x.code_cap = 4 * R + 8 + 64 * walks;
+    // The general VM schedules instruction indices below 8191.
+    if (x.code_cap > 8190) throw Unsupported("lowered programs longer than 8190 instructions");
     x.pool_cap = 2 * R + 8 + 8 * walks;
     x.stack_slots = std::min(kMaxStack, 2 * int(layout.slot_max_depth) + 2);
     x.walks = walks > 0;
@@ -1289,7 +1328,7 @@ struct SearchState {
     x.pool = a.alloc<Val>(std::size_t(code_rows) * x.pool_cap);
     x.code_len = a.alloc<int>(code_rows); x.pool_len = a.alloc<int>(code_rows); x.code_jumps = a.alloc<int>(code_rows);
     x.child_hash = a.alloc<std::uint64_t>(B);
-    x.pending = a.alloc<int>(B); x.n_pending = a.alloc<int>(2);
+    x.pending = a.alloc<int>(2 * std::size_t(B)); x.n_pending = a.alloc<int>(3);
     const int tsize = next_pow2(2 * P);
     x.table_key = a.alloc<std::uint64_t>(tsize); x.table = a.alloc<int>(tsize); x.table_mask = tsize - 1;
     x.mt = a.alloc<std::uint64_t>(312); x.mt_left = a.alloc<std::uint64_t>(kRing); x.n_left = a.alloc<int>(1);
@@ -1311,7 +1350,7 @@ struct SearchState {
     x.e_errors = a.alloc<unsigned>(code_rows); x.e_timeouts = a.alloc<unsigned>(code_rows);
     x.e_fuel = a.alloc<unsigned long long>(code_rows);
     x.error = a.alloc<int>(1);
-    HIP_OK(hipMemset((void*)x.n_pending, 0, 2 * sizeof(int)));
+    HIP_OK(hipMemset((void*)x.n_pending, 0, 3 * sizeof(int)));
     HIP_OK(hipMemset(x.counters, 0, sizeof(Counters)));
     HIP_OK(hipMemset(x.error, 0, sizeof(int)));
     HIP_OK(hipMemset(x.round, 0, sizeof(int)));
@@ -1376,7 +1415,8 @@ struct Batch::Impl {
     if (d_status) (void)hipFree(d_status);
     if (stream) (void)hipStreamDestroy(stream);
   }
-  std::size_t eval_fast_lds = 0, eval_general_lds = 0;  // dynamic LDS of the two evaluation kernels
+  std::size_t eval_fast_lds = 0, eval_general_lds = 0;  // dynamic LDS of the evaluation kernels
+  static constexpr std::size_t kLoopLds = std::size_t(kWavesPerBlock) * kLoopSlots * 64 * 8;  // plus the looping slots
   std::size_t vary_lds_bytes = 0;  // phase-table copy of the staged reproduction kernel
   bool vary_global = false;        // some search reproduces from global tables
   bool any_generational = false;
@@ -1391,6 +1431,14 @@ struct Batch::Impl {
       if (searches[std::size_t(id)]->d.walks) return true;
     return false;
   }
+  // Some grammar can lower to jumps (if/and/or or traversals): the looping
+  // evaluation kernel has work.
+  bool any_jumps(const std::vector<int>& ids) const {
+    for (int id : ids)
+      if (searches[std::size_t(id)]->may_jump) return true;
+    return false;
+  }
+  bool loop_kernel = false;  // any_jumps(live)
   template <class Blocks>
   Grid build_grid(DeviceArena& arena, const std::vector<int>& table, Blocks blocks_of) {
     std::vector<BlockMap> v;
@@ -1461,8 +1509,9 @@ struct Batch::Impl {
     };
     attr(reinterpret_cast<const void*>(&k_vary<true, false>), vary_lds_bytes);
     attr(reinterpret_cast<const void*>(&k_vary<true, true>), vary_lds_bytes);
-    attr(reinterpret_cast<const void*>(&k_eval<false>), eval_general_lds);
-    attr(reinterpret_cast<const void*>(&k_eval<true>), eval_fast_lds);
+    attr(reinterpret_cast<const void*>(&k_eval<0>), eval_general_lds);
+    attr(reinterpret_cast<const void*>(&k_eval<1>), eval_fast_lds);
+    attr(reinterpret_cast<const void*>(&k_eval<2>), eval_fast_lds + kLoopLds);
     attr(reinterpret_cast<const void*>(&k_place<true>), place_lds_bytes);
     attr(reinterpret_cast<const void*>(&k_place<false>), place_hbm_bytes);
   }
@@ -1487,9 +1536,13 @@ struct Batch::Impl {
       timer.time("k_lower_members_walks", stream,
                  [&] { hipLaunchKernelGGL(k_lower_members<true>, dim3(gl.blocks), dim3(64 * kVaryWaves), 0, stream, t, gl.map); });
     timer.time("k_eval_init", stream,
-               [&] { hipLaunchKernelGGL(k_eval<true>, dim3(ge.blocks), dim3(kEvalThreads), eval_fast_lds, stream, t, ge.map, 0); });
+               [&] { hipLaunchKernelGGL(k_eval<1>, dim3(ge.blocks), dim3(kEvalThreads), eval_fast_lds, stream, t, ge.map, 0); });
+    if (any_jumps(ids))
+      timer.time("k_eval_init_loop", stream, [&] {
+        hipLaunchKernelGGL(k_eval<2>, dim3(ge.blocks), dim3(kEvalThreads), eval_fast_lds + kLoopLds, stream, t, ge.map, 0);
+      });
     timer.time("k_eval_init_general", stream, [&] {
-      hipLaunchKernelGGL(k_eval<false>, dim3(ge.blocks), dim3(kEvalThreads), eval_general_lds, stream, t, ge.map, 0);
+      hipLaunchKernelGGL(k_eval<0>, dim3(ge.blocks), dim3(kEvalThreads), eval_general_lds, stream, t, ge.map, 0);
     });
     timer.time("k_tally_init", stream, [&] { hipLaunchKernelGGL(k_tally, dim3(kTallyBlocks, n), dim3(256), 0, stream, t, 0); });
     timer.time("k_init_keys", stream, [&] { hipLaunchKernelGGL(k_init_keys, dim3(n), dim3(256), 0, stream, t); });
@@ -1516,6 +1569,7 @@ struct Batch::Impl {
       HIP_OK(hipMemcpy(d_all, host.data(), sizeof(DSearch) * host.size(), hipMemcpyHostToDevice));
       size_kernels(live);
       build_round_grids();
+      loop_kernel = any_jumps(live);
       if (status_cap < live.size()) {
         if (d_status) (void)hipFree(d_status);
         status_cap = std::max<std::size_t>(64, 2 * live.size());
@@ -1539,8 +1593,9 @@ struct Batch::Impl {
     launch("k_clear", g_table, k_clear, 256, 0);
     launch("k_index", g_member, k_index, 128, 0);
     launch("k_reuse", g_child, k_reuse, 128, 0);
-    launch("k_eval", g_eval, k_eval<true>, kEvalThreads, eval_fast_lds, 1);
-    launch("k_eval_general", g_eval, k_eval<false>, kEvalThreads, eval_general_lds, 1);
+    launch("k_eval", g_eval, k_eval<1>, kEvalThreads, eval_fast_lds, 1);
+    if (loop_kernel) launch("k_eval_loop", g_eval, k_eval<2>, kEvalThreads, eval_fast_lds + kLoopLds, 1);
+    launch("k_eval_general", g_eval, k_eval<0>, kEvalThreads, eval_general_lds, 1);
     timer.time("k_tally", stream, [&] { hipLaunchKernelGGL(k_tally, dim3(kTallyBlocks, n), dim3(256), 0, stream, d_all, 1); });
     if (place_lds_bytes)
       timer.time("k_place", stream, [&] { hipLaunchKernelGGL(k_place<true>, dim3(n), dim3(kPlaceThreads), place_lds_bytes, stream, d_all); });

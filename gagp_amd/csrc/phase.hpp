@@ -550,12 +550,20 @@ constexpr int kMaxStack = 64;
 
 // Limits of the typed straight-line VM path (vm_fast in vm.hpp).
 constexpr int kFastStack = 16, kFastCode = 128, kFastPool = 128, kFastInputs = 4;
+// Limits of the typed looping VM path (vm_loop): hidden locals (one
+// traversal's temporaries; sequential traversals reuse them), code and
+// constants held in registers.
+constexpr int kLoopLocals = 9, kLoopCode = 256, kLoopPool = 256;
+constexpr int kLoopSlots = kFastStack + kLoopLocals;  // LDS slots per lane of the looping path
 // `fast`: every instruction and operand is certified for the typed
 // straight-line path (no jumps, every value statically Int or Bool, integer
 // builtins only, within the kFast* limits).
 // `locals`: evaluator locals plus the hidden temporaries of traversals.
-struct LowerResult { int status, code, constants, stack_bound, result_type, jumps, fast, locals; };
+// `loop`: the same certificate for programs with jumps, hidden locals or
+// IntList views (length and indexing only), for the typed looping path.
+struct LowerResult { int status, code, constants, stack_bound, result_type, jumps, fast, locals, loop; };
 GHD inline bool fast_type(int tag) { return tag == TAG_INT || tag == TAG_BOOL; }
+GHD inline bool loop_type(int tag) { return fast_type(tag) || tag == TAG_INT_LIST; }
 // a0..a2: operand tags (unused ones ignored); passed by value so the callers'
 // frames stay in registers.
 GHD inline bool fast_operation(int opcode, int builtin, int arity, int a0, int a1, int a2) {
@@ -576,6 +584,16 @@ GHD inline bool fast_operation(int opcode, int builtin, int arity, int a0, int a
       }
     default: return false;
   }
+}
+// Operations of the looping path: those of the straight-line path plus the
+// length and indexing of an IntList view.
+GHD inline bool loop_operation(int opcode, int builtin, int arity, int a0, int a1, int a2) {
+  if (opcode == OP_CALL_BUILTIN) {
+    if (builtin == B_LEN) return arity == 1 && a0 == TAG_INT_LIST;
+    if (builtin == B_INDEX || builtin == B_INDEX_NONNEG || builtin == B_INDEX_CLAMPED)
+      return arity == 2 && a0 == TAG_INT_LIST && a1 == TAG_INT;
+  }
+  return fast_operation(opcode, builtin, arity, a0, a1, a2);
 }
 
 // An open operation of the post-order walk: its node, operands seen so far,
@@ -616,12 +634,13 @@ template <class Frames, class OriginAt, bool kWalks = false>
 GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, OriginAt origin_at, const std::uint8_t* local_types,
                                         int local_count, int expected_type, Ins* code, int code_capacity, Val* pool,
                                         int pool_capacity, Frames& frames) {
-  LowerResult out{ST_INVALID, 0, 0, 0, TAG_INVALID, 0, 0, local_count};
+  LowerResult out{ST_INVALID, 0, 0, 0, TAG_INVALID, 0, 0, local_count, 0};
   if (t.n <= 0 || t.n > 4096) return out;
   [[maybe_unused]] WalkStore<kWalks> ws;
   if constexpr (kWalks)
     for (int i = 0; i < 16; ++i) { ws.env_slot[i] = -1; ws.env_type[i] = TAG_INVALID; }
   bool fast = local_count <= kFastInputs;
+  bool typed = local_count <= kFastInputs;  // every value Int, Bool or an IntList view; loop_operation only
   int depth = 0, cursor = 0, stack = 0, cconst = 0;
   int completed = TAG_INVALID;
   bool have_value = false;
@@ -753,6 +772,7 @@ GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, Origi
         if (out.constants >= pool_capacity) { out.status = ST_CAPACITY; return out; }
         pool[out.constants] = v;
         fast = fast && fast_type(v.tag);
+        typed = typed && fast_type(v.tag);
         if (!emit(OP_PUSH_CONST, out.constants, 0, node.fuel)) return out;
         ++out.constants;
         grow(1);
@@ -767,6 +787,7 @@ GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, Origi
           if (!emit(OP_LOAD, ws.env_slot[b], 0, node.fuel)) return out;
           grow(1);
           fast = false;
+          typed = typed && loop_type(completed);
           have_value = true;
         }
       } else if (kWalks && node.kind == TK_TRAVERSE) {
@@ -798,6 +819,7 @@ GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, Origi
         completed = local_types[node.operand];
         if (completed == TAG_INVALID || completed == TAG_FALLBACK) { out.status = ST_UNSUPPORTED; return out; }
         fast = fast && fast_type(completed);
+        typed = typed && loop_type(completed);
         if (!emit(OP_LOAD, node.operand, 0, node.fuel)) return out;
         grow(1);
         have_value = true;
@@ -820,6 +842,8 @@ GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, Origi
       out.result_type = completed;
       out.fast = fast && !out.jumps && fast_type(completed) && out.stack_bound <= kFastStack && out.code <= kFastCode &&
                  out.constants <= kFastPool;
+      out.loop = !out.fast && typed && fast_type(completed) && out.stack_bound <= kFastStack && out.code <= kLoopCode &&
+                 out.constants <= kLoopPool && out.locals - local_count <= kLoopLocals;
       return out;
     }
     Frame f = frames.get(depth - 1);
@@ -892,6 +916,7 @@ GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, Origi
       else if (node.type != TAG_INVALID && node.type != rt) return out;
       if (rt == TAG_INVALID || rt == TAG_FALLBACK) { out.status = ST_UNSUPPORTED; return out; }
       fast = fast && fast_operation(node.opcode, node.operand, node.arity, args[0], args[1], args[2]) && fast_type(rt);
+      typed = typed && loop_operation(node.opcode, node.operand, node.arity, args[0], args[1], args[2]) && fast_type(rt);
       // operands are on the stack: the instruction sees the stack with them
       if (node.opcode == OP_CALL_BUILTIN) { if (!emit(OP_CALL_BUILTIN, node.operand, node.arity, node.fuel)) return out; }
       else if (!emit(node.opcode, 0, 0, node.fuel)) return out;

@@ -59,6 +59,20 @@ __device__ inline int wave_min(int x) {
 }
 #endif
 
+// Smallest value of x over the wave for 0 <= x < 2^kBits, by a bitwise
+// search with ballots (scalar compares, no LDS round trips): bit b of the
+// minimum is 0 iff some lane lies in [m, m + 2^b - 1] for the bits m above it.
+#if defined(__HIPCC__)
+template <int kBits>
+__device__ inline int wave_min_bits(int x) {
+  int m = 0;
+#pragma unroll
+  for (int b = kBits - 1; b >= 0; --b)
+    if (__ballot(x <= (m | ((1 << b) - 1))) == 0) m |= 1 << b;
+  return m;
+}
+#endif
+
 GHD inline double fbits(std::int64_t i) { union { std::int64_t i; double f; } u{i}; return u.f; }
 
 struct LaneResult {
@@ -87,8 +101,9 @@ __device__ inline LaneResult vm_run(const ProgView& p, const CaseData& cd, int c
   for (;;) {
     int at;
     if (p.has_jumps) {
-      at = wave_min(done ? 0x7fffffff : ip);
-      if (at == 0x7fffffff) break;
+      // ip <= len < 8191 (code capacity), 8191 marks a finished lane.
+      at = __builtin_amdgcn_readfirstlane(wave_min_bits<13>(done ? 8191 : (ip < 8191 ? ip : 8190)));
+      if (at == 8191) break;
     } else {
       // Straight-line code: every live lane is at the same ip; take it from
       // the first live lane (finished lanes keep a stale ip).
@@ -489,6 +504,235 @@ __device__ inline LaneResult vm_fast(const FastProgram& f, const FastInputs& in,
     }
   }
   finish(ERR_VALUE);  // ran off the end without RETURN
+  return r;
+}
+
+// ---- typed looping path ----
+// A program certified like the straight-line path but with jumps, hidden
+// locals or IntList views (phase.hpp loop_operation): values are untagged
+// int64 (Bool as 0/1, an IntList as its view payload), the operand stack and
+// the hidden locals are 8-byte LDS slots per lane (conflict-free, addressed
+// by the instruction's static height and operand), and code and constants
+// sit in four register groups read with readlane. Lanes follow their own branches under the
+// reference's per-case semantics: fuel is charged per lane, and the wave
+// executes the smallest pending instruction index for the lanes that are at
+// it. That index is the previous one plus one except after a jump or when a
+// lane finished, where a wave minimum recomputes it.
+struct LoopProgram {
+  int w0[4], w1[4];  // instruction: op | argc << 8 | sp << 16 | fuel << 24, operand
+  int c0[4], c1[4];  // constant payload, low and high words
+  int len;
+};
+__device__ inline LoopProgram loop_load(const ProgView& p, int lane) {
+  LoopProgram f;
+  f.len = p.len;
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    const int j = g * 64 + lane;
+    int w0 = 0, w1 = 0;
+    if (j < p.len) {
+      const Ins ins = p.code[j];
+      w0 = int(ins.op) | int(ins.argc) << 8 | int(ins.sp) << 16 | int(unsigned(ins.fuel) << 24);
+      w1 = ins.a;
+    }
+    std::int64_t v = 0;
+    if (j < p.n_pool) {
+      const Val x = p.pool[j];
+      v = x.tag == TAG_BOOL ? std::int64_t(x.b != 0) : x.i;
+    }
+    f.w0[g] = w0;
+    f.w1[g] = w1;
+    f.c0[g] = int(std::uint32_t(std::uint64_t(v)));
+    f.c1[g] = int(std::uint32_t(std::uint64_t(v) >> 32));
+  }
+  return f;
+}
+__device__ inline int loop_pick(const int (&v)[4], int k) {
+  const int x0 = __builtin_amdgcn_readlane(v[0], k & 63), x1 = __builtin_amdgcn_readlane(v[1], k & 63);
+  const int x2 = __builtin_amdgcn_readlane(v[2], k & 63), x3 = __builtin_amdgcn_readlane(v[3], k & 63);
+  return k < 128 ? (k < 64 ? x0 : x1) : (k < 192 ? x2 : x3);
+}
+// `slots`: this wave's LDS area of kLoopSlots untagged slots, slot-major
+// (slot k of lane j at k * 64 + j): the operand stack, then the hidden locals.
+__device__ inline LaneResult vm_loop(const LoopProgram& f, const FastInputs& in, int n_inputs, GAS const std::int64_t* arena,
+                                     LAS std::int64_t* slots, int lane, bool active, int fuel0, int result_tag) {
+  LaneResult r{0, TAG_INVALID, 0, active ? ERR_NONE : ERR_VALUE, 0};
+  bool done = !active;
+  LAS std::int64_t* const sl = slots + lane;
+  const auto S = [&](int k) -> LAS std::int64_t& { return sl[(k & (kFastStack - 1)) * 64]; };
+  const auto L = [&](int k) -> LAS std::int64_t& { return sl[(kFastStack + k) * 64]; };
+  unsigned assigned = 0;  // hidden locals stored so far
+  int fuel = fuel0, ip = 0, at = 0;
+  bool resched = false;
+  // Every step runs one instruction of at least one live lane, which charges
+  // fuel, so this bound is never reached by a certified program.
+  const long long limit = 64LL * ((long long)fuel0 + 2LL * f.len + 2) + 64;
+  for (long long step = 0;; ++step) {
+    if (resched) {
+      at = __builtin_amdgcn_readfirstlane(wave_min_bits<9>(done ? 511 : (ip < 511 ? ip : 510)));  // ip <= len <= 256
+      if (at == 511) break;
+      resched = false;
+    } else if (__ballot(!done) == 0) {
+      break;
+    }
+    if (step > limit) {
+      if (!done) { r.err = ERR_VALUE; r.fuel_used = fuel0 - fuel; done = true; }
+      break;
+    }
+    const bool mine = !done && ip == at;
+    if (at >= f.len) {  // ran off the end without RETURN
+      if (mine) { r.err = ERR_VALUE; r.fuel_used = fuel0 - fuel; done = true; }
+      resched = true;
+      continue;
+    }
+    // The decoded fields are scalar values; readfirstlane states that for
+    // the compiler, which then indexes the register arrays with them.
+    const int w = __builtin_amdgcn_readfirstlane(loop_pick(f.w0, at)), a = __builtin_amdgcn_readfirstlane(loop_pick(f.w1, at));
+    const int op = w & 0xff, argc = (w >> 8) & 0xff, sp = (w >> 16) & 0xff, cost = int(unsigned(w) >> 24);
+    // Every lane computes the instruction; the lanes at `at` with fuel
+    // commit it (predication instead of divergent branches).
+    const bool starve = mine && fuel < cost;
+    const bool run = mine && !starve;
+    int err = starve ? ERR_TIMEOUT : ERR_NONE;
+    bool ret = false;
+    int next = ip + 1;
+    const int top = sp - 1, below = sp - 2, push = sp;
+    // Binary operations read both operands before any store.
+    switch (op) {
+      case OP_PUSH_CONST: {
+        const std::uint32_t lo = std::uint32_t(loop_pick(f.c0, a)), hi = std::uint32_t(loop_pick(f.c1, a));
+        if (run) S(push) = std::int64_t(std::uint64_t(hi) << 32 | lo);
+        break;
+      }
+      case OP_LOAD: {
+        std::int64_t v;
+        bool bound;
+        if (a < n_inputs) {  // uniform branch
+          v = in.pick(a);
+          bound = (in.bound >> (a & 31)) & 1u;
+        } else {
+          const int k = a - n_inputs < kLoopLocals ? a - n_inputs : 0;
+          v = L(k);
+          bound = (assigned >> k) & 1u;
+        }
+        if (run && !bound) err = ERR_NAME;
+        if (run) S(push) = v;
+        break;
+      }
+      case OP_STORE: {
+        const int k = a - n_inputs < kLoopLocals && a >= n_inputs ? a - n_inputs : 0;
+        if (run) { L(k) = S(top); assigned |= 1u << k; }
+        break;
+      }
+      case OP_JMP: next = a; break;
+      case OP_JMP_IF_FALSE: next = S(top) == 0 ? a : ip + 1; break;
+      case OP_JMP_IF_TRUE: next = S(top) != 0 ? a : ip + 1; break;
+      case OP_CHECK_INT: break;
+      case OP_NEG: if (run) S(top) = wrap_neg(S(top)); break;
+      case OP_NOT: if (run) S(top) = std::int64_t(S(top) == 0); break;
+      case OP_ADD: case OP_SUB: case OP_MUL: case OP_LT: case OP_LE: case OP_GT: case OP_GE: case OP_EQ: case OP_NE: {
+        const std::int64_t x = S(below), y = S(top);
+        std::int64_t v;
+        switch (op) {
+          case OP_ADD: v = wrap_add(x, y); break;
+          case OP_SUB: v = wrap_sub(x, y); break;
+          case OP_MUL: v = wrap_mul(x, y); break;
+          case OP_LT: v = x < y; break;
+          case OP_LE: v = x <= y; break;
+          case OP_GT: v = x > y; break;
+          case OP_GE: v = x >= y; break;
+          case OP_EQ: v = x == y; break;
+          default: v = x != y; break;
+        }
+        if (run) S(below) = v;
+        break;
+      }
+      case OP_MOD: {
+        // Python % with ZeroDiv on 0 (the shared truncating division of vm_fast).
+        const std::int64_t x = S(below), y = S(top);
+        const std::int64_t safe = (y == 0 || y == -1) ? 1 : y;
+        const std::int64_t rem = y == -1 ? 0 : x - (x / safe) * safe;
+        const std::int64_t floor_mod = (rem != 0 && ((rem < 0) != (y < 0))) ? rem + y : rem;
+        if (run && y == 0) err = ERR_ZERO_DIV;
+        if (run) S(below) = y == 0 ? 0 : floor_mod;
+        break;
+      }
+      case OP_CALL_BUILTIN: {
+        const int base = sp - argc;
+        const std::int64_t x = S(base), y = S(base + 1);
+        std::int64_t out = 0;
+        switch (a) {
+          case B_ABS: out = x < 0 ? wrap_neg(x) : x; break;
+          case B_MIN: out = x < y ? x : y; break;
+          case B_MAX: out = x > y ? x : y; break;
+          case B_CLIP: {
+            const std::int64_t hi = S(base + 2), m = x > y ? x : y;
+            out = m < hi ? m : hi;
+            break;
+          }
+          case B_IDIV0: case B_IMOD0: case B_TRUNC_DIV0: case B_TRUNC_MOD0: {
+            const std::int64_t safe = (y == 0 || y == -1) ? 1 : y;
+            std::int64_t q = x / safe, rem = x - q * safe;
+            if (y == -1) { q = wrap_neg(x); rem = 0; }
+            const std::int64_t floor_mod = (rem != 0 && ((rem < 0) != (y < 0))) ? rem + y : rem;
+            if (a == B_IDIV0) out = y == 0 ? 0 : q;
+            else if (a == B_IMOD0) out = y == 0 ? 0 : floor_mod;
+            else if (a == B_TRUNC_DIV0) out = (y == 0 || (x == INT64_MIN && y == -1)) ? 0 : q;
+            else out = (y == 0 || (x == INT64_MIN && y == -1)) ? 0 : rem;
+            break;
+          }
+          case B_LEN: out = std::int64_t(std::uint64_t(x) & 0xffff); break;
+          case B_INDEX: case B_INDEX_NONNEG: case B_INDEX_CLAMPED: {  // as in vm_run
+            const std::int64_t n = std::int64_t(std::uint64_t(x) & 0xffff);
+            const std::uint64_t off = std::uint64_t(x) >> 16;
+            std::int64_t j = y;
+            bool ok = true, zero = false;
+            if (a == B_INDEX_CLAMPED) {
+              zero = n == 0;
+              j = j < 0 ? 0 : (j >= n ? n - 1 : j);
+            } else {
+              if (a == B_INDEX_NONNEG && j < 0) ok = false;
+              if (j < 0) j += n;
+              if (j < 0 || j >= n) ok = false;
+            }
+            if (run && !ok) err = ERR_VALUE;
+            // Only running lanes with a valid index read the arena; the rest
+            // read its first element (the arena is never empty).
+            const bool read = run && ok && !zero;
+            const std::int64_t v = arena[read ? off + std::uint64_t(j) : 0];
+            out = read ? v : 0;
+            break;
+          }
+          default: if (run) err = ERR_NAME; break;
+        }
+        if (run) S(base) = out;
+        break;
+      }
+      case OP_RETURN: ret = true; break;
+      default: if (run) err = ERR_VALUE; break;
+    }
+    if (run) {
+      ip = next;
+      fuel -= cost;
+    }
+    const bool finished = mine && (err != ERR_NONE || ret);
+    if (finished) {
+      r.err = err;
+      if (err == ERR_NONE) {
+        const std::int64_t v = S(top);
+        if (result_tag == TAG_BOOL) { r.t = TAG_BOOL; r.b = v != 0; r.v = 0; }
+        else { r.t = TAG_INT; r.v = v; }
+      }
+      r.fuel_used = fuel0 - fuel;
+      done = true;
+    }
+    // The next instruction index: after a jump, or when a lane at this index
+    // finished, recompute the minimum; otherwise every lane that ran is at
+    // at + 1 and every other live lane was already past `at`.
+    const bool jump = op == OP_JMP || op == OP_JMP_IF_FALSE || op == OP_JMP_IF_TRUE;
+    if (jump || __ballot(finished) != 0) resched = true;
+    else ++at;
+  }
   return r;
 }
 #endif
