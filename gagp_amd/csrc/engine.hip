@@ -411,7 +411,8 @@ __device__ inline FastInputs staged_inputs(const StagedCases& sc, int C, int n_i
 // instead of by running the program again; the fast kernel always has it.
 template <int kKind>
 __device__ EvalOut eval_program(const EvalParams& e, const ProgView& pv, WaveStack& st, double* scores, int lane,
-                                StagedCases staged = StagedCases{}, LAS std::int64_t* loop_slots = nullptr) {
+                                StagedCases staged = StagedCases{}) {
+  static_assert(kKind == 0 || kKind == 1, "the looping path has its own kernel");
   constexpr bool kFast = kKind != 0;  // a typed path: scores in LDS, inputs untagged
   const int C = e.cd.C;
   double isum = 0;
@@ -419,14 +420,9 @@ __device__ EvalOut eval_program(const EvalParams& e, const ProgView& pv, WaveSta
   unsigned long long fuel = 0;
   bool integral = C <= 4096;
   FastProgram fp{};
-  LoopProgram lp{};
   if (kKind == 1) fp = fast_load(pv, lane);
-  if (kKind == 2) lp = loop_load(pv, lane);
   const int n_in = staged_inputs_of(pv.n_locals, e.cd.n_inputs);
-  const auto typed = [&](const FastInputs& in, bool active) {
-    if (kKind == 2) return vm_loop(lp, in, n_in, e.cd.arena, loop_slots, lane, active, e.fuel, pv.result_tag);
-    return vm_fast(fp, in, active, e.fuel, pv.result_tag);
-  };
+  const auto typed = [&](const FastInputs& in, bool active) { return vm_fast(fp, in, active, e.fuel, pv.result_tag); };
   const auto run = [&](int c, bool active) {
     if (kFast) return typed(fast_inputs(e.cd, pv.n_locals, c, active), active);
     return vm_run(pv, e.cd, active ? c : 0, active, e.fuel, st, lane);
@@ -524,6 +520,132 @@ __device__ EvalOut eval_program(const EvalParams& e, const ProgView& pv, WaveSta
   return out;
 }
 
+// The canonical total of C case scores held in `scores` (LDS), computed by
+// one wavefront: L logical lanes over contiguous case chunks with 48-bit
+// rounding, then 32-lane groups and the total (the legacy metric sums the
+// lane partials directly), as eval_program's replay.
+__device__ inline double canonical_from_scores(const EvalParams& e, int C, const double* scores, int lane) {
+  const int L = e.lanes;
+  double group_total = 0;
+  for (int base = 0; base < L; base += 64) {
+    const int l = base + lane;
+    const bool own = l < L;
+    const int begin = own ? int((std::int64_t(C) * l) / L) : 0;
+    const int end = own ? int((std::int64_t(C) * (l + 1)) / L) : 0;
+    int longest = end - begin;
+    for (int off = 32; off > 0; off >>= 1) { const int y = __shfl_xor(longest, off, 64); longest = y > longest ? y : longest; }
+    double local = 0;
+    for (int k = 0; k < longest; ++k) {
+      const int c = begin + k;
+      if (c < end) local = canon48(local + scores[c]);
+    }
+    for (int half = 0; half < 2; ++half) {
+      if (base + half * 32 >= L) break;
+      double group = 0;
+      for (int k = 0; k < 32; ++k) {
+        const int src = half * 32 + k;
+        const double v = __shfl(local, src, 64);
+        if (base + src < L) {
+          if (e.policy.metric == METRIC_LEGACY) group_total = canon48(group_total + v);
+          else group = canon48(group + v);
+        }
+      }
+      if (e.policy.metric != METRIC_LEGACY) group_total = canon48(group_total + group);
+    }
+  }
+  return group_total;
+}
+
+// The looping path, one program per block: the block's wavefronts score
+// interleaved 64-case tiles into one LDS score buffer, combine their counts
+// in LDS, and wavefront 0 forms the fitness (the exact integer sum, or the
+// canonical replay from the buffer). Programs of a search are spread over
+// the search's blocks. LDS: per wavefront kLoopSlots slots of 64 lanes,
+// the block's C scores, then the staged cases.
+__global__ __launch_bounds__(kEvalThreads) void k_eval_loop(DSearch* all, const BlockMap* map, int mode) {
+  const BlockMap bm = map[blockIdx.x];
+  const DSearch& d = all[bm.search];
+  if (mode == 1 ? !search_live(d) : (!d.active || *d.error)) return;
+  extern __shared__ unsigned char smem[];
+  __shared__ double red_sum[kWavesPerBlock];
+  __shared__ unsigned long long red_fuel[kWavesPerBlock];
+  __shared__ unsigned red_w[kWavesPerBlock], red_e[kWavesPerBlock], red_t[kWavesPerBlock], red_i[kWavesPerBlock];
+  const int wave = wave_index(), lane = threadIdx.x % 64;
+  const int C = d.cd.C;
+  constexpr std::size_t kLead = std::size_t(kWavesPerBlock) * kLoopSlots * 64 * 8;
+  LAS std::int64_t* const slots = (LAS std::int64_t*)smem + std::size_t(wave) * kLoopSlots * 64;
+  double* const scores = reinterpret_cast<double*>(smem + kLead);
+  const EvalParams e{d.cd, d.fuel, d.lanes, d.penalty, d.policy};
+  StagedCases staged{};
+  if (d.stage_cases) staged = stage_cases<kEvalThreads>(d.cd, d.n_locals, smem + kLead + 8 * std::size_t(C));
+  const int count = mode == 1 ? d.n_pending[2] : d.P;
+  for (int i = bm.block; i < count; i += bm.blocks) {
+    const int slot = mode == 1 ? d.pending[pending_at(d.B, 2, i)] : i;
+    const int pmode = d.code_jumps[slot];
+    if (mode == 0 && eval_class(pmode, C) != 2) continue;  // another kernel scores it (uniform per block)
+    const ProgView pv{d.code + std::size_t(slot) * d.code_cap, d.pool + std::size_t(slot) * d.pool_cap, d.code_len[slot],
+                      d.n_locals, pmode & 1, (pmode >> 1) & 1, d.pool_len[slot], (pmode & 4) ? TAG_BOOL : TAG_INT};
+    const LoopProgram lp = loop_load(pv, lane);
+    const int n_in = staged_inputs_of(pv.n_locals, e.cd.n_inputs);
+    double isum = 0;
+    unsigned w = 0, errs = 0, touts = 0;
+    unsigned long long fuel = 0;
+    bool integral = C <= 4096;
+    for (int tile = wave * 64; tile < C; tile += kWavesPerBlock * 64) {
+      const int c = tile + lane;
+      const bool active = c < C;
+      const FastInputs in =
+          staged.on ? staged_inputs(staged, C, n_in, c, active) : fast_inputs(e.cd, pv.n_locals, c, active);
+      const LaneResult r = vm_loop(lp, in, n_in, e.cd.arena, slots, lane, active, e.fuel, pv.result_tag);
+      if (active) {
+        Val expected{};
+        if (staged.on) { expected.i = staged.expected[c]; expected.tag = staged.exp_tag[c]; expected.b = staged.exp_b[c]; }
+        else expected = expected_of(e.cd, c);
+        bool exact = false;
+        const double sc = lane_score(e, r, expected, exact);
+        w += !exact;
+        errs += r.err != ERR_NONE;
+        touts += r.err == ERR_TIMEOUT;
+        fuel += unsigned(r.fuel_used);
+        integral = integral && sc == double(std::int64_t(sc)) && sc >= -68719476736.0 && sc <= 68719476736.0;
+        isum += sc;
+        scores[c] = sc;
+      }
+    }
+    const double ws = wave_sum_d(isum);
+    const unsigned ww = wave_sum_u(w), we = wave_sum_u(errs), wt = wave_sum_u(touts);
+    unsigned long long wf = fuel;
+    for (int off = 32; off > 0; off >>= 1) wf += __shfl_xor(wf, off, 64);
+    const bool wi = __ballot(!integral) == 0;
+    if (lane == 0) {
+      red_sum[wave] = ws; red_fuel[wave] = wf;
+      red_w[wave] = ww; red_e[wave] = we; red_t[wave] = wt; red_i[wave] = wi;
+    }
+    __syncthreads();
+    if (wave == 0) {
+      double total = 0;
+      unsigned long long tf = 0;
+      unsigned tw = 0, te = 0, tt = 0;
+      bool ti = true;
+      for (int k = 0; k < kWavesPerBlock; ++k) {
+        total += red_sum[k]; tf += red_fuel[k]; tw += red_w[k]; te += red_e[k]; tt += red_t[k]; ti = ti && red_i[k];
+      }
+      // Integral scores: every partial is an exact integer below 2^48, so
+      // any order gives the canonical total.
+      const double fitness = ti ? total : canonical_from_scores(e, C, scores, lane);
+      if (lane == 0) {
+        // Ranking rounds every published fitness to a 40-bit mantissa.
+        if (mode == 1) { d.c_fit[slot] = canon40(fitness); d.c_wrong[slot] = tw; }
+        else { d.fitness[slot] = canon40(fitness); d.wrong[slot] = tw; }
+        d.e_errors[slot] = te;
+        d.e_timeouts[slot] = tt;
+        d.e_fuel[slot] = tf;
+      }
+    }
+    __syncthreads();  // the score buffer and counts are reused by the next program
+  }
+}
+
 // A wavefront's LDS stack of `slots` operand slots followed by `temps`
 // hidden-local slots, 10 bytes per lane and slot.
 __device__ inline WaveStack stack_of(unsigned char* smem, int slots, int temps, int wave) {
@@ -545,14 +667,10 @@ __global__ __launch_bounds__(kEvalThreads) void k_eval(DSearch* all, const Block
   extern __shared__ unsigned char smem[];
   const int wave = wave_index(), lane = threadIdx.x % 64;
   WaveStack st = stack_of(smem, kFast ? 0 : d.stack_slots, kFast ? 0 : d.temp_slots, wave);
-  // The looping kernel's slot areas (one per wavefront) come first.
-  constexpr std::size_t kLead = kKind == 2 ? std::size_t(kWavesPerBlock) * kLoopSlots * 64 * 8 : 0;
-  LAS std::int64_t* const loop_slots = (LAS std::int64_t*)smem + std::size_t(wave) * kLoopSlots * 64;
   // Score buffers follow the stacks of the block's wavefronts.
   double* scores =
       (kFast ? d.cd.C <= kScoreCases : d.general_scores != 0)
-          ? reinterpret_cast<double*>(smem + kLead +
-                                      std::size_t(kWavesPerBlock) * (kFast ? 0 : d.stack_slots + d.temp_slots) * 64 * 10) +
+          ? reinterpret_cast<double*>(smem + std::size_t(kWavesPerBlock) * (kFast ? 0 : d.stack_slots + d.temp_slots) * 64 * 10) +
                 std::size_t(wave) * d.cd.C
           : nullptr;
   const EvalParams e{d.cd, d.fuel, d.lanes, d.penalty, d.policy};
@@ -561,7 +679,7 @@ __global__ __launch_bounds__(kEvalThreads) void k_eval(DSearch* all, const Block
   const bool stage = kFast && d.stage_cases;
   if (stage)
     staged = stage_cases<kEvalThreads>(d.cd, d.n_locals,
-                                       smem + kLead + std::size_t(kWavesPerBlock) * 8 * std::size_t(d.cd.C));
+                                       smem + std::size_t(kWavesPerBlock) * 8 * std::size_t(d.cd.C));
   const int count = mode == 1 ? d.n_pending[kKind] : d.P;
   for (int i = bm.block * kWavesPerBlock + wave; i < count; i += bm.blocks * kWavesPerBlock) {
     const int slot = mode == 1 ? d.pending[pending_at(d.B, kKind, i)] : i;
@@ -569,7 +687,7 @@ __global__ __launch_bounds__(kEvalThreads) void k_eval(DSearch* all, const Block
     if (mode == 0 && eval_class(pmode, d.cd.C) != kKind) continue;  // another kernel scores it
     const ProgView pv{d.code + std::size_t(slot) * d.code_cap, d.pool + std::size_t(slot) * d.pool_cap, d.code_len[slot],
                       d.n_locals, pmode & 1, (pmode >> 1) & 1, d.pool_len[slot], (pmode & 4) ? TAG_BOOL : TAG_INT};
-    const EvalOut r = eval_program<kKind>(e, pv, st, scores, lane, staged, loop_slots);
+    const EvalOut r = eval_program<kKind>(e, pv, st, scores, lane, staged);
     if (lane == 0) {
       // Ranking rounds every published fitness to a 40-bit mantissa.
       if (mode == 1) { d.c_fit[slot] = canon40(r.fitness); d.c_wrong[slot] = r.wrong; }
@@ -1416,7 +1534,8 @@ struct Batch::Impl {
     if (stream) (void)hipStreamDestroy(stream);
   }
   std::size_t eval_fast_lds = 0, eval_general_lds = 0;  // dynamic LDS of the evaluation kernels
-  static constexpr std::size_t kLoopLds = std::size_t(kWavesPerBlock) * kLoopSlots * 64 * 8;  // plus the looping slots
+  static constexpr std::size_t kLoopLds = std::size_t(kWavesPerBlock) * kLoopSlots * 64 * 8;  // the looping slots
+  std::size_t eval_loop_lds = 0;
   std::size_t vary_lds_bytes = 0;  // phase-table copy of the staged reproduction kernel
   bool vary_global = false;        // some search reproduces from global tables
   bool any_generational = false;
@@ -1425,7 +1544,7 @@ struct Batch::Impl {
   struct Grid { GAS BlockMap* map = nullptr; int blocks = 0; };
   DeviceArena map_arena;
   Grid g_init, g_lower, g_eval_init, g_vary_staged, g_vary_global, g_vary_staged_w, g_vary_global_w, g_table, g_member, g_child,
-      g_eval;
+      g_eval, g_eval_loop;
   bool any_walks(const std::vector<int>& ids) const {
     for (int id : ids)
       if (searches[std::size_t(id)]->d.walks) return true;
@@ -1467,6 +1586,8 @@ struct Batch::Impl {
     g_member = build_grid(map_arena, live, [&](const SearchState& s) { return up(s.P, 128); });
     g_child = build_grid(map_arena, live, [&](const SearchState& s) { return up(s.B, 128); });
     g_eval = build_grid(map_arena, live, [&](const SearchState& s) { return std::min(4096, up(s.B, kWavesPerBlock)); });
+    // One program per block for the looping path.
+    g_eval_loop = build_grid(map_arena, live, [&](const SearchState& s) { return s.may_jump ? std::min(8192, s.B) : 0; });
   }
 
   // Allocates one search and fixes its kernel variants and LDS needs.
@@ -1485,6 +1606,7 @@ struct Batch::Impl {
   // LDS sizes and kernel attributes for a table of searches.
   void size_kernels(const std::vector<int>& table) {
     eval_fast_lds = eval_general_lds = vary_lds_bytes = place_lds_bytes = place_hbm_bytes = 0;
+    eval_loop_lds = kLoopLds;
     vary_global = any_generational = false;
     max_P = max_B = max_slots = max_table = 0;
     for (int id : table) {
@@ -1493,7 +1615,10 @@ struct Batch::Impl {
       const std::size_t score_buf = std::size_t(kWavesPerBlock) * 8 * x.d.cd.C;
       eval_general_lds = std::max(eval_general_lds, stacks + (x.d.general_scores ? score_buf : 0));
       const std::size_t cases = x.d.stage_cases ? staged_case_bytes(x.d.cd.C, staged_inputs_of(x.d.n_locals, x.d.cd.n_inputs)) : 0;
-      if (x.d.cd.C <= kScoreCases) eval_fast_lds = std::max(eval_fast_lds, score_buf + cases);
+      if (x.d.cd.C <= kScoreCases) {
+        eval_fast_lds = std::max(eval_fast_lds, score_buf + cases);
+        eval_loop_lds = std::max(eval_loop_lds, kLoopLds + 8 * std::size_t(x.d.cd.C) + cases);
+      }
       if (x.d.stage_tables) vary_lds_bytes = std::max(vary_lds_bytes, phase_tables_bytes(x.d.g));
       else vary_global = true;
       if (x.d.place_lds) place_lds_bytes = std::max(place_lds_bytes, std::size_t(16) * x.P);
@@ -1511,7 +1636,7 @@ struct Batch::Impl {
     attr(reinterpret_cast<const void*>(&k_vary<true, true>), vary_lds_bytes);
     attr(reinterpret_cast<const void*>(&k_eval<0>), eval_general_lds);
     attr(reinterpret_cast<const void*>(&k_eval<1>), eval_fast_lds);
-    attr(reinterpret_cast<const void*>(&k_eval<2>), eval_fast_lds + kLoopLds);
+    attr(reinterpret_cast<const void*>(&k_eval_loop), eval_loop_lds);
     attr(reinterpret_cast<const void*>(&k_place<true>), place_lds_bytes);
     attr(reinterpret_cast<const void*>(&k_place<false>), place_hbm_bytes);
   }
@@ -1539,7 +1664,8 @@ struct Batch::Impl {
                [&] { hipLaunchKernelGGL(k_eval<1>, dim3(ge.blocks), dim3(kEvalThreads), eval_fast_lds, stream, t, ge.map, 0); });
     if (any_jumps(ids))
       timer.time("k_eval_init_loop", stream, [&] {
-        hipLaunchKernelGGL(k_eval<2>, dim3(ge.blocks), dim3(kEvalThreads), eval_fast_lds + kLoopLds, stream, t, ge.map, 0);
+        const Grid gp = build_grid(tmp, ids, [&](const SearchState& x) { return std::min(8192, x.P); });
+        hipLaunchKernelGGL(k_eval_loop, dim3(gp.blocks), dim3(kEvalThreads), eval_loop_lds, stream, t, gp.map, 0);
       });
     timer.time("k_eval_init_general", stream, [&] {
       hipLaunchKernelGGL(k_eval<0>, dim3(ge.blocks), dim3(kEvalThreads), eval_general_lds, stream, t, ge.map, 0);
@@ -1594,7 +1720,7 @@ struct Batch::Impl {
     launch("k_index", g_member, k_index, 128, 0);
     launch("k_reuse", g_child, k_reuse, 128, 0);
     launch("k_eval", g_eval, k_eval<1>, kEvalThreads, eval_fast_lds, 1);
-    if (loop_kernel) launch("k_eval_loop", g_eval, k_eval<2>, kEvalThreads, eval_fast_lds + kLoopLds, 1);
+    if (loop_kernel) launch("k_eval_loop", g_eval_loop, k_eval_loop, kEvalThreads, eval_loop_lds, 1);
     launch("k_eval_general", g_eval, k_eval<0>, kEvalThreads, eval_general_lds, 1);
     timer.time("k_tally", stream, [&] { hipLaunchKernelGGL(k_tally, dim3(kTallyBlocks, n), dim3(256), 0, stream, d_all, 1); });
     if (place_lds_bytes)
