@@ -555,6 +555,10 @@ constexpr int kFastStack = 16, kFastCode = 128, kFastPool = 128, kFastInputs = 4
 // constants held in registers.
 constexpr int kLoopLocals = 9, kLoopCode = 256, kLoopPool = 256;
 constexpr int kLoopSlots = kFastStack + kLoopLocals;  // LDS slots per lane of the looping path
+// Tags of a traversal's fixed instruction sequences, held in the argc byte
+// of their first instruction (a LOAD or STORE, whose argc is otherwise 0);
+// the low bits give the direction (1 forward, 2 reverse).
+constexpr int kWalkTest = 0x80, kWalkNext = 0x90;
 // `fast`: every instruction and operand is certified for the typed
 // straight-line path (no jumps, every value statically Int or Bool, integer
 // builtins only, within the kFast* limits).
@@ -724,6 +728,10 @@ GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, Origi
         w.loop_head = out.code;
         if (!put_load(w.counter) || !put_load(reverse ? w.begin : w.limit) || !put(reverse ? OP_GE : OP_LT, 0, 0, -1))
           return false;
+        // The loop head starts the fixed 12-instruction test/read/bind/index
+        // sequence: tagged in the otherwise unused argc byte of its LOAD for
+        // the looping VM (kWalkTest | direction).
+        code[w.loop_head].argc = std::uint8_t(kWalkTest | (reverse ? 2 : 1));
         w.done_jump = put_jump(OP_JMP_IF_FALSE, -1);
         if (w.done_jump < 0) return false;
         if (!put_load(w.xs) || !put_load(w.counter) || !put(OP_CALL_BUILTIN, B_INDEX, 2, -1)) return false;
@@ -744,10 +752,13 @@ GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, Origi
           ws.env_slot[node.operand + k] = static_cast<signed char>(w.saved_slot[k]);
           ws.env_type[node.operand + k] = static_cast<unsigned char>(w.saved_type[k]);
         }
+        const int update = out.code;
         if (!put_store(w.state)) return false;
         if (!put_load(w.counter) || !put_int(1) || !put(reverse ? OP_SUB : OP_ADD, 0, 0, -1) || !put_store(w.counter))
           return false;
         if (!emit(OP_JMP, w.loop_head, 0, 1)) return false;
+        // The fixed 6-instruction update/advance/repeat sequence.
+        code[update].argc = std::uint8_t(kWalkNext | (reverse ? 2 : 1));
         code[w.done_jump].a = out.code;
         if (!put_load(w.state)) return false;
       }

@@ -589,6 +589,71 @@ __device__ inline LaneResult vm_loop(const LoopProgram& f, const FastInputs& in,
     // the compiler, which then indexes the register arrays with them.
     const int w = __builtin_amdgcn_readfirstlane(loop_pick(f.w0, at)), a = __builtin_amdgcn_readfirstlane(loop_pick(f.w1, at));
     const int op = w & 0xff, argc = (w >> 8) & 0xff, sp = (w >> 16) & 0xff, cost = int(unsigned(w) >> 24);
+    if (argc & 0x80) {
+      // A traversal's fixed sequence as one step (phase.hpp kWalkTest,
+      // kWalkNext). Every instruction of it costs one unit of fuel and none
+      // can fail otherwise, so a lane with fewer units than the sequence
+      // charges times out inside it, with no fuel left.
+      const bool reverse = (argc & 3) == 2;
+      const int k_counter = a - n_inputs;
+      if ((argc & 0xf0) == kWalkTest) {
+        // LOAD counter, LOAD bound, LT/GE, JIF done, LOAD xs, LOAD counter,
+        // index, STORE element, LOAD start, LOAD counter, ADD, STORE index.
+        const int k_bound = __builtin_amdgcn_readfirstlane(loop_pick(f.w1, at + 1)) - n_inputs;
+        const int done_at = __builtin_amdgcn_readfirstlane(loop_pick(f.w1, at + 3));
+        const int k_xs = __builtin_amdgcn_readfirstlane(loop_pick(f.w1, at + 4)) - n_inputs;
+        const int k_element = __builtin_amdgcn_readfirstlane(loop_pick(f.w1, at + 7)) - n_inputs;
+        const int k_start = __builtin_amdgcn_readfirstlane(loop_pick(f.w1, at + 8)) - n_inputs;
+        const int k_index = __builtin_amdgcn_readfirstlane(loop_pick(f.w1, at + 11)) - n_inputs;
+        const std::int64_t counter = L(k_counter), bound = L(k_bound);
+        const bool more = reverse ? counter >= bound : counter < bound;
+        const int need = more ? 12 : 4;
+        const bool starve = mine && fuel < need;
+        const bool run = mine && !starve;
+        // The counter lies in [0, len): the element read is in bounds.
+        const std::int64_t xs = L(k_xs);
+        const bool read = run && more;
+        const std::int64_t element = arena[read ? (std::uint64_t(xs) >> 16) + std::uint64_t(counter) : 0];
+        if (read) {
+          L(k_element) = element;
+          L(k_index) = wrap_add(L(k_start), counter);
+          assigned |= (1u << k_element) | (1u << k_index);
+        }
+        if (run) {
+          fuel -= need;
+          ip = more ? at + 12 : done_at;
+        }
+        if (starve) {
+          fuel = 0;
+          r.err = ERR_TIMEOUT;
+          r.fuel_used = fuel0;
+          done = true;
+        }
+      } else {
+        // STORE state, LOAD counter, PUSH 1, ADD/SUB, STORE counter, JMP head.
+        const int k_state = k_counter;
+        const int k_step = __builtin_amdgcn_readfirstlane(loop_pick(f.w1, at + 1)) - n_inputs;
+        const int head = __builtin_amdgcn_readfirstlane(loop_pick(f.w1, at + 5));
+        const bool starve = mine && fuel < 6;
+        const bool run = mine && !starve;
+        if (run) {
+          L(k_state) = S(sp - 1);
+          assigned |= 1u << k_state;
+          const std::int64_t c = L(k_step);
+          L(k_step) = reverse ? wrap_sub(c, 1) : wrap_add(c, 1);
+          fuel -= 6;
+          ip = head;
+        }
+        if (starve) {
+          fuel = 0;
+          r.err = ERR_TIMEOUT;
+          r.fuel_used = fuel0;
+          done = true;
+        }
+      }
+      resched = true;
+      continue;
+    }
     // Every lane computes the instruction; the lanes at `at` with fuel
     // commit it (predication instead of divergent branches).
     const bool starve = mine && fuel < cost;
