@@ -561,7 +561,7 @@ __device__ inline double canonical_from_scores(const EvalParams& e, int C, const
 // in LDS, and wavefront 0 forms the fitness (the exact integer sum, or the
 // canonical replay from the buffer). Programs of a search are spread over
 // the search's blocks. LDS: per wavefront kLoopSlots slots of 64 lanes,
-// the block's C scores, then the staged cases.
+// then the block's C scores.
 __global__ __launch_bounds__(kEvalThreads) void k_eval_loop(DSearch* all, const BlockMap* map, int mode) {
   const BlockMap bm = map[blockIdx.x];
   const DSearch& d = all[bm.search];
@@ -576,8 +576,7 @@ __global__ __launch_bounds__(kEvalThreads) void k_eval_loop(DSearch* all, const 
   LAS std::int64_t* const slots = (LAS std::int64_t*)smem + std::size_t(wave) * kLoopSlots * 64;
   double* const scores = reinterpret_cast<double*>(smem + kLead);
   const EvalParams e{d.cd, d.fuel, d.lanes, d.penalty, d.policy};
-  StagedCases staged{};
-  if (d.stage_cases) staged = stage_cases<kEvalThreads>(d.cd, d.n_locals, smem + kLead + 8 * std::size_t(C));
+  const StagedCases staged{};  // inputs are read once per case tile: not staged, so more blocks fit the LDS
   const int count = mode == 1 ? d.n_pending[2] : d.P;
   for (int i = bm.block; i < count; i += bm.blocks) {
     const int slot = mode == 1 ? d.pending[pending_at(d.B, 2, i)] : i;
@@ -592,8 +591,10 @@ __global__ __launch_bounds__(kEvalThreads) void k_eval_loop(DSearch* all, const 
     unsigned long long fuel = 0;
     bool integral = C <= 4096;
     for (int tile = wave * 64; tile < C; tile += kWavesPerBlock * 64) {
-      const int c = tile + lane;
-      const bool active = c < C;
+      // Cases in the search's visiting order (similar loop lengths per
+      // tile); scores are kept by case index, so the totals are unchanged.
+      const bool active = tile + lane < C;
+      const int c = active ? e.cd.order[tile + lane] : 0;
       const FastInputs in =
           staged.on ? staged_inputs(staged, C, n_in, c, active) : fast_inputs(e.cd, pv.n_locals, c, active);
       const LaneResult r = vm_loop(lp, in, n_in, e.cd.arena, slots, lane, active, e.fuel, pv.result_tag);
@@ -1317,6 +1318,20 @@ struct DeviceCases {
     cd.exp_tag = a.upload(et);
     cd.exp_b = a.upload(eb);
     cd.arena = a.upload(arena);
+    // Visiting order of the looping path: cases by descending total list
+    // length (stable), so a tile's lanes run loops of similar length.
+    const std::size_t n = std::size_t(C);
+    std::vector<int> order(n);
+    std::vector<std::int64_t> work(n, 0);
+    for (int i = 0; i < C; ++i) {
+      order[std::size_t(i)] = i;
+      for (int k = 0; k < K; ++k) {
+        const std::size_t src = std::size_t(i) * K + k;
+        if (c.set[src] && c.inputs[src].tag == TAG_INT_LIST) work[std::size_t(i)] += std::int64_t(std::uint64_t(c.inputs[src].i) & 0xffff);
+      }
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work[std::size_t(x)] > work[std::size_t(y)]; });
+    cd.order = a.upload(order);
     cd.C = C;
     cd.n_inputs = K;
   }
@@ -1617,7 +1632,7 @@ struct Batch::Impl {
       const std::size_t cases = x.d.stage_cases ? staged_case_bytes(x.d.cd.C, staged_inputs_of(x.d.n_locals, x.d.cd.n_inputs)) : 0;
       if (x.d.cd.C <= kScoreCases) {
         eval_fast_lds = std::max(eval_fast_lds, score_buf + cases);
-        eval_loop_lds = std::max(eval_loop_lds, kLoopLds + 8 * std::size_t(x.d.cd.C) + cases);
+        eval_loop_lds = std::max(eval_loop_lds, kLoopLds + 8 * std::size_t(x.d.cd.C));
       }
       if (x.d.stage_tables) vary_lds_bytes = std::max(vary_lds_bytes, phase_tables_bytes(x.d.g));
       else vary_global = true;

@@ -553,8 +553,8 @@ constexpr int kFastStack = 16, kFastCode = 128, kFastPool = 128, kFastInputs = 4
 // Limits of the typed looping VM path (vm_loop): hidden locals (one
 // traversal's temporaries; sequential traversals reuse them), code and
 // constants held in registers.
-constexpr int kLoopLocals = 9, kLoopCode = 256, kLoopPool = 256;
-constexpr int kLoopSlots = kFastStack + kLoopLocals;  // LDS slots per lane of the looping path
+constexpr int kLoopStack = 12, kLoopLocals = 9, kLoopCode = 256, kLoopPool = 256;
+constexpr int kLoopSlots = kLoopStack + kLoopLocals;  // LDS slots per lane of the looping path
 // Tags of a traversal's fixed instruction sequences, held in the argc byte
 // of their first instruction (a LOAD or STORE, whose argc is otherwise 0);
 // the low bits give the direction (1 forward, 2 reverse).
@@ -853,7 +853,7 @@ GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, Origi
       out.result_type = completed;
       out.fast = fast && !out.jumps && fast_type(completed) && out.stack_bound <= kFastStack && out.code <= kFastCode &&
                  out.constants <= kFastPool;
-      out.loop = !out.fast && typed && fast_type(completed) && out.stack_bound <= kFastStack && out.code <= kLoopCode &&
+      out.loop = !out.fast && typed && fast_type(completed) && out.stack_bound <= kLoopStack && out.code <= kLoopCode &&
                  out.constants <= kLoopPool && out.locals - local_count <= kLoopLocals;
       return out;
     }

@@ -35,6 +35,7 @@ struct CaseData {
   GAS const std::uint8_t* exp_tag; // [C]
   GAS const std::uint8_t* exp_b;   // [C]
   GAS const std::int64_t* arena;   // IntList elements
+  GAS const int* order;            // [C] a permutation of the cases (visiting order of the looping path)
   int C, n_inputs;
 };
 
@@ -559,8 +560,9 @@ __device__ inline LaneResult vm_loop(const LoopProgram& f, const FastInputs& in,
   LaneResult r{0, TAG_INVALID, 0, active ? ERR_NONE : ERR_VALUE, 0};
   bool done = !active;
   LAS std::int64_t* const sl = slots + lane;
-  const auto S = [&](int k) -> LAS std::int64_t& { return sl[(k & (kFastStack - 1)) * 64]; };
-  const auto L = [&](int k) -> LAS std::int64_t& { return sl[(kFastStack + k) * 64]; };
+  // Heights are certified below kLoopStack; the clamp keeps every slot in range.
+  const auto S = [&](int k) -> LAS std::int64_t& { return sl[(k < 0 ? 0 : k < kLoopStack ? k : kLoopStack - 1) * 64]; };
+  const auto L = [&](int k) -> LAS std::int64_t& { return sl[(kLoopStack + (k < 0 ? 0 : k < kLoopLocals ? k : 0)) * 64]; };
   unsigned assigned = 0;  // hidden locals stored so far
   int fuel = fuel0, ip = 0, at = 0;
   bool resched = false;
