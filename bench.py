@@ -72,13 +72,21 @@ def profile(warm, steps):
     return ms, counts
 
 
+# The engine's timer names and the kernel symbols rocprofv3 records.
+PROFILED_NAME = {"k_vary": "k_vary<true, false>", "k_vary_global": "k_vary<false, false>",
+                 "k_vary_walks": "k_vary<true, true>", "k_vary_global_walks": "k_vary<false, true>",
+                 "k_eval": "k_eval<1>", "k_eval_general": "k_eval<0>", "k_place": "k_place<true>",
+                 "k_place_hbm": "k_place<false>"}
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE
     and WRITE_SIZE summary of this workload (None when absent)."""
     if not os.path.exists(TRAFFIC):
         return None
+    name = PROFILED_NAME.get(kernel, kernel)
     for line in open(TRAFFIC):
-        m = re.match(r"\|\s*%s\s*\|\s*([0-9.]+)\s*\|" % re.escape(kernel), line)
+        m = re.match(r"\|\s*%s\s*\|\s*([0-9.]+)\s*\|" % re.escape(name), line)
         if m:
             return float(m.group(1))
     return None
