@@ -155,6 +155,10 @@ __global__ void k_init_pop(DSearch* all, const BlockMap* map) {
     if (room <= 0 || room > d.R) { raise(d, 101); return; }
     const std::uint64_t s = splitmix_mix(d.seed ^ splitmix_mix((std::uint64_t(m) * 8 + 0) * 64 + std::uint64_t(attempt)));
     gen = generate_phase(g, g.root, 0, d.slot.max_depth, room, s, d.cur_origin + o, d.cur_cval + o, d.cur_site + o, logical);
+    // A statement program may read an unassigned local: the next seed is tried.
+    if (gen.status == ST_OK && d.slot.statement &&
+        !locals_defined(g, CRow{d.cur_origin + o, d.cur_site + o, d.cur_cval + o, gen.nodes, gen.constants}))
+      gen.status = ST_INVALID;
     if (gen.status != ST_OK && attempt % 4 == 3 && logical < sd) ++logical;
     if (attempt >= 256) { raise(d, 102); return; }
   }
@@ -1356,7 +1360,11 @@ struct SearchState {
   int P = 0, B = 0, R = 0, G = 0, rounds = 0, per_epoch = 0;
   int fuel = 0;
   double penalty = 0;
-  bool may_jump = false;  // the grammar has if/and/or or traversals
+  bool may_jump = false;   // the grammar has if/and/or, traversals or for_range
+  bool statement = false;  // a Program entry: statement programs with grammar locals
+  // The export name table: the names the first initial member uses, in
+  // declaration order (the layout prototype of phase_population_cpu.cpp:195).
+  std::vector<std::string> layout_names;
   DeviceArena arena;
   DSearch d{};
   double init_ms = 0;
@@ -1406,7 +1414,9 @@ struct SearchState {
     phase = compile_phase(grammar, grammar.entry, grammar.max_depth, unsigned(cfg.site_depth), train.input_names);
     R = std::min<int>(kPhaseCapacity, int(layout.max_variable_nodes));
     for (const auto& x : phase.exprs)
-      may_jump = may_jump || x.node.kind == TK_IF || x.node.kind == TK_AND || x.node.kind == TK_OR || x.node.kind == TK_TRAVERSE;
+      may_jump = may_jump || x.node.kind == TK_IF || x.node.kind == TK_AND || x.node.kind == TK_OR || x.node.kind == TK_TRAVERSE ||
+                 x.node.kind == TK_FOR_RANGE;
+    statement = grammar.nts[grammar.entry].cat == Cat::Program;
   }
 
   void allocate() {
@@ -1417,24 +1427,30 @@ struct SearchState {
     // A traversal lowers to at most 64 instructions and 8 constants of its
     // own besides those of its nodes, and keeps 9 temporaries live.
     const int walks = phase.traversals;
-    x.code_cap = 4 * R + 8 + 64 * walks;
+    // A for_range lowers to 23 instructions and 4 constants of its own, and
+    // takes at least three nodes (itself, its bound and its body).
+    const int loops = statement ? (R + 2) / 3 : 0;
+    x.code_cap = 4 * R + 8 + 64 * walks + 23 * loops;
     // The general VM schedules instruction indices below 8191.
     if (x.code_cap > 8190) throw Unsupported("lowered programs longer than 8190 instructions");
-    x.pool_cap = 2 * R + 8 + 8 * walks;
+    x.pool_cap = 2 * R + 8 + 8 * walks + 4 * loops;
     x.stack_slots = std::min(kMaxStack, 2 * int(layout.slot_max_depth) + 2);
-    x.walks = walks > 0;
-    x.temp_slots = walks ? std::max(0, std::min(64 - int(train.input_names.size()), 9 * std::min(walks, 8))) : 0;
+    // Statement programs lower through the hidden-locals instantiation too.
+    x.walks = walks > 0 || statement;
+    x.temp_slots = statement ? std::min(64 - int(train.input_names.size()), int(hidden_local_bound(grammar)))
+                   : walks  ? std::max(0, std::min(64 - int(train.input_names.size()), 9 * std::min(walks, 8)))
+                            : 0;
     x.rounds = rounds; x.per_epoch = per_epoch;
     x.fuel = fuel; x.lanes = 512; x.penalty = penalty; x.policy = task.policy;
     x.pressure = std::clamp(cfg.tournament, 1, P);
     x.replacement_size = cfg.replacement_size;
     x.p_mut = cfg.mutation_probability;
     x.pol = OpPolicy{cfg.counted, cfg.uniform_phase, cfg.crossover_probability, cfg.crossover_attempts, cfg.mutation_max_nodes};
-    x.slot = SlotInfo{phase.root, 0, int(layout.slot_max_depth), cfg.site_depth, int(layout.max_variable_nodes), R};
+    x.slot = SlotInfo{phase.root, 0, int(layout.slot_max_depth), cfg.site_depth, int(layout.max_variable_nodes), R, statement ? 1 : 0};
     x.fixed_nodes = int(layout.fixed_nodes);
     x.grammar_sites = cfg.grammar_sites;
     x.has_wrong = task.policy.metric != METRIC_LEGACY;
-    x.expected_type = grammar.nts[grammar.entry].type;
+    x.expected_type = statement ? kStatementType : grammar.nts[grammar.entry].type;
     x.n_locals = int(train.input_names.size());
     x.site_depth = cfg.site_depth;
     x.seed = cfg.seed;
@@ -1690,9 +1706,12 @@ struct Batch::Impl {
     timer.time("k_seed", stream, [&] { hipLaunchKernelGGL(k_seed, dim3(n), dim3(256), 0, stream, t); });
     HIP_OK(hipStreamSynchronize(stream));
     timer.collect();
-    for (int id : ids)
-      if (const int e = searches[std::size_t(id)]->device_error())
-        throw std::runtime_error("device initialization error " + std::to_string(e));
+    for (int id : ids) {
+      SearchState& x = *searches[std::size_t(id)];
+      if (const int e = x.device_error()) throw std::runtime_error("device initialization error " + std::to_string(e));
+      const int n0 = x.down(x.d.cur_n, 1)[0];
+      x.layout_names = names_used(x.grammar, x.down(x.d.cur_origin, std::size_t(n0)));
+    }
   }
   // Round table, grids, kernel sizes and graph for the current live set.
   void rebuild() {
@@ -1880,7 +1899,8 @@ double Batch::init_ms() const { return impl_->init_ms; }
 
 namespace {
 Ast export_row(const SearchState& s, const int* origin, const Val* cval, int n, int nc) {
-  return export_whole_program(s.grammar, s.phase, std::vector<int>(origin, origin + n), std::vector<Val>(cval, cval + nc));
+  return export_whole_program(s.grammar, s.phase, s.layout_names, std::vector<int>(origin, origin + n),
+                              std::vector<Val>(cval, cval + nc));
 }
 
 // The run's success criterion on held-out cases (cpp/src/cli/run.cpp:179-239).
@@ -2020,7 +2040,7 @@ SearchResult Batch::result(int index, bool save_all) {
   // Best member (the strict order's front after the last round).
   const std::vector<int> bo = s.down(d.cur_origin + std::size_t(best) * s.R, std::size_t(cur_n[best]));
   const std::vector<Val> bc = s.down(d.cur_cval + std::size_t(best) * s.R, std::size_t(cur_nc[best]));
-  const Ast best_ast = export_whole_program(s.grammar, s.phase, bo, bc);
+  const Ast best_ast = export_whole_program(s.grammar, s.phase, s.layout_names, bo, bc);
   json::Value bj = json::make_object();
   bj.obj["fitness"] = json::make_number(fitness[best]);
   bj.obj["wrong"] = json::make_number(wrong[best]);
@@ -2075,6 +2095,22 @@ std::vector<Ins> verify_program(const ProgramIn& p, int n_inputs, int& max_stack
   height[0] = 0;
   max_stack = 0;
   jumps = false;
+  // A NEG of a constant that is not a number is a guaranteed runtime error
+  // (the verifier's abstract types, bytecode_verify.cpp:629-637): nothing
+  // falls through it. Only the pushed constant's own path reaches it when no
+  // jump targets the NEG.
+  std::vector<char> targeted(std::size_t(n) + 1, 0);
+  for (int i = 0; i < n; ++i)
+    if ((p.op[i] == OP_JMP || p.op[i] == OP_JMP_IF_FALSE || p.op[i] == OP_JMP_IF_TRUE) && p.has_a[i] && p.a[i] >= 0 &&
+        p.a[i] <= n)
+      targeted[std::size_t(p.a[i])] = 1;
+  const auto failing_neg = [&](int ip) {
+    if (ip == 0 || targeted[std::size_t(ip)] || p.op[ip - 1] != OP_PUSH_CONST || !p.has_a[ip - 1] || p.a[ip - 1] < 0 ||
+        p.a[ip - 1] >= int(p.consts.size()))
+      return false;
+    const int tag = p.consts[std::size_t(p.a[ip - 1])].tag;
+    return tag != TAG_INT && tag != TAG_FLOAT;
+  };
   const auto flow = [&](int to, int h) {
     if (to < 0 || to > n) throw std::invalid_argument("jump target out of range");
     if (height[to] < 0) { height[to] = h; work.push_back(to); }
@@ -2099,7 +2135,8 @@ std::vector<Ins> verify_program(const ProgramIn& p, int n_inputs, int& max_stack
         if (!p.has_a[ip] || p.a[ip] < 0 || p.a[ip] >= 64) throw std::invalid_argument("local index out of range");
         if (p.a[ip] < n_inputs) throw Unsupported("stores into evaluator inputs");
         pop = 1; break;
-      case OP_NEG: case OP_NOT: case OP_CHECK_INT: case OP_CHECK_LIST: pop = 1; push = 1; break;
+      case OP_NEG: pop = 1; push = 1; falls = !failing_neg(ip); break;
+      case OP_NOT: case OP_CHECK_INT: case OP_CHECK_LIST: pop = 1; push = 1; break;
       case OP_ADD: case OP_SUB: case OP_MUL: case OP_DIV: case OP_MOD:
       case OP_LT: case OP_LE: case OP_GT: case OP_GE: case OP_EQ: case OP_NE: pop = 2; push = 1; break;
       case OP_JMP: jumps = true; if (!p.has_a[ip]) throw std::invalid_argument("jump target missing"); flow(p.a[ip], h); falls = false; break;

@@ -1,22 +1,52 @@
 // Genotype export (see export.hpp).
 #include "export.hpp"
 
+#include <algorithm>
 #include <functional>
 #include <stdexcept>
 
 namespace gagp_amd {
 
-Ast export_whole_program(const Grammar& g, const PhaseGrammar& phase, const std::vector<int>& origins,
-                         const std::vector<Val>& consts) {
+namespace {
+// The grammar name a node reads or writes (an input, a local read, or the
+// target of assign/for_range), else null.
+const std::string* node_name(const Grammar& g, const CExpr& e) {
+  if (e.kind == EKind::Input) return &g.inputs.at(e.target).name;
+  if (e.kind == EKind::Local) return &g.locals.at(e.target).name;
+  if (e.kind == EKind::Control && e.local >= 0) return &g.locals.at(std::size_t(e.local)).name;
+  return nullptr;
+}
+}  // namespace
+
+std::vector<std::string> names_used(const Grammar& g, const std::vector<int>& origins) {
+  const std::size_t ni = g.inputs.size();
+  std::vector<char> used(ni + g.locals.size(), 0);
+  for (const int o : origins) {
+    if (o < 0 || o >= int(g.exprs.size())) throw std::runtime_error("unknown expression origin");
+    const CExpr& e = g.exprs[std::size_t(o)];
+    if (e.kind == EKind::Input) used.at(e.target) = 1;
+    else if (e.kind == EKind::Local) used.at(ni + e.target) = 1;
+    else if (e.kind == EKind::Control && e.local >= 0) used.at(ni + std::size_t(e.local)) = 1;
+  }
+  std::vector<std::string> out;
+  for (std::size_t k = 0; k < used.size(); ++k)
+    if (used[k]) out.push_back(k < ni ? g.inputs[k].name : g.locals[k - ni].name);
+  return out;
+}
+
+Ast export_whole_program(const Grammar& g, const PhaseGrammar& phase, const std::vector<std::string>& layout_names,
+                         const std::vector<int>& origins, const std::vector<Val>& consts) {
   Ast ast;
-  ast.nodes.push_back({NK_PROGRAM, 0, 0});
-  ast.nodes.push_back({NK_BLOCK_CONS, 0, 0});
-  ast.nodes.push_back({NK_RETURN, 0, 0});
-  // Prototype name table: grammar inputs then locals; compacted to used names.
-  std::vector<std::string> table;
-  for (const auto& in : g.inputs) table.push_back(in.name);
-  for (const auto& lo : g.locals) table.push_back(lo.name);
-  std::vector<int> used(table.size(), 0);
+  const bool envelope = g.nts[g.entry].cat == Cat::Expression;
+  if (envelope) {
+    ast.nodes.push_back({NK_PROGRAM, 0, 0});
+    ast.nodes.push_back({NK_BLOCK_CONS, 0, 0});
+    ast.nodes.push_back({NK_RETURN, 0, 0});
+  }
+  const int base = int(ast.nodes.size());
+  // Name table: the layout names the program uses (their order), then its
+  // other names in first-use order; nodes are patched once it is complete.
+  std::vector<std::string> table = layout_names;
   std::vector<int> name_of_node(origins.size(), -1);
   std::size_t next_const = 0, j = 0;
   int next_binder = 6 * phase.traversals;
@@ -26,18 +56,19 @@ Ast export_whole_program(const Grammar& g, const PhaseGrammar& phase, const std:
     const std::size_t at = j++;
     const int o = origins[at];
     if (o < 0 || o >= int(g.exprs.size())) throw std::runtime_error("unknown expression origin");
-    const CExpr& e = g.exprs[o];
-    const PExpr& m = phase.exprs[o];
+    const CExpr& e = g.exprs[std::size_t(o)];
+    const PExpr& m = phase.exprs[std::size_t(o)];
     AstNode n{m.native_kind, 0, 0};
     int walk = -1;
+    if (const std::string* name = node_name(g, e)) {
+      auto it = std::find(table.begin(), table.end(), *name);
+      if (it == table.end()) it = table.insert(table.end(), *name);
+      name_of_node[at] = int(it - table.begin());
+    }
     if (e.kind == EKind::Constant) {
       if (next_const >= consts.size()) throw std::runtime_error("constant index");
       n.i0 = int(ast.consts.size());
       ast.consts.push_back(consts[next_const++]);
-    } else if (e.kind == EKind::Input) {
-      const int idx = int(e.target);  // grammar declaration order = prototype table order
-      used[idx] = 1;
-      name_of_node[at] = idx;
     } else if (e.kind == EKind::Bound) {
       const int b = m.node.operand;
       if (b < 0 || b >= 16 || env[std::size_t(b)] < 0) throw std::runtime_error("lexical index");
@@ -63,12 +94,15 @@ Ast export_whole_program(const Grammar& g, const PhaseGrammar& phase, const std:
   };
   emit(std::vector<int>(16, -1));
   if (j != origins.size()) throw std::runtime_error("invalid gene shape");
-  ast.nodes.push_back({NK_BLOCK_NIL, 0, 0});
-  std::vector<int> remap(table.size(), -1);
+  if (envelope) ast.nodes.push_back({NK_BLOCK_NIL, 0, 0});
+  // Stable compaction to the names in use.
+  std::vector<int> used(table.size(), 0), remap(table.size(), -1);
+  for (const int k : name_of_node)
+    if (k >= 0) used[std::size_t(k)] = 1;
   for (std::size_t i = 0; i < table.size(); ++i)
     if (used[i]) { remap[i] = int(ast.names.size()); ast.names.push_back(table[i]); }
-  for (std::size_t j = 0; j < origins.size(); ++j)
-    if (name_of_node[j] >= 0) ast.nodes[3 + j].i0 = remap[name_of_node[j]];
+  for (std::size_t k = 0; k < origins.size(); ++k)
+    if (name_of_node[k] >= 0) ast.nodes[std::size_t(base) + k].i0 = remap[std::size_t(name_of_node[k])];
   return ast;
 }
 

@@ -4,8 +4,7 @@
 // (cpp/src/evolution/repro/gpu/phase_layout.hpp:316-475,
 // cpp/src/evolution/repro/pack.cpp:11-77): the expression-entry envelope
 // PROGRAM, BLOCK_CONS, RETURN, <phase>, BLOCK_NIL; one constant per CONST node
-// in prefix order; names are the grammar inputs (declaration order) that the
-// program uses. Writers: write_ast_json (cpp/src/cli/program_json.cpp:211-232)
+// in prefix order. Writers: write_ast_json (cpp/src/cli/program_json.cpp:211-232)
 // and ProgramPrinter (program_json.cpp:876-946).
 //
 // Traversal binders take fresh public ids in prefix order, above every id of
@@ -35,9 +34,22 @@ struct Ast {
   std::vector<TraversalSpec> traversal_specs;
 };
 
+// Names of the grammar that the program reads or writes, in declaration order
+// (inputs, then locals). Applied to the first initial member this is the name
+// table of the search's layout prototype: the reference's generator lists
+// every declared name (cpp/src/evolution/grammar/generate.cpp:32-35) and the
+// member's export compacts that to the names it uses.
+std::vector<std::string> names_used(const Grammar& g, const std::vector<int>& origins);
+
 // `origins`/`consts` are one phase row; `phase` is its expression table.
-Ast export_whole_program(const Grammar& g, const PhaseGrammar& phase, const std::vector<int>& origins,
-                         const std::vector<Val>& consts);
+// `layout_names` is the name table of the search's layout prototype (its
+// first initial member's used names in declaration order): an exported
+// program lists those it uses in that order, then its other names in
+// first-use order (phase_layout.hpp name_index + pack.cpp compaction). An
+// Expression entry is wrapped in the PROGRAM, BLOCK_CONS, RETURN ... BLOCK_NIL
+// envelope; a Program entry is the phase itself.
+Ast export_whole_program(const Grammar& g, const PhaseGrammar& phase, const std::vector<std::string>& layout_names,
+                         const std::vector<int>& origins, const std::vector<Val>& consts);
 json::Value ast_json(const Ast& ast);
 std::string program_text(const Ast& ast);
 

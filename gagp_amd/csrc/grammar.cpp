@@ -192,7 +192,10 @@ class Compiler {
     g_.fuel = positive(exec.at("fuel"), 0x7fffffff, "fuel");
     g_.inputs = scope_of(doc.has("inputs") ? doc.at("inputs") : Value{});
     g_.locals = scope_of(doc.has("locals") ? doc.at("locals") : Value{});
-    if (!g_.locals.empty()) throw Unsupported("grammar locals (statement programs)");
+    if (g_.locals.size() > 16) throw Unsupported("more than 16 grammar locals");
+    for (const auto& l : g_.locals)
+      for (const auto& in : g_.inputs)
+        if (l.name == in.name) throw std::invalid_argument("local declaration conflicts with input name");
     // Resolution order: nonterminals and their alternatives sorted by id.
     std::vector<const Value*> rules;
     for (const auto& r : doc.at("nonterminals").as_array("nonterminals")) rules.push_back(&r);
@@ -205,7 +208,6 @@ class Compiler {
       if (ids_.count(nt.id)) throw std::invalid_argument("duplicate nonterminal id " + nt.id);
       nt.type = parse_type_name(r->at("type").as_string("type"));
       nt.cat = category_of(*r);
-      if (nt.cat != Cat::Expression) throw Unsupported("statement nonterminals");
       if (r->has("variation")) {
         if (r->at("variation").kind != Value::Bool) throw std::invalid_argument("nonterminal variation must be Boolean");
         nt.variation = r->at("variation").b;
@@ -219,7 +221,9 @@ class Compiler {
     g_.entry = reference(entry.at("nonterminal").as_string("entry nonterminal"));
     if (g_.nts[g_.entry].type != parse_type_name(entry.at("type").as_string("entry type")))
       throw std::invalid_argument("entry result type mismatch");
-    if (category_of(entry) != g_.nts[g_.entry].cat) throw std::invalid_argument("entry category mismatch");
+    if (category_of(entry) != g_.nts[g_.entry].cat ||
+        (g_.nts[g_.entry].cat != Cat::Expression && g_.nts[g_.entry].cat != Cat::Program))
+      throw std::invalid_argument("entry requires matching Expression or Program category");
     if (!g_.nts[g_.entry].scope.empty()) throw std::invalid_argument("entry nonterminal requires unavailable lexical bindings");
     for (std::size_t i = 0; i < rules.size(); ++i) {
       std::vector<const Value*> alts;
@@ -358,14 +362,90 @@ class Compiler {
           throw std::invalid_argument("primitive argument type mismatch");
         e.children.push_back(child);
       }
-    } else if (v.has("local") || v.has("control")) {
-      throw Unsupported("statement grammars (locals/controls)");
+    } else if (v.has("local")) {
+      keys(v, {"local", "fuel_events", "resource_charge"}, "local expression");
+      if (v.has("fuel_events") || v.has("resource_charge")) throw Unsupported("fuel profiles and resource charges");
+      e.kind = EKind::Local;
+      e.target = binding(g_.locals, v.at("local").as_string("local"));
+      e.type = g_.locals[e.target].type;
+      e.node_kind = NK_VAR;
+    } else if (v.has("control")) {
+      keys(v, {"control", "type", "args", "name", "input_name", "fuel_events", "resource_charge"}, "control expression");
+      if (v.has("fuel_events") || v.has("resource_charge")) throw Unsupported("fuel profiles and resource charges");
+      control(v, e, env, depth);
     } else if (v.has("template") || v.has("hole") || v.has("structured")) {
       throw Unsupported("templates and structured regions");
     } else {
       throw std::invalid_argument("unknown expression form");
     }
     return append(std::move(e));
+  }
+  // Structural controls (catalog.cpp control signatures; compiled.cpp:498-528):
+  // program(Block)->Program, block_nil()->Block, block_cons(Statement,Block)->Block,
+  // if_stmt(Bool,Block,Block)->Statement, for_range(Int,Block)->Statement and
+  // assign(T)->Statement / return(T)->Statement per value type. Structural
+  // children carry the enclosing result type; assign and for_range name a
+  // declared local of the signature's value type.
+  void control(const Value& v, CExpr& e, const std::vector<Binding>& env, unsigned depth) {
+    const std::string key = v.at("control").as_string("control");
+    const auto open = key.find('('), close = key.find(")->");
+    if (open == std::string::npos || close == std::string::npos || close < open)
+      throw std::invalid_argument("unknown control signature: " + key);
+    const std::string op = key.substr(0, open), list = key.substr(open + 1, close - open - 1), result = key.substr(close + 3);
+    std::vector<std::string> args;
+    {
+      std::stringstream ss(list);
+      std::string item;
+      while (!list.empty() && std::getline(ss, item, ',')) args.push_back(item);
+    }
+    struct Arg { Cat cat; std::uint8_t type; };  // type: TAG_INVALID for a structural argument
+    std::vector<Arg> want;
+    Cat cat;
+    bool named = false;
+    const auto is_value = [](const std::string& t) {
+      try { parse_type_name(t); return true; } catch (const std::invalid_argument&) { return false; }
+    };
+    if (op == "program" && args == std::vector<std::string>{"Block"} && result == "Program") {
+      e.node_kind = NK_PROGRAM; cat = Cat::Program; want = {{Cat::Block, TAG_INVALID}};
+    } else if (op == "block_nil" && args.empty() && result == "Block") {
+      e.node_kind = NK_BLOCK_NIL; cat = Cat::Block;
+    } else if (op == "block_cons" && args == std::vector<std::string>{"Statement", "Block"} && result == "Block") {
+      e.node_kind = NK_BLOCK_CONS; cat = Cat::Block; want = {{Cat::Statement, TAG_INVALID}, {Cat::Block, TAG_INVALID}};
+    } else if (op == "if_stmt" && args == std::vector<std::string>{"Bool", "Block", "Block"} && result == "Statement") {
+      e.node_kind = NK_IF_STMT; cat = Cat::Statement;
+      want = {{Cat::Expression, TAG_BOOL}, {Cat::Block, TAG_INVALID}, {Cat::Block, TAG_INVALID}};
+    } else if (op == "for_range" && args == std::vector<std::string>{"Int", "Block"} && result == "Statement") {
+      e.node_kind = NK_FOR_RANGE; cat = Cat::Statement; named = true;
+      want = {{Cat::Expression, TAG_INT}, {Cat::Block, TAG_INVALID}};
+    } else if ((op == "assign" || op == "return") && args.size() == 1 && is_value(args[0]) && result == "Statement") {
+      e.node_kind = op == "assign" ? NK_ASSIGN : NK_RETURN; cat = Cat::Statement; named = op == "assign";
+      want = {{Cat::Expression, parse_type_name(args[0])}};
+    } else {
+      throw std::invalid_argument("unknown control signature: " + key);
+    }
+    e.kind = EKind::Control;
+    e.cat = cat;
+    e.type = parse_type_name(v.at("type").as_string("control type"));
+    if (named) {
+      if (v.has("name") + v.has("input_name") != 1)
+        throw std::invalid_argument("control requires exactly one name or input_name target");
+      if (v.has("input_name")) throw Unsupported("writes to evaluator inputs (input_name targets)");
+      e.local = int(binding(g_.locals, v.at("name").as_string("name")));
+      if (g_.locals[std::size_t(e.local)].type != want[0].type) throw std::invalid_argument("control target local type mismatch");
+    } else if (v.has("name") || v.has("input_name")) {
+      throw std::invalid_argument("control does not accept a target name");
+    }
+    if (e.node_kind == NK_RETURN && want[0].type != e.type)
+      throw std::invalid_argument("return type disagrees with enclosing result contract");
+    const auto& children = v.at("args").as_array("args");
+    if (children.size() != want.size()) throw std::invalid_argument("control argument count mismatch");
+    for (std::size_t i = 0; i < children.size(); ++i) {
+      const auto child = expression(children[i], env, depth + 1);
+      const auto& c = g_.exprs[child];
+      const std::uint8_t expected = want[i].cat == Cat::Expression ? want[i].type : e.type;
+      if (c.cat != want[i].cat || c.type != expected) throw std::invalid_argument("control argument category or type mismatch");
+      e.children.push_back(child);
+    }
   }
   // minimum_expression_nodes without templates (budget.cpp:19-55).
   std::uint32_t cost(std::uint32_t id, std::uint32_t depth) const {
@@ -454,7 +534,7 @@ PhaseGrammar compile_phase(const Grammar& g, unsigned root, unsigned max_depth, 
     seen_nt[id] = true;
     nt_env[id] = env;
     const auto& nt = g.nts[id];
-    if (nt.cat != Cat::Expression || nt.scope.size() > 8) fail("nonterminal capability");
+    if (nt.scope.size() > 8) fail("nonterminal capability");
     auto& t = out.nts[id];
     t.scope_count = int(nt.scope.size());
     t.production_offset = int(out.prods.size());
@@ -476,7 +556,8 @@ PhaseGrammar compile_phase(const Grammar& g, unsigned root, unsigned max_depth, 
     const auto& e = g.exprs[id];
     auto& x = out.exprs[id];
     if (env.size() > 15) fail("lexical environment capacity");
-    x.node.type = e.type;
+    // A statement completes without a value; its type is the enclosing result.
+    x.node.type = e.cat == Cat::Expression ? e.type : kStatementType;
     if (e.kind == EKind::Reference) {
       Env inner;
       for (auto m : e.scope_mapping) inner.push_back(env.at(m));
@@ -520,6 +601,34 @@ PhaseGrammar compile_phase(const Grammar& g, unsigned root, unsigned max_depth, 
       x.native_kind = NK_VAR;
       x.node.kind = TK_LOCAL;
       x.node.operand = position;
+    } else if (e.kind == EKind::Local) {
+      // A read of a grammar local (phase_grammar.hpp:300-303).
+      if (e.target >= 16) fail("grammar local capacity");
+      x.kind = PK_OPERATION;
+      x.native_kind = NK_VAR;
+      x.node.kind = TK_VAR;
+      x.node.operand = int(e.target);
+      out.statements = true;
+    } else if (e.kind == EKind::Control) {
+      x.kind = PK_OPERATION;
+      x.native_kind = e.node_kind;
+      switch (e.node_kind) {
+        case NK_PROGRAM: x.node.kind = TK_PROGRAM; break;
+        case NK_BLOCK_CONS: x.node.kind = TK_BLOCK; break;
+        case NK_BLOCK_NIL: x.node.kind = TK_NIL; break;
+        case NK_ASSIGN: x.node.kind = TK_ASSIGN; break;
+        case NK_RETURN: x.node.kind = TK_RETURN; break;
+        case NK_IF_STMT: x.node.kind = TK_IF; break;
+        case NK_FOR_RANGE: x.node.kind = TK_FOR_RANGE; break;
+        default: fail("control capability");
+      }
+      if (e.local >= 0) {
+        if (e.local >= 16) fail("grammar local capacity");
+        x.node.operand = e.local;
+      }
+      // Assign/Return carry the exact stored or returned value type.
+      if (e.node_kind == NK_ASSIGN || e.node_kind == NK_RETURN) x.node.type = g.exprs[e.children.at(0)].type;
+      out.statements = true;
     } else if (e.kind == EKind::Bound) {
       // A lexical read: the binder number the environment gives the name.
       if (e.target >= env.size()) fail("bound reference outside its environment");
@@ -739,10 +848,54 @@ PhaseGrammar compile_phase(const Grammar& g, unsigned root, unsigned max_depth, 
   return out;
 }
 
+unsigned hidden_local_bound(const Grammar& g) {
+  const std::size_t levels = std::size_t(g.max_depth) + 1;
+  std::vector<int> memo(g.exprs.size() * levels, -1);
+  std::vector<char> open(g.exprs.size() * levels, 0);
+  // peak(e, d): temporaries live at once inside expression e placed with d
+  // levels of depth left; a reference takes its alternative's place.
+  std::function<int(std::uint32_t, int)> peak = [&](std::uint32_t id, int d) -> int {
+    if (d <= 0) return 0;
+    const std::size_t key = std::size_t(id) * levels + std::size_t(d);
+    if (memo[key] >= 0) return memo[key];
+    if (open[key]) return 64;  // a reference cycle that spends no depth: no finite bound
+    open[key] = 1;
+    const CExpr& e = g.exprs[id];
+    int v = 0;
+    if (e.kind == EKind::Reference) {
+      for (const auto p : g.nts[e.target].productions) v = std::max(v, peak(g.prods[p].expression, d));
+    } else if (e.kind == EKind::Control && e.node_kind == NK_FOR_RANGE) {
+      // The bound temporary is live over the bound expression; the counter
+      // joins it over the body.
+      v = std::max(1 + peak(e.children.at(0), d - 1), 2 + peak(e.children.at(1), d - 1));
+    } else {
+      for (const auto c : e.children) v = std::max(v, peak(c, d - 1));
+      if (e.direction != 0) v += 9;  // a traversal's temporaries
+    }
+    open[key] = 0;
+    return memo[key] = std::min(v, 64);
+  };
+  int v = 0;
+  for (const auto p : g.nts[g.entry].productions) v = std::max(v, peak(g.prods[p].expression, int(g.max_depth)));
+  return unsigned(std::min<std::size_t>(64, g.locals.size() + std::size_t(v)));
+}
+
 Layout whole_program_layout(const Grammar& g) {
   Layout l;
   const auto& entry = g.nts[g.entry];
-  if (entry.cat != Cat::Expression) throw Unsupported("program-entry grammars");
+  if (entry.cat == Cat::Program) {
+    // A Program entry already contains its native envelope: the phase is the
+    // whole program, at the full depth budget, with no terminator.
+    if (g.max_nodes > 2048 || g.max_depth > 64) throw Unsupported("GPU genotype capacity requires fallback");
+    if (entry.minimum_by_depth.at(g.max_depth) > g.max_nodes) throw Unsupported("generation request has no feasible derivation");
+    l.fixed_nodes = 0;
+    l.slot_max_depth = g.max_depth;
+    l.max_variable_nodes = g.max_nodes;
+    l.entry_nt = g.entry;
+    l.program_terminator = false;
+    return l;
+  }
+  if (entry.cat != Cat::Expression) throw Unsupported("entry category");
   if (g.max_depth <= 3 || g.max_nodes <= 4)
     throw Unsupported("expression entry needs four envelope nodes and three prefix levels");
   if (g.max_nodes > 2048 || g.max_depth > 64) throw Unsupported("GPU genotype capacity requires fallback");

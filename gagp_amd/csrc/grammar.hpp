@@ -63,6 +63,8 @@ struct CExpr {
   int direction = 0;
   int region_argument = -1;
   std::vector<Binding> region_bindings;
+  // A named control (assign, for_range): its target grammar local.
+  int local = -1;
 };
 struct CNt {
   std::string id;
@@ -103,6 +105,8 @@ enum TKind : int {
   TK_ASSIGN, TK_RETURN, TK_FOR_RANGE, TK_VAR, TK_LET, TK_BOUND, TK_REGION, TK_TRAVERSE
 };
 constexpr int kMaxArity = 8;
+// Completion type of a statement node (never a runtime value tag).
+constexpr int kStatementType = 0xF0;
 constexpr int kFixedSite = 1 << 30;
 constexpr unsigned kNoId = 0xffffffffu;
 
@@ -160,5 +164,11 @@ struct Layout {
   bool program_terminator = true;
 };
 Layout whole_program_layout(const Grammar& g);
+
+// An upper bound on the hidden locals (slots after the evaluator inputs) a
+// program of the entry nonterminal keeps live at once: every declared
+// grammar local, plus the deepest nesting of for_range bound and counter
+// temporaries and traversal temporaries within the depth budget.
+unsigned hidden_local_bound(const Grammar& g);
 
 }  // namespace gagp_amd

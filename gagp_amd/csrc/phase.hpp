@@ -407,7 +407,59 @@ struct SlotInfo {
   int root, scope, max_depth, logical_depth;
   int max_variable_nodes;
   int row_capacity;  // physical row size
+  int statement;     // the phase is a statement program: variation checks definite assignment
 };
+
+// Definite assignment of grammar locals (phase_compile.cuh:170-200): a local
+// is readable after an assign to it, inside a for_range body that names it,
+// and after an if_stmt only when both branches assigned it; a for_range
+// defines nothing after its body. `OriginAt(i)` gives node i's origin in
+// prefix order; `Frames` holds the open nodes (at most 64).
+struct DefFrame { int origin, seen; unsigned saved, branch; };
+template <class OriginAt, class Frames>
+GHD inline bool locals_defined_with(const PhaseView& g, int n, OriginAt origin_at, Frames& frames) {
+  int depth = 0;
+  unsigned mask = 0;
+  const auto bit = [](int operand) { return operand >= 0 && operand < 16 ? 1u << operand : 0u; };
+  for (int i = 0; i < n; ++i) {
+    const int o = origin_at(i);
+    const PNode& node = g.expr[o].node;
+    if (node.kind == TK_VAR && !(mask & bit(node.operand))) return false;
+    if (node.arity > 0) {
+      if (depth == 64) return false;
+      frames.set(depth++, DefFrame{o, 0, mask, 0});
+      continue;
+    }
+    while (depth) {
+      DefFrame f = frames.get(depth - 1);
+      const PNode& owner = g.expr[f.origin].node;
+      ++f.seen;
+      if (owner.kind == TK_FOR_RANGE && f.seen == 1) {
+        f.saved = mask;
+        mask |= bit(owner.operand);
+      } else if (owner.kind == TK_IF && owner.type == kStatementType) {
+        if (f.seen == 1) f.saved = mask;
+        else if (f.seen == 2) { f.branch = mask; mask = f.saved; }
+        else mask &= f.branch;
+      }
+      if (f.seen < owner.arity) { frames.set(depth - 1, f); break; }
+      if (owner.kind == TK_ASSIGN) mask |= bit(owner.operand);
+      else if (owner.kind == TK_FOR_RANGE) mask = f.saved;
+      --depth;
+    }
+  }
+  return depth == 0;
+}
+struct LocalDefFrames {
+  DefFrame f[64];
+  GHD DefFrame get(int k) const { return f[k]; }
+  GHD void set(int k, const DefFrame& x) { f[k] = x; }
+};
+GHD inline bool locals_defined(const PhaseView& g, const CRow& t) {
+  LocalDefFrames frames;
+  const auto origin_at = [&](int i) { return t.origin[i]; };
+  return locals_defined_with(g, t.n, origin_at, frames);
+}
 
 // Crossover of one child into `out` (a private row per child). `in(m)` gives
 // the current member rows.
@@ -442,7 +494,7 @@ GHD inline void crossover_child(const PhaseView& g, const SlotInfo& s, int child
     // The budget check above bounds the splice by the node budget, which the
     // physical row holds, so the row size never changes an outcome.
     const GenResult r = splice(g, a, dest, b, src, out, s.row_capacity, s.max_depth, s.logical_depth);
-    if (r.status == ST_OK) {
+    if (r.status == ST_OK && (!s.statement || locals_defined(g, CRow{out.origin, out.site, out.cval, r.nodes, r.constants}))) {
       *out_n = r.nodes;
       *out_nc = r.constants;
       st.crossover = 1;
@@ -486,6 +538,8 @@ GHD inline void mutate_child(const PhaseView& g, const SlotInfo& s, int child, R
   const GenResult r = splice(g, a, dest, sub, 0, spliced, s.row_capacity, s.max_depth, s.logical_depth);
   if (r.status != ST_OK) { st.error = 8; return; }
   const CRow sp{spliced.origin, spliced.site, spliced.cval, r.nodes, r.constants};
+  // An undefined local read rejects this mutation; the crossed child stands.
+  if (s.statement && !locals_defined(g, sp)) return;
   copy_row(sp, x, x_n, x_nc);
   st.mutation = 1;
 }
@@ -604,6 +658,7 @@ GHD inline bool loop_operation(int opcode, int builtin, int arity, int a0, int a
 // pending jump sites, the stack height at its start and its operand types.
 struct Frame { int origin, seen, first_jump, second_jump, base_stack; std::uint8_t args[3]; };
 struct LocalFrames {
+  using Def = LocalDefFrames;  // frames of the definite-assignment check
   Frame f[kMaxStack];
   GHD Frame get(int k) const { return f[k]; }
   GHD void set(int k, const Frame& x) { f[k] = x; }
@@ -618,22 +673,33 @@ struct Walk {
   int live_begin, xs, start, begin, limit, length, counter, element, index, state, loop_head, done_jump;
   int saved_slot[3], saved_type[3];
 };
+// An open for_range (phase_compile.cuh:251): its bound and counter
+// temporaries, loop head, exit jump, the pool index of its constant 1 and
+// the live temporaries at its start.
+struct Loop { int bound, counter, head, end_jump, one, live_begin; };
 template <bool kWalks> struct WalkStore {};
 template <> struct WalkStore<true> {
-  int live_count = 0, free_count = 0, walk_depth = 0;
+  int live_count = 0, free_count = 0, walk_depth = 0, loop_depth = 0;
+  unsigned named_mask = 0;  // grammar locals given a slot (on first use)
   signed char live[64], freed[64];
   signed char env_slot[16];
   unsigned char env_type[16];
+  signed char named[16];
   Walk walks[8];
+  Loop loops[16];
 };
 
 // Post-order lowering of an expression phase plus the program terminator,
 // with the reference's jump shapes for if/and/or. Constants are emitted in
 // first-use order (one pool entry per constant node). `OriginAt(i)` gives the
-// origin of node i in prefix order. kWalks adds lexical reads and the
-// traversal lowering of phase_compile.cuh:444-542,596-637,730-740 without
-// source fuel profiles: each emitted instruction costs one, no operand is
-// aliased and begin is a stored temporary.
+// origin of node i in prefix order. kWalks is the instantiation with hidden
+// locals. It adds lexical reads and the traversal lowering of
+// phase_compile.cuh:444-542,596-637,730-740 without source fuel profiles
+// (each emitted instruction costs one, no operand is aliased and begin is a
+// stored temporary). It also adds statement programs (expected type
+// kStatementType, phase_compile.cuh:589-594,660-667,766-860): grammar locals
+// take a slot on first use, for_range keeps a bound and a counter temporary,
+// and no terminator follows the code.
 template <class Frames, class OriginAt, bool kWalks = false>
 GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, OriginAt origin_at, const std::uint8_t* local_types,
                                         int local_count, int expected_type, Ins* code, int code_capacity, Val* pool,
@@ -686,6 +752,28 @@ GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, Origi
     }
     return slot;
   };
+  // The slot of grammar local `operand`, allocated on first use; never freed.
+  const auto variable_slot = [&](int operand) -> int {
+    int slot = -1;
+    if constexpr (kWalks) {
+      if (operand < 0 || operand >= 16) return -1;
+      if (!(ws.named_mask >> operand & 1u)) {
+        if (out.locals >= 64) { out.status = ST_CAPACITY; return -1; }
+        ws.named[operand] = static_cast<signed char>(out.locals++);
+        ws.named_mask |= 1u << operand;
+      }
+      slot = ws.named[operand];
+    }
+    return slot;
+  };
+  const bool statement = expected_type == kStatementType;
+  int returned = TAG_INVALID;  // the value type of the program's returns
+  if (statement) {
+    if (!kWalks) { out.status = ST_UNSUPPORTED; return out; }
+    typename Frames::Def def;
+    if (!locals_defined_with(g, t.n, origin_at, def)) return out;
+    fast = false;
+  }
   // Code after child `c` of traversal `w` (source, start, [begin, end,]
   // initial accumulator, body) completed (walk_after without aliases).
   const auto walk_after = [&](Walk& w, const PNode& node, int c) -> bool {
@@ -825,6 +913,40 @@ GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, Origi
           frames.set(depth++, Frame{o, 0, -1, -1, stack, {TAG_INVALID, TAG_INVALID, TAG_INVALID}});
           continue;
         }
+      } else if (kWalks && node.kind == TK_VAR) {
+        // A read of a grammar local.
+        if (node.arity != 0 || node.operand < 0) return out;
+        if (node.type == TAG_INVALID || node.type == TAG_FALLBACK || node.type == kStatementType) {
+          out.status = ST_UNSUPPORTED;
+          return out;
+        }
+        const int slot = variable_slot(node.operand);
+        if (slot < 0 || !emit(OP_LOAD, slot, 0, node.fuel)) return out;
+        grow(1);
+        completed = node.type;
+        typed = typed && loop_type(completed);
+        have_value = true;
+      } else if (kWalks && node.kind == TK_NIL) {
+        if (node.arity != 0) return out;
+        completed = kStatementType;
+        have_value = true;
+      } else if (kWalks && (node.kind == TK_PROGRAM || node.kind == TK_ASSIGN || node.kind == TK_RETURN ||
+                            node.kind == TK_BLOCK || node.kind == TK_FOR_RANGE)) {
+        if constexpr (kWalks) {
+          const bool pair = node.kind == TK_BLOCK || node.kind == TK_FOR_RANGE;
+          if (node.arity != (pair ? 2 : 1)) return out;
+          if (depth == 64) { out.status = ST_CAPACITY; return out; }
+          if (node.kind == TK_FOR_RANGE) {
+            // The bound temporary precedes the bound expression's own.
+            if (ws.loop_depth == 16) { out.status = ST_CAPACITY; return out; }
+            Loop& loop = ws.loops[ws.loop_depth++];
+            loop.live_begin = ws.live_count;
+            loop.bound = temporary();
+            if (loop.bound < 0) return out;
+          }
+          frames.set(depth++, Frame{o, 0, -1, -1, stack, {TAG_INVALID, TAG_INVALID, TAG_INVALID}});
+          continue;
+        }
       } else if (node.kind == TK_LOCAL) {
         if (node.operand < 0 || node.operand >= local_count) return out;
         completed = local_types[node.operand];
@@ -845,16 +967,18 @@ GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, Origi
       }
     }
     if (depth == 0) {
-      if (cursor != t.n || stack != 1 || completed != expected_type) return out;
+      if (cursor != t.n || stack != (statement ? 0 : 1) || completed != expected_type) return out;
       if (out.stack_bound > kMaxStack) { out.status = ST_CAPACITY; return out; }
-      // program terminator (whole-program expression entry)
-      if (!emit(OP_RETURN, 0, 0, 1)) return out;
+      // program terminator (whole-program expression entry); a statement
+      // program that runs off its end fails with ValueError
+      if (!statement && !emit(OP_RETURN, 0, 0, 1)) return out;
       out.status = ST_OK;
-      out.result_type = completed;
+      // A statement program's value is that of its returns.
+      out.result_type = statement ? returned : completed;
       out.fast = fast && !out.jumps && fast_type(completed) && out.stack_bound <= kFastStack && out.code <= kFastCode &&
                  out.constants <= kFastPool;
-      out.loop = !out.fast && typed && fast_type(completed) && out.stack_bound <= kLoopStack && out.code <= kLoopCode &&
-                 out.constants <= kLoopPool && out.locals - local_count <= kLoopLocals;
+      out.loop = !out.fast && typed && fast_type(out.result_type) && out.stack_bound <= kLoopStack &&
+                 out.code <= kLoopCode && out.constants <= kLoopPool && out.locals - local_count <= kLoopLocals;
       return out;
     }
     Frame f = frames.get(depth - 1);
@@ -890,8 +1014,77 @@ GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, Origi
       code[f.first_jump].a = out.code;
       stack = f.base_stack;
     }
+    if constexpr (kWalks) {
+      if (node.kind == TK_FOR_RANGE && f.seen == 1) {
+        // Validate the stored bound (a non-Int or negative bound reaches
+        // NEG(true), a runtime error), then count from zero; every
+        // instruction costs one.
+        Loop& loop = ws.loops[ws.loop_depth - 1];
+        if (completed == kStatementType) return out;
+        // On an Int bound IS_INT is always true and NEG(true) is reached only
+        // by a negative bound; the looping path then raises TypeError for the
+        // NEG that argc 1 marks.
+        typed = typed && completed == TAG_INT;
+        if (!put_store(loop.bound)) return out;
+        if (!put_load(loop.bound) || !put(OP_CALL_BUILTIN, B_IS_INT, 1, 0)) return out;
+        const int bad = put_jump(OP_JMP_IF_FALSE, -1);
+        if (bad < 0 || !put_load(loop.bound) || !put_int(0) || !put(OP_LT, 0, 0, -1)) return out;
+        const int valid = put_jump(OP_JMP_IF_FALSE, -1);
+        if (valid < 0) return out;
+        code[bad].a = out.code;
+        if (out.constants >= pool_capacity) { out.status = ST_CAPACITY; return out; }
+        pool[out.constants] = val_bool(true);
+        if (!put(OP_PUSH_CONST, out.constants, 0, 1)) return out;
+        ++out.constants;
+        if (!put(OP_NEG, 0, 1, 0)) return out;
+        stack = f.base_stack;
+        code[valid].a = out.code;
+        if (out.constants + 2 > pool_capacity) { out.status = ST_CAPACITY; return out; }
+        const int zero = out.constants++;
+        pool[zero] = val_int(0);
+        loop.one = out.constants++;
+        pool[loop.one] = val_int(1);
+        loop.counter = temporary();
+        const int user = variable_slot(node.operand);
+        if (loop.counter < 0 || user < 0) return out;
+        if (!put(OP_PUSH_CONST, zero, 0, 1) || !put_store(loop.counter)) return out;
+        loop.head = out.code;
+        if (!put_load(loop.counter) || !put_load(loop.bound) || !put(OP_LT, 0, 0, -1)) return out;
+        loop.end_jump = put_jump(OP_JMP_IF_FALSE, -1);
+        if (loop.end_jump < 0 || !put_load(loop.counter) || !put_store(user)) return out;
+      }
+    }
     if (f.seen < node.arity) { frames.set(depth - 1, f); have_value = false; continue; }
-    if (node.kind == TK_IF) {
+    if (kWalks && node.kind == TK_PROGRAM) {
+      if (completed != kStatementType) return out;
+    } else if (kWalks && node.kind == TK_BLOCK) {
+      if (f.args[0] != kStatementType || f.args[1] != kStatementType) return out;
+    } else if (kWalks && (node.kind == TK_ASSIGN || node.kind == TK_RETURN)) {
+      // The grammar fixes the stored or returned type; an assigned local
+      // takes its slot after its value expression.
+      if (completed != node.type || completed == kStatementType) return out;
+      if (node.kind == TK_ASSIGN) {
+        const int slot = variable_slot(node.operand);
+        if (slot < 0 || !emit(OP_STORE, slot, 0, node.fuel)) return out;
+      } else {
+        if (!emit(OP_RETURN, 0, 0, node.fuel)) return out;
+        typed = typed && (returned == TAG_INVALID || returned == completed);
+        returned = completed;
+      }
+      stack = f.base_stack;
+      completed = kStatementType;
+    } else if (kWalks && node.kind == TK_FOR_RANGE) {
+      if constexpr (kWalks) {
+        if (completed != kStatementType) return out;
+        const Loop loop = ws.loops[--ws.loop_depth];
+        if (!put_load(loop.counter) || !put(OP_PUSH_CONST, loop.one, 0, 1) || !put(OP_ADD, 0, 0, -1) ||
+            !put_store(loop.counter) || !emit(OP_JMP, loop.head, 0, 1))
+          return out;
+        code[loop.end_jump].a = out.code;
+        stack = f.base_stack;
+        while (ws.live_count > loop.live_begin) ws.freed[ws.free_count++] = ws.live[--ws.live_count];
+      }
+    } else if (node.kind == TK_IF) {
       if (f.args[1] != f.args[2]) return out;
       code[f.second_jump].a = out.code;
     } else if (node.kind == TK_AND || node.kind == TK_OR) {

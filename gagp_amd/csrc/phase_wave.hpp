@@ -106,8 +106,22 @@ struct WaveTasks {
     else { e2 = wrlane(x.e, ln, e2); d2 = wrlane(x.d, ln, d2); s2 = wrlane(x.sc, ln, s2); l2 = wrlane(x.l, ln, l2); }
   }
 };
+// Open nodes of the definite-assignment check: frame k in lane k.
+struct WaveDefFrames {
+  int origin = 0, seen = 0, saved = 0, branch = 0;
+  __device__ DefFrame get(int k) const {
+    return DefFrame{rdlane(origin, k), rdlane(seen, k), unsigned(rdlane(saved, k)), unsigned(rdlane(branch, k))};
+  }
+  __device__ void set(int k, const DefFrame& f) {
+    origin = wrlane(f.origin, k, origin);
+    seen = wrlane(f.seen, k, seen);
+    saved = wrlane(int(f.saved), k, saved);
+    branch = wrlane(int(f.branch), k, branch);
+  }
+};
 // Open operations of the lowering walk: frame k in lane k.
 struct WaveFrames {
+  using Def = WaveDefFrames;
   int origin = 0, packed = 0, first = 0, second = 0, base = 0;  // packed: seen | args << 8
   __device__ Frame get(int k) const {
     const int p = rdlane(packed, k);
@@ -330,6 +344,27 @@ __device__ inline GenResult splice_w(const PhaseView& g, const CRow& base, int d
   return r;
 }
 
+// Origins of a row read 64 at a time, one per lane, and handed out uniformly.
+struct WaveOrigins {
+  const CRow& t;
+  int tile = -64, tile_origin = 0;
+  __device__ int operator()(int i) {
+    if (i - tile >= 64 || i < tile) {
+      tile = i & ~63;
+      const int k = tile + wlane();
+      tile_origin = k < t.n ? t.origin[k] : 0;
+    }
+    return rdlane(tile_origin, i - tile);
+  }
+};
+// locals_defined by the whole wavefront (phase_operators.cuh:148-154).
+__device__ inline bool locals_defined_w(const PhaseView& g, const CRow& t) {
+  wave_fence();
+  WaveDefFrames frames;
+  WaveOrigins origin_at{t};
+  return locals_defined_with(g, t.n, origin_at, frames);
+}
+
 // crossover_child (phase_operators.cuh:156-199).
 template <class MemberRow>
 __device__ inline void crossover_w(const PhaseView& g, const SlotInfo& s, int child, MemberRow in, Row& out, const Keys& keys,
@@ -358,7 +393,7 @@ __device__ inline void crossover_w(const PhaseView& g, const SlotInfo& s, int ch
     if (ae < 0 || be < 0) { st.error = 3; return; }
     if (total - (ae - dest) + (be - src) > s.max_variable_nodes) { ++st.crossover_rejected; continue; }
     const GenResult r = splice_w(g, a, dest, ae, b, src, be, out, s.row_capacity, s.max_depth, s.logical_depth);
-    if (r.status == ST_OK) { st.crossover = 1; return; }
+    if (r.status == ST_OK && (!s.statement || locals_defined_w(g, as_const(out)))) { st.crossover = 1; return; }
     copy_row_w(a, out);  // failed speculation restores the phase
     ++st.crossover_rejected;
   }
@@ -401,6 +436,8 @@ __device__ inline void mutate_w(const PhaseView& g, const SlotInfo& s, int child
   const int sub_end = subtree_end_w(g, sub, 0);
   const GenResult r = splice_w(g, a, dest, end, sub, 0, sub_end, spliced, s.row_capacity, s.max_depth, s.logical_depth);
   if (r.status != ST_OK) { st.error = 8; return; }
+  // An undefined local read rejects this mutation; the crossed child stands.
+  if (s.statement && !locals_defined_w(g, as_const(spliced))) return;
   copy_row_w(as_const(spliced), x);
   st.mutation = 1;
 }
@@ -439,17 +476,9 @@ template <bool kWalks = false>
 __device__ inline LowerResult lower_phase_w(const PhaseView& g, const CRow& t, const std::uint8_t* local_types, int local_count,
                                             int expected_type, Ins* code, int code_capacity, Val* pool, int pool_capacity) {
   WaveFrames frames;
-  int tile = -64, tile_origin = 0;
-  const auto origin_at = [&](int i) {
-    if (i - tile >= 64 || i < tile) {
-      tile = i & ~63;
-      const int k = tile + wlane();
-      tile_origin = k < t.n ? t.origin[k] : 0;
-    }
-    return rdlane(tile_origin, i - tile);
-  };
-  return lower_phase_with<WaveFrames, decltype(origin_at), kWalks>(g, t, origin_at, local_types, local_count, expected_type, code,
-                                                                   code_capacity, pool, pool_capacity, frames);
+  WaveOrigins origin_at{t};
+  return lower_phase_with<WaveFrames, WaveOrigins&, kWalks>(g, t, origin_at, local_types, local_count, expected_type, code,
+                                                            code_capacity, pool, pool_capacity, frames);
 }
 
 }  // namespace gagp_amd

@@ -695,7 +695,11 @@ __device__ inline LaneResult vm_loop(const LoopProgram& f, const FastInputs& in,
       case OP_JMP_IF_FALSE: next = S(top) == 0 ? a : ip + 1; break;
       case OP_JMP_IF_TRUE: next = S(top) != 0 ? a : ip + 1; break;
       case OP_CHECK_INT: break;
-      case OP_NEG: if (run) S(top) = wrap_neg(S(top)); break;
+      // argc 1 marks a NEG of a Bool (the for_range bound check): TypeError.
+      case OP_NEG:
+        if (run && argc == 1) err = ERR_TYPE;
+        else if (run) S(top) = wrap_neg(S(top));
+        break;
       case OP_NOT: if (run) S(top) = std::int64_t(S(top) == 0); break;
       case OP_ADD: case OP_SUB: case OP_MUL: case OP_LT: case OP_LE: case OP_GT: case OP_GE: case OP_EQ: case OP_NE: {
         const std::int64_t x = S(below), y = S(top);
@@ -749,6 +753,7 @@ __device__ inline LaneResult vm_loop(const LoopProgram& f, const FastInputs& in,
             break;
           }
           case B_LEN: out = std::int64_t(std::uint64_t(x) & 0xffff); break;
+          case B_IS_INT: out = 1; break;  // certified on an Int operand
           case B_INDEX: case B_INDEX_NONNEG: case B_INDEX_CLAMPED: {  // as in vm_run
             const std::int64_t n = std::int64_t(std::uint64_t(x) & 0xffff);
             const std::uint64_t off = std::uint64_t(x) >> 16;

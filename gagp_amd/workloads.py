@@ -197,6 +197,91 @@ def list_grammar(constants, max_nodes: int = 40, max_depth: int = 11, fuel: int 
     }
 
 
+def statement_grammar(constants, max_nodes: int = 60, max_depth: int = 12, fuel: int = 4096, with_temp: bool = False,
+                      with_traverse: bool = False) -> dict:
+    """A statement program over the list ``xs``: it sets the accumulator
+    ``acc``, then runs statements (assignments to ``acc``, a ``for_range``
+    loop over ``i``, ``if_stmt``) and ends with a ``return``. Reads of ``i``
+    outside a loop body fail definite assignment and are never generated.
+    ``with_temp`` adds a second local ``t``; ``with_traverse`` makes the
+    first assignment a forward traversal of ``xs`` (exactly one per
+    program)."""
+    i_ = {"ref": "St.Int"}
+    b_ = {"ref": "St.Bool"}
+    body = {"ref": "St.Body"}
+    xs = {"input": "xs"}
+    local = lambda name: {"local": name}  # noqa: E731
+    assign = lambda name, arg: {"control": "assign(Int)->Statement", "type": "Int", "name": name, "args": [arg]}  # noqa: E731
+    cons = lambda a, b: {"control": "block_cons(Statement,Block)->Block", "type": "Int", "args": [a, b]}  # noqa: E731
+    nil = {"control": "block_nil()->Block", "type": "Int", "args": []}
+    ints = [
+        ("acc", local("acc")),
+        ("i", local("i")),
+        ("elem", _op("index(IntList,Int)->Int", xs, local("i"))),
+        ("len", _op("len(IntList)->Int", xs)),
+        ("const", _const(constants)),
+        ("add", _op("add(Int,Int)->Int", i_, i_)),
+        ("sub", _op("sub(Int,Int)->Int", i_, i_)),
+        ("mod", _op("mod(Int,Int)->Int", i_, i_)),
+    ]
+    stmts = [
+        ("acc", 3, assign("acc", i_)),
+        ("for", 2, {"control": "for_range(Int,Block)->Statement", "type": "Int", "name": "i",
+                    "args": [{"ref": "St.Bound"}, body]}),
+        ("if", 2, {"control": "if_stmt(Bool,Block,Block)->Statement", "type": "Int", "args": [b_, body, body]}),
+    ]
+    locals_ = [{"name": "acc", "type": "Int"}, {"name": "i", "type": "Int"}]
+    if with_temp:
+        ints.append(("t", local("t")))
+        stmts.append(("t", 1, assign("t", i_)))
+        locals_.append({"name": "t", "type": "Int"})
+    first = i_
+    extra = []
+    if with_traverse:
+        first = {"signature": "traverse(IntList,Int,Int,Int)->Int",
+                 "args": [xs, _const([0]), _const([0]), {"ref": "St.Fold"}], "bind": {"3": ["e", "k", "a"]}}
+        scope = [{"name": "e", "type": "Int"}, {"name": "k", "type": "Int"}, {"name": "a", "type": "Int"}]
+        fold = {"ref": "St.Fold"}
+        extra = [{"id": "St.Fold", "type": "Int", "scope": scope, "alternatives": [
+            {"id": "add", "weight": 2, "expression": _op("add(Int,Int)->Int", fold, fold)},
+            {"id": "mod", "weight": 1, "expression": _op("mod(Int,Int)->Int", fold, fold)},
+            {"id": "e", "weight": 2, "expression": {"bound": "e"}},
+            {"id": "k", "weight": 1, "expression": {"bound": "k"}},
+            {"id": "a", "weight": 2, "expression": {"bound": "a"}},
+            {"id": "const", "weight": 1, "expression": _const(constants)}]}]
+    return {
+        "format_version": "grammar-definition-v2",
+        "entry": {"category": "Program", "nonterminal": "St.Program", "type": "Int"},
+        "inputs": [{"name": "xs", "type": "IntList"}],
+        "locals": locals_,
+        "search_limits": {"max_nodes": max_nodes, "max_depth": max_depth},
+        "execution_limits": {"fuel": fuel},
+        "nonterminals": [
+            {"id": "St.Program", "category": "Program", "type": "Int", "scope": [], "variation": False, "alternatives": [
+                {"id": "program", "weight": 1, "expression": {
+                    "control": "program(Block)->Program", "type": "Int",
+                    "args": [cons(assign("acc", first), {"ref": "St.Tail"})]}}]},
+            {"id": "St.Tail", "category": "Block", "type": "Int", "scope": [], "alternatives": [
+                {"id": "more", "weight": 3, "expression": cons({"ref": "St.Stmt"}, {"ref": "St.Tail"})},
+                {"id": "final", "weight": 1, "expression": cons(
+                    {"control": "return(Int)->Statement", "type": "Int", "args": [i_]}, nil)}]},
+            {"id": "St.Body", "category": "Block", "type": "Int", "scope": [], "alternatives": [
+                {"id": "nil", "weight": 2, "expression": nil},
+                {"id": "cons", "weight": 2, "expression": cons({"ref": "St.Stmt"}, body)}]},
+            {"id": "St.Stmt", "category": "Statement", "type": "Int", "scope": [], "alternatives": [
+                {"id": name, "weight": w, "expression": e} for name, w, e in stmts]},
+            {"id": "St.Bound", "type": "Int", "scope": [], "alternatives": [
+                {"id": "len", "weight": 3, "expression": _op("len(IntList)->Int", xs)},
+                {"id": "const", "weight": 1, "expression": _const([0, 2, 3])}]},
+            {"id": "St.Int", "type": "Int", "scope": [], "alternatives": [
+                {"id": name, "weight": 1, "expression": e} for name, e in ints]},
+            {"id": "St.Bool", "type": "Bool", "scope": [], "alternatives": [
+                {"id": "eq", "weight": 2, "expression": _op("eq(Int,Int)->Bool", i_, i_)},
+                {"id": "lt", "weight": 1, "expression": _op("lt(Int,Int)->Bool", i_, i_)}]},
+        ] + extra,
+    }
+
+
 LIST_KINDS = ("sum", "max", "count_gt", "sum_even")
 
 
@@ -217,21 +302,28 @@ def _list_cases(rng: SplitMix64, kind: str, k: int, n: int, max_len: int):
 
 
 def list_task(directory: str, kind: str, seed: int, n_train: int = 200, n_test: int = 400, max_len: int = 64,
-              fold: str = "traverse", nested: bool = False, fuel: int = 4096) -> str:
+              fold: str = "traverse", nested: bool = False, fuel: int = 4096, **statement) -> str:
     """A list reduction (``LIST_KINDS``); count_gt draws k ~ U{-100..100},
-    the max of an empty list is -1001."""
+    the max of an empty list is -1001. ``fold`` "statement" writes a
+    statement program grammar (``statement`` holds its options)."""
     rng = SplitMix64(seed)
     k = rng.uniform_int(-100, 100)
     train = _list_cases(rng, kind, k, n_train, max_len)
     test = _list_cases(rng, kind, k, n_test, max_len)
     constants = {"sum": [0, 1, 2], "max": [0, 1, -1001], "count_gt": [0, 1, k], "sum_even": [0, 1, 2]}[kind]
-    return write_task(directory, "list_%s" % kind, list_grammar(constants, fuel=fuel, fold=fold, nested=nested), train, test)
+    if fold == "statement":
+        grammar = statement_grammar(constants, fuel=fuel, **statement)
+    else:
+        grammar = list_grammar(constants, fuel=fuel, fold=fold, nested=nested)
+    return write_task(directory, "list_%s" % kind, grammar, train, test)
 
 
-def sum_even(directory: str, seed: int = 3) -> str:
+def sum_even(directory: str, seed: int = 3, fold: str = "traverse") -> str:
     """The loop-heavy search: sum of the even elements of an Int list,
-    200 train / 400 test cases, lengths 0..50, values ~ U{-1000..1000}."""
-    return list_task(directory, "sum_even", seed, 200, 400, 50)
+    200 train / 400 test cases, lengths 0..50, values ~ U{-1000..1000};
+    ``fold`` "statement" searches statement programs (a for_range loop over
+    the list, an accumulator local, if_stmt) instead of a traversal."""
+    return list_task(directory, "sum_even", seed, 200, 400, 50, fold=fold)
 
 
 def log_uniform_int(rng: SplitMix64, lo: int, hi: int) -> int:

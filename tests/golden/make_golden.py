@@ -9,9 +9,11 @@ expected values or error names) into tests/golden/bytecode/.
 
 It also writes tests/golden/traversal/<task>.json: the reference's
 `gagp eval --split train` of the programs tests/host_lowering.cpp exports for
-the list tasks (seeded random traversal programs), which
+the list tasks (seeded random traversal and statement programs), which
 tests/test_host_lowering.py compares with the engine's lowering run on the
-oracle VM.
+oracle VM, and tests/golden/definite/<task>.json: whether the reference's
+verifier accepts random statement programs, some of which read unassigned
+locals.
 
     python tests/golden/make_golden.py            (from the repository root)
     python tests/golden/make_golden.py --only 'list_'   (runs whose name matches)
@@ -66,9 +68,19 @@ RUNS = {
     "list_lowfuel_p400_g8_s8": ("list_lowfuel", 400, 8, 8, False),
     "list_max_range_p200_g6_s9": ("list_max_range", 200, 6, 9, False),
     "list_count_gt_rev_gen_e1_p256_g8_s2": ("list_count_gt_rev", 256, 8, 2, False, "generational_e1.json"),
+    # statement programs: the loop-heavy sum of even elements as a for_range
+    # loop with an accumulator local (full size and small), two locals with a
+    # fixed traversal, and a low-fuel variant whose long loops time out
+    "stmt_sum_even_p8192_g64_s0": ("stmt_sum_even", 8192, 64, 0, False),
+    "stmt_sum_even_p512_g10_s5": ("stmt_sum_even", 512, 10, 5, True),
+    "stmt_count_gt_p300_g8_s6": ("stmt_count_gt", 300, 8, 6, False),
+    "stmt_lowfuel_p400_g8_s8": ("stmt_lowfuel", 400, 8, 8, False),
+    "stmt_count_gt_gen_e1_p256_g8_s2": ("stmt_count_gt", 256, 8, 2, False, "generational_e1.json"),
 }
 # Tasks whose random traversal programs are checked against `gagp eval`.
-TRAVERSAL_TASKS = ["list-sum", "list_count_gt_rev", "list_range_nested", "list_lowfuel", "list_max_range", "list_sum_even"]
+STATEMENT_TASKS = ["stmt_sum_even", "stmt_count_gt", "stmt_lowfuel"]
+TRAVERSAL_TASKS = ["list-sum", "list_count_gt_rev", "list_range_nested", "list_lowfuel", "list_max_range",
+                   "list_sum_even"] + STATEMENT_TASKS
 
 
 def list_tasks():
@@ -78,6 +90,10 @@ def list_tasks():
                         nested=True, fuel=420)
     workloads.list_task(os.path.join(TASKS, "list_lowfuel"), "sum_even", 12, 64, 32, 40, fuel=300)
     workloads.list_task(os.path.join(TASKS, "list_max_range"), "max", 9, 48, 32, 40, fold="traverse_range")
+    workloads.sum_even(os.path.join(TASKS, "stmt_sum_even"), 3, fold="statement")
+    workloads.list_task(os.path.join(TASKS, "stmt_count_gt"), "count_gt", 22, 64, 32, 40, fold="statement", with_temp=True,
+                        with_traverse=True)
+    workloads.list_task(os.path.join(TASKS, "stmt_lowfuel"), "sum_even", 23, 64, 32, 40, fold="statement", fuel=300)
 
 
 def traversal_goldens(n=40):
@@ -86,16 +102,11 @@ def traversal_goldens(n=40):
     out_dir = os.path.join(GOLDEN, "traversal")
     os.makedirs(out_dir, exist_ok=True)
     with tempfile.TemporaryDirectory() as tmp:
-        driver = os.path.join(tmp, "host_lowering")
-        build = os.path.join(ROOT, "gagp_amd", "_build")
-        subprocess.run(["g++", "-O1", "-std=c++17", "-I" + os.path.join(ROOT, "gagp_amd", "csrc"),
-                        os.path.join(ROOT, "tests", "host_lowering.cpp"), os.path.join(build, "grammar.o"),
-                        os.path.join(build, "task.o"), os.path.join(build, "export.o"),
-                        os.path.join(ROOT, "oracle", "liboracle.so"), "-Wl,-rpath," + os.path.join(ROOT, "oracle"),
-                        "-o", driver], check=True)
+        driver = build_driver(tmp)
         for task_dir in TRAVERSAL_TASKS:
             task = os.path.join(TASKS, task_dir, "task.json")
-            lines = subprocess.run([driver, task, str(n)], check=True, capture_output=True, text=True).stdout.splitlines()
+            lines = subprocess.run([driver, task, str(n)], check=True, capture_output=True, text=True,
+                                   cwd=ROOT).stdout.splitlines()
             programs = []
             for line in lines:
                 row = json.loads(line)
@@ -109,6 +120,44 @@ def traversal_goldens(n=40):
             with open(os.path.join(out_dir, task_dir + ".json"), "w") as f:
                 json.dump({"task": os.path.relpath(task, GOLDEN), "n": n, "programs": programs}, f, indent=1)
             print("traversal golden", task_dir, len(programs))
+
+
+def definite_assignment_goldens(n=150):
+    """For the statement tasks: one random draw per program (no retry), and
+    whether the reference's verifier accepts the exported AST (`gagp eval`
+    rejects a read of an unassigned local with exit 2, undefined_local)."""
+    out_dir = os.path.join(GOLDEN, "definite")
+    os.makedirs(out_dir, exist_ok=True)
+    with tempfile.TemporaryDirectory() as tmp:
+        driver = build_driver(tmp)
+        for task_dir in STATEMENT_TASKS:
+            task = os.path.join(TASKS, task_dir, "task.json")
+            lines = subprocess.run([driver, task, str(n), "defs"], check=True, capture_output=True, text=True,
+                                   cwd=ROOT).stdout.splitlines()
+            programs = []
+            for line in lines:
+                row = json.loads(line)
+                path = os.path.join(tmp, "p.json")
+                with open(path, "w") as f:
+                    json.dump({"best": {"program": row["program"]}}, f)
+                r = subprocess.run([REF, "eval", task, path, "--split", "train"], capture_output=True, text=True)
+                if r.returncode not in (0, 2) or (r.returncode == 2 and "undefined_local" not in r.stderr):
+                    raise RuntimeError("unexpected eval outcome: %s" % r.stderr)
+                programs.append({"i": row["i"], "text": row["text"], "accepted": r.returncode == 0})
+            with open(os.path.join(out_dir, task_dir + ".json"), "w") as f:
+                json.dump({"task": os.path.relpath(task, GOLDEN), "n": n, "programs": programs}, f, indent=1)
+            print("definite assignment golden", task_dir, len(programs), sum(p["accepted"] for p in programs))
+
+
+def build_driver(tmp):
+    driver = os.path.join(tmp, "host_lowering")
+    build = os.path.join(ROOT, "gagp_amd", "_build")
+    subprocess.run(["g++", "-O1", "-std=c++17", "-I" + os.path.join(ROOT, "gagp_amd", "csrc"),
+                    os.path.join(ROOT, "tests", "host_lowering.cpp"), os.path.join(build, "grammar.o"),
+                    os.path.join(build, "task.o"), os.path.join(build, "export.o"),
+                    os.path.join(ROOT, "oracle", "liboracle.so"), "-Wl,-rpath," + os.path.join(ROOT, "oracle"),
+                    "-o", driver], check=True)
+    return driver
 
 
 def run_reference(task, population, generations, seed, out, save_all, search=None):
@@ -161,6 +210,8 @@ def main():
         json.dump(manifest, f, indent=1, sort_keys=True)
     if not args.only or re.search(args.only, "traversal"):
         traversal_goldens()
+    if not args.only or re.search(args.only, "definite"):
+        definite_assignment_goldens()
     if os.path.isdir(REFERENCE_FIXTURES):
         dst = os.path.join(GOLDEN, "bytecode")
         os.makedirs(dst, exist_ok=True)

@@ -6,6 +6,8 @@
 // reference's own evaluation (`gagp eval --split train`) of the same ASTs.
 //
 //   host_lowering TASK.json N
+//   host_lowering TASK.json N defs   (statement tasks: one draw per program,
+//                                     its definite-assignment verdict and AST)
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -33,7 +35,8 @@ void orc_eval_fitness(const OProg* programs, int n_programs, const gagp_amd::Val
 using namespace gagp_amd;
 
 int main(int argc, char** argv) {
-  if (argc != 3) { std::fprintf(stderr, "usage: host_lowering TASK.json N\n"); return 2; }
+  if (argc != 3 && argc != 4) { std::fprintf(stderr, "usage: host_lowering TASK.json N [defs]\n"); return 2; }
+  const bool defs = argc == 4 && std::string(argv[3]) == "defs";
   const Task task = load_task(argv[1]);
   const int n = std::atoi(argv[2]);
   const Grammar g = load_grammar_file(task.grammar_path);
@@ -48,17 +51,39 @@ int main(int argc, char** argv) {
   std::vector<int> origin(cap), site(cap);
   std::vector<Val> cval(cap);
   // The engine's per-program capacities.
-  std::vector<Ins> code(std::size_t(4 * R + 8 + 64 * ph.traversals));
-  std::vector<Val> pool(std::size_t(2 * R + 8 + 8 * ph.traversals));
-  for (int i = 0; i < n; ++i) {
+  const bool statement = g.nts[g.entry].cat == Cat::Program;
+  const int loops = statement ? (R + 2) / 3 : 0;
+  std::vector<Ins> code(std::size_t(4 * R + 8 + 64 * ph.traversals + 23 * loops));
+  std::vector<Val> pool(std::size_t(2 * R + 8 + 8 * ph.traversals + 4 * loops));
+  for (int i = 0; defs && i < n; ++i) {
+    const GenResult gen = generate_phase(gv, gv.root, 0, int(layout.slot_max_depth), R, 0x51ed2701ull * std::uint64_t(i + 1),
+                                         origin.data(), cval.data(), site.data(), 2 + i % 6);
+    if (gen.status != ST_OK) continue;
+    const std::vector<int> row_origin(origin.begin(), origin.begin() + gen.nodes);
+    const Ast ast = export_whole_program(g, ph, names_used(g, row_origin), row_origin,
+                                         std::vector<Val>(cval.begin(), cval.begin() + gen.constants));
+    json::Value line = json::make_object();
+    line.obj["i"] = json::make_number(i);
+    line.obj["defined"] = json::make_bool(locals_defined(gv, CRow{origin.data(), site.data(), cval.data(), gen.nodes, gen.constants}));
+    line.obj["text"] = json::make_string(program_text(ast));
+    line.obj["program"] = ast_json(ast);
+    std::printf("%s\n", json::canonical(line).c_str());
+  }
+  for (int i = 0; !defs && i < n; ++i) {
     GenResult gen{ST_INVALID, 0, 0, 0};
     for (int attempt = 0; gen.status != ST_OK && attempt < 64; ++attempt)
+    {
       gen = generate_phase(gv, gv.root, 0, int(layout.slot_max_depth), R, 0x9e3779b97f4a7c15ull * std::uint64_t(i + 1) + attempt,
                            origin.data(), cval.data(), site.data(), 2 + i % 6);
+      // A statement program must read only assigned locals.
+      if (gen.status == ST_OK && statement && !locals_defined(gv, CRow{origin.data(), site.data(), cval.data(), gen.nodes, gen.constants}))
+        gen.status = ST_INVALID;
+    }
     if (gen.status != ST_OK) continue;
     const CRow row{origin.data(), site.data(), cval.data(), gen.nodes, gen.constants};
     const LowerResult lr = lower_phase<true>(gv, row, cases.input_types.data(), int(cases.input_types.size()),
-                                             g.nts[g.entry].type, code.data(), int(code.size()), pool.data(), int(pool.size()));
+                                             statement ? kStatementType : int(g.nts[g.entry].type), code.data(), int(code.size()),
+                                             pool.data(), int(pool.size()));
     if (lr.status != ST_OK) { std::printf("{\"i\":%d,\"lowering_status\":%d}\n", i, lr.status); continue; }
     std::vector<OIns> oc;
     std::vector<std::uint32_t> fuel;
@@ -75,7 +100,8 @@ int main(int argc, char** argv) {
     orc_eval_fitness(&p, 1, cases.inputs.data(), cases.set.data(), cases.n, int(cases.input_names.size()), cases.expected.data(),
                      cases.arena.data(), int(g.fuel), task.case_cap, 512, task.policy.metric, task.policy.proximity, &fit, &wrong,
                      counts);
-    const Ast ast = export_whole_program(g, ph, std::vector<int>(origin.begin(), origin.begin() + gen.nodes),
+    const std::vector<int> row_origin(origin.begin(), origin.begin() + gen.nodes);
+    const Ast ast = export_whole_program(g, ph, names_used(g, row_origin), row_origin,
                                          std::vector<Val>(cval.begin(), cval.begin() + gen.constants));
     json::Value line = json::make_object();
     line.obj["i"] = json::make_number(i);

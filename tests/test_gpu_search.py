@@ -13,7 +13,8 @@ from _util import GOLDEN, strip_timing
 
 pytestmark = pytest.mark.gpu
 MANIFEST = json.load(open(os.path.join(GOLDEN, "runs", "manifest.json")))
-FULL = ("quintic_p4096_g128_s0", "list_sum_even_p8192_g64_s0")  # the full-size single searches
+# the full-size single searches
+FULL = ("quintic_p4096_g128_s0", "list_sum_even_p8192_g64_s0", "stmt_sum_even_p8192_g64_s0")
 SMALL = sorted(n for n in MANIFEST if n not in FULL)
 
 
@@ -67,12 +68,18 @@ def test_graph_and_stream_execution_agree():
     assert results[0] == results[1] == strip_timing(golden(name))
 
 
-def test_cli_drop_in(tmp_path):
-    name = "polynomial_p1024_g64_s0"
+@pytest.mark.parametrize("name", sorted(n for n in MANIFEST if MANIFEST[n]["save_all"]))
+def test_cli_drop_in(tmp_path, name):
+    # The command line, with every final member exported (population.json):
+    # polynomial, traversal and statement grammars.
+    print("cli", name, flush=True)
     m = MANIFEST[name]
     out = tmp_path / "run"
-    subprocess.run([gagp_amd.CLI_PATH, "run", os.path.join(GOLDEN, m["task"]), "--engine", "gpu", "--seed", "0",
-                    "--out", str(out), "--save-all", "--quiet"], check=True, timeout=300)
+    cmd = [gagp_amd.CLI_PATH, "run", os.path.join(GOLDEN, m["task"]), "--engine", "gpu", "--population", str(m["population"]),
+           "--generations", str(m["generations"]), "--seed", str(m["seed"]), "--out", str(out), "--save-all", "--quiet"]
+    if "search" in m:
+        cmd += ["--search", os.path.join(GOLDEN, m["search"])]
+    subprocess.run(cmd, check=True, timeout=300)
     mine = json.load(open(out / "result.json"))
     assert strip_timing(mine) == strip_timing(golden(name))
     ref_pop = json.load(open(os.path.join(GOLDEN, "runs", name, "population.json")))
