@@ -137,19 +137,27 @@ def cpu_baseline(generations):
                       "history identical to the device run: %s" % (POPULATION, generations, cases, search_s, wall, same)}
 
 
-LOOP_TASK = os.path.join(ROOT, "tests", "golden", "tasks", "list_sum_even", "task.json")
-LOOP_GOLDEN = os.path.join(ROOT, "tests", "golden", "runs", "list_sum_even_p8192_g64_s0", "result.json")
+# The loop-heavy search in its two grammar forms: one traversal per program,
+# and statement programs (a for_range loop, an accumulator local, if_stmt).
+LOOP_FORMS = {
+    "traversal": ("list_sum_even", "one traversal per program with mod/eq/if/add/sub/max in its body"),
+    "statement": ("stmt_sum_even", "statement programs: for_range over the list, accumulator local, if_stmt, "
+                                   "mod/eq/lt/add/sub, index and len"),
+}
 
 
-def loop_heavy(cpu_generations):
+def loop_heavy(form, cpu_generations):
     """The loop-heavy search: the sum of the even elements of an Int list
-    (200 train / 400 test cases, lengths 0-50, values in [-1000,1000]; one
-    traversal per program with mod/eq/if/add/sub/max in its body), population
-    8192, 64 generations, seed 0, on the device, checked against the
-    reference's golden result; and the reference CPU engine on the first
-    `cpu_generations` generations of the same search (one thread), or None."""
+    (200 train / 400 test cases, lengths 0-50, values in [-1000,1000]) in the
+    grammar `form` of LOOP_FORMS, population 8192, 64 generations, seed 0, on
+    the device, checked against the reference's golden result; and the
+    reference CPU engine on the first `cpu_generations` generations of the
+    same search (one thread), or None."""
+    task_dir, grammar = LOOP_FORMS[form]
+    task = os.path.join(ROOT, "tests", "golden", "tasks", task_dir, "task.json")
+    golden = os.path.join(ROOT, "tests", "golden", "runs", task_dir + "_p8192_g64_s0", "result.json")
     b = gagp_amd.Batch()
-    b.add(LOOP_TASK, None, 8192, 64, 0)
+    b.add(task, None, 8192, 64, 0)
     t0 = time.perf_counter()
     b.prepare()
     t1 = time.perf_counter()
@@ -157,18 +165,19 @@ def loop_heavy(cpu_generations):
     b.finish()
     t2 = time.perf_counter()
     cases = b.evaluated_program_cases(0)
-    same = strip_timing(b.result(0)) == strip_timing(json.load(open(LOOP_GOLDEN)))
-    out = {"workload": "sum of the even elements of an Int list, population 8192, 64 generations, 200 train cases",
+    same = strip_timing(b.result(0)) == strip_timing(json.load(open(golden)))
+    out = {"workload": "sum of the even elements of an Int list, population 8192, 64 generations, 200 train cases; "
+                       + grammar,
            "value": round(cases / (t2 - t0), 1), "unit": "program-case evals/s", "seconds": round(t2 - t0, 3),
            "prepare_s": round(t1 - t0, 3), "ms_per_generation": round((t2 - t1) * 1e3 / 64, 3),
            "identical_to_reference": same, "cpu_reference": None}
     if cpu_generations and os.path.exists(REF_GAGP):
         with tempfile.TemporaryDirectory() as tmp:
-            subprocess.run([REF_GAGP, "run", LOOP_TASK, "--engine", "cpu", "--population", "8192", "--generations",
+            subprocess.run([REF_GAGP, "run", task, "--engine", "cpu", "--population", "8192", "--generations",
                             str(cpu_generations), "--seed", "0", "--out", tmp, "--quiet"], check=True)
             ref = json.load(open(os.path.join(tmp, "result.json")))
         c = gagp_amd.Batch()
-        c.add(LOOP_TASK, None, 8192, cpu_generations, 0)
+        c.add(task, None, 8192, cpu_generations, 0)
         c.run()
         ref_cases = c.evaluated_program_cases(0)
         search_s = ref["time_ms"]["search"] / 1e3
@@ -331,7 +340,7 @@ def main():
     psteps = min(args.steps, 16)
     ms, counts = profile(args.warmup, psteps)
     roof, kernels, vm = roofline(ms, counts, psteps)
-    loop = loop_heavy(0 if args.no_cpu else 2) if args.loop_heavy else None
+    loop = {form: loop_heavy(form, 0 if args.no_cpu else 2) for form in LOOP_FORMS} if args.loop_heavy else None
     batch = batching(args.batch_searches, 64) if args.batch_searches else None
     serve = serving(args.serve_requests, 20.0) if args.serve_requests else None
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_generations)

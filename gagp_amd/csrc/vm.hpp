@@ -100,23 +100,19 @@ __device__ inline LaneResult vm_run(const ProgView& p, const CaseData& cd, int c
   long long iterations = 0;
   for (int k = 0; k < st.temps; ++k) st.t[(st.slots + k) * 64 + lane] = TAG_INVALID;  // unassigned
   for (;;) {
-    int at;
-    if (p.has_jumps) {
-      // ip <= len < 8191 (code capacity), 8191 marks a finished lane.
+    // Converged live lanes (always so in straight-line code) share their ip:
+    // take it from the first live lane (finished lanes keep a stale ip);
+    // otherwise take the smallest one (ip <= len < 8191, the code capacity).
+    const unsigned long long live = __ballot(!done);
+    if (live == 0) break;
+    int at = __builtin_amdgcn_readlane(ip, __ffsll((long long)live) - 1);
+    if (p.has_jumps && __ballot(!done && ip != at) != 0)
       at = __builtin_amdgcn_readfirstlane(wave_min_bits<13>(done ? 8191 : (ip < 8191 ? ip : 8190)));
-      if (at == 8191) break;
-    } else {
-      // Straight-line code: every live lane is at the same ip; take it from
-      // the first live lane (finished lanes keep a stale ip).
-      const unsigned long long live = __ballot(!done);
-      if (live == 0) break;
-      const int first = __ffsll((long long)live) - 1;
-      at = __builtin_amdgcn_readlane(ip, first);
-    }
     if (++iterations > iteration_limit) {
       if (!done) { r.err = ERR_VALUE; done = true; }
       break;
     }
+    at = __builtin_amdgcn_readfirstlane(at);  // uniform: keeps the fetch scalar
     const bool mine = !done && ip == at;
     if (at >= p.len) {  // ran off the end without RETURN
       if (mine) { r.err = ERR_VALUE; done = true; }
@@ -571,8 +567,13 @@ __device__ inline LaneResult vm_loop(const LoopProgram& f, const FastInputs& in,
   const long long limit = 64LL * ((long long)fuel0 + 2LL * f.len + 2) + 64;
   for (long long step = 0;; ++step) {
     if (resched) {
-      at = __builtin_amdgcn_readfirstlane(wave_min_bits<9>(done ? 511 : (ip < 511 ? ip : 510)));  // ip <= len <= 256
-      if (at == 511) break;
+      // Converged live lanes (the common case) share their ip; otherwise
+      // take the smallest one.
+      const unsigned long long live = __ballot(!done);
+      if (live == 0) break;
+      const int lead = __builtin_amdgcn_readlane(ip, __ffsll((long long)live) - 1);
+      if (__ballot(!done && ip != lead) == 0) at = lead;
+      else at = __builtin_amdgcn_readfirstlane(wave_min_bits<9>(done ? 511 : (ip < 511 ? ip : 510)));  // ip <= len <= 256
       resched = false;
     } else if (__ballot(!done) == 0) {
       break;
@@ -581,6 +582,9 @@ __device__ inline LaneResult vm_loop(const LoopProgram& f, const FastInputs& in,
       if (!done) { r.err = ERR_VALUE; r.fuel_used = fuel0 - fuel; done = true; }
       break;
     }
+    // `at` is wave-uniform on every path; restating that keeps it (and the
+    // decode below) in scalar registers across the loop's merges.
+    at = __builtin_amdgcn_readfirstlane(at);
     const bool mine = !done && ip == at;
     if (at >= f.len) {  // ran off the end without RETURN
       if (mine) { r.err = ERR_VALUE; r.fuel_used = fuel0 - fuel; done = true; }
