@@ -613,6 +613,10 @@ constexpr int kLoopSlots = kLoopStack + kLoopLocals;  // LDS slots per lane of t
 // of their first instruction (a LOAD or STORE, whose argc is otherwise 0);
 // the low bits give the direction (1 forward, 2 reverse).
 constexpr int kWalkTest = 0x80, kWalkNext = 0x90;
+// Likewise a for_range loop's 6-instruction test/bind sequence at its head
+// (LOAD counter, LOAD bound, LT, JIF end, LOAD counter, STORE variable) and
+// its 5-instruction advance (LOAD counter, PUSH 1, ADD, STORE counter, JMP head).
+constexpr int kRangeTest = 0xA0, kRangeNext = 0xB0;
 // `fast`: every instruction and operand is certified for the typed
 // straight-line path (no jumps, every value statically Int or Bool, integer
 // builtins only, within the kFast* limits).
@@ -1052,6 +1056,7 @@ GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, Origi
         if (!put_load(loop.counter) || !put_load(loop.bound) || !put(OP_LT, 0, 0, -1)) return out;
         loop.end_jump = put_jump(OP_JMP_IF_FALSE, -1);
         if (loop.end_jump < 0 || !put_load(loop.counter) || !put_store(user)) return out;
+        code[loop.head].argc = std::uint8_t(kRangeTest);
       }
     }
     if (f.seen < node.arity) { frames.set(depth - 1, f); have_value = false; continue; }
@@ -1077,9 +1082,11 @@ GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, Origi
       if constexpr (kWalks) {
         if (completed != kStatementType) return out;
         const Loop loop = ws.loops[--ws.loop_depth];
+        const int advance = out.code;
         if (!put_load(loop.counter) || !put(OP_PUSH_CONST, loop.one, 0, 1) || !put(OP_ADD, 0, 0, -1) ||
             !put_store(loop.counter) || !emit(OP_JMP, loop.head, 0, 1))
           return out;
+        code[advance].argc = std::uint8_t(kRangeNext);
         code[loop.end_jump].a = out.code;
         stack = f.base_stack;
         while (ws.live_count > loop.live_begin) ws.freed[ws.free_count++] = ws.live[--ws.live_count];

@@ -596,13 +596,44 @@ __device__ inline LaneResult vm_loop(const LoopProgram& f, const FastInputs& in,
     const int w = __builtin_amdgcn_readfirstlane(loop_pick(f.w0, at)), a = __builtin_amdgcn_readfirstlane(loop_pick(f.w1, at));
     const int op = w & 0xff, argc = (w >> 8) & 0xff, sp = (w >> 16) & 0xff, cost = int(unsigned(w) >> 24);
     if (argc & 0x80) {
-      // A traversal's fixed sequence as one step (phase.hpp kWalkTest,
-      // kWalkNext). Every instruction of it costs one unit of fuel and none
+      // A traversal's or for_range loop's fixed sequence as one step
+      // (phase.hpp kWalkTest, kWalkNext, kRangeTest, kRangeNext). Every
+      // instruction of it costs one unit of fuel and none
       // can fail otherwise, so a lane with fewer units than the sequence
       // charges times out inside it, with no fuel left.
       const bool reverse = (argc & 3) == 2;
       const int k_counter = a - n_inputs;
-      if ((argc & 0xf0) == kWalkTest) {
+      const int tag = argc & 0xf0;
+      if (tag == kRangeTest || tag == kRangeNext) {
+        // A for_range step (phase.hpp kRangeTest, kRangeNext): unit costs
+        // and no failure but fuel, as for the traversal sequences.
+        const bool test = tag == kRangeTest;
+        const int k_bound = __builtin_amdgcn_readfirstlane(loop_pick(f.w1, at + 1)) - n_inputs;
+        const int exit_at = __builtin_amdgcn_readfirstlane(loop_pick(f.w1, at + 3));
+        const int k_var = __builtin_amdgcn_readfirstlane(loop_pick(f.w1, at + 5)) - n_inputs;
+        const int head = __builtin_amdgcn_readfirstlane(loop_pick(f.w1, at + 4));
+        const std::int64_t counter = L(k_counter);
+        const bool more = test && counter < L(k_bound);
+        const int need = test ? (more ? 6 : 4) : 5;
+        const bool starve = mine && fuel < need;
+        const bool run = mine && !starve;
+        if (run) {
+          if (more) {
+            L(k_var) = counter;
+            assigned |= 1u << k_var;
+          } else if (!test) {
+            L(k_counter) = wrap_add(counter, 1);
+          }
+          fuel -= need;
+          ip = test ? (more ? at + 6 : exit_at) : head;
+        }
+        if (starve) {
+          fuel = 0;
+          r.err = ERR_TIMEOUT;
+          r.fuel_used = fuel0;
+          done = true;
+        }
+      } else if (tag == kWalkTest) {
         // LOAD counter, LOAD bound, LT/GE, JIF done, LOAD xs, LOAD counter,
         // index, STORE element, LOAD start, LOAD counter, ADD, STORE index.
         const int k_bound = __builtin_amdgcn_readfirstlane(loop_pick(f.w1, at + 1)) - n_inputs;
