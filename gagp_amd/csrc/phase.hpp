@@ -617,6 +617,12 @@ constexpr int kWalkTest = 0x80, kWalkNext = 0x90;
 // (LOAD counter, LOAD bound, LT, JIF end, LOAD counter, STORE variable) and
 // its 5-instruction advance (LOAD counter, PUSH 1, ADD, STORE counter, JMP head).
 constexpr int kRangeTest = 0xA0, kRangeNext = 0xB0;
+// A LOAD or PUSH_CONST whose next instruction is a binary operation taking it
+// as its second operand, entered by no jump: the pair runs as one step.
+constexpr int kFuse = 0xC0;
+GHD inline bool fusable_binary(int op) {
+  return op == OP_ADD || op == OP_SUB || op == OP_MUL || op == OP_MOD || (op >= OP_LT && op <= OP_NE);
+}
 // `fast`: every instruction and operand is certified for the typed
 // straight-line path (no jumps, every value statically Int or Bool, integer
 // builtins only, within the kFast* limits).
@@ -718,14 +724,25 @@ GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, Origi
   int depth = 0, cursor = 0, stack = 0, cconst = 0;
   int completed = TAG_INVALID;
   bool have_value = false;
+  int last_op = -1;  // the previous instruction's opcode and argc (fusion)
+  int last_argc = 0;
   const auto emit = [&](int op, int a, int argc, unsigned fuel) -> bool {
     if (out.code >= code_capacity) { out.status = ST_CAPACITY; return false; }
     if (fuel > 255 || stack < 0 || stack > kMaxStack) { out.status = ST_UNSUPPORTED; return false; }
     code[out.code++] = Ins{std::uint8_t(op), std::uint8_t(argc), std::uint8_t(stack), std::uint8_t(fuel), a};
     out.jumps |= op == OP_JMP || op == OP_JMP_IF_FALSE || op == OP_JMP_IF_TRUE;
+    last_op = op;
+    last_argc = argc;
     return true;
   };
   const auto grow = [&](int by) { stack += by; if (stack > out.stack_bound) out.stack_bound = stack; };
+  // The last position a jump targets when it is the next instruction's: a
+  // binary operation there is not fused with the instruction before it.
+  int target_at = -1;
+  const auto patch = [&](int jump) {
+    code[jump].a = out.code;
+    target_at = out.code;
+  };
   // Traversal emission helpers: one unit of fuel per instruction.
   const auto put = [&](int op, int a, int argc, int delta) -> bool {
     if (!emit(op, a, argc, 1)) return false;
@@ -800,12 +817,12 @@ GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, Origi
             if (nonnegative < 0 || !put_int(0) || !put_store(slot)) return false;
             const int clamped = put_jump(OP_JMP, 0);
             if (clamped < 0) return false;
-            code[nonnegative].a = out.code;
+            patch(nonnegative);
             if (!put_load(slot) || !put_load(w.length) || !put(OP_GT, 0, 0, -1)) return false;
             const int above = put_jump(OP_JMP_IF_FALSE, -1);
             if (above < 0 || !put_load(w.length) || !put_store(slot)) return false;
-            code[clamped].a = out.code;
-            code[above].a = out.code;
+            patch(clamped);
+            patch(above);
           }
         } else {
           if (!put_int(0) || !put_store(w.begin)) return false;
@@ -851,7 +868,7 @@ GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, Origi
         if (!emit(OP_JMP, w.loop_head, 0, 1)) return false;
         // The fixed 6-instruction update/advance/repeat sequence.
         code[update].argc = std::uint8_t(kWalkNext | (reverse ? 2 : 1));
-        code[w.done_jump].a = out.code;
+        patch(w.done_jump);
         if (!put_load(w.state)) return false;
       }
       return true;
@@ -1015,7 +1032,7 @@ GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, Origi
     } else if (node.kind == TK_IF && f.seen == 2) {
       f.second_jump = out.code;
       if (!emit(OP_JMP, 0, 0, node.merge_fuel)) return out;
-      code[f.first_jump].a = out.code;
+      patch(f.first_jump);
       stack = f.base_stack;
     }
     if constexpr (kWalks) {
@@ -1035,14 +1052,14 @@ GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, Origi
         if (bad < 0 || !put_load(loop.bound) || !put_int(0) || !put(OP_LT, 0, 0, -1)) return out;
         const int valid = put_jump(OP_JMP_IF_FALSE, -1);
         if (valid < 0) return out;
-        code[bad].a = out.code;
+        patch(bad);
         if (out.constants >= pool_capacity) { out.status = ST_CAPACITY; return out; }
         pool[out.constants] = val_bool(true);
         if (!put(OP_PUSH_CONST, out.constants, 0, 1)) return out;
         ++out.constants;
         if (!put(OP_NEG, 0, 1, 0)) return out;
         stack = f.base_stack;
-        code[valid].a = out.code;
+        patch(valid);
         if (out.constants + 2 > pool_capacity) { out.status = ST_CAPACITY; return out; }
         const int zero = out.constants++;
         pool[zero] = val_int(0);
@@ -1087,19 +1104,19 @@ GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, Origi
             !put_store(loop.counter) || !emit(OP_JMP, loop.head, 0, 1))
           return out;
         code[advance].argc = std::uint8_t(kRangeNext);
-        code[loop.end_jump].a = out.code;
+        patch(loop.end_jump);
         stack = f.base_stack;
         while (ws.live_count > loop.live_begin) ws.freed[ws.free_count++] = ws.live[--ws.live_count];
       }
     } else if (node.kind == TK_IF) {
       if (f.args[1] != f.args[2]) return out;
-      code[f.second_jump].a = out.code;
+      patch(f.second_jump);
     } else if (node.kind == TK_AND || node.kind == TK_OR) {
       if (completed != TAG_BOOL) return out;
       if (!emit(OP_NOT, 0, 0, node.fuel) || !emit(OP_NOT, 0, 0, node.fuel)) return out;
       const int finish = out.code;
       if (!emit(OP_JMP, 0, 0, node.fuel)) return out;
-      code[f.first_jump].a = out.code;
+      patch(f.first_jump);
       stack = f.base_stack;
       // push the short-circuit constant
       if (out.constants >= pool_capacity) { out.status = ST_CAPACITY; return out; }
@@ -1107,7 +1124,7 @@ GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, Origi
       if (!emit(OP_PUSH_CONST, out.constants, 0, node.fuel)) return out;
       ++out.constants;
       grow(1);
-      code[finish].a = out.code;
+      patch(finish);
     } else {
       // operation result type: the certified Int/Bool rules, else the grammar type
       int rt = TAG_INVALID;
@@ -1128,6 +1145,11 @@ GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, Origi
       if (rt == TAG_INVALID || rt == TAG_FALLBACK) { out.status = ST_UNSUPPORTED; return out; }
       fast = fast && fast_operation(node.opcode, node.operand, node.arity, args[0], args[1], args[2]) && fast_type(rt);
       typed = typed && loop_operation(node.opcode, node.operand, node.arity, args[0], args[1], args[2]) && fast_type(rt);
+      // A binary operation right after the LOAD or PUSH of its second
+      // operand, which no jump enters between, fuses with it (kFuse).
+      if (node.arity == 2 && fusable_binary(node.opcode) && (last_op == OP_LOAD || last_op == OP_PUSH_CONST) &&
+          last_argc == 0 && target_at != out.code && out.code > 0)
+        code[out.code - 1].argc = std::uint8_t(kFuse);
       // operands are on the stack: the instruction sees the stack with them
       if (node.opcode == OP_CALL_BUILTIN) { if (!emit(OP_CALL_BUILTIN, node.operand, node.arity, node.fuel)) return out; }
       else if (!emit(node.opcode, 0, 0, node.fuel)) return out;

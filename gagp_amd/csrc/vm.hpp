@@ -544,6 +544,28 @@ __device__ inline LoopProgram loop_load(const ProgView& p, int lane) {
   }
   return f;
 }
+// The binary operations of the looping path on untagged operands; `zero`:
+// MOD by zero (ZeroDivisionError). MOD is Python's, by the shared truncating
+// division of vm_fast.
+__device__ inline std::int64_t loop_binary(int op, std::int64_t x, std::int64_t y, bool& zero) {
+  switch (op) {
+    case OP_ADD: return wrap_add(x, y);
+    case OP_SUB: return wrap_sub(x, y);
+    case OP_MUL: return wrap_mul(x, y);
+    case OP_LT: return x < y;
+    case OP_LE: return x <= y;
+    case OP_GT: return x > y;
+    case OP_GE: return x >= y;
+    case OP_EQ: return x == y;
+    case OP_NE: return x != y;
+    default: {  // OP_MOD
+      const std::int64_t safe = (y == 0 || y == -1) ? 1 : y;
+      const std::int64_t rem = y == -1 ? 0 : x - (x / safe) * safe;
+      zero = y == 0;
+      return y == 0 ? 0 : ((rem != 0 && ((rem < 0) != (y < 0))) ? rem + y : rem);
+    }
+  }
+}
 __device__ inline int loop_pick(const int (&v)[4], int k) {
   const int x0 = __builtin_amdgcn_readlane(v[0], k & 63), x1 = __builtin_amdgcn_readlane(v[1], k & 63);
   const int x2 = __builtin_amdgcn_readlane(v[2], k & 63), x3 = __builtin_amdgcn_readlane(v[3], k & 63);
@@ -595,7 +617,7 @@ __device__ inline LaneResult vm_loop(const LoopProgram& f, const FastInputs& in,
     // the compiler, which then indexes the register arrays with them.
     const int w = __builtin_amdgcn_readfirstlane(loop_pick(f.w0, at)), a = __builtin_amdgcn_readfirstlane(loop_pick(f.w1, at));
     const int op = w & 0xff, argc = (w >> 8) & 0xff, sp = (w >> 16) & 0xff, cost = int(unsigned(w) >> 24);
-    if (argc & 0x80) {
+    if ((argc & 0x80) && (argc & 0xf0) != kFuse) {
       // A traversal's or for_range loop's fixed sequence as one step
       // (phase.hpp kWalkTest, kWalkNext, kRangeTest, kRangeNext). Every
       // instruction of it costs one unit of fuel and none
@@ -691,6 +713,49 @@ __device__ inline LaneResult vm_loop(const LoopProgram& f, const FastInputs& in,
       resched = true;
       continue;
     }
+    if ((argc & 0xf0) == kFuse) {
+      // A LOAD or PUSH_CONST and the binary operation after it (phase.hpp
+      // kFuse) as one step, when every lane at `at` has the fuel of both;
+      // otherwise the first runs alone below.
+      const int w2 = __builtin_amdgcn_readfirstlane(loop_pick(f.w0, at + 1));
+      const int op2 = w2 & 0xff, cost2 = int(unsigned(w2) >> 24);
+      if (__ballot(mine && fuel < cost + cost2) == 0) {
+        std::int64_t y;
+        bool bound = true;
+        if (op == OP_PUSH_CONST) {
+          const std::uint32_t lo = std::uint32_t(loop_pick(f.c0, a)), hi = std::uint32_t(loop_pick(f.c1, a));
+          y = std::int64_t(std::uint64_t(hi) << 32 | lo);
+        } else if (a < n_inputs) {  // uniform branch
+          y = in.pick(a);
+          bound = (in.bound >> (a & 31)) & 1u;
+        } else {
+          const int k = a - n_inputs < kLoopLocals ? a - n_inputs : 0;
+          y = L(k);
+          bound = (assigned >> k) & 1u;
+        }
+        bool zero = false;
+        const std::int64_t v = loop_binary(op2, S(sp - 1), y, zero);
+        int err2 = ERR_NONE;
+        if (mine) {
+          if (!bound) {
+            err2 = ERR_NAME;
+            fuel -= cost;
+          } else {
+            fuel -= cost + cost2;
+            if (zero) err2 = ERR_ZERO_DIV;
+            S(sp - 1) = v;
+            ip = at + 2;
+          }
+        }
+        if (mine && err2 != ERR_NONE) {
+          r.err = err2;
+          r.fuel_used = fuel0 - fuel;
+          done = true;
+        }
+        resched = true;
+        continue;
+      }
+    }
     // Every lane computes the instruction; the lanes at `at` with fuel
     // commit it (predication instead of divergent branches).
     const bool starve = mine && fuel < cost;
@@ -736,31 +801,12 @@ __device__ inline LaneResult vm_loop(const LoopProgram& f, const FastInputs& in,
         else if (run) S(top) = wrap_neg(S(top));
         break;
       case OP_NOT: if (run) S(top) = std::int64_t(S(top) == 0); break;
-      case OP_ADD: case OP_SUB: case OP_MUL: case OP_LT: case OP_LE: case OP_GT: case OP_GE: case OP_EQ: case OP_NE: {
-        const std::int64_t x = S(below), y = S(top);
-        std::int64_t v;
-        switch (op) {
-          case OP_ADD: v = wrap_add(x, y); break;
-          case OP_SUB: v = wrap_sub(x, y); break;
-          case OP_MUL: v = wrap_mul(x, y); break;
-          case OP_LT: v = x < y; break;
-          case OP_LE: v = x <= y; break;
-          case OP_GT: v = x > y; break;
-          case OP_GE: v = x >= y; break;
-          case OP_EQ: v = x == y; break;
-          default: v = x != y; break;
-        }
+      case OP_ADD: case OP_SUB: case OP_MUL: case OP_MOD: case OP_LT: case OP_LE: case OP_GT: case OP_GE: case OP_EQ:
+      case OP_NE: {
+        bool zero = false;
+        const std::int64_t v = loop_binary(op, S(below), S(top), zero);
+        if (run && zero) err = ERR_ZERO_DIV;
         if (run) S(below) = v;
-        break;
-      }
-      case OP_MOD: {
-        // Python % with ZeroDiv on 0 (the shared truncating division of vm_fast).
-        const std::int64_t x = S(below), y = S(top);
-        const std::int64_t safe = (y == 0 || y == -1) ? 1 : y;
-        const std::int64_t rem = y == -1 ? 0 : x - (x / safe) * safe;
-        const std::int64_t floor_mod = (rem != 0 && ((rem < 0) != (y < 0))) ? rem + y : rem;
-        if (run && y == 0) err = ERR_ZERO_DIV;
-        if (run) S(below) = y == 0 ? 0 : floor_mod;
         break;
       }
       case OP_CALL_BUILTIN: {
