@@ -170,11 +170,11 @@ __global__ void k_init_pop(DSearch* all, const BlockMap* map) {
 // ---- lowering: one wavefront per row ----
 // kWalks: the lowering instantiation with traversals (grammars that have them).
 template <bool kWalks>
-__device__ inline void lower_into_w(const PhaseView& g, const DSearch& d, const CRow& row, int slot) {
+__device__ inline void lower_into_w(const PhaseView& g, const DSearch& d, const CRow& row, int slot, WalkStore<kWalks>& ws) {
   Ins* code = d.code + std::size_t(slot) * d.code_cap;
   Val* pool = d.pool + std::size_t(slot) * d.pool_cap;
   const LowerResult r =
-      lower_phase_w<kWalks>(g, row, d.local_types, d.n_locals, d.expected_type, code, d.code_cap, pool, d.pool_cap);
+      lower_phase_w<kWalks>(g, row, d.local_types, d.n_locals, d.expected_type, code, d.code_cap, pool, d.pool_cap, ws);
   const bool ok = r.status == ST_OK && r.stack_bound >= 1 && r.stack_bound <= d.stack_slots &&
                   r.locals - d.n_locals <= d.temp_slots;
   if (wlane() == 0) {
@@ -192,7 +192,8 @@ __global__ __launch_bounds__(64 * kVaryWaves) void k_lower_members(DSearch* all,
   const DSearch& d = all[bm.search];
   const int m = bm.block * kVaryWaves + wave_index();
   if (!d.active || (d.walks != 0) != kWalks || m >= d.P || *d.error) return;
-  lower_into_w<kWalks>(view_of(d.g), d, member_row(d, m), m);
+  __shared__ WalkStore<kWalks> walk_lds[kVaryWaves];
+  lower_into_w<kWalks>(view_of(d.g), d, member_row(d, m), m, walk_lds[wave_index()]);
 }
 
 // ---- reproduction: one wavefront per child (phase_operators.cuh:156-265) ----
@@ -239,7 +240,8 @@ __global__ __launch_bounds__(64 * kVaryWaves) void k_vary(DSearch* all, const Bl
     if (st.error) raise(d, 300 + st.error);
   }
   if (st.error) return;
-  lower_into_w<kWalks>(g, d, child, c);
+  __shared__ WalkStore<kWalks> walk_lds[kVaryWaves];
+  lower_into_w<kWalks>(g, d, child, c, walk_lds[wave_index()]);
 }
 
 // ---- member hash table (parent_fitness_cache.cuh:11-58) ----

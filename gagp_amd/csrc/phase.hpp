@@ -687,10 +687,12 @@ struct Walk {
 // temporaries, loop head, exit jump, the pool index of its constant 1 and
 // the live temporaries at its start.
 struct Loop { int bound, counter, head, end_jump, one, live_begin; };
+// The lowering sets every field before use (the record may live in LDS, one
+// per wavefront, where all lanes store the same values).
 template <bool kWalks> struct WalkStore {};
 template <> struct WalkStore<true> {
-  int live_count = 0, free_count = 0, walk_depth = 0, loop_depth = 0;
-  unsigned named_mask = 0;  // grammar locals given a slot (on first use)
+  int live_count, free_count, walk_depth, loop_depth;
+  unsigned named_mask;  // grammar locals given a slot (on first use)
   signed char live[64], freed[64];
   signed char env_slot[16];
   unsigned char env_type[16];
@@ -713,12 +715,14 @@ template <> struct WalkStore<true> {
 template <class Frames, class OriginAt, bool kWalks = false>
 GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, OriginAt origin_at, const std::uint8_t* local_types,
                                         int local_count, int expected_type, Ins* code, int code_capacity, Val* pool,
-                                        int pool_capacity, Frames& frames) {
+                                        int pool_capacity, Frames& frames, [[maybe_unused]] WalkStore<kWalks>& ws) {
   LowerResult out{ST_INVALID, 0, 0, 0, TAG_INVALID, 0, 0, local_count, 0};
   if (t.n <= 0 || t.n > 4096) return out;
-  [[maybe_unused]] WalkStore<kWalks> ws;
-  if constexpr (kWalks)
+  if constexpr (kWalks) {
+    ws.live_count = ws.free_count = ws.walk_depth = ws.loop_depth = 0;
+    ws.named_mask = 0;
     for (int i = 0; i < 16; ++i) { ws.env_slot[i] = -1; ws.env_type[i] = TAG_INVALID; }
+  }
   bool fast = local_count <= kFastInputs;
   bool typed = local_count <= kFastInputs;  // every value Int, Bool or an IntList view; loop_operation only
   int depth = 0, cursor = 0, stack = 0, cconst = 0;
@@ -1163,9 +1167,10 @@ template <bool kWalks = false>
 GHD inline LowerResult lower_phase(const PhaseView& g, const CRow& t, const std::uint8_t* local_types, int local_count,
                                    int expected_type, Ins* code, int code_capacity, Val* pool, int pool_capacity) {
   LocalFrames frames;
+  WalkStore<kWalks> ws;
   const auto origin_at = [&](int i) { return t.origin[i]; };
   return lower_phase_with<LocalFrames, decltype(origin_at), kWalks>(g, t, origin_at, local_types, local_count, expected_type, code,
-                                                                    code_capacity, pool, pool_capacity, frames);
+                                                                    code_capacity, pool, pool_capacity, frames, ws);
 }
 
 }  // namespace gagp_amd

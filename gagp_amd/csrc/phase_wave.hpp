@@ -471,14 +471,16 @@ __device__ inline std::uint64_t row_hash_w(const CRow& r) {
 }
 
 // Lowering of one row by the whole wavefront (uniform), origins read from a
-// per-tile register by lane.
+// per-tile register by lane. `ws`: this wavefront's lowering record (in LDS
+// for the kWalks instantiation; every lane stores the same values).
 template <bool kWalks = false>
 __device__ inline LowerResult lower_phase_w(const PhaseView& g, const CRow& t, const std::uint8_t* local_types, int local_count,
-                                            int expected_type, Ins* code, int code_capacity, Val* pool, int pool_capacity) {
+                                            int expected_type, Ins* code, int code_capacity, Val* pool, int pool_capacity,
+                                            WalkStore<kWalks>& ws) {
   WaveFrames frames;
   WaveOrigins origin_at{t};
   return lower_phase_with<WaveFrames, WaveOrigins&, kWalks>(g, t, origin_at, local_types, local_count, expected_type, code,
-                                                            code_capacity, pool, pool_capacity, frames);
+                                                            code_capacity, pool, pool_capacity, frames, ws);
 }
 
 }  // namespace gagp_amd
