@@ -111,6 +111,8 @@ int main(int argc, char** argv) {
     line.obj["timeouts"] = json::make_number(counts[2]);
     line.obj["code"] = json::make_number(lr.code);
     line.obj["locals"] = json::make_number(lr.locals);
+    line.obj["stack"] = json::make_number(lr.stack_bound);
+    line.obj["loop"] = json::make_number(lr.loop);
     line.obj["text"] = json::make_string(program_text(ast));
     line.obj["program"] = ast_json(ast);
     std::printf("%s\n", json::canonical(line).c_str());
