@@ -119,6 +119,7 @@ struct DSearch {
   int temp_slots;      // hidden-local slots per lane of the general VM (traversal temporaries)
   int walks;           // the grammar has traversals: lowering runs the kWalks instantiations
   int general_scores;  // the general evaluation kernel has an LDS score buffer for this search
+  GAS double* loop_scores;  // the looping kernel's case scores: C per block of its grid (at most 8192 blocks)
   int stage_cases;     // the typed evaluation kernel stages the case set in LDS
 };
 
@@ -566,8 +567,9 @@ __device__ inline double canonical_from_scores(const EvalParams& e, int C, const
 // interleaved 64-case tiles into one LDS score buffer, combine their counts
 // in LDS, and wavefront 0 forms the fitness (the exact integer sum, or the
 // canonical replay from the buffer). Programs of a search are spread over
-// the search's blocks. LDS: per wavefront kLoopSlots slots of 64 lanes,
-// then the block's C scores.
+// the search's blocks. LDS: per wavefront kLoopSlots slots of 64 lanes; the
+// block's C scores are in HBM (its share of loop_scores), so the LDS size does
+// not depend on the case count and four blocks fit a CU.
 __global__ __launch_bounds__(kEvalThreads) void k_eval_loop(DSearch* all, const BlockMap* map, int mode) {
   const BlockMap bm = map[blockIdx.x];
   const DSearch& d = all[bm.search];
@@ -578,9 +580,8 @@ __global__ __launch_bounds__(kEvalThreads) void k_eval_loop(DSearch* all, const 
   __shared__ unsigned red_w[kWavesPerBlock], red_e[kWavesPerBlock], red_t[kWavesPerBlock], red_i[kWavesPerBlock];
   const int wave = wave_index(), lane = threadIdx.x % 64;
   const int C = d.cd.C;
-  constexpr std::size_t kLead = std::size_t(kWavesPerBlock) * kLoopSlots * 64 * 8;
   LAS std::int64_t* const slots = (LAS std::int64_t*)smem + std::size_t(wave) * kLoopSlots * 64;
-  double* const scores = reinterpret_cast<double*>(smem + kLead);
+  GAS double* const scores = d.loop_scores + std::size_t(bm.block) * C;
   const EvalParams e{d.cd, d.fuel, d.lanes, d.penalty, d.policy};
   const StagedCases staged{};  // inputs are read once per case tile: not staged, so more blocks fit the LDS
   const int count = mode == 1 ? d.n_pending[2] : d.P;
@@ -1464,6 +1465,7 @@ struct SearchState {
     DeviceCases dc;
     dc.upload(a, train);
     x.cd = dc.cd;
+    if (x.cd.C <= kScoreCases) x.loop_scores = a.alloc<double>(std::size_t(std::min(8192, std::max(P, B))) * std::size_t(x.cd.C));
     const std::size_t rows = std::size_t(P) * R, crows = std::size_t(B) * R;
     x.cur_origin = a.alloc<int>(rows); x.cur_site = a.alloc<int>(rows); x.cur_cval = a.alloc<Val>(rows);
     x.cur_n = a.alloc<int>(P); x.cur_nc = a.alloc<int>(P);
@@ -1650,7 +1652,7 @@ struct Batch::Impl {
       const std::size_t cases = x.d.stage_cases ? staged_case_bytes(x.d.cd.C, staged_inputs_of(x.d.n_locals, x.d.cd.n_inputs)) : 0;
       if (x.d.cd.C <= kScoreCases) {
         eval_fast_lds = std::max(eval_fast_lds, score_buf + cases);
-        eval_loop_lds = std::max(eval_loop_lds, kLoopLds + 8 * std::size_t(x.d.cd.C));
+        eval_loop_lds = kLoopLds;
       }
       if (x.d.stage_tables) vary_lds_bytes = std::max(vary_lds_bytes, phase_tables_bytes(x.d.g));
       else vary_global = true;
