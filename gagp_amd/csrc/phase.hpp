@@ -607,7 +607,7 @@ constexpr int kFastStack = 16, kFastCode = 128, kFastPool = 128, kFastInputs = 4
 // Limits of the typed looping VM path (vm_loop): hidden locals (one
 // traversal's temporaries; sequential traversals reuse them), code and
 // constants held in registers.
-constexpr int kLoopStack = 8, kLoopLocals = 9, kLoopCode = 256, kLoopPool = 256;
+constexpr int kLoopStack = 6, kLoopLocals = 9, kLoopCode = 256, kLoopPool = 256;
 constexpr int kLoopSlots = kLoopStack + kLoopLocals;  // LDS slots per lane of the looping path
 // Tags of a traversal's fixed instruction sequences, held in the argc byte
 // of their first instruction (a LOAD or STORE, whose argc is otherwise 0);
