@@ -583,9 +583,12 @@ __device__ inline LaneResult vm_loop(const LoopProgram& f, const FastInputs& in,
   const auto L = [&](int k) -> LAS std::int64_t& { return sl[(kLoopStack + (k < 0 ? 0 : k < kLoopLocals ? k : 0)) * 64]; };
   unsigned assigned = 0;  // hidden locals stored so far
   int fuel = fuel0, ip = 0, at = 0;
-  bool resched = false;
+  // Every step that finishes a lane, jumps or runs a fixed sequence sets
+  // `resched`; any other step leaves the live lanes as they were, at + 1.
+  bool resched = true;
   // Every step runs one instruction of at least one live lane, which charges
-  // fuel, so this bound is never reached by a certified program.
+  // fuel, so this bound is never reached by a certified program. A wave that
+  // never stops must jump, so the bound is checked when rescheduling.
   const long long limit = 64LL * ((long long)fuel0 + 2LL * f.len + 2) + 64;
   for (long long step = 0;; ++step) {
     if (resched) {
@@ -593,16 +596,14 @@ __device__ inline LaneResult vm_loop(const LoopProgram& f, const FastInputs& in,
       // take the smallest one.
       const unsigned long long live = __ballot(!done);
       if (live == 0) break;
+      if (step > limit) {
+        if (!done) { r.err = ERR_VALUE; r.fuel_used = fuel0 - fuel; done = true; }
+        break;
+      }
       const int lead = __builtin_amdgcn_readlane(ip, __ffsll((long long)live) - 1);
       if (__ballot(!done && ip != lead) == 0) at = lead;
       else at = __builtin_amdgcn_readfirstlane(wave_min_bits<9>(done ? 511 : (ip < 511 ? ip : 510)));  // ip <= len <= 256
       resched = false;
-    } else if (__ballot(!done) == 0) {
-      break;
-    }
-    if (step > limit) {
-      if (!done) { r.err = ERR_VALUE; r.fuel_used = fuel0 - fuel; done = true; }
-      break;
     }
     // `at` is wave-uniform on every path; restating that keeps it (and the
     // decode below) in scalar registers across the loop's merges.
