@@ -34,6 +34,53 @@ void orc_eval_fitness(const OProg* programs, int n_programs, const gagp_amd::Val
 
 using namespace gagp_amd;
 
+namespace {
+// The fixed sequences the lowering tags for the looping VM (the argc byte of
+// their first instruction), checked against the code: the VM runs each as one
+// step, so the pattern must hold and no jump may enter the sequence after its
+// first instruction. Returns the number of violations; `tags` counts each kind.
+int tag_violations(const std::vector<Ins>& code, int n, int (&tags)[5]) {
+  std::vector<char> target(std::size_t(n) + 1, 0);
+  for (int i = 0; i < n; ++i)
+    if (code[i].op == OP_JMP || code[i].op == OP_JMP_IF_FALSE || code[i].op == OP_JMP_IF_TRUE) target.at(std::size_t(code[i].a)) = 1;
+  const auto ops = [&](int at, std::initializer_list<int> want) {
+    int k = 0;
+    for (const int op : want) {
+      if (at + k >= n || code[at + k].op != op || (k > 0 && target[std::size_t(at + k)])) return false;
+      ++k;
+    }
+    return true;
+  };
+  int bad = 0;
+  for (int i = 0; i < n; ++i) {
+    const int tag = code[i].op == OP_CALL_BUILTIN || code[i].op == OP_NEG ? 0 : code[i].argc & 0xf0;
+    if (tag == kFuse) {
+      ++tags[0];
+      bad += !((code[i].op == OP_LOAD || code[i].op == OP_PUSH_CONST) && i + 1 < n && fusable_binary(code[i + 1].op) &&
+               !target[std::size_t(i + 1)]);
+    } else if (tag == kRangeTest) {
+      ++tags[1];
+      bad += !ops(i, {OP_LOAD, OP_LOAD, OP_LT, OP_JMP_IF_FALSE, OP_LOAD, OP_STORE});
+    } else if (tag == kRangeNext) {
+      ++tags[2];
+      bad += !(ops(i, {OP_LOAD, OP_PUSH_CONST, OP_ADD, OP_STORE, OP_JMP}) && code[i + 4].a >= 0 && code[code[i + 4].a].argc == kRangeTest);
+    } else if (tag == kWalkTest) {
+      ++tags[3];
+      const bool reverse = (code[i].argc & 3) == 2;
+      bad += !ops(i, {OP_LOAD, OP_LOAD, reverse ? OP_GE : OP_LT, OP_JMP_IF_FALSE, OP_LOAD, OP_LOAD, OP_CALL_BUILTIN, OP_STORE,
+                      OP_LOAD, OP_LOAD, OP_ADD, OP_STORE});
+    } else if (tag == kWalkNext) {
+      ++tags[4];
+      const bool reverse = (code[i].argc & 3) == 2;
+      bad += !ops(i, {OP_STORE, OP_LOAD, OP_PUSH_CONST, reverse ? OP_SUB : OP_ADD, OP_STORE, OP_JMP});
+    } else if (code[i].argc != 0 && code[i].op != OP_CALL_BUILTIN && !(code[i].op == OP_NEG && code[i].argc == 1)) {
+      ++bad;  // an unknown tag
+    }
+  }
+  return bad;
+}
+}  // namespace
+
 int main(int argc, char** argv) {
   if (argc != 3 && argc != 4) { std::fprintf(stderr, "usage: host_lowering TASK.json N [defs]\n"); return 2; }
   const bool defs = argc == 4 && std::string(argv[3]) == "defs";
@@ -113,6 +160,11 @@ int main(int argc, char** argv) {
     line.obj["locals"] = json::make_number(lr.locals);
     line.obj["stack"] = json::make_number(lr.stack_bound);
     line.obj["loop"] = json::make_number(lr.loop);
+    int tags[5] = {0, 0, 0, 0, 0};
+    line.obj["tag_violations"] = json::make_number(tag_violations(code, lr.code, tags));
+    json::Value t = json::make_array();
+    for (const int k : tags) t.arr.push_back(json::make_number(k));
+    line.obj["tags"] = t;
     line.obj["text"] = json::make_string(program_text(ast));
     line.obj["program"] = ast_json(ast);
     std::printf("%s\n", json::canonical(line).c_str());

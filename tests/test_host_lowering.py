@@ -49,6 +49,13 @@ def test_lowering_matches_reference_evaluation(driver, task):
         assert r["fitness"] == pytest.approx(-p["mean_loss"] * p["cases"], rel=1e-12, abs=1e-9), r["text"]
         assert r["wrong"] == p["wrong"], r["text"]
         assert r["runtime_errors"] == p["runtime_errors"], r["text"]
+    # The looping VM's single-step sequences: every tag on its pattern, no jump
+    # into a sequence after its first instruction.
+    assert all(r["tag_violations"] == 0 for r in rows)
+    tags = [sum(r["tags"][k] for r in rows) for k in range(5)]  # fuse, range test/next, walk test/next
+    assert tags[0] > 0
+    assert (tags[1] > 0 and tags[1] == tags[2]) if task.startswith("stmt_sum") else True
+    assert tags[3] == tags[4] and (tags[3] > 0) == (task != "stmt_sum_even" and task != "stmt_lowfuel")
     if task in ("list_lowfuel", "list_range_nested", "stmt_lowfuel"):
         # The fuel schedule is only pinned where cases actually run out of fuel.
         assert sum(r["timeouts"] for r in rows) > 100
