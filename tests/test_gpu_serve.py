@@ -31,7 +31,7 @@ def test_staggered_requests_match_reference():
     server = gagp_amd.Server(budget_bytes=8 << 30, poll_rounds=2)
     plain = ["polynomial_p512_g32_s7", "quintic_p512_g16_s3", "mixed_0", "mixed_3", "polynomial_gen_e1_p256_g12_s3",
              "polynomial_steady_b128_rs3_p512_g16_s4", "polynomial_p100_g10_s5", "list_count_gt_rev_p300_g8_s6",
-             "list_range_nested_p256_g8_s7", "mixedl_1"]
+             "list_range_nested_p256_g8_s7", "mixedl_1", "stmt_count_gt_p300_g8_s6", "stmt_lowfuel_p400_g8_s8"]
     tickets = {name: submit(server, name, 0.004 * i) for i, name in enumerate(plain)}
     # Early exit: the 64-generation requests stop where the reference's
     # shorter runs end.
