@@ -140,11 +140,15 @@ __device__ inline void raise(const DSearch& d, int code) { atomicCAS(d.error, 0,
 // largest search).
 struct BlockMap { int search, block, blocks; };
 
+constexpr int kVaryWaves = 4;  // wavefronts (rows) per block of the row kernels
+
 // ---- initial population (phase_population_cpu.cpp:54-113) ----
-__global__ void k_init_pop(DSearch* all, const BlockMap* map) {
+// One wavefront per member runs the derivation uniformly (as mutation does),
+// so members with different retry counts do not diverge within a wavefront.
+__global__ __launch_bounds__(64 * kVaryWaves) void k_init_pop(DSearch* all, const BlockMap* map) {
   const BlockMap bm = map[blockIdx.x];
   const DSearch& d = all[bm.search];
-  const int m = bm.block * blockDim.x + threadIdx.x;
+  const int m = bm.block * kVaryWaves + wave_index();
   if (!d.active || m >= d.P) return;
   const int sd = d.site_depth;
   const int ramp = sd < 2 ? sd : 2 + m % (sd - 1);
@@ -155,17 +159,21 @@ __global__ void k_init_pop(DSearch* all, const BlockMap* map) {
   for (int logical = ramp, attempt = 0; gen.status != ST_OK; ++attempt) {
     if (room <= 0 || room > d.R) { raise(d, 101); return; }
     const std::uint64_t s = splitmix_mix(d.seed ^ splitmix_mix((std::uint64_t(m) * 8 + 0) * 64 + std::uint64_t(attempt)));
-    gen = generate_phase(g, g.root, 0, d.slot.max_depth, room, s, d.cur_origin + o, d.cur_cval + o, d.cur_site + o, logical);
+    WaveTasks pending;
+    gen = generate_phase_with(g, g.root, 0, d.slot.max_depth, room, s, d.cur_origin + o, d.cur_cval + o, d.cur_site + o, logical,
+                              pending);
     // A statement program may read an unassigned local: the next seed is tried.
     if (gen.status == ST_OK && d.slot.statement &&
-        !locals_defined(g, CRow{d.cur_origin + o, d.cur_site + o, d.cur_cval + o, gen.nodes, gen.constants}))
+        !locals_defined_w(g, CRow{d.cur_origin + o, d.cur_site + o, d.cur_cval + o, gen.nodes, gen.constants}))
       gen.status = ST_INVALID;
     if (gen.status != ST_OK && attempt % 4 == 3 && logical < sd) ++logical;
     if (attempt >= 256) { raise(d, 102); return; }
   }
-  d.cur_n[m] = gen.nodes;
-  d.cur_nc[m] = gen.constants;
-  d.placed[m] = -1;
+  if (wlane() == 0) {
+    d.cur_n[m] = gen.nodes;
+    d.cur_nc[m] = gen.constants;
+    d.placed[m] = -1;
+  }
 }
 
 // ---- lowering: one wavefront per row ----
@@ -186,7 +194,6 @@ __device__ inline void lower_into_w(const PhaseView& g, const DSearch& d, const 
     d.code_jumps[slot] = r.jumps | (r.fast ? 2 : 0) | (r.result_type == TAG_BOOL ? 4 : 0) | (r.loop ? 8 : 0);
   }
 }
-constexpr int kVaryWaves = 4;  // wavefronts (rows) per block of the row kernels
 template <bool kWalks>
 __global__ __launch_bounds__(64 * kVaryWaves) void k_lower_members(DSearch* all, const BlockMap* map) {
   const BlockMap bm = map[blockIdx.x];
@@ -1686,10 +1693,11 @@ struct Batch::Impl {
     GAS DSearch* table = tmp.upload(host);
     DSearch* t = (DSearch*)table;
     size_kernels(ids);
-    const Grid gi = build_grid(tmp, ids, [&](const SearchState& x) { return up(x.P, 64); });
+    const Grid gi = build_grid(tmp, ids, [&](const SearchState& x) { return up(x.P, kVaryWaves); });
     const Grid gl = build_grid(tmp, ids, [&](const SearchState& x) { return up(x.P, kVaryWaves); });
     const Grid ge = build_grid(tmp, ids, [&](const SearchState& x) { return std::min(4096, up(x.P, kWavesPerBlock)); });
-    timer.time("k_init_pop", stream, [&] { hipLaunchKernelGGL(k_init_pop, dim3(gi.blocks), dim3(64), 0, stream, t, gi.map); });
+    timer.time("k_init_pop", stream,
+               [&] { hipLaunchKernelGGL(k_init_pop, dim3(gi.blocks), dim3(64 * kVaryWaves), 0, stream, t, gi.map); });
     timer.time("k_lower_members", stream,
                [&] { hipLaunchKernelGGL(k_lower_members<false>, dim3(gl.blocks), dim3(64 * kVaryWaves), 0, stream, t, gl.map); });
     if (any_walks(ids))
