@@ -4,15 +4,13 @@
 // are read from the expression table, constants live in a per-row value list
 // in prefix order.
 //
-// The draw order and every accept/reject rule restate the reference bodies:
+// The draw order and every accept/reject rule restate the reference bodies
+// (the wavefront operators that use these pieces are in phase_wave.hpp):
 //   generate_phase        cpp/src/evolution/repro/gpu/device/phase_generate.cuh:125-270
-//   subtree end / depths  .../device/phase_variation.cuh:81-142
-//   splice_phase_tree     .../device/phase_variation.cuh:147-207
 //   tournament, sites     cpp/src/evolution/repro/gpu/phase_operators.cuh:46-83
-//   crossover_child       phase_operators.cuh:156-199
-//   mutate_child          phase_operators.cuh:202-245
-//   compare_child         phase_operators.cuh:247-265
-//   compile_phase_tree    .../device/phase_compile.cuh:206-895 (expression subset)
+//   locals_defined        .../device/phase_compile.cuh:170-200
+//   compile_phase_tree    .../device/phase_compile.cuh:206-895 (expressions,
+//                         traversals and statements)
 #pragma once
 #include "core.hpp"
 #include "grammar.hpp"
@@ -222,113 +220,6 @@ GHD inline GenResult generate_phase(const PhaseView& g, int root_expression, int
 GHD inline int arity_of(const PhaseView& g, int origin) { return g.expr[origin].node.arity; }
 GHD inline bool is_constant(const PhaseView& g, int origin) { return g.expr[origin].kind == PK_CONSTANT; }
 
-GHD inline int subtree_end(const PhaseView& g, const CRow& t, int start) {
-  if (start < 0 || start >= t.n || t.n > kPhaseCapacity) return -1;
-  int pending = 1;
-  for (int i = start; i < t.n; ++i) {
-    const int a = arity_of(g, t.origin[i]);
-    if (a < 0 || a > kMaxArity) return -1;
-    pending += a - 1;
-    if (!pending) return i + 1;
-  }
-  return -1;
-}
-GHD inline int node_depth(const PhaseView& g, const CRow& t, int position) {
-  if (position < 0 || position >= t.n || t.n > kPhaseCapacity) return -1;
-  int pending[64];
-  int depth = 1;
-  pending[0] = 1;
-  for (int i = 0; i <= position; ++i) {
-    while (depth && !pending[depth - 1]) --depth;
-    if (!depth) return -1;
-    if (i == position) return depth;
-    --pending[depth - 1];
-    const int a = arity_of(g, t.origin[i]);
-    if (a < 0 || a > kMaxArity) return -1;
-    if (a) { if (depth == 64) return -1; pending[depth++] = a; }
-  }
-  return -1;
-}
-// Nested grammar-invocation depth; with `position`, the incoming depth at it.
-GHD inline int site_depth_of(const PhaseView& g, const int* origin, const int* site, int count, int position = -1) {
-  if (count <= 0 || count > kPhaseCapacity || position >= count) return -1;
-  int pending[64], incoming[64];
-  int depth = 1, maximum = 0;
-  pending[0] = 1;
-  incoming[0] = 0;
-  for (int i = 0; i < count; ++i) {
-    while (depth && !pending[depth - 1]) --depth;
-    if (!depth) return -1;
-    if (i == position) return incoming[depth - 1];
-    const int a = arity_of(g, origin[i]);
-    if (a < 0 || a > kMaxArity) return -1;
-    const int value = incoming[depth - 1] + (site[i] >= 0 ? 1 : 0);
-    maximum = value > maximum ? value : maximum;
-    --pending[depth - 1];
-    if (a) { if (depth == 64) return -1; pending[depth] = a; incoming[depth++] = value; }
-  }
-  while (depth && !pending[depth - 1]) --depth;
-  return depth ? -1 : maximum;
-}
-GHD inline int constants_before(const PhaseView& g, const CRow& t, int end) {
-  int c = 0;
-  for (int i = 0; i < end; ++i) c += is_constant(g, t.origin[i]);
-  return c;
-}
-
-// Replace base[destination's subtree] by donor[source's subtree] into `out`
-// (which must alias neither input). Fused depth tracking.
-GHD inline GenResult splice(const PhaseView& g, const CRow& base, int destination, const CRow& donor, int source, Row out,
-                            int out_capacity, int max_depth, int site_limit) {
-  GenResult r{ST_INVALID, 0, 0, 0};
-  if (out_capacity <= 0 || out_capacity > kPhaseCapacity || max_depth <= 0 || max_depth > 64 || base.n <= 0 ||
-      base.n > kPhaseCapacity || donor.n <= 0 || donor.n > kPhaseCapacity || destination < 0 || destination >= base.n ||
-      source < 0 || source >= donor.n)
-    return r;
-  const int a = base.site[destination], b = donor.site[source];
-  if (a < 0 || b < 0 || site_nt(g, a) != site_nt(g, b) || site_scope(g, a) != site_scope(g, b)) return r;
-  const int end = subtree_end(g, base, destination), donor_end = subtree_end(g, donor, source);
-  if (end < 0 || donor_end < 0) return r;
-  const int length = base.n - (end - destination) + donor_end - source;
-  if (length > out_capacity) { r.status = ST_CAPACITY; return r; }
-  int pending[64];
-  int depth = 1, maximum = 1;
-  pending[0] = 1;
-  // Constant cursors of the three copied ranges (constants are in prefix order).
-  int c_base = 0, c_donor = constants_before(g, donor, source), c_tail = constants_before(g, base, end);
-  for (int i = 0; i < length; ++i) {
-    const bool incoming = i >= destination && i < destination + donor_end - source;
-    const CRow& tree = incoming ? donor : base;
-    const int index = incoming ? source + i - destination : i < destination ? i : i + (end - destination) - (donor_end - source);
-    const int o = tree.origin[index];
-    const int ar = arity_of(g, o);
-    while (depth && !pending[depth - 1]) --depth;
-    if (!depth || ar < 0 || ar > kMaxArity) return r;
-    maximum = depth > maximum ? depth : maximum;
-    --pending[depth - 1];
-    if (ar) { if (depth == 64) return r; pending[depth++] = ar; }
-    if (is_constant(g, o)) {
-      int& cursor = incoming ? c_donor : (i < destination ? c_base : c_tail);
-      if (cursor >= tree.nc) return r;
-      out.cval[r.constants++] = tree.cval[cursor++];
-    }
-    out.origin[i] = o;
-    out.site[i] = tree.site[index];
-  }
-  out.site[destination] = a;
-  while (depth && !pending[depth - 1]) --depth;
-  if (depth) return r;
-  if (maximum > max_depth) { r.status = ST_CAPACITY; return r; }
-  if (site_limit) {
-    const int h = site_depth_of(g, out.origin, out.site, length);
-    if (h < 0) return r;
-    if (h > site_limit) { r.status = ST_CAPACITY; return r; }
-  }
-  r.nodes = length;
-  r.status = ST_OK;
-  return r;
-}
-
 struct OpStats { int parent, crossover_attempted, crossover, crossover_rejected, mutation, changed, error, reusable_parent, donor; };
 struct OpPolicy {
   int counted_sites;
@@ -375,25 +266,6 @@ GHD inline bool site_matches(const PhaseView& g, const CRow& t, int i, int nt, i
   const int sc = site_scope(g, s);
   if (sc & kFixedSite) return false;
   return nt < 0 || (site_nt(g, s) == nt && sc == scope);
-}
-// Single-phase rows: slot selection is trivially slot 0 (or none).
-GHD inline int choose_site(const PhaseView& g, const CRow& t, SplitMix& random, bool counted, int nt, int scope) {
-  if (t.n <= 0) return -1;
-  if (counted) {
-    unsigned count = 0;
-    for (int i = 0; i < t.n; ++i) count += site_matches(g, t, i, nt, scope);
-    if (!count) return -1;
-    unsigned rank = unsigned(random.bounded(count));
-    for (int i = 0; i < t.n; ++i)
-      if (site_matches(g, t, i, nt, scope)) { if (rank == 0) return i; --rank; }
-    return -1;
-  }
-  int selected = -1;
-  unsigned count = 0;
-  for (int i = 0; i < t.n; ++i)
-    if (site_matches(g, t, i, nt, scope))
-      if (random.bounded(++count) == 0) selected = i;
-  return selected;
 }
 
 GHD inline void copy_row(const CRow& from, Row to, int* to_n, int* to_nc) {
@@ -461,104 +333,6 @@ GHD inline bool locals_defined(const PhaseView& g, const CRow& t) {
   return locals_defined_with(g, t.n, origin_at, frames);
 }
 
-// Crossover of one child into `out` (a private row per child). `in(m)` gives
-// the current member rows.
-template <class MemberRow>
-GHD inline void crossover_child(const PhaseView& g, const SlotInfo& s, int child, MemberRow in, Row out, int* out_n, int* out_nc,
-                                const Keys& keys, int population, int pressure, std::uint64_t seed, const OpPolicy& pol, OpStats& st) {
-  SplitMix random(seed ^ (0x617089e168ULL + std::uint64_t(child)));
-  st = OpStats{-1, 0, 0, 0, 0, 0, 0, -1, -1};
-  const int parent = tournament(keys, population, pressure, random);
-  const int donor = tournament(keys, population, pressure, random);
-  st.parent = parent;
-  st.donor = donor;
-  const CRow a = in(parent);
-  if (a.n <= 0) { st.error = 1; return; }
-  const int total = a.n;
-  copy_row(a, out, out_n, out_nc);
-  if (pol.crossover_probability < 1 && random.unit() >= pol.crossover_probability) return;
-  st.crossover_attempted = 1;
-  const int selected_slot = pol.uniform_phase ? int(random.bounded(1)) : -1;
-  (void)selected_slot;
-  const CRow b = in(donor);
-  for (int attempt = 0; attempt < pol.crossover_attempts; ++attempt) {
-    const int dest = choose_site(g, a, random, pol.counted_sites != 0, -1, -1);
-    if (dest < 0) break;
-    const int contract = a.site[dest];
-    // An invalid donor row draws nothing and rejects the attempt.
-    const int src = b.n > 0 ? choose_site(g, b, random, pol.counted_sites != 0, site_nt(g, contract), site_scope(g, contract)) : -1;
-    if (src < 0) { ++st.crossover_rejected; continue; }
-    const int ae = subtree_end(g, a, dest), be = subtree_end(g, b, src);
-    if (ae < 0 || be < 0) { st.error = 3; return; }
-    if (total - (ae - dest) + (be - src) > s.max_variable_nodes) { ++st.crossover_rejected; continue; }
-    // The budget check above bounds the splice by the node budget, which the
-    // physical row holds, so the row size never changes an outcome.
-    const GenResult r = splice(g, a, dest, b, src, out, s.row_capacity, s.max_depth, s.logical_depth);
-    if (r.status == ST_OK && (!s.statement || locals_defined(g, CRow{out.origin, out.site, out.cval, r.nodes, r.constants}))) {
-      *out_n = r.nodes;
-      *out_nc = r.constants;
-      st.crossover = 1;
-      return;
-    }
-    copy_row(a, out, out_n, out_nc);  // failed speculation restores the phase
-    ++st.crossover_rejected;
-  }
-}
-
-// Mutation of the crossed child row `x` in place; `tmp` holds the generated
-// subtree and `spliced` the candidate (private rows, never aliasing `x`).
-GHD inline void mutate_child(const PhaseView& g, const SlotInfo& s, int child, Row x, int* x_n, int* x_nc, Row tmp, Row spliced,
-                             double probability, std::uint64_t seed, const OpPolicy& pol, OpStats& st) {
-  if (st.error) return;
-  if (x.n <= 0) { st.error = 4; return; }
-  const int total = x.n;
-  SplitMix random(seed ^ (0x124ab512bf01ULL + std::uint64_t(child)));
-  if (random.unit() >= probability) return;
-  if (pol.uniform_phase) (void)random.bounded(1);
-  const CRow a = as_const(x);
-  const int dest = choose_site(g, a, random, pol.counted_sites != 0, -1, -1);
-  if (dest < 0) return;
-  const int end = subtree_end(g, a, dest), depth = node_depth(g, a, dest);
-  if (end < 0 || depth < 0) { st.error = 6; return; }
-  const int site = a.site[dest];
-  const int incoming = s.logical_depth ? site_depth_of(g, a.origin, a.site, a.n, dest) : 0;
-  if (incoming < 0 || incoming > s.logical_depth) { st.error = 6; return; }
-  const int logical = s.logical_depth - incoming;
-  const int replaced = end - dest;
-  const int whole = s.max_variable_nodes - (total - replaced), phase = kPhaseCapacity - (a.n - replaced);
-  const int available = whole < phase ? whole : phase, uncapped = available < kPhaseCapacity ? available : kPhaseCapacity;
-  const bool capped = pol.mutation_max_nodes > 0 && pol.mutation_max_nodes < uncapped;
-  const int bounded = capped ? pol.mutation_max_nodes : uncapped;
-  if (bounded > s.row_capacity) { st.error = 10; return; }  // node budget exceeds the physical row
-  const GenResult gen = generate_phase(g, site_entry(site), site_parent_scope(site), s.max_depth - depth + 1, bounded,
-                                       random.next(), tmp.origin, tmp.cval, tmp.site, logical);
-  if (capped && gen.status == ST_CAPACITY) return;
-  if (gen.status != ST_OK) { st.error = 7; return; }
-  const CRow sub{tmp.origin, tmp.site, tmp.cval, gen.nodes, gen.constants};
-  const GenResult r = splice(g, a, dest, sub, 0, spliced, s.row_capacity, s.max_depth, s.logical_depth);
-  if (r.status != ST_OK) { st.error = 8; return; }
-  const CRow sp{spliced.origin, spliced.site, spliced.cval, r.nodes, r.constants};
-  // An undefined local read rejects this mutation; the crossed child stands.
-  if (s.statement && !locals_defined(g, sp)) return;
-  copy_row(sp, x, x_n, x_nc);
-  st.mutation = 1;
-}
-
-// Is the child identical to its first parent? (constants compared by tag and
-// payload; reuse additionally requires identical raw bits).
-GHD inline void compare_child(const CRow& parent_row, const CRow& child_row, int population, OpStats& st) {
-  if (st.error) return;
-  if (st.parent < 0 || st.parent >= population) { st.error = 9; return; }
-  bool equal = parent_row.n == child_row.n && parent_row.nc == child_row.nc, reusable = true;
-  for (int i = 0; equal && i < parent_row.n; ++i) equal = parent_row.origin[i] == child_row.origin[i];
-  for (int i = 0; equal && i < parent_row.nc; ++i) {
-    const Val x = parent_row.cval[i], y = child_row.cval[i];
-    equal = x.tag == y.tag && (x.tag == TAG_BOOL ? x.b == y.b : x.i == y.i);
-    reusable = reusable && x.i == y.i && x.b == y.b;
-  }
-  st.changed = !equal;
-  st.reusable_parent = equal && reusable ? st.parent : -1;
-}
 
 // Genotype hash: a wrapping sum of one mixed term per node (origin and
 // position) and per constant (position, tag and the payload that equality
