@@ -570,13 +570,24 @@ __device__ inline double canonical_from_scores(const EvalParams& e, int C, const
   return group_total;
 }
 
+// Blocks of the looping kernel for n programs (a block per program, or a
+// wavefront per program when one 64-case tile covers the cases; at most 8192
+// blocks, which loop over the rest) and the score rows they use.
+__host__ __device__ inline int loop_blocks(int C, int n) {
+  return std::min(8192, C <= 64 ? (n + kWavesPerBlock - 1) / kWavesPerBlock : n);
+}
+__host__ __device__ inline std::size_t loop_score_rows(int C, int n) {
+  return std::size_t(loop_blocks(C, n)) * (C <= 64 ? kWavesPerBlock : 1);
+}
+
 // The looping path, one program per block: the block's wavefronts score
-// interleaved 64-case tiles into one LDS score buffer, combine their counts
-// in LDS, and wavefront 0 forms the fitness (the exact integer sum, or the
-// canonical replay from the buffer). Programs of a search are spread over
+// interleaved 64-case tiles into one score buffer, combine their counts in
+// LDS, and wavefront 0 forms the fitness (the exact integer sum, or the
+// canonical replay from the buffer). With at most 64 cases each wavefront
+// scores programs of its own instead. Programs of a search are spread over
 // the search's blocks. LDS: per wavefront kLoopSlots slots of 64 lanes; the
-// block's C scores are in HBM (its share of loop_scores), so the LDS size does
-// not depend on the case count and four blocks fit a CU.
+// scores are in HBM (the block's or wavefront's share of loop_scores), so the
+// LDS size does not depend on the case count and five blocks fit a CU.
 __global__ __launch_bounds__(kEvalThreads) void k_eval_loop(DSearch* all, const BlockMap* map, int mode) {
   const BlockMap bm = map[blockIdx.x];
   const DSearch& d = all[bm.search];
@@ -588,10 +599,51 @@ __global__ __launch_bounds__(kEvalThreads) void k_eval_loop(DSearch* all, const 
   const int wave = wave_index(), lane = threadIdx.x % 64;
   const int C = d.cd.C;
   LAS std::int64_t* const slots = (LAS std::int64_t*)smem + std::size_t(wave) * kLoopSlots * 64;
-  GAS double* const scores = d.loop_scores + std::size_t(bm.block) * C;
   const EvalParams e{d.cd, d.fuel, d.lanes, d.penalty, d.policy};
   const StagedCases staged{};  // inputs are read once per case tile: not staged, so more blocks fit the LDS
   const int count = mode == 1 ? d.n_pending[2] : d.P;
+  if (C <= 64) {
+    // One tile per program: each wavefront scores programs of its own, so
+    // no wavefront of the block idles (uniform per search).
+    GAS double* const scores = d.loop_scores + (std::size_t(bm.block) * kWavesPerBlock + wave) * C;
+    for (int i = bm.block * kWavesPerBlock + wave; i < count; i += bm.blocks * kWavesPerBlock) {
+      const int slot = mode == 1 ? d.pending[pending_at(d.B, 2, i)] : i;
+      const int pmode = d.code_jumps[slot];
+      if (mode == 0 && eval_class(pmode, C) != 2) continue;  // another kernel scores it (uniform per wavefront)
+      const ProgView pv{d.code + std::size_t(slot) * d.code_cap, d.pool + std::size_t(slot) * d.pool_cap, d.code_len[slot],
+                        d.n_locals, pmode & 1, (pmode >> 1) & 1, d.pool_len[slot], (pmode & 4) ? TAG_BOOL : TAG_INT};
+      const LoopProgram lp = loop_load(pv, lane);
+      const int n_in = staged_inputs_of(pv.n_locals, e.cd.n_inputs);
+      const bool active = lane < C;
+      const int c = active ? e.cd.order[lane] : 0;
+      const FastInputs in = fast_inputs(e.cd, pv.n_locals, c, active);
+      const LaneResult r = vm_loop(lp, in, n_in, e.cd.arena, slots, lane, active, e.fuel, pv.result_tag);
+      double sc = 0;
+      bool exact = true;
+      if (active) {
+        sc = lane_score(e, r, expected_of(e.cd, c), exact);
+        scores[c] = sc;
+      }
+      const bool integral = !active || (sc == double(std::int64_t(sc)) && sc >= -68719476736.0 && sc <= 68719476736.0);
+      const double ws = wave_sum_d(sc);
+      const unsigned tw = wave_sum_u(active && !exact), te = wave_sum_u(active && r.err != ERR_NONE);
+      const unsigned tt = wave_sum_u(active && r.err == ERR_TIMEOUT);
+      unsigned long long tf = active ? unsigned(r.fuel_used) : 0u;
+      for (int off = 32; off > 0; off >>= 1) tf += __shfl_xor(tf, off, 64);
+      wave_fence();  // the canonical replay reads the other lanes' scores
+      const double fitness = __ballot(!integral) == 0 ? ws : canonical_from_scores(e, C, scores, lane);
+      if (lane == 0) {
+        if (mode == 1) { d.c_fit[slot] = canon40(fitness); d.c_wrong[slot] = tw; }
+        else { d.fitness[slot] = canon40(fitness); d.wrong[slot] = tw; }
+        d.e_errors[slot] = te;
+        d.e_timeouts[slot] = tt;
+        d.e_fuel[slot] = tf;
+      }
+      wave_fence();  // the score buffer is reused by the next program
+    }
+    return;
+  }
+  GAS double* const scores = d.loop_scores + std::size_t(bm.block) * C;
   for (int i = bm.block; i < count; i += bm.blocks) {
     const int slot = mode == 1 ? d.pending[pending_at(d.B, 2, i)] : i;
     const int pmode = d.code_jumps[slot];
@@ -1472,7 +1524,7 @@ struct SearchState {
     DeviceCases dc;
     dc.upload(a, train);
     x.cd = dc.cd;
-    if (x.cd.C <= kScoreCases) x.loop_scores = a.alloc<double>(std::size_t(std::min(8192, std::max(P, B))) * std::size_t(x.cd.C));
+    if (x.cd.C <= kScoreCases) x.loop_scores = a.alloc<double>(loop_score_rows(x.cd.C, std::max(P, B)) * std::size_t(x.cd.C));
     const std::size_t rows = std::size_t(P) * R, crows = std::size_t(B) * R;
     x.cur_origin = a.alloc<int>(rows); x.cur_site = a.alloc<int>(rows); x.cur_cval = a.alloc<Val>(rows);
     x.cur_n = a.alloc<int>(P); x.cur_nc = a.alloc<int>(P);
@@ -1629,7 +1681,7 @@ struct Batch::Impl {
     g_child = build_grid(map_arena, live, [&](const SearchState& s) { return up(s.B, 128); });
     g_eval = build_grid(map_arena, live, [&](const SearchState& s) { return std::min(4096, up(s.B, kWavesPerBlock)); });
     // One program per block for the looping path.
-    g_eval_loop = build_grid(map_arena, live, [&](const SearchState& s) { return s.may_jump ? std::min(8192, s.B) : 0; });
+    g_eval_loop = build_grid(map_arena, live, [&](const SearchState& s) { return s.may_jump ? loop_blocks(s.d.cd.C, s.B) : 0; });
   }
 
   // Allocates one search and fixes its kernel variants and LDS needs.
@@ -1707,7 +1759,7 @@ struct Batch::Impl {
                [&] { hipLaunchKernelGGL(k_eval<1>, dim3(ge.blocks), dim3(kEvalThreads), eval_fast_lds, stream, t, ge.map, 0); });
     if (any_jumps(ids))
       timer.time("k_eval_init_loop", stream, [&] {
-        const Grid gp = build_grid(tmp, ids, [&](const SearchState& x) { return std::min(8192, x.P); });
+        const Grid gp = build_grid(tmp, ids, [&](const SearchState& x) { return loop_blocks(x.d.cd.C, x.P); });
         hipLaunchKernelGGL(k_eval_loop, dim3(gp.blocks), dim3(kEvalThreads), eval_loop_lds, stream, t, gp.map, 0);
       });
     timer.time("k_eval_init_general", stream, [&] {
