@@ -55,6 +55,13 @@ struct Counters {
 
 // Device state of one search. Every pointer is device memory owned by the
 // host Search object.
+// Totals of one program's tiles in the looping kernel (device atomics).
+struct LoopAcc {
+  unsigned long long sum;   // two's-complement int64 sum of the integral scores
+  unsigned long long fuel;  // charged fuel
+  unsigned wrong, errors, timeouts, fractional, tiles, pad;
+};
+
 struct DSearch {
   int active;
   int P, B, R, code_cap, pool_cap, stack_slots;
@@ -119,7 +126,8 @@ struct DSearch {
   int temp_slots;      // hidden-local slots per lane of the general VM (traversal temporaries)
   int walks;           // the grammar has traversals: lowering runs the kWalks instantiations
   int general_scores;  // the general evaluation kernel has an LDS score buffer for this search
-  GAS double* loop_scores;  // the looping kernel's case scores: C per block of its grid (at most 8192 blocks)
+  GAS double* loop_scores;  // the looping kernel's case scores: C per program slot
+  GAS LoopAcc* loop_acc;    // the looping kernel's per-program tile totals (zero between evaluations)
   int stage_cases;     // the typed evaluation kernel stages the case set in LDS
 };
 
@@ -570,146 +578,108 @@ __device__ inline double canonical_from_scores(const EvalParams& e, int C, const
   return group_total;
 }
 
-// Blocks of the looping kernel for n programs (a block per program, or a
-// wavefront per program when one 64-case tile covers the cases; at most 8192
-// blocks, which loop over the rest) and the score rows they use.
+// Items of the looping kernel: (program, 64-case tile) pairs, one wavefront
+// each; blocks for n programs (at most 8192, which loop over the rest).
 __host__ __device__ inline int loop_blocks(int C, int n) {
-  return std::min(8192, C <= 64 ? (n + kWavesPerBlock - 1) / kWavesPerBlock : n);
-}
-__host__ __device__ inline std::size_t loop_score_rows(int C, int n) {
-  return std::size_t(loop_blocks(C, n)) * (C <= 64 ? kWavesPerBlock : 1);
+  const long long waves = (long long)n * ((C + 63) / 64);
+  return int(std::min<long long>(8192, (waves + kWavesPerBlock - 1) / kWavesPerBlock));
 }
 
-// The looping path, one program per block: the block's wavefronts score
-// interleaved 64-case tiles into one score buffer, combine their counts in
-// LDS, and wavefront 0 forms the fitness (the exact integer sum, or the
-// canonical replay from the buffer). With at most 64 cases each wavefront
-// scores programs of its own instead. Programs of a search are spread over
-// the search's blocks. LDS: per wavefront kLoopSlots slots of 64 lanes; the
-// scores are in HBM (the block's or wavefront's share of loop_scores), so the
-// LDS size does not depend on the case count and five blocks fit a CU.
+// The looping path as a tile queue: every wavefront takes (program, tile)
+// items of its own, program-major, so no wavefront waits for another. Scores
+// go to the program's row of loop_scores (HBM). A program with one tile is
+// finished by its wavefront; otherwise each tile adds its counts and its
+// exact integer score sum to the program's LoopAcc with device atomics, and
+// the wavefront that completes the last tile forms the fitness: the sum, or
+// the canonical replay from the scores when some tile had a non-integral
+// score. It then clears the record for the program's next evaluation. LDS:
+// per wavefront kLoopSlots slots of 64 lanes, independent of the case count.
 __global__ __launch_bounds__(kEvalThreads) void k_eval_loop(DSearch* all, const BlockMap* map, int mode) {
   const BlockMap bm = map[blockIdx.x];
   const DSearch& d = all[bm.search];
   if (mode == 1 ? !search_live(d) : (!d.active || *d.error)) return;
   extern __shared__ unsigned char smem[];
-  __shared__ double red_sum[kWavesPerBlock];
-  __shared__ unsigned long long red_fuel[kWavesPerBlock];
-  __shared__ unsigned red_w[kWavesPerBlock], red_e[kWavesPerBlock], red_t[kWavesPerBlock], red_i[kWavesPerBlock];
   const int wave = wave_index(), lane = threadIdx.x % 64;
-  const int C = d.cd.C;
+  const int C = d.cd.C, tiles = (C + 63) / 64;
   LAS std::int64_t* const slots = (LAS std::int64_t*)smem + std::size_t(wave) * kLoopSlots * 64;
   const EvalParams e{d.cd, d.fuel, d.lanes, d.penalty, d.policy};
-  const StagedCases staged{};  // inputs are read once per case tile: not staged, so more blocks fit the LDS
   const int count = mode == 1 ? d.n_pending[2] : d.P;
-  if (C <= 64) {
-    // One tile per program: each wavefront scores programs of its own, so
-    // no wavefront of the block idles (uniform per search).
-    GAS double* const scores = d.loop_scores + (std::size_t(bm.block) * kWavesPerBlock + wave) * C;
-    for (int i = bm.block * kWavesPerBlock + wave; i < count; i += bm.blocks * kWavesPerBlock) {
-      const int slot = mode == 1 ? d.pending[pending_at(d.B, 2, i)] : i;
-      const int pmode = d.code_jumps[slot];
-      if (mode == 0 && eval_class(pmode, C) != 2) continue;  // another kernel scores it (uniform per wavefront)
-      const ProgView pv{d.code + std::size_t(slot) * d.code_cap, d.pool + std::size_t(slot) * d.pool_cap, d.code_len[slot],
-                        d.n_locals, pmode & 1, (pmode >> 1) & 1, d.pool_len[slot], (pmode & 4) ? TAG_BOOL : TAG_INT};
-      const LoopProgram lp = loop_load(pv, lane);
-      const int n_in = staged_inputs_of(pv.n_locals, e.cd.n_inputs);
-      const bool active = lane < C;
-      const int c = active ? e.cd.order[lane] : 0;
-      const FastInputs in = fast_inputs(e.cd, pv.n_locals, c, active);
-      const LaneResult r = vm_loop(lp, in, n_in, e.cd.arena, slots, lane, active, e.fuel, pv.result_tag);
-      double sc = 0;
-      bool exact = true;
-      if (active) {
-        sc = lane_score(e, r, expected_of(e.cd, c), exact);
-        scores[c] = sc;
-      }
-      const bool integral = !active || (sc == double(std::int64_t(sc)) && sc >= -68719476736.0 && sc <= 68719476736.0);
-      const double ws = wave_sum_d(sc);
-      const unsigned tw = wave_sum_u(active && !exact), te = wave_sum_u(active && r.err != ERR_NONE);
-      const unsigned tt = wave_sum_u(active && r.err == ERR_TIMEOUT);
-      unsigned long long tf = active ? unsigned(r.fuel_used) : 0u;
-      for (int off = 32; off > 0; off >>= 1) tf += __shfl_xor(tf, off, 64);
-      wave_fence();  // the canonical replay reads the other lanes' scores
-      const double fitness = __ballot(!integral) == 0 ? ws : canonical_from_scores(e, C, scores, lane);
-      if (lane == 0) {
-        if (mode == 1) { d.c_fit[slot] = canon40(fitness); d.c_wrong[slot] = tw; }
-        else { d.fitness[slot] = canon40(fitness); d.wrong[slot] = tw; }
-        d.e_errors[slot] = te;
-        d.e_timeouts[slot] = tt;
-        d.e_fuel[slot] = tf;
-      }
-      wave_fence();  // the score buffer is reused by the next program
-    }
-    return;
-  }
-  GAS double* const scores = d.loop_scores + std::size_t(bm.block) * C;
-  for (int i = bm.block; i < count; i += bm.blocks) {
+  const long long items = (long long)count * tiles;
+  for (long long k = (long long)bm.block * kWavesPerBlock + wave; k < items; k += (long long)bm.blocks * kWavesPerBlock) {
+    const int i = int(k / tiles), tile = int(k % tiles) * 64;
     const int slot = mode == 1 ? d.pending[pending_at(d.B, 2, i)] : i;
     const int pmode = d.code_jumps[slot];
-    if (mode == 0 && eval_class(pmode, C) != 2) continue;  // another kernel scores it (uniform per block)
+    if (mode == 0 && eval_class(pmode, C) != 2) continue;  // another kernel scores it (uniform per wavefront)
     const ProgView pv{d.code + std::size_t(slot) * d.code_cap, d.pool + std::size_t(slot) * d.pool_cap, d.code_len[slot],
                       d.n_locals, pmode & 1, (pmode >> 1) & 1, d.pool_len[slot], (pmode & 4) ? TAG_BOOL : TAG_INT};
     const LoopProgram lp = loop_load(pv, lane);
     const int n_in = staged_inputs_of(pv.n_locals, e.cd.n_inputs);
-    double isum = 0;
-    unsigned w = 0, errs = 0, touts = 0;
-    unsigned long long fuel = 0;
-    bool integral = C <= 4096;
-    for (int tile = wave * 64; tile < C; tile += kWavesPerBlock * 64) {
-      // Cases in the search's visiting order (similar loop lengths per
-      // tile); scores are kept by case index, so the totals are unchanged.
-      const bool active = tile + lane < C;
-      const int c = active ? e.cd.order[tile + lane] : 0;
-      const FastInputs in =
-          staged.on ? staged_inputs(staged, C, n_in, c, active) : fast_inputs(e.cd, pv.n_locals, c, active);
-      const LaneResult r = vm_loop(lp, in, n_in, e.cd.arena, slots, lane, active, e.fuel, pv.result_tag);
-      if (active) {
-        Val expected{};
-        if (staged.on) { expected.i = staged.expected[c]; expected.tag = staged.exp_tag[c]; expected.b = staged.exp_b[c]; }
-        else expected = expected_of(e.cd, c);
-        bool exact = false;
-        const double sc = lane_score(e, r, expected, exact);
-        w += !exact;
-        errs += r.err != ERR_NONE;
-        touts += r.err == ERR_TIMEOUT;
-        fuel += unsigned(r.fuel_used);
-        integral = integral && sc == double(std::int64_t(sc)) && sc >= -68719476736.0 && sc <= 68719476736.0;
-        isum += sc;
-        scores[c] = sc;
-      }
+    // Cases in the search's visiting order (similar loop lengths per tile);
+    // scores are kept by case index, so the totals are unchanged.
+    const bool active = tile + lane < C;
+    const int c = active ? e.cd.order[tile + lane] : 0;
+    const LaneResult r = vm_loop(lp, fast_inputs(e.cd, pv.n_locals, c, active), n_in, e.cd.arena, slots, lane, active, e.fuel,
+                                 pv.result_tag);
+    GAS double* const scores = d.loop_scores + std::size_t(slot) * C;
+    double sc = 0;
+    bool exact = true;
+    if (active) {
+      sc = lane_score(e, r, expected_of(e.cd, c), exact);
+      scores[c] = sc;
     }
-    const double ws = wave_sum_d(isum);
-    const unsigned ww = wave_sum_u(w), we = wave_sum_u(errs), wt = wave_sum_u(touts);
-    unsigned long long wf = fuel;
-    for (int off = 32; off > 0; off >>= 1) wf += __shfl_xor(wf, off, 64);
-    const bool wi = __ballot(!integral) == 0;
-    if (lane == 0) {
-      red_sum[wave] = ws; red_fuel[wave] = wf;
-      red_w[wave] = ww; red_e[wave] = we; red_t[wave] = wt; red_i[wave] = wi;
-    }
-    __syncthreads();
-    if (wave == 0) {
-      double total = 0;
-      unsigned long long tf = 0;
-      unsigned tw = 0, te = 0, tt = 0;
-      bool ti = true;
-      for (int k = 0; k < kWavesPerBlock; ++k) {
-        total += red_sum[k]; tf += red_fuel[k]; tw += red_w[k]; te += red_e[k]; tt += red_t[k]; ti = ti && red_i[k];
-      }
-      // Integral scores: every partial is an exact integer below 2^48, so
-      // any order gives the canonical total.
-      const double fitness = ti ? total : canonical_from_scores(e, C, scores, lane);
+    // Integral scores below 2^36 in magnitude: every partial sum is exact.
+    const bool fractional =
+        __ballot(active && !(sc == double(std::int64_t(sc)) && sc >= -68719476736.0 && sc <= 68719476736.0)) != 0;
+    const double ws = wave_sum_d(sc);
+    unsigned tw = wave_sum_u(active && !exact), te = wave_sum_u(active && r.err != ERR_NONE);
+    unsigned tt = wave_sum_u(active && r.err == ERR_TIMEOUT);
+    unsigned long long tf = active ? unsigned(r.fuel_used) : 0u;
+    for (int off = 32; off > 0; off >>= 1) tf += __shfl_xor(tf, off, 64);
+    long long total = fractional ? 0 : (long long)ws;
+    bool frac = fractional;
+    if (tiles > 1) {
+      GAS LoopAcc* const acc = d.loop_acc + slot;
+      __threadfence();  // this tile's scores before its arrival
+      unsigned prev = 0;
       if (lane == 0) {
-        // Ranking rounds every published fitness to a 40-bit mantissa.
-        if (mode == 1) { d.c_fit[slot] = canon40(fitness); d.c_wrong[slot] = tw; }
-        else { d.fitness[slot] = canon40(fitness); d.wrong[slot] = tw; }
-        d.e_errors[slot] = te;
-        d.e_timeouts[slot] = tt;
-        d.e_fuel[slot] = tf;
+        atomicAdd(&acc->sum, (unsigned long long)total);
+        atomicAdd(&acc->fuel, tf);
+        atomicAdd(&acc->wrong, tw);
+        atomicAdd(&acc->errors, te);
+        atomicAdd(&acc->timeouts, tt);
+        if (fractional) atomicOr(&acc->fractional, 1u);
+        __threadfence();
+        prev = atomicAdd(&acc->tiles, 1u);
       }
+      if (__shfl(prev, 0, 64) != unsigned(tiles - 1)) continue;  // another tile finishes the program
+      __threadfence();  // every tile's scores and counts after the last arrival
+      unsigned long long v[2] = {0, 0};
+      unsigned u[4] = {0, 0, 0, 0};
+      if (lane == 0) {
+        v[0] = atomicAdd(&acc->sum, 0ull); v[1] = atomicAdd(&acc->fuel, 0ull);
+        u[0] = atomicAdd(&acc->wrong, 0u); u[1] = atomicAdd(&acc->errors, 0u);
+        u[2] = atomicAdd(&acc->timeouts, 0u); u[3] = atomicAdd(&acc->fractional, 0u);
+        // cleared for the program's next evaluation
+        atomicExch(&acc->sum, 0ull); atomicExch(&acc->fuel, 0ull); atomicExch(&acc->wrong, 0u);
+        atomicExch(&acc->errors, 0u); atomicExch(&acc->timeouts, 0u); atomicExch(&acc->fractional, 0u);
+        atomicExch(&acc->tiles, 0u);
+      }
+      total = (long long)__shfl(v[0], 0, 64);
+      tf = __shfl(v[1], 0, 64);
+      tw = __shfl(u[0], 0, 64); te = __shfl(u[1], 0, 64); tt = __shfl(u[2], 0, 64);
+      frac = __shfl(u[3], 0, 64) != 0;
+    } else {
+      wave_fence();  // the canonical replay reads the other lanes' scores
     }
-    __syncthreads();  // the score buffer and counts are reused by the next program
+    const double fitness = frac ? canonical_from_scores(e, C, scores, lane) : double(total);
+    if (lane == 0) {
+      // Ranking rounds every published fitness to a 40-bit mantissa.
+      if (mode == 1) { d.c_fit[slot] = canon40(fitness); d.c_wrong[slot] = tw; }
+      else { d.fitness[slot] = canon40(fitness); d.wrong[slot] = tw; }
+      d.e_errors[slot] = te;
+      d.e_timeouts[slot] = tt;
+      d.e_fuel[slot] = tf;
+    }
   }
 }
 
@@ -1524,7 +1494,10 @@ struct SearchState {
     DeviceCases dc;
     dc.upload(a, train);
     x.cd = dc.cd;
-    if (x.cd.C <= kScoreCases) x.loop_scores = a.alloc<double>(loop_score_rows(x.cd.C, std::max(P, B)) * std::size_t(x.cd.C));
+    if (x.cd.C <= kScoreCases) {
+      x.loop_scores = a.alloc<double>(std::size_t(std::max(P, B)) * std::size_t(x.cd.C));
+      x.loop_acc = a.upload(std::vector<LoopAcc>(std::size_t(std::max(P, B))));
+    }
     const std::size_t rows = std::size_t(P) * R, crows = std::size_t(B) * R;
     x.cur_origin = a.alloc<int>(rows); x.cur_site = a.alloc<int>(rows); x.cur_cval = a.alloc<Val>(rows);
     x.cur_n = a.alloc<int>(P); x.cur_nc = a.alloc<int>(P);
