@@ -123,7 +123,8 @@ struct DSearch {
   int early_exit;            // stop at the first generation boundary whose best member has no wrong case
   GAS int* stop;             // generations completed when it stopped early, else 0
   GAS int* elite;            // elite members of the current generation, best first
-  int temp_slots;      // hidden-local slots per lane of the general VM (traversal temporaries)
+  int temp_slots;      // hidden-local slots per lane of the general VM (temporaries, grammar locals, input shadows)
+  int input_shadow;    // the last input_shadow of them shadow stored inputs (statement input targets), else 0
   int walks;           // the grammar has traversals: lowering runs the kWalks instantiations
   int general_scores;  // the general evaluation kernel has an LDS score buffer for this search
   GAS double* loop_scores;  // the looping kernel's case scores: C per program slot
@@ -193,7 +194,7 @@ __device__ inline void lower_into_w(const PhaseView& g, const DSearch& d, const 
   const LowerResult r =
       lower_phase_w<kWalks>(g, row, d.local_types, d.n_locals, d.expected_type, code, d.code_cap, pool, d.pool_cap, ws);
   const bool ok = r.status == ST_OK && r.stack_bound >= 1 && r.stack_bound <= d.stack_slots &&
-                  r.locals - d.n_locals <= d.temp_slots;
+                  r.locals - d.n_locals <= d.temp_slots - d.input_shadow;
   if (wlane() == 0) {
     if (!ok) raise(d, 200 + r.status);
     d.code_len[slot] = ok ? r.code : 0;
@@ -684,12 +685,13 @@ __global__ __launch_bounds__(kEvalThreads) void k_eval_loop(DSearch* all, const 
 }
 
 // A wavefront's LDS stack of `slots` operand slots followed by `temps`
-// hidden-local slots, 10 bytes per lane and slot.
-__device__ inline WaveStack stack_of(unsigned char* smem, int slots, int temps, int wave) {
+// hidden-local slots, 10 bytes per lane and slot; the last `shadow` of them
+// shadow stored inputs.
+__device__ inline WaveStack stack_of(unsigned char* smem, int slots, int temps, int wave, int shadow) {
   const int all = slots + temps;
   unsigned char* base = smem + std::size_t(wave) * all * 64 * 10;
   return WaveStack{reinterpret_cast<std::int64_t*>(base), base + std::size_t(all) * 64 * 8,
-                   base + std::size_t(all) * 64 * 9, slots, temps};
+                   base + std::size_t(all) * 64 * 9, slots, temps, shadow};
 }
 
 // mode 0: score members 0..P-1 (initial population), mode 1: pending
@@ -703,7 +705,7 @@ __global__ __launch_bounds__(kEvalThreads) void k_eval(DSearch* all, const Block
   if (mode == 1 ? !search_live(d) : (!d.active || *d.error)) return;
   extern __shared__ unsigned char smem[];
   const int wave = wave_index(), lane = threadIdx.x % 64;
-  WaveStack st = stack_of(smem, kFast ? 0 : d.stack_slots, kFast ? 0 : d.temp_slots, wave);
+  WaveStack st = stack_of(smem, kFast ? 0 : d.stack_slots, kFast ? 0 : d.temp_slots, wave, kFast ? 0 : d.input_shadow);
   // Score buffers follow the stacks of the block's wavefronts.
   double* scores =
       (kFast ? d.cd.C <= kScoreCases : d.general_scores != 0)
@@ -1251,10 +1253,10 @@ __device__ inline ProgView view_of(const DProg& p) {
 // doubles per wavefront); otherwise the rest (LDS: the general stacks).
 template <bool kFast>
 __global__ __launch_bounds__(kEvalThreads) void k_eval_programs(const DProg* progs, int n, EvalParams e, int slots, int temps,
-                                                                 double* fitness, unsigned* wrong, unsigned* counts) {
+                                                                 int shadow, double* fitness, unsigned* wrong, unsigned* counts) {
   extern __shared__ unsigned char smem[];
   const int wave = wave_index(), lane = threadIdx.x % 64;
-  WaveStack st = stack_of(smem, kFast ? 0 : slots, kFast ? 0 : temps, wave);
+  WaveStack st = stack_of(smem, kFast ? 0 : slots, kFast ? 0 : temps, wave, kFast ? 0 : shadow);
   double* scores = kFast ? reinterpret_cast<double*>(smem) + std::size_t(wave) * e.cd.C : nullptr;
   for (int i = blockIdx.x * kWavesPerBlock + wave; i < n; i += gridDim.x * kWavesPerBlock) {
     const DProg p = progs[i];
@@ -1270,11 +1272,11 @@ __global__ __launch_bounds__(kEvalThreads) void k_eval_programs(const DProg* pro
   }
 }
 template <bool kFast>
-__global__ __launch_bounds__(kEvalThreads) void k_eval_cases(DProg p, CaseData cd, int fuel, int slots, int temps, Val* values,
-                                                              int* errors) {
+__global__ __launch_bounds__(kEvalThreads) void k_eval_cases(DProg p, CaseData cd, int fuel, int slots, int temps, int shadow,
+                                                              Val* values, int* errors) {
   extern __shared__ unsigned char smem[];
   const int wave = wave_index(), lane = threadIdx.x % 64;
-  WaveStack st = stack_of(smem, kFast ? 0 : slots, kFast ? 0 : temps, wave);
+  WaveStack st = stack_of(smem, kFast ? 0 : slots, kFast ? 0 : temps, wave, kFast ? 0 : shadow);
   const ProgView pv = view_of(p);
   FastProgram fp{};
   if (kFast) fp = fast_load(pv, lane);
@@ -1472,6 +1474,8 @@ struct SearchState {
     x.temp_slots = statement ? std::min(64 - int(train.input_names.size()), int(hidden_local_bound(grammar)))
                    : walks  ? std::max(0, std::min(64 - int(train.input_names.size()), 9 * std::min(walks, 8)))
                             : 0;
+    x.input_shadow = phase.input_writes ? int(train.input_names.size()) : 0;
+    x.temp_slots += x.input_shadow;
     x.rounds = rounds; x.per_epoch = per_epoch;
     x.fuel = fuel; x.lanes = 512; x.penalty = penalty; x.policy = task.policy;
     x.pressure = std::clamp(cfg.tournament, 1, P);
@@ -2121,7 +2125,7 @@ namespace {
 // Static stack heights of external bytecode (the verifier's stack-shape
 // pass, cpp/src/runtime/bytecode_verify.cpp): every instruction gets one
 // height; joins must agree; operand/constant/jump domains are checked.
-std::vector<Ins> verify_program(const ProgramIn& p, int n_inputs, int& max_stack, bool& jumps) {
+std::vector<Ins> verify_program(const ProgramIn& p, int n_inputs, int& max_stack, bool& jumps, bool* input_stores = nullptr) {
   const int n = int(p.op.size());
   if (n == 0) throw std::invalid_argument("empty program");
   if (p.a.size() != std::size_t(n) || p.b.size() != std::size_t(n) || p.has_a.size() != std::size_t(n) ||
@@ -2170,7 +2174,8 @@ std::vector<Ins> verify_program(const ProgramIn& p, int n_inputs, int& max_stack
         push = 1; break;
       case OP_STORE:
         if (!p.has_a[ip] || p.a[ip] < 0 || p.a[ip] >= 64) throw std::invalid_argument("local index out of range");
-        if (p.a[ip] < n_inputs) throw Unsupported("stores into evaluator inputs");
+        // A store into an input slot: the evaluators shadow it per execution.
+        if (p.a[ip] < n_inputs && input_stores) *input_stores = true;
         pop = 1; break;
       case OP_NEG: pop = 1; push = 1; falls = !failing_neg(ip); break;
       case OP_NOT: case OP_CHECK_INT: case OP_CHECK_LIST: pop = 1; push = 1; break;
@@ -2324,11 +2329,12 @@ void Session::eval(const std::vector<ProgramIn>& programs, double* fitness, std:
   std::vector<std::size_t> code_off, pool_off;
   std::vector<int> lens, jumps_v, locals, fast_v, pool_n, result_v;
   int slots = 1, temps = 0;
+  bool stores = false;  // some program stores into an input
   for (const auto& p : programs) {
     temps = std::max(temps, std::min(64, p.n_locals) - I.cases.cd.n_inputs);
     int ms = 0;
     bool j = false;
-    const auto ins = verify_program(p, I.cases.cd.n_inputs, ms, j);
+    const auto ins = verify_program(p, I.cases.cd.n_inputs, ms, j, &stores);
     for (const auto& v : p.consts) check_view(v, I.arena_len);
     const FastCert cert = certify_fast(ins, p.consts, I.input_tags, p.n_locals, ms, j, I.cases.cd.C);
     fast_v.push_back(cert.fast);
@@ -2355,6 +2361,8 @@ void Session::eval(const std::vector<ProgramIn>& programs, double* fitness, std:
   auto dwrong = tmp.alloc<unsigned>(std::size_t(n));
   auto dcounts = tmp.alloc<unsigned>(std::size_t(n) * 3);
   const EvalParams e{I.cases.cd, I.fuel, I.lanes, I.penalty, I.policy};
+  const int shadow = stores ? I.cases.cd.n_inputs : 0;
+  temps += shadow;
   const std::size_t lds = std::size_t(kWavesPerBlock) * (slots + temps) * 64 * 10;
   const std::size_t fast_lds = std::size_t(kWavesPerBlock) * 8 * std::size_t(I.cases.cd.C);
   const bool any_fast = std::count(fast_v.begin(), fast_v.end(), 1) > 0, any_general = std::count(fast_v.begin(), fast_v.end(), 0) > 0;
@@ -2370,11 +2378,11 @@ void Session::eval(const std::vector<ProgramIn>& programs, double* fitness, std:
   const int blocks = std::min(4096, (n + kWavesPerBlock - 1) / kWavesPerBlock);
   HIP_OK(hipEventRecord(a, nullptr));
   if (any_fast)
-    hipLaunchKernelGGL(k_eval_programs<true>, dim3(blocks), dim3(kEvalThreads), fast_lds, nullptr, dprogs, n, e, slots, temps, dfit,
-                       dwrong, dcounts);
+    hipLaunchKernelGGL(k_eval_programs<true>, dim3(blocks), dim3(kEvalThreads), fast_lds, nullptr, dprogs, n, e, slots, temps, shadow,
+                       dfit, dwrong, dcounts);
   if (any_general)
-    hipLaunchKernelGGL(k_eval_programs<false>, dim3(blocks), dim3(kEvalThreads), lds, nullptr, dprogs, n, e, slots, temps, dfit,
-                       dwrong, dcounts);
+    hipLaunchKernelGGL(k_eval_programs<false>, dim3(blocks), dim3(kEvalThreads), lds, nullptr, dprogs, n, e, slots, temps, shadow,
+                       dfit, dwrong, dcounts);
   HIP_OK(hipGetLastError());
   HIP_OK(hipEventRecord(b, nullptr));
   HIP_OK(hipEventSynchronize(b));
@@ -2390,9 +2398,11 @@ void Session::eval_cases(const ProgramIn& program, std::vector<Val>& values, std
   auto& I = *impl_;
   int ms = 0;
   bool j = false;
-  const auto ins = verify_program(program, I.cases.cd.n_inputs, ms, j);
+  bool stores = false;
+  const auto ins = verify_program(program, I.cases.cd.n_inputs, ms, j, &stores);
   for (const auto& v : program.consts) check_view(v, I.arena_len);
-  const int slots = ms + 1, temps = std::max(0, std::min(64, program.n_locals) - I.cases.cd.n_inputs);
+  const int shadow = stores ? I.cases.cd.n_inputs : 0;
+  const int slots = ms + 1, temps = std::max(0, std::min(64, program.n_locals) - I.cases.cd.n_inputs) + shadow;
   DeviceArena tmp;
   std::vector<Val> consts = program.consts;
   if (consts.empty()) consts.push_back(val_invalid());
@@ -2404,14 +2414,15 @@ void Session::eval_cases(const ProgramIn& program, std::vector<Val>& values, std
   auto de = tmp.alloc<int>(std::size_t(C));
   const int blocks = std::max(1, std::min(1024, (C + 255) / 256));
   if (cert.fast) {
-    hipLaunchKernelGGL(k_eval_cases<true>, dim3(blocks), dim3(kEvalThreads), 0, nullptr, p, I.cases.cd, I.fuel, slots, temps, dv, de);
+    hipLaunchKernelGGL(k_eval_cases<true>, dim3(blocks), dim3(kEvalThreads), 0, nullptr, p, I.cases.cd, I.fuel, slots, temps, shadow,
+                       dv, de);
   } else {
     const std::size_t lds = std::size_t(kWavesPerBlock) * (slots + temps) * 64 * 10;
     if (lds > 65536)
       HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_eval_cases<false>), hipFuncAttributeMaxDynamicSharedMemorySize,
                                  int(lds)));
-    hipLaunchKernelGGL(k_eval_cases<false>, dim3(blocks), dim3(kEvalThreads), lds, nullptr, p, I.cases.cd, I.fuel, slots, temps, dv,
-                       de);
+    hipLaunchKernelGGL(k_eval_cases<false>, dim3(blocks), dim3(kEvalThreads), lds, nullptr, p, I.cases.cd, I.fuel, slots, temps, shadow,
+                       dv, de);
   }
   HIP_OK(hipGetLastError());
   HIP_OK(hipDeviceSynchronize());

@@ -14,6 +14,7 @@ const std::string* node_name(const Grammar& g, const CExpr& e) {
   if (e.kind == EKind::Input) return &g.inputs.at(e.target).name;
   if (e.kind == EKind::Local) return &g.locals.at(e.target).name;
   if (e.kind == EKind::Control && e.local >= 0) return &g.locals.at(std::size_t(e.local)).name;
+  if (e.kind == EKind::Control && e.input >= 0) return &g.inputs.at(std::size_t(e.input)).name;
   return nullptr;
 }
 }  // namespace
@@ -27,6 +28,7 @@ std::vector<std::string> names_used(const Grammar& g, const std::vector<int>& or
     if (e.kind == EKind::Input) used.at(e.target) = 1;
     else if (e.kind == EKind::Local) used.at(ni + e.target) = 1;
     else if (e.kind == EKind::Control && e.local >= 0) used.at(ni + std::size_t(e.local)) = 1;
+    else if (e.kind == EKind::Control && e.input >= 0) used.at(std::size_t(e.input)) = 1;
   }
   std::vector<std::string> out;
   for (std::size_t k = 0; k < used.size(); ++k)

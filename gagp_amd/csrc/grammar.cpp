@@ -429,9 +429,14 @@ class Compiler {
     if (named) {
       if (v.has("name") + v.has("input_name") != 1)
         throw std::invalid_argument("control requires exactly one name or input_name target");
-      if (v.has("input_name")) throw Unsupported("writes to evaluator inputs (input_name targets)");
-      e.local = int(binding(g_.locals, v.at("name").as_string("name")));
-      if (g_.locals[std::size_t(e.local)].type != want[0].type) throw std::invalid_argument("control target local type mismatch");
+      if (v.has("input_name")) {
+        // A write to an evaluator input: its slot, for this execution only.
+        e.input = int(binding(g_.inputs, v.at("input_name").as_string("input_name")));
+        if (g_.inputs[std::size_t(e.input)].type != want[0].type) throw std::invalid_argument("control target input type mismatch");
+      } else {
+        e.local = int(binding(g_.locals, v.at("name").as_string("name")));
+        if (g_.locals[std::size_t(e.local)].type != want[0].type) throw std::invalid_argument("control target local type mismatch");
+      }
     } else if (v.has("name") || v.has("input_name")) {
       throw std::invalid_argument("control does not accept a target name");
     }
@@ -625,6 +630,12 @@ PhaseGrammar compile_phase(const Grammar& g, unsigned root, unsigned max_depth, 
       if (e.local >= 0) {
         if (e.local >= 16) fail("grammar local capacity");
         x.node.operand = e.local;
+      } else if (e.input >= 0) {
+        // An input target: -1 - its evaluator slot (phase_compile.cuh:281-283).
+        const auto it = std::find(input_order.begin(), input_order.end(), g.inputs.at(std::size_t(e.input)).name);
+        if (it == input_order.end()) fail("input missing from evaluator order");
+        x.node.operand = -1 - int(it - input_order.begin());
+        out.input_writes = true;
       }
       // Assign/Return carry the exact stored or returned value type.
       if (e.node_kind == NK_ASSIGN || e.node_kind == NK_RETURN) x.node.type = g.exprs[e.children.at(0)].type;

@@ -63,8 +63,10 @@ struct CExpr {
   int direction = 0;
   int region_argument = -1;
   std::vector<Binding> region_bindings;
-  // A named control (assign, for_range): its target grammar local.
+  // A named control (assign, for_range): its target grammar local, or the
+  // evaluator input it writes (input_name).
   int local = -1;
+  int input = -1;
 };
 struct CNt {
   std::string id;
@@ -143,6 +145,7 @@ struct PhaseGrammar {
   int root = -1, depth_count = 0, logical_count = 1;
   bool site_depth = false;
   bool statements = false, lets = false;
+  bool input_writes = false;  // a control writes an evaluator input
   // Traversal nodes in every program of the entry nonterminal (the grammar
   // admits traversals only where that count is the same for all programs).
   int traversals = 0;

@@ -551,11 +551,13 @@ GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, Origi
     }
     return slot;
   };
-  // The slot of grammar local `operand`, allocated on first use; never freed.
+  // The slot of grammar local `operand`, allocated on first use and never
+  // freed; a negative operand names evaluator input slot -1 - operand.
   const auto variable_slot = [&](int operand) -> int {
     int slot = -1;
     if constexpr (kWalks) {
-      if (operand < 0 || operand >= 16) return -1;
+      if (operand < 0) return -1 - operand < local_count ? -1 - operand : -1;
+      if (operand >= 16) return -1;
       if (!(ws.named_mask >> operand & 1u)) {
         if (out.locals >= 64) { out.status = ST_CAPACITY; return -1; }
         ws.named[operand] = static_cast<signed char>(out.locals++);

@@ -48,6 +48,10 @@ struct WaveStack {
   std::uint8_t* b;
   int slots;
   int temps;
+  // The last `shadow` hidden slots shadow the evaluator inputs of programs
+  // that store into them (statement targets): a store lands there and later
+  // loads of that input read it; 0 when inputs are read-only.
+  int shadow;
 };
 
 #if defined(__HIPCC__)
@@ -141,9 +145,16 @@ __device__ inline LaneResult vm_run(const ProgView& p, const CaseData& cd, int c
         if (ins.a >= cd.n_inputs) {  // a hidden local
           const int k = ins.a - cd.n_inputs;
           const int at_t = (st.slots + k) * 64 + lane;
-          if (k >= st.temps || st.t[at_t] == TAG_INVALID) { r.err = ERR_NAME; done = true; break; }
+          if (k >= st.temps - st.shadow || st.t[at_t] == TAG_INVALID) { r.err = ERR_NAME; done = true; break; }
           st.v[(sp) * 64 + lane] = st.v[at_t]; st.t[(sp) * 64 + lane] = st.t[at_t]; st.b[(sp) * 64 + lane] = st.b[at_t];
           break;
+        }
+        if (ins.a < st.shadow) {  // a stored input reads its shadow
+          const int at_s = (st.slots + st.temps - st.shadow + ins.a) * 64 + lane;
+          if (st.t[at_s] != TAG_INVALID) {
+            st.v[(sp) * 64 + lane] = st.v[at_s]; st.t[(sp) * 64 + lane] = st.t[at_s]; st.b[(sp) * 64 + lane] = st.b[at_s];
+            break;
+          }
         }
         const std::size_t at_in = std::size_t(ins.a) * cd.C + c;
         const std::uint8_t tag = cd.in_tag[at_in];
@@ -151,9 +162,11 @@ __device__ inline LaneResult vm_run(const ProgView& p, const CaseData& cd, int c
         st.v[(sp) * 64 + lane] = cd.in_val[at_in]; st.t[(sp) * 64 + lane] = tag; st.b[(sp) * 64 + lane] = cd.in_b[at_in];
         break;
       }
-      case OP_STORE: {  // lowering stores only hidden locals
-        const int k = ins.a - cd.n_inputs;
-        if (ins.a < cd.n_inputs || k >= st.temps) { r.err = ERR_VALUE; done = true; break; }
+      case OP_STORE: {  // a hidden local, or an input's shadow
+        const int k = ins.a < cd.n_inputs ? st.temps - st.shadow + ins.a : ins.a - cd.n_inputs;
+        if (ins.a < 0 || (ins.a < cd.n_inputs ? ins.a >= st.shadow : k >= st.temps - st.shadow)) {
+          r.err = ERR_VALUE; done = true; break;
+        }
         const int at_t = (st.slots + k) * 64 + lane;
         st.v[at_t] = st.v[(sp - 1) * 64 + lane]; st.t[at_t] = st.t[(sp - 1) * 64 + lane]; st.b[at_t] = st.b[(sp - 1) * 64 + lane];
         break;
@@ -386,6 +399,14 @@ struct FastInputs {
   std::int64_t v0, v1, v2, v3;
   unsigned bound;
   __device__ std::int64_t pick(int k) const { return k == 0 ? v0 : k == 1 ? v1 : k == 2 ? v2 : v3; }
+  // A store into input k (uniform): later loads read it, and it is bound.
+  __device__ void set(int k, std::int64_t v) {
+    if (k == 0) v0 = v;
+    else if (k == 1) v1 = v;
+    else if (k == 2) v2 = v;
+    else v3 = v;
+    bound |= 1u << (k & 31);
+  }
 };
 __device__ inline FastInputs fast_inputs(const CaseData& cd, int n_inputs, int c, bool active) {
   FastInputs in{0, 0, 0, 0, 0};
@@ -573,8 +594,9 @@ __device__ inline int loop_pick(const int (&v)[4], int k) {
 }
 // `slots`: this wave's LDS area of kLoopSlots untagged slots, slot-major
 // (slot k of lane j at k * 64 + j): the operand stack, then the hidden locals.
-__device__ inline LaneResult vm_loop(const LoopProgram& f, const FastInputs& in, int n_inputs, GAS const std::int64_t* arena,
+__device__ inline LaneResult vm_loop(const LoopProgram& f, FastInputs in, int n_inputs, GAS const std::int64_t* arena,
                                      LAS std::int64_t* slots, int lane, bool active, int fuel0, int result_tag) {
+  // `in` is this lane's copy: statement targets may store into inputs.
   LaneResult r{0, TAG_INVALID, 0, active ? ERR_NONE : ERR_VALUE, 0};
   bool done = !active;
   LAS std::int64_t* const sl = slots + lane;
@@ -641,7 +663,9 @@ __device__ inline LaneResult vm_loop(const LoopProgram& f, const FastInputs& in,
         const bool starve = mine && fuel < need;
         const bool run = mine && !starve;
         if (run) {
-          if (more) {
+          if (more && k_var < 0) {
+            in.set(k_var + n_inputs, counter);  // an input target (uniform)
+          } else if (more) {
             L(k_var) = counter;
             assigned |= 1u << k_var;
           } else if (!test) {
@@ -788,7 +812,11 @@ __device__ inline LaneResult vm_loop(const LoopProgram& f, const FastInputs& in,
         break;
       }
       case OP_STORE: {
-        const int k = a - n_inputs < kLoopLocals && a >= n_inputs ? a - n_inputs : 0;
+        if (a < n_inputs) {  // an input target (uniform branch)
+          if (run) in.set(a, S(top));
+          break;
+        }
+        const int k = a - n_inputs < kLoopLocals ? a - n_inputs : 0;
         if (run) { L(k) = S(top); assigned |= 1u << k; }
         break;
       }

@@ -282,6 +282,65 @@ def statement_grammar(constants, max_nodes: int = 60, max_depth: int = 12, fuel:
     }
 
 
+def input_target_grammar(max_nodes: int = 48, max_depth: int = 10, fuel: int = 600) -> dict:
+    """Statement programs over an Int input ``x`` that also write ``x``: an
+    ``assign`` and a ``for_range`` with ``input_name`` targets besides the
+    locals ``acc`` and ``i``."""
+    i_, b_, body = {"ref": "In.Int"}, {"ref": "In.Bool"}, {"ref": "In.Body"}
+    cons = lambda a, b: {"control": "block_cons(Statement,Block)->Block", "type": "Int", "args": [a, b]}  # noqa: E731
+    nil = {"control": "block_nil()->Block", "type": "Int", "args": []}
+    stmts = [
+        ("acc", 3, {"control": "assign(Int)->Statement", "type": "Int", "name": "acc", "args": [i_]}),
+        ("x", 1, {"control": "assign(Int)->Statement", "type": "Int", "input_name": "x", "args": [i_]}),
+        ("for", 1, {"control": "for_range(Int,Block)->Statement", "type": "Int", "name": "i", "args": [_const([0, 1, 2, 3]), body]}),
+        ("for_x", 1, {"control": "for_range(Int,Block)->Statement", "type": "Int", "input_name": "x",
+                      "args": [_const([0, 1, 2, 3]), body]}),
+        ("if", 1, {"control": "if_stmt(Bool,Block,Block)->Statement", "type": "Int", "args": [b_, body, body]}),
+    ]
+    ints = [("x", 3, {"input": "x"}), ("acc", 2, {"local": "acc"}), ("i", 1, {"local": "i"}), ("const", 2, _const([0, 1, 2])),
+            ("add", 1, _op("add(Int,Int)->Int", i_, i_)), ("sub", 1, _op("sub(Int,Int)->Int", i_, i_)),
+            ("mul", 1, _op("mul(Int,Int)->Int", i_, i_))]
+    return {
+        "format_version": "grammar-definition-v2",
+        "entry": {"category": "Program", "nonterminal": "In.Program", "type": "Int"},
+        "inputs": [{"name": "x", "type": "Int"}],
+        "locals": [{"name": "acc", "type": "Int"}, {"name": "i", "type": "Int"}],
+        "search_limits": {"max_nodes": max_nodes, "max_depth": max_depth},
+        "execution_limits": {"fuel": fuel},
+        "nonterminals": [
+            {"id": "In.Program", "category": "Program", "type": "Int", "scope": [], "variation": False, "alternatives": [
+                {"id": "program", "weight": 1, "expression": {
+                    "control": "program(Block)->Program", "type": "Int",
+                    "args": [cons({"control": "assign(Int)->Statement", "type": "Int", "name": "acc", "args": [i_]},
+                                  {"ref": "In.Tail"})]}}]},
+            {"id": "In.Tail", "category": "Block", "type": "Int", "scope": [], "alternatives": [
+                {"id": "more", "weight": 3, "expression": cons({"ref": "In.Stmt"}, {"ref": "In.Tail"})},
+                {"id": "final", "weight": 1, "expression": cons(
+                    {"control": "return(Int)->Statement", "type": "Int", "args": [i_]}, nil)}]},
+            {"id": "In.Body", "category": "Block", "type": "Int", "scope": [], "alternatives": [
+                {"id": "nil", "weight": 2, "expression": nil},
+                {"id": "cons", "weight": 2, "expression": cons({"ref": "In.Stmt"}, body)}]},
+            {"id": "In.Stmt", "category": "Statement", "type": "Int", "scope": [], "alternatives": [
+                {"id": name, "weight": w, "expression": e} for name, w, e in stmts]},
+            {"id": "In.Int", "type": "Int", "scope": [], "alternatives": [
+                {"id": name, "weight": w, "expression": e} for name, w, e in ints]},
+            {"id": "In.Bool", "type": "Bool", "scope": [], "alternatives": [
+                {"id": "lt", "weight": 1, "expression": _op("lt(Int,Int)->Bool", i_, i_)},
+                {"id": "eq", "weight": 1, "expression": _op("eq(Int,Int)->Bool", i_, i_)}]},
+        ],
+    }
+
+
+def input_target_task(directory: str, seed: int = 5) -> str:
+    """y = x^2 - 2x + 3 over x ~ U{-20..20} (32 train / 40 test cases),
+    searched with input_target_grammar."""
+    rng = SplitMix64(seed)
+    f = lambda x: x * x - 2 * x + 3  # noqa: E731
+    train = [_int_case(x, f(x)) for x in (rng.uniform_int(-20, 20) for _ in range(32))]
+    test = [_int_case(x, f(x)) for x in (rng.uniform_int(-20, 20) for _ in range(40))]
+    return write_task(directory, "stmt_input", input_target_grammar(), train, test)
+
+
 LIST_KINDS = ("sum", "max", "count_gt", "sum_even")
 
 

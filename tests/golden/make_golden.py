@@ -76,9 +76,11 @@ RUNS = {
     "stmt_count_gt_p300_g8_s6": ("stmt_count_gt", 300, 8, 6, False),
     "stmt_lowfuel_p400_g8_s8": ("stmt_lowfuel", 400, 8, 8, False),
     "stmt_count_gt_gen_e1_p256_g8_s2": ("stmt_count_gt", 256, 8, 2, False, "generational_e1.json"),
+    # statement programs that write their Int input (assign and for_range with input_name targets)
+    "stmt_input_p300_g10_s3": ("stmt_input", 300, 10, 3, True),
 }
 # Tasks whose random traversal programs are checked against `gagp eval`.
-STATEMENT_TASKS = ["stmt_sum_even", "stmt_count_gt", "stmt_lowfuel"]
+STATEMENT_TASKS = ["stmt_sum_even", "stmt_count_gt", "stmt_lowfuel", "stmt_input"]
 TRAVERSAL_TASKS = ["list-sum", "list_count_gt_rev", "list_range_nested", "list_lowfuel", "list_max_range",
                    "list_sum_even"] + STATEMENT_TASKS
 
@@ -94,6 +96,7 @@ def list_tasks():
     workloads.list_task(os.path.join(TASKS, "stmt_count_gt"), "count_gt", 22, 64, 32, 40, fold="statement", with_temp=True,
                         with_traverse=True)
     workloads.list_task(os.path.join(TASKS, "stmt_lowfuel"), "sum_even", 23, 64, 32, 40, fold="statement", fuel=300)
+    workloads.input_target_task(os.path.join(TASKS, "stmt_input"), 5)
 
 
 def traversal_goldens(n=40):
