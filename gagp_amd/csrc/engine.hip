@@ -1394,7 +1394,7 @@ struct SearchState {
   int P = 0, B = 0, R = 0, G = 0, rounds = 0, per_epoch = 0;
   int fuel = 0;
   double penalty = 0;
-  bool may_jump = false;   // the grammar has if/and/or, traversals or for_range
+  bool may_jump = false;   // programs may go to the looping path: if/and/or, traversals, statements, lets
   bool statement = false;  // a Program entry: statement programs with grammar locals
   // The export name table: the names the first initial member uses, in
   // declaration order (the layout prototype of phase_population_cpu.cpp:195).
@@ -1451,6 +1451,14 @@ struct SearchState {
       may_jump = may_jump || x.node.kind == TK_IF || x.node.kind == TK_AND || x.node.kind == TK_OR || x.node.kind == TK_TRAVERSE ||
                  x.node.kind == TK_FOR_RANGE;
     statement = grammar.nts[grammar.entry].cat == Cat::Program;
+    // Hidden locals (statements, lets) also make programs for the looping
+    // path, with or without jumps, and so does straight-line code beyond the
+    // straight-line VM's limits: one instruction per node plus RETURN, and a
+    // stack of at most (arity - 1) pending operands per level plus one.
+    int arity = 1;
+    for (const auto& x : phase.exprs) arity = std::max(arity, x.node.arity);
+    may_jump = may_jump || statement || phase.has_let || R + 1 > kFastCode ||
+               (arity - 1) * int(layout.slot_max_depth) + 1 > kFastStack;
   }
 
   void allocate() {
@@ -1470,8 +1478,8 @@ struct SearchState {
     x.pool_cap = 2 * R + 8 + 8 * walks + 4 * loops;
     x.stack_slots = std::min(kMaxStack, 2 * int(layout.slot_max_depth) + 2);
     // Statement programs lower through the hidden-locals instantiation too.
-    x.walks = walks > 0 || statement;
-    x.temp_slots = statement ? std::min(64 - int(train.input_names.size()), int(hidden_local_bound(grammar)))
+    x.walks = walks > 0 || statement || phase.has_let;
+    x.temp_slots = statement || phase.has_let ? std::min(64 - int(train.input_names.size()), int(hidden_local_bound(grammar)))
                    : walks  ? std::max(0, std::min(64 - int(train.input_names.size()), 9 * std::min(walks, 8)))
                             : 0;
     x.input_shadow = phase.input_writes ? int(train.input_names.size()) : 0;

@@ -51,7 +51,9 @@ Ast export_whole_program(const Grammar& g, const PhaseGrammar& phase, const std:
   std::vector<std::string> table = layout_names;
   std::vector<int> name_of_node(origins.size(), -1);
   std::size_t next_const = 0, j = 0;
-  int next_binder = 6 * phase.traversals;
+  // Binder ids follow those of the first member and of the prototype it was
+  // exported against (phase_layout.hpp:334-342): twice the fixed count.
+  int next_binder = 2 * phase.binder_ids;
   // Prefix-order emission; `env` maps binder numbers to public ids.
   std::function<void(const std::vector<int>&)> emit = [&](const std::vector<int>& env) {
     if (j >= origins.size()) throw std::runtime_error("invalid gene shape");
@@ -61,7 +63,7 @@ Ast export_whole_program(const Grammar& g, const PhaseGrammar& phase, const std:
     const CExpr& e = g.exprs[std::size_t(o)];
     const PExpr& m = phase.exprs[std::size_t(o)];
     AstNode n{m.native_kind, 0, 0};
-    int walk = -1;
+    int walk = -1, let_binder = -1;
     if (const std::string* name = node_name(g, e)) {
       auto it = std::find(table.begin(), table.end(), *name);
       if (it == table.end()) it = table.insert(table.end(), *name);
@@ -75,6 +77,10 @@ Ast export_whole_program(const Grammar& g, const PhaseGrammar& phase, const std:
       const int b = m.node.operand;
       if (b < 0 || b >= 16 || env[std::size_t(b)] < 0) throw std::runtime_error("lexical index");
       n.i0 = env[std::size_t(b)];
+    } else if (m.node.kind == TK_LET) {
+      // Recorded before its subtrees (phase_layout.hpp:400-404).
+      let_binder = next_binder++;
+      ast.lexical_regions.push_back(LexicalRegion{int(ast.nodes.size()), 1, {{let_binder, e.region_bindings.at(0).type}}});
     } else if (m.node.kind == TK_TRAVERSE) {
       // Recorded before its subtrees.
       walk = next_binder;
@@ -85,7 +91,11 @@ Ast export_whole_program(const Grammar& g, const PhaseGrammar& phase, const std:
     }
     ast.nodes.push_back(n);
     for (int c = 0; c < m.node.arity; ++c) {
-      if (walk >= 0 && c == m.node.arity - 1) {
+      if (let_binder >= 0 && c == 1) {
+        std::vector<int> inner = env;
+        inner[std::size_t(m.node.operand)] = let_binder;
+        emit(inner);
+      } else if (walk >= 0 && c == m.node.arity - 1) {
         std::vector<int> inner = env;
         for (int k = 0; k < 3; ++k) inner[std::size_t(m.node.operand + k)] = walk + k;
         emit(inner);

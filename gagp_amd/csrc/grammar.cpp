@@ -174,6 +174,18 @@ bool resolve_traversal(const Sig& s, CExpr& e) {
   return true;
 }
 
+// let(T,U)->U (catalog.cpp:91-92): the body argument binds the value.
+bool resolve_let(const Sig& s, CExpr& e) {
+  if (s.op != "let") return false;
+  if (s.args.size() != 2 || s.args[1] != s.result) throw std::invalid_argument("unknown exact primitive overload: let");
+  const auto value = [](std::uint8_t t) { return t == TAG_INT || t == TAG_BOOL || t == TAG_INT_LIST; };
+  if (!value(s.args[0]) || (s.result != TAG_INT && s.result != TAG_BOOL)) throw Unsupported("let outside Int/Bool values");
+  e.node_kind = NK_LET_REGION;
+  e.region_argument = 1;
+  e.region_bindings = {{"value", s.args[0]}};
+  return true;
+}
+
 class Compiler {
  public:
   Grammar compile(const Value& doc) {
@@ -332,7 +344,7 @@ class Compiler {
       if (v.has("resource_charge")) throw Unsupported("resource charges");
       const Sig s = parse_signature(v.at("signature").as_string("signature"));
       e.kind = EKind::Primitive;
-      if (!resolve_traversal(s, e)) resolve_primitive(s, e);
+      if (!resolve_traversal(s, e) && !resolve_let(s, e)) resolve_primitive(s, e);
       e.type = s.result;
       const auto& args = v.at("args").as_array("args");
       if (args.size() != s.args.size()) throw std::invalid_argument("primitive argument count mismatch");
@@ -531,6 +543,7 @@ PhaseGrammar compile_phase(const Grammar& g, unsigned root, unsigned max_depth, 
   using Env = std::vector<int>;
   std::vector<Env> nt_env(g.nts.size()), expr_env(g.exprs.size());
   std::function<void(unsigned, const Env&)> visit_expr, visit_nt;
+  bool has_let = false;
   visit_nt = [&](unsigned id, const Env& env) {
     if (seen_nt[id]) {
       if (nt_env[id] != env) fail("nonterminal entered under two lexical environments");
@@ -573,6 +586,7 @@ PhaseGrammar compile_phase(const Grammar& g, unsigned root, unsigned max_depth, 
       return;
     }
     const bool traversal = e.kind == EKind::Primitive && (e.node_kind == NK_TRAVERSE || e.node_kind == NK_TRAVERSE_RANGE);
+    const bool let = e.kind == EKind::Primitive && e.node_kind == NK_LET_REGION;
     if (e.children.size() > (traversal ? std::size_t(kMaxArity) : 3)) fail("arity capacity");
     x.node.arity = int(e.children.size());
     std::vector<Env> child_env(e.children.size(), env);
@@ -648,6 +662,18 @@ PhaseGrammar compile_phase(const Grammar& g, unsigned root, unsigned max_depth, 
       x.native_kind = NK_REGION_VAR;
       x.node.kind = TK_BOUND;
       x.node.operand = env[e.target];
+    } else if (let) {
+      // The body (argument 1) binds the value to the binder number above
+      // every visible one (phase_grammar.hpp let binders).
+      const int binder = env.empty() ? 0 : *std::max_element(env.begin(), env.end()) + 1;
+      if (binder >= 16 || e.region_argument != 1 || e.region_bindings.size() != 1) fail("let capability");
+      x.kind = PK_OPERATION;
+      x.native_kind = NK_LET_REGION;
+      x.node.kind = TK_LET;
+      x.node.operand = binder;
+      child_env.back().push_back(binder);
+      out.lets = true;
+      out.has_let = has_let = true;
     } else if (traversal) {
       // Element, index and accumulator bind the three binder numbers above
       // every visible one for the body (the last argument); the node keeps
@@ -715,12 +741,13 @@ PhaseGrammar compile_phase(const Grammar& g, unsigned root, unsigned max_depth, 
     const auto is_traversal = [&](const CExpr& e) {
       return e.kind == EKind::Primitive && (e.node_kind == NK_TRAVERSE || e.node_kind == NK_TRAVERSE_RANGE);
     };
+    const auto is_let = [&](const CExpr& e) { return e.kind == EKind::Primitive && e.node_kind == NK_LET_REGION; };
     // Nonterminals that can derive a traversal (fixpoint over references).
     std::vector<char> reach(g.nts.size(), 0);
     std::function<bool(unsigned)> expr_reach = [&](unsigned id) -> bool {
       const auto& e = g.exprs[id];
       if (e.kind == EKind::Reference) return reach[e.target] != 0;
-      if (is_traversal(e)) return true;
+      if (is_traversal(e) || is_let(e)) return true;
       for (auto c : e.children) if (expr_reach(c)) return true;
       return false;
     };
@@ -733,11 +760,12 @@ PhaseGrammar compile_phase(const Grammar& g, unsigned root, unsigned max_depth, 
       }
     }
     std::vector<int> nt_count(g.nts.size(), -2);  // -2 unvisited, -3 in progress
+    int w_traversal = 3, w_let = 1;  // binder numbers first, then traversals
     std::function<int(unsigned)> count_expr, count_nt;
     count_expr = [&](unsigned id) -> int {
       const auto& e = g.exprs[id];
       if (e.kind == EKind::Reference) return count_nt(e.target);
-      int n = is_traversal(e) ? 1 : 0;
+      int n = is_traversal(e) ? w_traversal : is_let(e) ? w_let : 0;
       for (auto c : e.children) {
         const int k = count_expr(c);
         if (k < 0) return -1;
@@ -760,8 +788,15 @@ PhaseGrammar compile_phase(const Grammar& g, unsigned root, unsigned max_depth, 
       return nt_count[id];
     };
     const int k = count_nt(root);
-    if (k < 0) fail("traversal count that varies between programs");
-    out.traversals = k;
+    if (k < 0) fail("traversal or let count that varies between programs");
+    out.binder_ids = k;
+    std::fill(nt_count.begin(), nt_count.end(), -2);
+    w_traversal = 1;
+    w_let = 0;
+    out.traversals = count_nt(root);
+    // The reference's traversal lowering consults facts about let-bound
+    // values (phase_compile.cuh:254-266), which this lowering does not restate.
+    if (has_let && out.traversals > 0) fail("let together with traversals");
   }
   // Each production root must resolve to a distinct node (site annotation
   // recovers a production from its first node's origin).
@@ -881,7 +916,8 @@ unsigned hidden_local_bound(const Grammar& g) {
       v = std::max(1 + peak(e.children.at(0), d - 1), 2 + peak(e.children.at(1), d - 1));
     } else {
       for (const auto c : e.children) v = std::max(v, peak(c, d - 1));
-      if (e.direction != 0) v += 9;  // a traversal's temporaries
+      if (e.direction != 0) v += 9;                // a traversal's temporaries
+      if (e.node_kind == NK_LET_REGION) v += 1;    // a let's value slot
     }
     open[key] = 0;
     return memo[key] = std::min(v, 64);

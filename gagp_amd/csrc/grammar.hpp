@@ -146,9 +146,13 @@ struct PhaseGrammar {
   bool site_depth = false;
   bool statements = false, lets = false;
   bool input_writes = false;  // a control writes an evaluator input
+  bool has_let = false;       // let bindings (lexical binders with one value slot)
   // Traversal nodes in every program of the entry nonterminal (the grammar
   // admits traversals only where that count is the same for all programs).
   int traversals = 0;
+  // Lexical binder numbers of every program (3 per traversal, 1 per let),
+  // the same for all programs of the grammar.
+  int binder_ids = 0;
 };
 
 // compile_phase_grammar + variation flags + static normalization.
@@ -170,7 +174,7 @@ Layout whole_program_layout(const Grammar& g);
 
 // An upper bound on the hidden locals (slots after the evaluator inputs) a
 // program of the entry nonterminal keeps live at once: every declared
-// grammar local, plus the deepest nesting of for_range bound and counter
+// grammar local, plus the deepest nesting of let value slots, for_range bound and counter
 // temporaries and traversal temporaries within the depth budget.
 unsigned hidden_local_bound(const Grammar& g);
 

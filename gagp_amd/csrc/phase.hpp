@@ -461,11 +461,14 @@ struct Walk {
 // temporaries, loop head, exit jump, the pool index of its constant 1 and
 // the live temporaries at its start.
 struct Loop { int bound, counter, head, end_jump, one, live_begin; };
+// An open let (phase_compile.cuh:268): its binder number, value slot, the
+// live temporaries at its start and the binder's outer slot and type.
+struct LetScope { int index, slot, live_begin, saved_slot, saved_type; };
 // The lowering sets every field before use (the record may live in LDS, one
 // per wavefront, where all lanes store the same values).
 template <bool kWalks> struct WalkStore {};
 template <> struct WalkStore<true> {
-  int live_count, free_count, walk_depth, loop_depth;
+  int live_count, free_count, walk_depth, loop_depth, let_depth;
   unsigned named_mask;  // grammar locals given a slot (on first use)
   signed char live[64], freed[64];
   signed char env_slot[16];
@@ -473,6 +476,7 @@ template <> struct WalkStore<true> {
   signed char named[16];
   Walk walks[8];
   Loop loops[16];
+  LetScope lets[16];
 };
 
 // Post-order lowering of an expression phase plus the program terminator,
@@ -482,7 +486,8 @@ template <> struct WalkStore<true> {
 // locals. It adds lexical reads and the traversal lowering of
 // phase_compile.cuh:444-542,596-637,730-740 without source fuel profiles
 // (each emitted instruction costs one, no operand is aliased and begin is a
-// stored temporary). It also adds statement programs (expected type
+// stored temporary), `let` (its value slot allocated before the value
+// expression, phase_compile.cuh:662-667,802-815), and statement programs (expected type
 // kStatementType, phase_compile.cuh:589-594,660-667,766-860): grammar locals
 // take a slot on first use, for_range keeps a bound and a counter temporary,
 // and no terminator follows the code.
@@ -493,7 +498,7 @@ GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, Origi
   LowerResult out{ST_INVALID, 0, 0, 0, TAG_INVALID, 0, 0, local_count, 0};
   if (t.n <= 0 || t.n > 4096) return out;
   if constexpr (kWalks) {
-    ws.live_count = ws.free_count = ws.walk_depth = ws.loop_depth = 0;
+    ws.live_count = ws.free_count = ws.walk_depth = ws.loop_depth = ws.let_depth = 0;
     ws.named_mask = 0;
     for (int i = 0; i < 16; ++i) { ws.env_slot[i] = -1; ws.env_type[i] = TAG_INVALID; }
   }
@@ -732,9 +737,9 @@ GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, Origi
         completed = kStatementType;
         have_value = true;
       } else if (kWalks && (node.kind == TK_PROGRAM || node.kind == TK_ASSIGN || node.kind == TK_RETURN ||
-                            node.kind == TK_BLOCK || node.kind == TK_FOR_RANGE)) {
+                            node.kind == TK_BLOCK || node.kind == TK_FOR_RANGE || node.kind == TK_LET)) {
         if constexpr (kWalks) {
-          const bool pair = node.kind == TK_BLOCK || node.kind == TK_FOR_RANGE;
+          const bool pair = node.kind == TK_BLOCK || node.kind == TK_FOR_RANGE || node.kind == TK_LET;
           if (node.arity != (pair ? 2 : 1)) return out;
           if (depth == 64) { out.status = ST_CAPACITY; return out; }
           if (node.kind == TK_FOR_RANGE) {
@@ -744,6 +749,16 @@ GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, Origi
             loop.live_begin = ws.live_count;
             loop.bound = temporary();
             if (loop.bound < 0) return out;
+          } else if (node.kind == TK_LET) {
+            // The value slot precedes the value expression's temporaries.
+            if (node.operand < 0 || node.operand >= 16) return out;
+            if (ws.let_depth == 16) { out.status = ST_CAPACITY; return out; }
+            LetScope& scope = ws.lets[ws.let_depth++];
+            scope.index = node.operand;
+            scope.live_begin = ws.live_count;
+            scope.slot = temporary();
+            if (scope.slot < 0) return out;
+            fast = false;  // its STORE is outside the straight-line VM, read or not
           }
           frames.set(depth++, Frame{o, 0, -1, -1, stack, {TAG_INVALID, TAG_INVALID, TAG_INVALID}});
           continue;
@@ -856,8 +871,29 @@ GHD inline LowerResult lower_phase_with(const PhaseView& g, const CRow& t, Origi
         code[loop.head].argc = std::uint8_t(kRangeTest);
       }
     }
+    if constexpr (kWalks) {
+      if (node.kind == TK_LET && f.seen == 1) {
+        // The value is stored, then the body reads it through the binder.
+        LetScope& scope = ws.lets[ws.let_depth - 1];
+        if (completed == kStatementType) return out;
+        if (!emit(OP_STORE, scope.slot, 0, node.fuel)) return out;
+        stack = f.base_stack;
+        scope.saved_slot = ws.env_slot[scope.index];
+        scope.saved_type = ws.env_type[scope.index];
+        ws.env_slot[scope.index] = static_cast<signed char>(scope.slot);
+        ws.env_type[scope.index] = static_cast<unsigned char>(completed);
+      }
+    }
     if (f.seen < node.arity) { frames.set(depth - 1, f); have_value = false; continue; }
-    if (kWalks && node.kind == TK_PROGRAM) {
+    if (kWalks && node.kind == TK_LET) {
+      if constexpr (kWalks) {
+        if (completed == kStatementType || (node.type != TAG_INVALID && node.type != completed)) return out;
+        const LetScope scope = ws.lets[--ws.let_depth];
+        ws.env_slot[scope.index] = static_cast<signed char>(scope.saved_slot);
+        ws.env_type[scope.index] = static_cast<unsigned char>(scope.saved_type);
+        while (ws.live_count > scope.live_begin) ws.freed[ws.free_count++] = ws.live[--ws.live_count];
+      }
+    } else if (kWalks && node.kind == TK_PROGRAM) {
       if (completed != kStatementType) return out;
     } else if (kWalks && node.kind == TK_BLOCK) {
       if (f.args[0] != kStatementType || f.args[1] != kStatementType) return out;

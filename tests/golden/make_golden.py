@@ -78,11 +78,14 @@ RUNS = {
     "stmt_count_gt_gen_e1_p256_g8_s2": ("stmt_count_gt", 256, 8, 2, False, "generational_e1.json"),
     # statement programs that write their Int input (assign and for_range with input_name targets)
     "stmt_input_p300_g10_s3": ("stmt_input", 300, 10, 3, True),
+    # let bindings: one let per program whose body reads the bound value
+    "let_sum_p256_g8_s1": ("let-sum", 256, 8, 1, True),
+    "let_sum_p512_g12_s4": ("let-sum", 512, 12, 4, False),
 }
 # Tasks whose random traversal programs are checked against `gagp eval`.
 STATEMENT_TASKS = ["stmt_sum_even", "stmt_count_gt", "stmt_lowfuel", "stmt_input"]
 TRAVERSAL_TASKS = ["list-sum", "list_count_gt_rev", "list_range_nested", "list_lowfuel", "list_max_range",
-                   "list_sum_even"] + STATEMENT_TASKS
+                   "list_sum_even", "let-sum"] + STATEMENT_TASKS
 
 
 def list_tasks():

@@ -41,11 +41,11 @@ def test_errors_are_reported_without_a_device():
 def test_batch_rejects_unsupported_grammar_on_the_host():
     b = gagp_amd.Batch()
     try:
-        b.add(os.path.join(GOLDEN, "tasks", "let-sum", "task.json"))
+        b.add(os.path.join(GOLDEN, "tasks", "fuel-profile", "task.json"))
     except gagp_amd.GagpError as e:
         assert e.status == gagp_amd.GAGP_ERR_UNSUPPORTED
     else:
-        raise AssertionError("let-sum binds a let region, which the engine declines")
+        raise AssertionError("fuel-profile has a source fuel profile (fuel_events), which the engine declines")
 
 
 def test_cli_usage_and_exit_codes():

@@ -88,6 +88,6 @@ def test_cli_drop_in(tmp_path, name):
 
 
 def test_cli_rejects_unsupported_with_exit_2(tmp_path):
-    r = subprocess.run([gagp_amd.CLI_PATH, "run", os.path.join(GOLDEN, "tasks", "let-sum", "task.json"), "--quiet"],
+    r = subprocess.run([gagp_amd.CLI_PATH, "run", os.path.join(GOLDEN, "tasks", "fuel-profile", "task.json"), "--quiet"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2 and "gagp:" in r.stderr

@@ -67,7 +67,7 @@ def test_budget_admits_one_search_at_a_time():
 def test_unsupported_request_fails_alone():
     print("serve: unsupported request", flush=True)
     server = gagp_amd.Server(poll_rounds=2)
-    bad = server.submit(os.path.join(GOLDEN, "tasks", "let-sum", "task.json"), arrival=0.0)
+    bad = server.submit(os.path.join(GOLDEN, "tasks", "fuel-profile", "task.json"), arrival=0.0)
     good = submit(server, "polynomial_p100_g10_s5", 0.0)
     server.run()
     assert server.outcome(bad)["status"] == gagp_amd.GAGP_ERR_UNSUPPORTED
