@@ -320,7 +320,7 @@ def main():
     ap.add_argument("--no-loop-heavy", dest="loop_heavy", action="store_false", help="skip the loop-heavy search")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: each search runs on a single device and this build has no multi-GPU "
+        print("bench.py measures one GPU: each search runs on a single device and the engine has no multi-GPU "
               "path; --gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
     if args.warmup + args.steps > GENERATIONS:

@@ -28,7 +28,7 @@ const char* kRunUsage =
     "  --population N      population size (default 1024, or the search file's)\n"
     "  --generations N     generations (default 64, or the search file's)\n"
     "  --seed N            random seed (default 0)\n"
-    "  --engine gpu        the device engine (the only engine of this build)\n"
+    "  --engine gpu        the device engine (the only engine of this command)\n"
     "  --out DIR           write DIR/result.json\n"
     "  --save-all          also write DIR/population.json and DIR/history.json\n"
     "  --quiet             no progress output\n";
@@ -75,7 +75,7 @@ int run_command(const std::vector<std::string>& args) {
     else usage_error("unexpected argument " + a);
   }
   if (spec.task_path.empty()) usage_error("missing TASK");
-  if (engine == "cpu") throw Unsupported("--engine cpu: this build is the device engine; the CPU engine is the reference's");
+  if (engine == "cpu") throw Unsupported("--engine cpu: this command runs the device engine; the CPU engine is the reference's");
   if (engine != "gpu") usage_error("--engine must be gpu or cpu");
   Batch batch;
   batch.add(spec);

@@ -1,8 +1,8 @@
 # Builds the reference's single-threaded C++ CPU engine (`gagp`) straight from
 # its sources with g++ -- no CMake, no CUDA, no OpenSSL (the grammar identity
 # hash then uses the reference's own built-in SHA-256).  Outputs go only to
-# oracle/_ref/.  Used as the checker that writes golden fixtures in the build
-# container, and as the CPU baseline ("kind": "reference") timed by bench.py.
+# oracle/_ref/.  Used to write the golden fixtures of the parity tests, and as
+# the CPU baseline ("kind": "reference") timed by bench.py.
 #
 #   make -f oracle/ref.mk -j8            (from the repository root)
 REF ?= /root/reference/cpp

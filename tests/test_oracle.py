@@ -1,5 +1,5 @@
-"""Pins the oracle (oracle/vm_oracle.c) before it is trusted as the checker of
-the device engine: reference bytecode fixtures, reference RNG known answers,
+"""Pins the oracle (oracle/vm_oracle.c) before the parity tests of the device
+engine rely on it: reference bytecode fixtures, reference RNG known answers,
 the C++ standard's mt19937_64 value, closed-form fitness totals and
 hand-derived edge cases."""
 import ctypes
