@@ -256,8 +256,8 @@ def serving(n, rate):
     reductions."""
     from gagp_amd import workloads
 
-    def serve(requests, budget):
-        server = gagp_amd.Server(budget_bytes=budget, poll_rounds=2)
+    def serve(requests, max_live):
+        server = gagp_amd.Server(budget_bytes=64 << 30, poll_rounds=2, max_live=max_live)
         tickets = [server.submit(task, None, population, generations, seed, True, arrival)
                    for task, population, generations, seed, arrival in requests]
         t0 = time.perf_counter()
@@ -272,10 +272,10 @@ def serving(n, rate):
 
     with tempfile.TemporaryDirectory() as tmp:
         requests = workloads.served_requests(tmp, n=n, seed=5, rate=rate, lists=True)
-        batched, rb = serve(requests, 64 << 30)
+        batched, rb = serve(requests, 0)
         serial, rs = serve(requests, 1)
         burst = [(task, population, generations, seed, 0.0) for task, population, generations, seed, _ in requests]
-        burst_batched, bb = serve(burst, 64 << 30)
+        burst_batched, bb = serve(burst, 0)
         burst_serial, bs = serve(burst, 1)
     return {"requests": n, "rate_per_s": rate, "early_exit": True,
             "populations": "log-uniform 256-16384", "train_cases": "log-uniform 16-2048",

@@ -77,6 +77,7 @@ def lib():
     L.gagp_batch_add.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32,
                                  ctypes.c_int32, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int32)]
     L.gagp_batch_run.argtypes = [ctypes.c_void_p, ctypes.c_int32]
+    L.gagp_batch_planned_bytes.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_uint64)]
     L.gagp_batch_result_json.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_char_p, ctypes.c_size_t,
                                          ctypes.POINTER(ctypes.c_size_t)]
     L.gagp_batch_prepare.argtypes = [ctypes.c_void_p]
@@ -88,6 +89,7 @@ def lib():
                                      ctypes.c_int32, ctypes.c_uint64, ctypes.c_int32, ctypes.c_double,
                                      ctypes.POINTER(ctypes.c_int32)]
     L.gagp_server_run.argtypes = [ctypes.c_void_p]
+    L.gagp_server_set_max_live.argtypes = [ctypes.c_void_p, ctypes.c_int32]
     L.gagp_server_outcome.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
     L.gagp_server_result_json.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_char_p, ctypes.c_size_t,
@@ -211,6 +213,11 @@ class Batch:
                                     int(seed is not None), seed or 0, ctypes.byref(idx)))
         return idx.value
 
+    def planned_bytes(self, index):
+        out = ctypes.c_uint64()
+        _check(lib().gagp_batch_planned_bytes(self.h, index, ctypes.byref(out)))
+        return out.value
+
     def set_profile(self, on=True):
         _check(lib().gagp_batch_set_profile(self.h, int(on)))
 
@@ -260,9 +267,11 @@ class Server:
     """Request loop with continuous batching (gagp_server_*): submit requests
     with arrival offsets, run() serves them in real time."""
 
-    def __init__(self, budget_bytes=64 << 30, poll_rounds=2):
+    def __init__(self, budget_bytes=64 << 30, poll_rounds=2, max_live=0):
         self.h = ctypes.c_void_p()
         _check(lib().gagp_server_create(budget_bytes, poll_rounds, ctypes.byref(self.h)))
+        if max_live:
+            _check(lib().gagp_server_set_max_live(self.h, max_live))
 
     def __del__(self):
         if getattr(self, "h", None):

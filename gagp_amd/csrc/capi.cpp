@@ -155,6 +155,13 @@ int gagp_batch_add(gagp_batch* b, const char* task_path, const char* search_path
   });
 }
 
+int gagp_batch_planned_bytes(gagp_batch* b, int32_t index, uint64_t* out) {
+  return guarded([&] {
+    if (!b || index < 0 || std::size_t(index) >= b->b.size() || !out) throw std::invalid_argument("no such search");
+    *out = b->b.planned_footprint(index);
+  });
+}
+
 int gagp_batch_run(gagp_batch* b, int32_t use_graph) {
   return guarded([&] {
     if (!b) throw std::invalid_argument("null batch");
@@ -249,6 +256,13 @@ int gagp_server_create(uint64_t budget_bytes, int32_t poll_rounds, gagp_server**
   return guarded([&] {
     if (!out) throw std::invalid_argument("invalid output pointer");
     *out = new gagp_server{Server(std::size_t(budget_bytes), poll_rounds)};
+  });
+}
+
+int gagp_server_set_max_live(gagp_server* s, int32_t max_live) {
+  return guarded([&] {
+    if (!s || s->ran || max_live < 0) throw std::invalid_argument("invalid server or limit");
+    s->s.set_max_live(max_live);
   });
 }
 

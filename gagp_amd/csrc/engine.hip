@@ -1297,26 +1297,59 @@ __global__ __launch_bounds__(kEvalThreads) void k_eval_cases(DProg p, CaseData c
 // ---- device buffers ----
 // Bump allocator over device chunks that double in size (1 MiB up to
 // 256 MiB): a search's arrays take a handful of hipMalloc calls in all.
-// Arrays are 256-byte aligned.
+// Arrays are 256-byte aligned. A counting arena (`dry`) runs the same chunk
+// arithmetic without touching the device, so the footprint of a search is
+// known before any of its memory is allocated. The arena owns its chunks:
+// it can be moved but not copied, and release() (or destruction) returns
+// them to the device.
 struct DeviceArena {
   std::vector<void*> blocks;
-  std::size_t bytes = 0;
+  std::size_t bytes = 0;     // bytes handed out
+  std::size_t reserved = 0;  // bytes of device chunks held (the footprint)
+  bool dry = false;
   char* cursor = nullptr;
-  std::size_t left = 0, next_chunk = std::size_t(1) << 20;
+  std::size_t left = 0, next_chunk = kFirstChunk;
+  static constexpr std::size_t kFirstChunk = std::size_t(1) << 20;
+
+  DeviceArena() = default;
+  explicit DeviceArena(bool count_only) : dry(count_only) {}
+  DeviceArena(const DeviceArena&) = delete;
+  DeviceArena& operator=(const DeviceArena&) = delete;
+  DeviceArena(DeviceArena&& o) noexcept { take(o); }
+  DeviceArena& operator=(DeviceArena&& o) noexcept {
+    if (this != &o) {
+      release();
+      take(o);
+    }
+    return *this;
+  }
+  ~DeviceArena() { release(); }
+
+  void release() {
+    for (void* p : blocks) (void)hipFree(p);
+    blocks.clear();
+    bytes = reserved = 0;
+    cursor = nullptr;
+    left = 0;
+    next_chunk = kFirstChunk;
+  }
   template <class T>
   GAS T* alloc(std::size_t n) {
     const std::size_t sz = (std::max<std::size_t>(n, 1) * sizeof(T) + 255) & ~std::size_t(255);
     if (sz > left) {
       const std::size_t chunk = std::max(sz, next_chunk);
       void* p = nullptr;
-      HIP_OK(hipMalloc(&p, chunk));
-      blocks.push_back(p);
+      if (!dry) {
+        HIP_OK(hipMalloc(&p, chunk));
+        blocks.push_back(p);
+      }
       cursor = static_cast<char*>(p);
       left = chunk;
+      reserved += chunk;
       next_chunk = std::min(next_chunk * 2, std::size_t(256) << 20);
     }
     void* p = cursor;
-    cursor += sz;
+    if (!dry) cursor += sz;
     left -= sz;
     bytes += sz;
     return gbl(static_cast<T*>(p));
@@ -1324,10 +1357,30 @@ struct DeviceArena {
   template <class T>
   GAS T* upload(const std::vector<T>& v) {
     GAS T* p = alloc<T>(v.size());
-    if (!v.empty()) HIP_OK(hipMemcpy((void*)p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    if (!dry && !v.empty()) HIP_OK(hipMemcpy((void*)p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
     return p;
   }
-  ~DeviceArena() { for (void* p : blocks) (void)hipFree(p); }
+  // Clears `n` elements (skipped when counting).
+  template <class T>
+  void zero(GAS T* p, std::size_t n) {
+    if (!dry && n) HIP_OK(hipMemset((void*)p, 0, n * sizeof(T)));
+  }
+
+ private:
+  void take(DeviceArena& o) {
+    blocks = std::move(o.blocks);
+    bytes = o.bytes;
+    reserved = o.reserved;
+    dry = o.dry;
+    cursor = o.cursor;
+    left = o.left;
+    next_chunk = o.next_chunk;
+    o.blocks.clear();
+    o.bytes = o.reserved = 0;
+    o.cursor = nullptr;
+    o.left = 0;
+    o.next_chunk = kFirstChunk;
+  }
 };
 
 struct DeviceCases {
@@ -1461,8 +1514,17 @@ struct SearchState {
                (arity - 1) * int(layout.slot_max_depth) + 1 > kFastStack;
   }
 
-  void allocate() {
-    auto& a = arena;
+  // Allocates every device array of the search in its arena.
+  void allocate() { d = build(arena); }
+  // Device bytes the search holds once allocated (its arena's chunks),
+  // computed without allocating.
+  std::size_t planned_bytes() const {
+    DeviceArena count(true);
+    (void)build(count);
+    return count.reserved;
+  }
+
+  DSearch build(DeviceArena& a) const {
     DSearch x{};
     x.active = 1;
     x.P = P; x.B = B; x.R = R;
@@ -1547,12 +1609,12 @@ struct SearchState {
     x.e_errors = a.alloc<unsigned>(code_rows); x.e_timeouts = a.alloc<unsigned>(code_rows);
     x.e_fuel = a.alloc<unsigned long long>(code_rows);
     x.error = a.alloc<int>(1);
-    HIP_OK(hipMemset((void*)x.n_pending, 0, 3 * sizeof(int)));
-    HIP_OK(hipMemset(x.counters, 0, sizeof(Counters)));
-    HIP_OK(hipMemset(x.error, 0, sizeof(int)));
-    HIP_OK(hipMemset(x.round, 0, sizeof(int)));
-    HIP_OK(hipMemset((void*)x.stop, 0, sizeof(int)));
-    d = x;
+    a.zero(x.n_pending, 3);
+    a.zero(x.counters, 1);
+    a.zero(x.error, 1);
+    a.zero(x.round, 1);
+    a.zero(x.stop, 1);
+    return x;
   }
 
   template <class T>
@@ -1651,7 +1713,7 @@ struct Batch::Impl {
   }
   static int up(int x, int k) { return (x + k - 1) / k; }
   void build_round_grids() {
-    map_arena = DeviceArena{};
+    map_arena.release();
     const auto vary = [&](bool staged, bool walks) {
       return build_grid(map_arena, live, [&](const SearchState& s) {
         return (s.d.stage_tables != 0) == staged && (s.d.walks != 0) == walks ? up(s.B, kVaryWaves) : 0;
@@ -1823,7 +1885,7 @@ void Batch::set_profile(bool on) { impl_->timer.on = on; }
 int Batch::max_rounds() const { return impl_->max_rounds; }
 std::size_t Batch::device_bytes() const {
   std::size_t b = 0;
-  for (const auto& s : impl_->searches) b += s->arena.bytes;
+  for (const auto& s : impl_->searches) b += s->arena.reserved;
   return b;
 }
 std::vector<std::pair<std::string, double>> Batch::kernel_times() const {
@@ -1852,15 +1914,34 @@ void Batch::prepare() {
   I.init_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
-int Batch::admit(const SearchSpec& spec) {
+std::size_t Batch::planned_footprint(int index) const { return impl_->searches.at(std::size_t(index))->planned_bytes(); }
+
+void Batch::admit(int index) {
   auto& I = *impl_;
+  SearchState& s = *I.searches.at(std::size_t(index));
+  if (s.configured || s.retired) throw std::invalid_argument("search already admitted");
   if (!I.stream) HIP_OK(hipStreamCreateWithFlags(&I.stream, hipStreamNonBlocking));
-  const int id = add(spec);
-  I.configure(*I.searches[std::size_t(id)]);
-  I.init_searches({id});
-  I.live.push_back(id);
+  try {
+    I.configure(s);
+    I.init_searches({index});
+  } catch (...) {
+    // Undo the partial admission: nothing of the search stays on the
+    // device, and the round table never saw it.
+    (void)hipStreamSynchronize(I.stream);
+    s.arena.release();
+    s.d.active = 0;
+    s.retired = true;
+    throw;
+  }
+  I.live.push_back(index);
   I.dirty = true;
-  return id;
+}
+
+void Batch::drop(int index) {
+  SearchState& s = *impl_->searches.at(std::size_t(index));
+  if (s.configured && !s.retired) throw std::invalid_argument("search is admitted: retire it");
+  s.arena.release();
+  s.retired = true;
 }
 
 void Batch::retire(int index) {
@@ -1870,7 +1951,7 @@ void Batch::retire(int index) {
   I.live.erase(it);
   I.dirty = true;
   SearchState& s = *I.searches.at(std::size_t(index));
-  s.arena = DeviceArena{};  // the stream is idle between steps: the memory is free to return
+  s.arena.release();  // the stream is idle between steps: the memory is free to return
   s.d.active = 0;
   s.retired = true;
 }
@@ -1895,7 +1976,7 @@ std::vector<Batch::Status> Batch::status() {
   return out;
 }
 
-std::size_t Batch::footprint(int index) const { return impl_->searches.at(std::size_t(index))->arena.bytes; }
+std::size_t Batch::footprint(int index) const { return impl_->searches.at(std::size_t(index))->arena.reserved; }
 
 void Batch::step(int rounds, bool use_graph) {
   auto& I = *impl_;

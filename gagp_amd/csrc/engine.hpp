@@ -72,15 +72,22 @@ class Batch {
   // Generation budget of the batch (max over searches).
   int max_rounds() const;
 
-  // Continuous batching. admit() adds, allocates and initializes one search
-  // between rounds (it joins the round table at the next step); status()
-  // reports every search of the table; retire() drops a finished search and
-  // returns its device memory (assemble its result first).
+  // Continuous batching. add() parses a search on the host only;
+  // planned_footprint() is the device memory admitting it will take (its
+  // arena's chunks), computed without allocating; admit() allocates and
+  // initializes it between rounds (it joins the round table at the next
+  // step) -- if that fails, the search's device memory is returned and the
+  // search never joins; drop() discards a search that was added but never
+  // admitted; status() reports every search of the table; retire() drops a
+  // finished search and returns its device memory (assemble its result
+  // first).
   struct Status { int index, round, stopped_at, error; bool done; };
-  int admit(const SearchSpec& spec);
+  std::size_t planned_footprint(int index) const;
+  void admit(int index);
+  void drop(int index);
   std::vector<Status> status();
   void retire(int index);
-  std::size_t footprint(int index) const;  // device bytes of one search
+  std::size_t footprint(int index) const;  // device bytes one admitted search holds
 
  private:
   struct Impl;

@@ -504,7 +504,7 @@ class Compiler {
 
 std::string read_file(const std::string& path) {
   std::ifstream in(path, std::ios::binary);
-  if (!in) throw std::runtime_error("cannot read " + path);
+  if (!in) throw std::invalid_argument("cannot read " + path);
   std::ostringstream s;
   s << in.rdbuf();
   return s.str();

@@ -31,39 +31,68 @@ void Server::run() {
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return requests_[a].arrival_s < requests_[b].arrival_s; });
   Batch batch;
-  std::map<int, int> ticket_of;  // batch index -> ticket
+  std::map<int, int> ticket_of;  // batch index -> ticket of a live search
   std::map<int, std::size_t> bytes_of;
-  std::size_t used = 0, largest = 0, next = 0, finished = 0;
+  std::size_t used = 0, next = 0, finished = 0;
+  int pending = -1;              // batch index of the head request, parsed and planned
+  std::size_t pending_bytes = 0;
   const auto t0 = Clock::now();
   const auto now = [&] { return std::chrono::duration<double>(Clock::now() - t0).count(); };
-  const auto fail = [&](ServeOutcome& o, int status, const char* what) {
+  const auto fail = [&](ServeOutcome& o, int status, const std::string& what) {
     o.status = status;
     o.message = what;
     o.done_s = now();
     ++finished;
   };
+  // Runs `f` for request `o`; any exception fails that request alone.
+  const auto guarded = [&](ServeOutcome& o, auto&& f) {
+    try {
+      f();
+      return true;
+    } catch (const Unsupported& e) {
+      fail(o, GAGP_ERR_UNSUPPORTED, e.what());
+    } catch (const std::invalid_argument& e) {
+      fail(o, GAGP_ERR_INVALID, e.what());
+    } catch (const std::exception& e) {
+      fail(o, GAGP_ERR_DEVICE, e.what());
+    }
+    return false;
+  };
   while (finished < n) {
-    // Admission in arrival order while the next search fits (head-of-line:
-    // a search that does not fit waits for memory, later ones wait behind it).
+    // Admission in arrival order while the head request fits (head-of-line:
+    // a request that does not fit waits for memory, later ones wait behind
+    // it).
     while (next < n && requests_[std::size_t(order[next])].arrival_s <= now()) {
-      if (!ticket_of.empty() && used + largest > budget_) break;
       const int ticket = order[next];
       ServeOutcome& o = outcomes_[std::size_t(ticket)];
-      try {
-        const int index = batch.admit(requests_[std::size_t(ticket)].spec);
-        o.admit_s = now();
-        ticket_of[index] = ticket;
-        bytes_of[index] = batch.footprint(index);
-        used += bytes_of[index];
-        largest = std::max(largest, bytes_of[index]);
-        peak_bytes_ = std::max(peak_bytes_, used);
-        peak_live_ = std::max(peak_live_, int(ticket_of.size()));
-      } catch (const Unsupported& e) {
-        fail(o, GAGP_ERR_UNSUPPORTED, e.what());
-      } catch (const std::invalid_argument& e) {
-        fail(o, GAGP_ERR_INVALID, e.what());
+      if (pending < 0) {
+        const bool planned = guarded(o, [&] {
+          const int index = batch.add(requests_[std::size_t(ticket)].spec);
+          pending = index;
+          pending_bytes = batch.planned_footprint(index);
+          if (pending_bytes > budget_)
+            throw std::invalid_argument("search needs " + std::to_string(pending_bytes) + " device bytes, above the serving budget of " +
+                                        std::to_string(budget_));
+        });
+        if (!planned) {
+          if (pending >= 0) batch.drop(pending);
+          pending = -1;
+          ++next;
+          continue;
+        }
       }
+      if (used + pending_bytes > budget_) break;
+      if (max_live_ > 0 && int(ticket_of.size()) >= max_live_) break;
+      const int index = pending;
+      pending = -1;
       ++next;
+      if (!guarded(o, [&] { batch.admit(index); })) continue;
+      o.admit_s = now();
+      ticket_of[index] = ticket;
+      bytes_of[index] = pending_bytes;
+      used += pending_bytes;
+      peak_bytes_ = std::max(peak_bytes_, used);
+      peak_live_ = std::max(peak_live_, int(ticket_of.size()));
     }
     if (ticket_of.empty()) {
       if (next < n) {
@@ -80,9 +109,14 @@ void Server::run() {
         o.status = GAGP_ERR_DEVICE;
         o.message = "device search error " + std::to_string(st.error);
       } else {
-        const SearchResult r = batch.result(st.index, false);
-        o.result_json = json::canonical(r.result);
-        o.generations = int(r.result.at("generations").num);
+        try {
+          const SearchResult r = batch.result(st.index, false);
+          o.result_json = json::canonical(r.result);
+          o.generations = int(r.result.at("generations").num);
+        } catch (const std::exception& e) {
+          o.status = GAGP_ERR_DEVICE;
+          o.message = e.what();
+        }
       }
       o.done_s = now();
       used -= bytes_of.at(st.index);

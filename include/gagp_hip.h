@@ -119,6 +119,14 @@ int gagp_batch_create(gagp_batch** out);
 int gagp_batch_add(gagp_batch* b, const char* task_path, const char* search_path, int32_t population, int32_t generations,
                    int32_t seed_is_set, uint64_t seed, int32_t* out_index);
 
+/* Device bytes search `index` will hold once allocated (its arena's
+ * chunks: population and child rows, bytecode, cases, history, keys),
+ * computed on the host from its documents without allocating. This is the
+ * footprint serving admits by. No reference counterpart: the reference
+ * sizes its buffers inside the owner's constructor,
+ * cpp/src/evolution/repro/gpu/phase_population.cu:75-79. */
+int gagp_batch_planned_bytes(gagp_batch* b, int32_t index, uint64_t* out);
+
 /* Runs every admitted search to completion (use_graph: capture one round as
  * a HIP graph and replay it). */
 int gagp_batch_run(gagp_batch* b, int32_t use_graph);
@@ -175,14 +183,22 @@ typedef struct gagp_server gagp_server;
  * poll_rounds: rounds stepped between completion checks. */
 int gagp_server_create(uint64_t budget_bytes, int32_t poll_rounds, gagp_server** out);
 
+/* At most max_live searches on the device at once (0, the default: only
+ * the budget limits them; 1 serves one search at a time). */
+int gagp_server_set_max_live(gagp_server* s, int32_t max_live);
+
 /* Queues one request arriving `arrival_s` seconds after gagp_server_run
  * starts; arguments as gagp_batch_add. *out_ticket identifies it. */
 int gagp_server_submit(gagp_server* s, const char* task_path, const char* search_path, int32_t population, int32_t generations,
                        int32_t seed_is_set, uint64_t seed, int32_t early_exit, double arrival_s, int32_t* out_ticket);
 
 /* Serves every submitted request to completion in real time: a request is
- * admitted once it has arrived and fits the budget, stepped with all live
- * searches in one kernel sequence per round, and retired when done. */
+ * admitted once it has arrived and its device footprint (computed from its
+ * documents before anything is allocated) fits the budget beside the live
+ * searches, stepped with all live searches in one kernel sequence per round,
+ * and retired when done. A request that cannot be served (invalid or
+ * unsupported documents, a footprint above the whole budget, a failed device
+ * allocation) fails alone with its own status; the others are served. */
 int gagp_server_run(gagp_server* s);
 
 /* times[3] = {arrival, admission, completion} in seconds from the start of

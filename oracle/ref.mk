@@ -10,12 +10,17 @@ OUT := oracle/_ref
 SRCS := $(filter-out $(REF)/src/runtime/gpu/%,$(shell find $(REF)/src -name '*.cpp' 2>/dev/null | sort))
 OBJS := $(patsubst $(REF)/src/%.cpp,$(OUT)/obj/%.o,$(SRCS))
 CXXFLAGS ?= -std=c++17 -O3 -DNDEBUG -ffp-contract=off
-all: $(OUT)/gagp
+# ref_fitness: the fitness-reduction known-answer driver (oracle/ref_fitness.cpp,
+# test infrastructure) over the same objects, without the command line's main.
+LIB_OBJS := $(filter-out $(OUT)/obj/cli/gagp_main.o,$(OBJS))
+all: $(OUT)/gagp $(OUT)/ref_fitness
 $(OUT)/obj/%.o: $(REF)/src/%.cpp
 	@mkdir -p $(dir $@)
 	g++ $(CXXFLAGS) -I$(REF)/include -c $< -o $@
 $(OUT)/gagp: $(OBJS)
 	g++ $(CXXFLAGS) $(OBJS) -o $@ -lpthread
+$(OUT)/ref_fitness: oracle/ref_fitness.cpp $(LIB_OBJS)
+	g++ $(CXXFLAGS) -I$(REF)/include oracle/ref_fitness.cpp $(LIB_OBJS) -o $@ -lpthread
 clean:
 	rm -rf $(OUT)
 .PHONY: all clean

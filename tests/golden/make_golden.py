@@ -82,6 +82,62 @@ RUNS = {
     "let_sum_p256_g8_s1": ("let-sum", 256, 8, 1, True),
     "let_sum_p512_g12_s4": ("let-sum", 512, 12, 4, False),
 }
+# Non-integral fitness: discounted proximity (-e/(1+e) per case), a
+# fractional case cap and the exact metric, over straight-line, general-VM,
+# traversal and statement programs, including case counts above 64 (several
+# tiles per program) and above the 512 reduction lanes (several cases per
+# lane), so every case-score sum goes through the canonical 48-bit order.
+FITNESS_RUNS = {
+    "fit_quintic_disc_p512_g16_s3": ("fitness/quintic_disc", 512, 16, 3, False),
+    "fit_quintic600_disc_p256_g12_s4": ("fitness/quintic600_disc", 256, 12, 4, False),
+    "fit_polynomial_cap2p5_p256_g12_s2": ("fitness/polynomial_cap2p5", 256, 12, 2, True),
+    "fit_polynomial_exact_p256_g12_s4": ("fitness/polynomial_exact", 256, 12, 4, False),
+    "fit_poly_c1_disc_gen_e3r_p256_g10_s6": ("fitness/poly_c1_disc", 256, 10, 6, False, "generational_e3_reservoir.json"),
+    "fit_list_sum_even_disc_p512_g10_s5": ("fitness/list_sum_even_disc", 512, 10, 5, True),
+    "fit_list1100_disc_p256_g6_s7": ("fitness/list1100_disc", 256, 6, 7, False),
+    "fit_list_count_gt_rev_exact_p300_g8_s6": ("fitness/list_count_gt_rev_exact", 300, 8, 6, False),
+    "fit_stmt_sum_even_disc_p512_g8_s2": ("fitness/stmt_sum_even_disc", 512, 8, 2, False),
+    "fit_stmt_count_gt_cap7p25_p300_g8_s3": ("fitness/stmt_count_gt_cap7p25", 300, 8, 3, False),
+    "fit_let_sum_disc_p256_g8_s1": ("fitness/let_sum_disc", 256, 8, 1, False),
+}
+# (variant dir, base task dir, fitness member)
+FITNESS_VARIANTS = [
+    ("quintic_disc", "quintic", {"metric": "absolute", "proximity": "discounted"}),
+    ("polynomial_cap2p5", "polynomial", {"metric": "absolute", "case_cap": 2.5}),
+    ("polynomial_exact", "polynomial", {"metric": "exact"}),
+    ("poly_c1_disc", "poly_c1", {"metric": "absolute", "proximity": "discounted"}),
+    ("list_sum_even_disc", "list_sum_even", {"metric": "absolute", "proximity": "discounted"}),
+    ("list_count_gt_rev_exact", "list_count_gt_rev", {"metric": "exact"}),
+    ("stmt_sum_even_disc", "stmt_sum_even", {"metric": "absolute", "proximity": "discounted"}),
+    ("stmt_count_gt_cap7p25", "stmt_count_gt", {"metric": "absolute", "case_cap": 7.25}),
+    ("let_sum_disc", "let-sum", {"metric": "absolute", "proximity": "discounted"}),
+]
+
+
+def fitness_tasks():
+    """The non-integral fitness tasks: variants that reuse a base task's
+    grammar and cases with another fitness member, and two tasks of their
+    own with 600 and 1100 train cases."""
+    root = os.path.join(TASKS, "fitness")
+    for name, base, fitness in FITNESS_VARIANTS:
+        base_task = json.load(open(os.path.join(TASKS, base, "task.json")))
+        task = {"format": "gagp-task-v1", "name": name}
+        for key in ("grammar", "train", "test"):
+            task[key] = os.path.join("..", "..", base, base_task[key])
+        task["fitness"] = fitness
+        os.makedirs(os.path.join(root, name), exist_ok=True)
+        with open(os.path.join(root, name, "task.json"), "w") as f:
+            json.dump(task, f, sort_keys=True)
+    disc = {"metric": "absolute", "proximity": "discounted"}
+    for name, path in (("quintic600_disc", workloads.quintic(os.path.join(root, "quintic600_disc"), 600, 64, 7)),
+                       ("list1100_disc", workloads.list_task(os.path.join(root, "list1100_disc"), "sum", 17, 1100, 64,
+                                                             40))):
+        task = json.load(open(path))
+        task["fitness"] = disc
+        with open(path, "w") as f:
+            json.dump(task, f, sort_keys=True)
+
+
 # Tasks whose random traversal programs are checked against `gagp eval`.
 STATEMENT_TASKS = ["stmt_sum_even", "stmt_count_gt", "stmt_lowfuel", "stmt_input"]
 TRAVERSAL_TASKS = ["list-sum", "list_count_gt_rev", "list_range_nested", "list_lowfuel", "list_max_range",
@@ -188,10 +244,12 @@ def main():
     workloads.poly_c1(os.path.join(TASKS, "poly_c1"))
     workloads.quintic(os.path.join(TASKS, "quintic"))
     list_tasks()
+    fitness_tasks()
     mixed = workloads.mixed_searches(os.path.join(TASKS, "mixed"), n=8, seed=4, p_range=(64, 512), c_range=(16, 256))
     mixedl = workloads.mixed_searches(os.path.join(TASKS, "mixedl"), n=6, seed=6, p_range=(64, 512), c_range=(16, 256),
                                       lists=True)
     runs = dict(RUNS)
+    runs.update(FITNESS_RUNS)
     for i, (task, population) in enumerate(mixed):
         runs["mixed_%d" % i] = (os.path.relpath(os.path.dirname(task), TASKS), population, 8, 11 + i, False)
     for i, (task, population) in enumerate(mixedl):

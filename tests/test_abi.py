@@ -73,3 +73,32 @@ def test_server_rejects_requests_before_and_after_run_on_the_host():
     assert st == gagp_amd.GAGP_ERR_INVALID and b"no such request" in L.gagp_last_error()
     assert L.gagp_server_submit(None, b"x", None, 0, 0, 0, 0, 0, 0.0, None) == gagp_amd.GAGP_ERR_INVALID
     assert L.gagp_server_run(None) == gagp_amd.GAGP_ERR_INVALID
+
+
+def test_planned_footprint_grows_with_population_and_cases_on_the_host():
+    # The footprint is computed from the documents alone (no device calls).
+    b = gagp_amd.Batch()
+    task = os.path.join(GOLDEN, "tasks", "quintic", "task.json")
+    small = b.planned_bytes(b.add(task, None, 512, 8, 0))
+    large = b.planned_bytes(b.add(task, None, 4096, 8, 0))
+    longer = b.planned_bytes(b.add(task, None, 512, 128, 0))
+    assert 1 << 20 <= small < large and small < longer
+    # Rows of 60 nodes, 24 B per node for members and 72 B per child row, the
+    # bytecode rows and the hash table: at least 100 B per node at P = 4096.
+    assert large >= 4096 * 60 * 100
+
+
+def test_server_fails_bad_requests_alone_on_the_host():
+    # Every request fails before any device call: an unreadable task, a
+    # grammar the engine declines, and a search whose footprint exceeds the
+    # whole budget. run() itself succeeds and each ticket carries its status.
+    s = gagp_amd.Server(budget_bytes=1 << 20, poll_rounds=1)
+    bad = s.submit(os.path.join(GOLDEN, "tasks", "no-such-task.json"))
+    declined = s.submit(os.path.join(GOLDEN, "tasks", "fuel-profile", "task.json"))
+    big = s.submit(os.path.join(GOLDEN, "tasks", "quintic", "task.json"), None, 4096, 8, 0)
+    s.run()
+    assert s.outcome(bad)["status"] == gagp_amd.GAGP_ERR_INVALID
+    assert s.outcome(declined)["status"] == gagp_amd.GAGP_ERR_UNSUPPORTED
+    assert s.outcome(big)["status"] == gagp_amd.GAGP_ERR_INVALID
+    with __import__("pytest").raises(gagp_amd.GagpError, match="above the serving budget"):
+        s.result(big)

@@ -47,12 +47,28 @@ def test_staggered_requests_match_reference():
     assert server.outcome(early["poly_c1_p1024_g10_s0"])["generations"] == 10
 
 
-def test_budget_admits_one_search_at_a_time():
-    # A budget smaller than any two searches serializes admission; results
-    # are unchanged.
-    print("serve: one-at-a-time budget", flush=True)
-    server = gagp_amd.Server(budget_bytes=1, poll_rounds=3)
+def planned(names):
+    b = gagp_amd.Batch()
+    out = []
+    for name in names:
+        m = MANIFEST[name]
+        search = os.path.join(GOLDEN, m["search"]) if "search" in m else None
+        out.append(b.planned_bytes(b.add(os.path.join(GOLDEN, m["task"]), search, m["population"], m["generations"],
+                                         m["seed"])))
+    return out
+
+
+@pytest.mark.parametrize("limit", ["footprint", "max_live"])
+def test_budget_admits_one_search_at_a_time(limit):
+    # A budget that holds the largest search but no two of them (admission
+    # by planned footprint), or a limit of one live search, serializes
+    # admission; results are unchanged.
+    print("serve: one-at-a-time", limit, flush=True)
     names = ["mixed_1", "mixed_2", "polynomial_p100_g10_s5"]
+    if limit == "footprint":
+        server = gagp_amd.Server(budget_bytes=max(planned(names)), poll_rounds=3)
+    else:
+        server = gagp_amd.Server(budget_bytes=8 << 30, poll_rounds=3, max_live=1)
     tickets = [submit(server, n, 0.0) for n in names]
     server.run()
     outcomes = [server.outcome(t) for t in tickets]
@@ -72,3 +88,37 @@ def test_unsupported_request_fails_alone():
     server.run()
     assert server.outcome(bad)["status"] == gagp_amd.GAGP_ERR_UNSUPPORTED
     assert strip_timing(server.result(good)) == golden("polynomial_p100_g10_s5")
+
+
+def hip_free_bytes():
+    """Free device memory as the HIP runtime the engine links reports it
+    (hipMemGetInfo of the same libamdhip64 instance)."""
+    import ctypes
+    gagp_amd.lib()
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    free, total = ctypes.c_size_t(), ctypes.c_size_t()
+    assert hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) == 0
+    return free.value
+
+
+def test_retired_searches_return_their_device_memory():
+    # Searches admitted and retired one after another (two live at a time)
+    # give every byte back: free device memory after the run equals free
+    # memory before it, while the searches held far more between them.
+    print("serve: device memory returned on retirement", flush=True)
+    names = ["quintic_p512_g16_s3", "mixed_0", "mixed_3", "polynomial_p512_g32_s7", "list_sum_even_p512_g10_s5",
+             "stmt_sum_even_p512_g10_s5", "mixedl_1", "polynomial_p1024_g18_s0"]
+    warm = gagp_amd.Server(poll_rounds=2)  # runtime state of a first run (code objects, streams)
+    submit(warm, "polynomial_p100_g10_s5", 0.0)
+    warm.run()
+    before = hip_free_bytes()
+    server = gagp_amd.Server(budget_bytes=8 << 30, poll_rounds=2, max_live=2)
+    tickets = [submit(server, n, 0.0) for n in names]
+    server.run()
+    after = hip_free_bytes()
+    for name, t in zip(names, tickets):
+        assert server.outcome(t)["status"] == 0, name
+        assert strip_timing(server.result(t)) == golden(name), name
+    held = sum(planned(names))
+    assert held > 64 << 20
+    assert before - after < 8 << 20, (before, after, held)
