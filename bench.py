@@ -19,6 +19,7 @@ import argparse
 import json
 import os
 import re
+import shutil
 import subprocess
 import sys
 import tempfile
@@ -31,6 +32,9 @@ import gagp_amd  # noqa: E402
 TASK = os.path.join(ROOT, "tests", "golden", "tasks", "quintic", "task.json")
 POPULATION, GENERATIONS, TRAIN_CASES, INPUTS = 4096, 128, 256, 1
 HBM_PEAK_GBS = 8000.0
+# Graph replays (the engine's default) or plain stream launches (--no-graph,
+# for tools that trace every dispatch).
+GRAPH = True
 REF_GAGP = os.path.join(ROOT, "oracle", "_ref", "gagp")
 TRAFFIC = os.path.join(ROOT, "profiles", "quintic_p4096_hbm_traffic.md")
 ROUND_KERNELS = ("k_record", "k_vary", "k_vary_global", "k_clear", "k_index", "k_reuse", "k_eval", "k_eval_general",
@@ -50,7 +54,10 @@ def algorithmic_bytes(c, steps):
         "k_vary": row(per["parent_nodes"], per["parent_consts"]) + row(per["child_nodes"], per["child_consts"])
         + 8 * per["child_code_words"] + 16 * per["child_pool_words"] + 60 * per["children"],
         "k_place": 16 * POPULATION + 16 * per["children"] + 20 * per["children"],
-        "k_reuse": 8 * per["children"] + 16 * per["children"],
+        # hash and stats per child, plus both rows of every child whose hash
+        # matches a member (compared in full before its result is reused)
+        "k_reuse": 24 * per["children"] + 2 * max(0.0, per["children"] - per["programs"]) * (
+            (4 * per["child_nodes"] + 16 * per["child_consts"]) / max(1.0, per["children"])),
         "k_index": 8 * POPULATION + 12 * POPULATION,
         "k_commit": row(per["child_nodes"], per["child_consts"]) * 2 + 4 * POPULATION,
     }
@@ -119,22 +126,84 @@ def cpu_baseline(generations):
         return None
     with tempfile.TemporaryDirectory() as out:
         t0 = time.perf_counter()
-        subprocess.run([REF_GAGP, "run", TASK, "--engine", "cpu", "--population", str(POPULATION), "--generations",
-                        str(generations), "--seed", "0", "--out", out, "--quiet"], check=True)
+        subprocess.run(pinned([REF_GAGP, "run", TASK, "--engine", "cpu", "--population", str(POPULATION), "--generations",
+                               str(generations), "--seed", "0", "--out", out, "--quiet"]), check=True)
         wall = time.perf_counter() - t0
         ref = json.load(open(os.path.join(out, "result.json")))
     b = gagp_amd.Batch()
     b.add(TASK, None, POPULATION, generations, 0)
-    b.run()
+    b.run(use_graph=GRAPH)
     mine = b.result(0)
     same = mine["best"] == ref["best"] and [h["best_fitness"] for h in mine["history"]] == \
         [h["best_fitness"] for h in ref["history"]]
     cases = b.evaluated_program_cases(0)
     search_s = ref["time_ms"]["search"] / 1e3
     return {"value": round(cases / search_s, 1), "unit": "program-case evals/s", "cores": 1, "kind": "reference",
+            "pinned": "taskset -c 0" if shutil.which("taskset") else None, "host": host_cores(),
             "sample": "reference CPU engine (gagp run --engine cpu, built from its sources), quintic P%d seed 0, "
                       "%d generations: %d program-cases in %.2f s of search time (%.2f s wall); best program and "
                       "history identical to the device run: %s" % (POPULATION, generations, cases, search_s, wall, same)}
+
+
+def host_cores():
+    """Host CPU description: cores this process may use and the model."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": len(os.sched_getaffinity(0)), "model": model}
+
+
+def pinned(cmd, core=0):
+    """`cmd` pinned to one host core with taskset (when present)."""
+    return (["taskset", "-c", str(core)] if shutil.which("taskset") else []) + cmd
+
+
+def valu_peak():
+    """Measured int32 VALU issue rate of this device: the best of a few
+    launch sizes of the library's probe kernel (8 VALU operations per
+    iteration per lane), in lane operations per second."""
+    best = 0.0
+    for blocks in (2048, 4096, 8192):
+        iters = 8192
+        ms = gagp_amd.valu_probe(blocks, iters)
+        best = max(best, 8.0 * iters * 256 * blocks / (ms * 1e-3))
+    return best
+
+
+def end_to_end(name, task, population, generations, seed, with_cpu):
+    """One whole search end to end: in-process on the device (prepare,
+    every round, result), the `gagp run --engine gpu` command line (process
+    start to exit), and the reference CPU engine on one pinned core with the
+    same arguments, whose result must be identical."""
+    b = gagp_amd.Batch()
+    b.add(task, None, population, generations, seed)
+    t0 = time.perf_counter()
+    b.run(use_graph=GRAPH)
+    device_s = time.perf_counter() - t0
+    mine = strip_timing(b.result(0))
+    cases = b.evaluated_program_cases(0)
+    out = {"workload": name, "population": population, "generations": generations, "seed": seed,
+           "device_search_s": round(device_s, 4), "program_case_evals_per_s": round(cases / device_s, 1)}
+    with tempfile.TemporaryDirectory() as tmp:
+        t0 = time.perf_counter()
+        subprocess.run([gagp_amd.CLI_PATH, "run", task, "--engine", "gpu", "--population", str(population), "--generations",
+                        str(generations), "--seed", str(seed), "--out", tmp, "--quiet"], check=True)
+        out["cli_wall_s"] = round(time.perf_counter() - t0, 3)
+        out["cli_identical"] = strip_timing(json.load(open(os.path.join(tmp, "result.json")))) == mine
+        if with_cpu and os.path.exists(REF_GAGP):
+            t0 = time.perf_counter()
+            subprocess.run(pinned([REF_GAGP, "run", task, "--engine", "cpu", "--population", str(population), "--generations",
+                                   str(generations), "--seed", str(seed), "--out", tmp, "--quiet"]), check=True)
+            ref = json.load(open(os.path.join(tmp, "result.json")))
+            out["reference_cpu"] = {"wall_s": round(time.perf_counter() - t0, 3),
+                                    "search_s": round(ref["time_ms"]["search"] / 1e3, 4), "cores": 1,
+                                    "identical": strip_timing(ref) == mine}
+    return out
 
 
 # The loop-heavy search in its two grammar forms: one traversal per program,
@@ -161,7 +230,7 @@ def loop_heavy(form, cpu_generations):
     t0 = time.perf_counter()
     b.prepare()
     t1 = time.perf_counter()
-    b.step(b.max_rounds())
+    b.step(b.max_rounds(), use_graph=GRAPH)
     b.finish()
     t2 = time.perf_counter()
     cases = b.evaluated_program_cases(0)
@@ -173,12 +242,12 @@ def loop_heavy(form, cpu_generations):
            "identical_to_reference": same, "cpu_reference": None}
     if cpu_generations and os.path.exists(REF_GAGP):
         with tempfile.TemporaryDirectory() as tmp:
-            subprocess.run([REF_GAGP, "run", task, "--engine", "cpu", "--population", "8192", "--generations",
-                            str(cpu_generations), "--seed", "0", "--out", tmp, "--quiet"], check=True)
+            subprocess.run(pinned([REF_GAGP, "run", task, "--engine", "cpu", "--population", "8192", "--generations",
+                                   str(cpu_generations), "--seed", "0", "--out", tmp, "--quiet"]), check=True)
             ref = json.load(open(os.path.join(tmp, "result.json")))
         c = gagp_amd.Batch()
         c.add(task, None, 8192, cpu_generations, 0)
-        c.run()
+        c.run(use_graph=GRAPH)
         ref_cases = c.evaluated_program_cases(0)
         search_s = ref["time_ms"]["search"] / 1e3
         out["cpu_reference"] = {
@@ -215,10 +284,20 @@ def batching(n, generations):
         for i, (task, population) in enumerate(tasks):
             b = gagp_amd.Batch()
             b.add(task, None, population, generations, 11 + i)
-            b.run()
+            b.run(use_graph=GRAPH)
             alone.append(strip(b.result(0)))
             lat_alone.append(time.perf_counter() - t0)
         alone_s = time.perf_counter() - t0
+        # The batch with the evaluation launch shapes fixed (autotuner off):
+        # the gain of the tuned shapes below is measured against it.
+        t0 = time.perf_counter()
+        f = gagp_amd.Batch()
+        f.set_autotune(0)
+        for i, (task, population) in enumerate(tasks):
+            f.add(task, None, population, generations, 11 + i)
+        f.run(use_graph=GRAPH)
+        fixed = [strip(f.result(i)) for i in range(n)]
+        fixed_s = time.perf_counter() - t0
         t0 = time.perf_counter()
         b = gagp_amd.Batch()
         for i, (task, population) in enumerate(tasks):
@@ -226,14 +305,27 @@ def batching(n, generations):
         t1 = time.perf_counter()
         b.prepare()
         t2 = time.perf_counter()
-        b.step(b.max_rounds())
+        b.step(b.max_rounds(), use_graph=GRAPH)
         b.finish()
         t3 = time.perf_counter()
         together = [strip(b.result(i)) for i in range(n)]
         batched_s = time.perf_counter() - t0
+        tuning = b.tuning()
         phases = {"load_s": round(t1 - t0, 4), "prepare_s": round(t2 - t1, 4), "rounds_s": round(t3 - t2, 4),
                   "results_s": round(batched_s - (t3 - t0), 4)}
         cases = sum(b.evaluated_program_cases(i) for i in range(n))
+        # The same batch once more with every launch bracketed by HIP events
+        # on the engine's stream (stream launches, no graph): device
+        # milliseconds per kernel over the whole batched run.
+        p = gagp_amd.Batch()
+        for i, (task, population) in enumerate(tasks):
+            p.add(task, None, population, generations, 11 + i)
+        p.set_profile(True)
+        p.prepare()
+        p.step(p.max_rounds(), use_graph=False)
+        p.finish()
+        kernel_ms = {k: round(v, 3) for k, v in sorted(p.kernel_times().items(), key=lambda kv: -kv[1])}
+        profiled_same = sum(strip(p.result(i)) == x for i, x in enumerate(alone))
     same = sum(x == y for x, y in zip(alone, together))
     return {"searches": n, "generations": generations,
             "one_at_a_time": {"searches_per_s": round(n / alone_s, 3), "seconds": round(alone_s, 3),
@@ -242,49 +334,115 @@ def batching(n, generations):
             "batched": {"searches_per_s": round(n / batched_s, 3), "seconds": round(batched_s, 3),
                         "p50_latency_s": round(batched_s, 3), "p99_latency_s": round(batched_s, 3),
                         "program_case_evals_per_s": round(cases / batched_s, 1), "phases": phases},
-            "speedup": round(alone_s / batched_s, 2), "identical_to_alone": "%d/%d" % (same, n)}
+            "speedup": round(alone_s / batched_s, 2), "identical_to_alone": "%d/%d" % (same, n),
+            "batched_kernel_ms": kernel_ms, "profiled_identical_to_alone": "%d/%d" % (profiled_same, n),
+            "autotune": {"tuning": tuning, "fixed_shapes_s": round(fixed_s, 3), "tuned_s": round(batched_s, 3),
+                         "gain": round(fixed_s / batched_s, 3),
+                         "fixed_identical_to_alone": "%d/%d" % (sum(x == y for x, y in zip(alone, fixed)), n)}}
 
 
-def serving(n, rate):
+def serve_stream(requests, max_live):
+    """Serves `requests` [(task, population, generations, seed, arrival_s)]
+    with early exit: max_live 0 is continuous batching, 1 one search at a
+    time. Returns the summary, the canonical result documents and the
+    outcomes."""
+    n = len(requests)
+    server = gagp_amd.Server(budget_bytes=64 << 30, poll_rounds=2, max_live=max_live)
+    tickets = [server.submit(task, None, population, generations, seed, True, arrival)
+               for task, population, generations, seed, arrival in requests]
+    t0 = time.perf_counter()
+    server.run()
+    makespan = time.perf_counter() - t0
+    outcomes = [server.outcome(t) for t in tickets]
+    results = [json.dumps(strip_timing(server.result(t)), sort_keys=True) for t in tickets]
+    lat = [o["done"] - o["arrival"] for o in outcomes]
+    return {"searches_per_s": round(n / makespan, 3), "makespan_s": round(makespan, 3),
+            "p50_latency_s": round(percentile(lat, 50), 4), "p99_latency_s": round(percentile(lat, 99), 4),
+            "generations_run": sum(o["generations"] for o in outcomes)}, results, outcomes
+
+
+def reference_sample(requests, results, outcomes, fraction=0.05, seed=5, budget=4e8):
+    """Bit-exactness of served results against the reference CPU engine on
+    a seeded sample of the requests: requests are visited in a seeded random
+    order and taken while the sample is below `fraction` of all requests,
+    skipping any whose reference run would evaluate more than `budget`
+    program-cases (population x completed generations x train cases, about a
+    minute of one core); each sampled request's published result (early exit
+    at generation g) must equal `gagp run --engine cpu --generations g` of
+    the reference, run one per core in parallel."""
+    import random
+    from concurrent.futures import ThreadPoolExecutor
+    if not os.path.exists(REF_GAGP):
+        return None
+    want = max(1, int(round(fraction * len(requests))))
+    order = list(range(len(requests)))
+    random.Random(seed).shuffle(order)
+    picked, skipped = [], 0
+    for i in order:
+        task, population, _, rseed, _ = requests[i]
+        cases = len(json.load(open(os.path.join(os.path.dirname(task), "train.cases.json")))["cases"])
+        cost = population * max(1, outcomes[i]["generations"]) * cases
+        if cost > budget:
+            skipped += 1
+            continue
+        picked.append((i, cost))
+        if len(picked) == want:
+            break
+
+    def run(item):
+        i, _ = item
+        task, population, _, rseed, _ = requests[i]
+        with tempfile.TemporaryDirectory() as out:
+            subprocess.run([REF_GAGP, "run", task, "--engine", "cpu", "--population", str(population), "--generations",
+                            str(outcomes[i]["generations"]), "--seed", str(rseed), "--out", out, "--quiet"], check=True)
+            ref = json.load(open(os.path.join(out, "result.json")))
+        return json.dumps(strip_timing(ref), sort_keys=True) == results[i]
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=min(len(picked), 8) or 1) as pool:
+        same = list(pool.map(run, picked))
+    return {"sampled": len(picked), "of": len(requests), "identical": "%d/%d" % (sum(same), len(same)),
+            "requests": [{"index": i, "population": requests[i][1], "generations": outcomes[i]["generations"],
+                          "program_cases": cost} for i, cost in picked],
+            "skipped_over_budget": skipped, "budget_program_cases": budget, "seconds": round(time.perf_counter() - t0, 1)}
+
+
+def serving(n, rate, stress_rate, check_reference):
     """The same request stream served with continuous batching and by a
-    server that admits one search at a time (a budget of one search):
-    completed searches/s over the makespan, p50/p99 latency from arrival to
-    published result, and whether every result is identical in both. The
-    stream is Poisson at `rate` requests/s (where both servers may keep up
-    with arrivals, so latency shows the difference) and, for capacity, the
-    same requests all arriving at t=0. Half the requests are list
-    reductions."""
+    server that admits one search at a time (max_live 1): completed
+    searches/s over the makespan, p50/p99 latency from arrival to published
+    result, and whether every result is identical in both. The stream is
+    Poisson at `rate` requests/s, again at `stress_rate` (above the
+    one-at-a-time capacity, so throughput shows the difference), and, for
+    capacity, the same requests all arriving at t=0. Half the requests are
+    list reductions. With `check_reference`, a seeded 5% sample of the
+    results is compared with the reference CPU engine."""
     from gagp_amd import workloads
 
-    def serve(requests, max_live):
-        server = gagp_amd.Server(budget_bytes=64 << 30, poll_rounds=2, max_live=max_live)
-        tickets = [server.submit(task, None, population, generations, seed, True, arrival)
-                   for task, population, generations, seed, arrival in requests]
-        t0 = time.perf_counter()
-        server.run()
-        makespan = time.perf_counter() - t0
-        outcomes = [server.outcome(t) for t in tickets]
-        results = [json.dumps(strip_timing(server.result(t)), sort_keys=True) for t in tickets]
-        lat = [o["done"] - o["arrival"] for o in outcomes]
-        return {"searches_per_s": round(n / makespan, 3), "makespan_s": round(makespan, 3),
-                "p50_latency_s": round(percentile(lat, 50), 4), "p99_latency_s": round(percentile(lat, 99), 4),
-                "generations_run": sum(o["generations"] for o in outcomes)}, results
+    def compare(requests):
+        batched, rb, ob = serve_stream(requests, 0)
+        serial, rs, _ = serve_stream(requests, 1)
+        return {"continuous_batching": batched, "one_at_a_time": serial,
+                "throughput_ratio": round(batched["searches_per_s"] / serial["searches_per_s"], 2),
+                "identical_results": "%d/%d" % (sum(a == b for a, b in zip(rb, rs)), n)}, rb, ob
 
     with tempfile.TemporaryDirectory() as tmp:
         requests = workloads.served_requests(tmp, n=n, seed=5, rate=rate, lists=True)
-        batched, rb = serve(requests, 0)
-        serial, rs = serve(requests, 1)
+        at_rate, results, outcomes = compare(requests)
+        stress = [(task, population, generations, seed, arrival * rate / stress_rate)
+                  for task, population, generations, seed, arrival in requests]
+        at_stress, _, _ = compare(stress)
         burst = [(task, population, generations, seed, 0.0) for task, population, generations, seed, _ in requests]
-        burst_batched, bb = serve(burst, 0)
-        burst_serial, bs = serve(burst, 1)
-    return {"requests": n, "rate_per_s": rate, "early_exit": True,
-            "populations": "log-uniform 256-16384", "train_cases": "log-uniform 16-2048",
-            "generation_caps": "uniform 32-256", "continuous_batching": batched, "one_at_a_time": serial,
-            "throughput_ratio": round(batched["searches_per_s"] / serial["searches_per_s"], 2),
-            "identical_results": "%d/%d" % (sum(a == b for a, b in zip(rb, rs)), n),
-            "all_at_once": {"continuous_batching": burst_batched, "one_at_a_time": burst_serial,
-                            "throughput_ratio": round(burst_batched["searches_per_s"] / burst_serial["searches_per_s"], 2),
-                            "identical_results": "%d/%d" % (sum(a == b for a, b in zip(bb, bs)), n)}}
+        all_at_once, _, _ = compare(burst)
+        sample = reference_sample(requests, results, outcomes) if check_reference else None
+    out = {"requests": n, "rate_per_s": rate, "early_exit": True,
+           "populations": "log-uniform 256-16384", "train_cases": "log-uniform 16-2048",
+           "generation_caps": "uniform 32-256"}
+    out.update(at_rate)
+    out["at_%g_per_s" % stress_rate] = at_stress
+    out["all_at_once"] = all_at_once
+    out["reference_sample"] = sample
+    return out
 
 
 def strip_timing(r):
@@ -318,7 +476,11 @@ def main():
     ap.add_argument("--batch-searches", type=int, default=64, help="searches of the batching comparison (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=60, help="requests of the serving comparison (0: skip)")
     ap.add_argument("--no-loop-heavy", dest="loop_heavy", action="store_false", help="skip the loop-heavy search")
+    ap.add_argument("--no-graph", action="store_true", help="stream launches instead of graph replays everywhere")
+    ap.add_argument("--no-whole", dest="whole", action="store_false", help="skip the whole-search timings")
     args = ap.parse_args()
+    global GRAPH
+    GRAPH = not args.no_graph
     if args.gpus != 1:
         print("bench.py measures one GPU: each search runs on a single device and the engine has no multi-GPU "
               "path; --gpus %d is not supported" % args.gpus, file=sys.stderr)
@@ -330,19 +492,31 @@ def main():
     b = gagp_amd.Batch()
     b.add(TASK, None, POPULATION, GENERATIONS, 0)
     b.prepare()
-    b.step(args.warmup)
+    b.step(args.warmup, use_graph=GRAPH)
     before = b.counters(0)
     t0 = time.perf_counter()
-    b.step(args.steps)  # graph replays, synchronized before returning
+    b.step(args.steps, use_graph=GRAPH)  # graph replays, synchronized before returning
     seconds = time.perf_counter() - t0
     after = b.counters(0)
     value = (after["program_cases"] - before["program_cases"]) / seconds
     psteps = min(args.steps, 16)
     ms, counts = profile(args.warmup, psteps)
     roof, kernels, vm = roofline(ms, counts, psteps)
+    peak_lane_ops = valu_peak()
+    # Window-independent figures: whole searches end to end.
+    whole = None if not args.whole else {
+        "quintic": end_to_end("quintic symbolic regression (the headline search), all 128 generations", TASK, POPULATION,
+                              GENERATIONS, 0, False),
+        "c1_polynomial_example": end_to_end(
+            "the reference's examples/polynomial (y = x^3 - 2x + 1, constants 1..3; the reference publishes 0.4 s per "
+            "run on an RTX 3090, README.md:53)", os.path.join(ROOT, "tests", "golden", "tasks", "polynomial", "task.json"),
+            1024, 64, 0, not args.no_cpu),
+        "c1": end_to_end("y = x^3 - 2x + 1 with constants {1, 2}, 21 train / 40 test cases",
+                         os.path.join(ROOT, "tests", "golden", "tasks", "poly_c1", "task.json"), 1024, 64, 0, not args.no_cpu),
+    }
     loop = {form: loop_heavy(form, 0 if args.no_cpu else 2) for form in LOOP_FORMS} if args.loop_heavy else None
     batch = batching(args.batch_searches, 64) if args.batch_searches else None
-    serve = serving(args.serve_requests, 20.0) if args.serve_requests else None
+    serve = serving(args.serve_requests, 20.0, 60.0, not args.no_cpu) if args.serve_requests else None
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_generations)
     line = {
         "metric": "program-case evals/sec", "value": round(value, 1), "unit": "program-case evals/s", "n_gpus": 1,
@@ -352,7 +526,8 @@ def main():
         "config": {"workload": "quintic symbolic regression, population 4096, 128 generations, 256 train cases, "
                                "default resident steady-state search; one step = one generation",
                    "population": POPULATION, "generations": GENERATIONS, "train_cases": TRAIN_CASES},
-        "roofline": roof, "cpu_baseline": cpu, "kernels": kernels, "vm": vm, "loop_heavy": loop, "batching": batch,
+        "roofline": roof, "valu_peak_lane_ops_per_s": round(peak_lane_ops, 1), "cpu_baseline": cpu, "kernels": kernels, "vm": vm, "whole_searches": whole, "loop_heavy": loop,
+        "batching": batch,
         "serving": serve,
     }
     if cpu:

@@ -100,6 +100,9 @@ def lib():
     L.gagp_batch_evaluated_program_cases.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_uint64)]
     L.gagp_batch_set_profile.argtypes = [ctypes.c_void_p, ctypes.c_int32]
     L.gagp_batch_kernel_times_json.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
+    L.gagp_batch_set_autotune.argtypes = [ctypes.c_void_p, ctypes.c_int32]
+    L.gagp_valu_probe.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_float)]
+    L.gagp_batch_tuning_json.argtypes = L.gagp_batch_kernel_times_json.argtypes
     L.gagp_batch_destroy.argtypes = [ctypes.c_void_p]
     _lib = L
     return L
@@ -200,6 +203,14 @@ class Session:
             self.h = None
 
 
+def valu_probe(blocks, iterations):
+    """Device milliseconds of the int32 VALU probe (8 * iterations * 256 *
+    blocks lane operations)."""
+    ms = ctypes.c_float()
+    _check(lib().gagp_valu_probe(blocks, iterations, ctypes.byref(ms)))
+    return ms.value
+
+
 class Batch:
     """Searches stepped together on the device (`gagp run --engine gpu`)."""
 
@@ -256,6 +267,13 @@ class Batch:
 
     def kernel_times(self):
         return json.loads(_text(lib().gagp_batch_kernel_times_json, self.h))
+
+    def set_autotune(self, mode=1):
+        """0 off, 1 when a round evaluates at least 2^22 program-cases (the default), 2 always."""
+        _check(lib().gagp_batch_set_autotune(self.h, int(mode)))
+
+    def tuning(self):
+        return json.loads(_text(lib().gagp_batch_tuning_json, self.h))
 
     def __del__(self):
         if getattr(self, "h", None):

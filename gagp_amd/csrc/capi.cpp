@@ -237,6 +237,30 @@ int gagp_batch_set_profile(gagp_batch* b, int32_t on) {
   });
 }
 
+int gagp_valu_probe(int32_t blocks, int32_t iterations, float* out_ms) {
+  return guarded([&] {
+    if (!out_ms) throw std::invalid_argument("null output");
+    *out_ms = valu_probe_ms(blocks, iterations);
+  });
+}
+
+int gagp_batch_set_autotune(gagp_batch* b, int32_t mode) {
+  return guarded([&] {
+    if (!b) throw std::invalid_argument("null batch");
+    b->b.set_autotune(mode);
+  });
+}
+
+int gagp_batch_tuning_json(gagp_batch* b, char* buf, size_t cap, size_t* needed) {
+  std::string text;
+  const int st = guarded([&] {
+    if (!b) throw std::invalid_argument("null batch");
+    text = b->b.tuning_report();
+  });
+  if (st != GAGP_OK) return st;
+  return copy_out(text, buf, cap, needed);
+}
+
 int gagp_batch_kernel_times_json(gagp_batch* b, char* buf, size_t cap, size_t* needed) {
   std::string text;
   const int st = guarded([&] {

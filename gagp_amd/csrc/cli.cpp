@@ -102,6 +102,7 @@ int run_command(const std::vector<std::string>& args) {
     if (save_all) {
       json::Value pop = json::make_object();
       pop.obj["format"] = json::make_string("gagp-population-v1");
+      pop.obj["grammar_hash"] = json::make_string(r.grammar_hash);
       json::Value programs = json::make_array();
       programs.arr = r.population;
       pop.obj["programs"] = programs;

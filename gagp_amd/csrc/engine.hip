@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <fstream>
 #include <numeric>
 #include <sstream>
@@ -579,35 +580,40 @@ __device__ inline double canonical_from_scores(const EvalParams& e, int C, const
   return group_total;
 }
 
-// Items of the looping kernel: (program, 64-case tile) pairs, one wavefront
-// each; blocks for n programs (at most 8192, which loop over the rest).
-__host__ __device__ inline int loop_blocks(int C, int n) {
-  const long long waves = (long long)n * ((C + 63) / 64);
+// Items of the looping kernel: (program, group of `group` consecutive
+// 64-case tiles) pairs, one wavefront each; blocks for n programs (at most
+// 8192, which loop over the rest).
+__host__ __device__ inline int loop_blocks(int C, int n, int group = 1) {
+  const int tiles = (C + 63) / 64, groups = (tiles + group - 1) / group;
+  const long long waves = (long long)n * groups;
   return int(std::min<long long>(8192, (waves + kWavesPerBlock - 1) / kWavesPerBlock));
 }
 
-// The looping path as a tile queue: every wavefront takes (program, tile)
-// items of its own, program-major, so no wavefront waits for another. Scores
-// go to the program's row of loop_scores (HBM). A program with one tile is
-// finished by its wavefront; otherwise each tile adds its counts and its
-// exact integer score sum to the program's LoopAcc with device atomics, and
-// the wavefront that completes the last tile forms the fitness: the sum, or
-// the canonical replay from the scores when some tile had a non-integral
-// score. It then clears the record for the program's next evaluation. LDS:
-// per wavefront kLoopSlots slots of 64 lanes, independent of the case count.
-__global__ __launch_bounds__(kEvalThreads) void k_eval_loop(DSearch* all, const BlockMap* map, int mode) {
+// The looping path as a tile queue: every wavefront takes (program, tile
+// group) items of its own, program-major, so no wavefront waits for another.
+// A group is `group` consecutive 64-case tiles of one program, run one after
+// another by the wavefront (the program's decoded code stays in registers).
+// Scores go to the program's row of loop_scores (HBM). A program with one
+// group is finished by its wavefront; otherwise each group adds its counts
+// and its exact integer score sum to the program's LoopAcc with device
+// atomics, and the wavefront that completes the last group forms the
+// fitness: the sum, or the canonical replay from the scores when some tile
+// had a non-integral score. It then clears the record for the program's next
+// evaluation. LDS: per wavefront kLoopSlots slots of 64 lanes, independent of
+// the case count.
+__global__ __launch_bounds__(kEvalThreads) void k_eval_loop(DSearch* all, const BlockMap* map, int mode, int group) {
   const BlockMap bm = map[blockIdx.x];
   const DSearch& d = all[bm.search];
   if (mode == 1 ? !search_live(d) : (!d.active || *d.error)) return;
   extern __shared__ unsigned char smem[];
   const int wave = wave_index(), lane = threadIdx.x % 64;
-  const int C = d.cd.C, tiles = (C + 63) / 64;
+  const int C = d.cd.C, tiles = (C + 63) / 64, groups = (tiles + group - 1) / group;
   LAS std::int64_t* const slots = (LAS std::int64_t*)smem + std::size_t(wave) * kLoopSlots * 64;
   const EvalParams e{d.cd, d.fuel, d.lanes, d.penalty, d.policy};
   const int count = mode == 1 ? d.n_pending[2] : d.P;
-  const long long items = (long long)count * tiles;
+  const long long items = (long long)count * groups;
   for (long long k = (long long)bm.block * kWavesPerBlock + wave; k < items; k += (long long)bm.blocks * kWavesPerBlock) {
-    const int i = int(k / tiles), tile = int(k % tiles) * 64;
+    const int i = int(k / groups), first = int(k % groups) * group, last = min(tiles, first + group);
     const int slot = mode == 1 ? d.pending[pending_at(d.B, 2, i)] : i;
     const int pmode = d.code_jumps[slot];
     if (mode == 0 && eval_class(pmode, C) != 2) continue;  // another kernel scores it (uniform per wavefront)
@@ -615,32 +621,42 @@ __global__ __launch_bounds__(kEvalThreads) void k_eval_loop(DSearch* all, const 
                       d.n_locals, pmode & 1, (pmode >> 1) & 1, d.pool_len[slot], (pmode & 4) ? TAG_BOOL : TAG_INT};
     const LoopProgram lp = loop_load(pv, lane);
     const int n_in = staged_inputs_of(pv.n_locals, e.cd.n_inputs);
-    // Cases in the search's visiting order (similar loop lengths per tile);
-    // scores are kept by case index, so the totals are unchanged.
-    const bool active = tile + lane < C;
-    const int c = active ? e.cd.order[tile + lane] : 0;
-    const LaneResult r = vm_loop(lp, fast_inputs(e.cd, pv.n_locals, c, active), n_in, e.cd.arena, slots, lane, active, e.fuel,
-                                 pv.result_tag);
     GAS double* const scores = d.loop_scores + std::size_t(slot) * C;
-    double sc = 0;
-    bool exact = true;
-    if (active) {
-      sc = lane_score(e, r, expected_of(e.cd, c), exact);
-      scores[c] = sc;
+    long long total = 0;  // exact while no score is fractional
+    bool frac = false;
+    unsigned tw = 0, te = 0, tt = 0;
+    unsigned long long tf = 0;
+    for (int tile = first; tile < last; ++tile) {
+      // Cases in the search's visiting order (similar loop lengths per
+      // tile); scores are kept by case index, so the totals are unchanged.
+      const int at = tile * 64 + lane;
+      const bool active = at < C;
+      const int c = active ? e.cd.order[at] : 0;
+      const LaneResult r = vm_loop(lp, fast_inputs(e.cd, pv.n_locals, c, active), n_in, e.cd.arena, slots, lane, active, e.fuel,
+                                   pv.result_tag);
+      double sc = 0;
+      bool exact = true;
+      if (active) {
+        sc = lane_score(e, r, expected_of(e.cd, c), exact);
+        scores[c] = sc;
+      }
+      // Integral scores below 2^36 in magnitude: every partial sum is exact.
+      const bool fractional =
+          __ballot(active && !(sc == double(std::int64_t(sc)) && sc >= -68719476736.0 && sc <= 68719476736.0)) != 0;
+      const double ws = wave_sum_d(sc);
+      frac = frac || fractional;
+      if (!frac) total += (long long)ws;
+      tw += wave_sum_u(active && !exact);
+      te += wave_sum_u(active && r.err != ERR_NONE);
+      tt += wave_sum_u(active && r.err == ERR_TIMEOUT);
+      unsigned long long f = active ? unsigned(r.fuel_used) : 0u;
+      for (int off = 32; off > 0; off >>= 1) f += __shfl_xor(f, off, 64);
+      tf += f;
     }
-    // Integral scores below 2^36 in magnitude: every partial sum is exact.
-    const bool fractional =
-        __ballot(active && !(sc == double(std::int64_t(sc)) && sc >= -68719476736.0 && sc <= 68719476736.0)) != 0;
-    const double ws = wave_sum_d(sc);
-    unsigned tw = wave_sum_u(active && !exact), te = wave_sum_u(active && r.err != ERR_NONE);
-    unsigned tt = wave_sum_u(active && r.err == ERR_TIMEOUT);
-    unsigned long long tf = active ? unsigned(r.fuel_used) : 0u;
-    for (int off = 32; off > 0; off >>= 1) tf += __shfl_xor(tf, off, 64);
-    long long total = fractional ? 0 : (long long)ws;
-    bool frac = fractional;
-    if (tiles > 1) {
+    if (frac) total = 0;
+    if (groups > 1) {
       GAS LoopAcc* const acc = d.loop_acc + slot;
-      __threadfence();  // this tile's scores before its arrival
+      __threadfence();  // this group's scores before its arrival
       unsigned prev = 0;
       if (lane == 0) {
         atomicAdd(&acc->sum, (unsigned long long)total);
@@ -648,12 +664,12 @@ __global__ __launch_bounds__(kEvalThreads) void k_eval_loop(DSearch* all, const 
         atomicAdd(&acc->wrong, tw);
         atomicAdd(&acc->errors, te);
         atomicAdd(&acc->timeouts, tt);
-        if (fractional) atomicOr(&acc->fractional, 1u);
+        if (frac) atomicOr(&acc->fractional, 1u);
         __threadfence();
         prev = atomicAdd(&acc->tiles, 1u);
       }
-      if (__shfl(prev, 0, 64) != unsigned(tiles - 1)) continue;  // another tile finishes the program
-      __threadfence();  // every tile's scores and counts after the last arrival
+      if (__shfl(prev, 0, 64) != unsigned(groups - 1)) continue;  // another group finishes the program
+      __threadfence();  // every group's scores and counts after the last arrival
       unsigned long long v[2] = {0, 0};
       unsigned u[4] = {0, 0, 0, 0};
       if (lane == 0) {
@@ -1654,6 +1670,19 @@ struct KernelTimer {
   }
 };
 
+// ---- evaluation launch autotuner ----
+// Launch shapes of the evaluation kernels that leave every result unchanged:
+// pending programs per wavefront of the typed straight-line kernel (its
+// grid, and so how many blocks stage the case set), and tiles per work item
+// of the looping kernel (cases per item = 64 * loop_group).
+struct TuneChoice {
+  int fast_programs = 1;
+  int loop_group = 1;
+};
+// Choices measured so far in this process, by batch shape.
+std::mutex tune_mutex;
+std::map<std::string, TuneChoice> tune_cache;
+
 struct Batch::Impl {
   std::vector<std::unique_ptr<SearchState>> searches;  // every search ever added (stable indices)
   std::vector<int> live;                                // searches of the round table, in table order
@@ -1685,7 +1714,13 @@ struct Batch::Impl {
   struct Grid { GAS BlockMap* map = nullptr; int blocks = 0; };
   DeviceArena map_arena;
   Grid g_init, g_lower, g_eval_init, g_vary_staged, g_vary_global, g_vary_staged_w, g_vary_global_w, g_table, g_member, g_child,
-      g_eval, g_eval_loop;
+      g_eval, g_eval_general, g_eval_loop;
+  // Evaluation launch shapes of the round (the autotuner's choice, or the
+  // fixed default: one program per wavefront, one 64-case tile per item).
+  TuneChoice tune;
+  int autotune = 1;  // 0 off, 1 above kTuneWork, 2 always
+  bool tuned = false;
+  std::string tune_report;  // JSON of the last tuning pass
   bool any_walks(const std::vector<int>& ids) const {
     for (int id : ids)
       if (searches[std::size_t(id)]->d.walks) return true;
@@ -1726,9 +1761,18 @@ struct Batch::Impl {
     g_table = build_grid(map_arena, live, [&](const SearchState& s) { return up(s.d.table_mask + 1, 256); });
     g_member = build_grid(map_arena, live, [&](const SearchState& s) { return up(s.P, 128); });
     g_child = build_grid(map_arena, live, [&](const SearchState& s) { return up(s.B, 128); });
-    g_eval = build_grid(map_arena, live, [&](const SearchState& s) { return std::min(4096, up(s.B, kWavesPerBlock)); });
-    // One program per block for the looping path.
-    g_eval_loop = build_grid(map_arena, live, [&](const SearchState& s) { return s.may_jump ? loop_blocks(s.d.cd.C, s.B) : 0; });
+    g_eval = eval_grid(map_arena, tune.fast_programs);
+    g_eval_general = eval_grid(map_arena, 1);
+    g_eval_loop = loop_grid(map_arena, tune.loop_group);
+  }
+  // Evaluation grids of the live searches: `per_wave` pending programs per
+  // wavefront of the typed and general kernels; `group` tiles per work item
+  // of the looping kernel.
+  Grid eval_grid(DeviceArena& arena, int per_wave) {
+    return build_grid(arena, live, [&](const SearchState& s) { return std::min(4096, up(s.B, kWavesPerBlock * per_wave)); });
+  }
+  Grid loop_grid(DeviceArena& arena, int group) {
+    return build_grid(arena, live, [&](const SearchState& s) { return s.may_jump ? loop_blocks(s.d.cd.C, s.B, group) : 0; });
   }
 
   // Allocates one search and fixes its kernel variants and LDS needs.
@@ -1807,7 +1851,7 @@ struct Batch::Impl {
     if (any_jumps(ids))
       timer.time("k_eval_init_loop", stream, [&] {
         const Grid gp = build_grid(tmp, ids, [&](const SearchState& x) { return loop_blocks(x.d.cd.C, x.P); });
-        hipLaunchKernelGGL(k_eval_loop, dim3(gp.blocks), dim3(kEvalThreads), eval_loop_lds, stream, t, gp.map, 0);
+        hipLaunchKernelGGL(k_eval_loop, dim3(gp.blocks), dim3(kEvalThreads), eval_loop_lds, stream, t, gp.map, 0, 1);
       });
     timer.time("k_eval_init_general", stream, [&] {
       hipLaunchKernelGGL(k_eval<0>, dim3(ge.blocks), dim3(kEvalThreads), eval_general_lds, stream, t, ge.map, 0);
@@ -1825,9 +1869,140 @@ struct Batch::Impl {
     }
   }
   // Round table, grids, kernel sizes and graph for the current live set.
+  // Batch shape for the tuning cache: per kernel, log2 buckets of the
+  // searches' programs per round and cases, summed over the live searches.
+  std::string shape_key() const {
+    const auto bucket = [](double x) { return x <= 1 ? 0 : int(std::floor(std::log2(x))); };
+    double fast_programs = 0, fast_cases = 0, loop_items = 0, loop_cases = 0;
+    int loops = 0;
+    for (int id : live) {
+      const SearchState& x = *searches[std::size_t(id)];
+      fast_programs += x.B;
+      fast_cases += double(x.B) * x.d.cd.C;
+      if (x.may_jump) {
+        loop_items += double(x.B) * ((x.d.cd.C + 63) / 64);
+        loop_cases += double(x.B) * x.d.cd.C;
+        ++loops;
+      }
+    }
+    std::ostringstream k;
+    k << "typed:" << bucket(fast_programs) << "/" << bucket(fast_cases / std::max(1.0, fast_programs)) << " loop:" << loops << "/"
+      << bucket(loop_items) << "/" << bucket(loop_cases / std::max(1.0, loop_items * 64) * 64);
+    return k.str();
+  }
+  // Evaluation work of one round (program-cases over the live searches):
+  // below kTuneWork the tuning pass would cost more than it can gain, and
+  // the fixed shapes run.
+  static constexpr double kTuneWork = double(1 << 22);
+  double round_work() const {
+    double w = 0;
+    for (int id : live) w += double(searches[std::size_t(id)]->B) * searches[std::size_t(id)]->d.cd.C;
+    return w;
+  }
+  // Measures the evaluation kernels over the current round's pending
+  // programs under each candidate launch shape and keeps the fastest.
+  // Re-running an evaluation kernel over the same pending list rewrites the
+  // same results (each program's fitness, wrong count and tallies; the
+  // looping kernel's per-program totals are cleared by the program's last
+  // item), so tuning changes no search. Call between enqueue_front and
+  // enqueue_eval of a round, with the stream idle.
+  void tune_launches() {
+    const std::string key = shape_key();
+    if (autotune == 1 && round_work() < kTuneWork) {
+      apply_tuning(TuneChoice{});
+      std::ostringstream r;
+      r << "{\"key\":\"" << key << "\",\"cached\":false,\"skipped\":\"round work below " << kTuneWork
+        << " program-cases\",\"fast_programs\":1,\"loop_group\":1}";
+      tune_report = r.str();
+      return;
+    }
+    {
+      std::lock_guard<std::mutex> lock(tune_mutex);
+      auto it = tune_cache.find(key);
+      if (it != tune_cache.end()) {
+        apply_tuning(it->second);
+        std::ostringstream r;
+        r << "{\"key\":\"" << key << "\",\"cached\":true,\"fast_programs\":" << tune.fast_programs
+          << ",\"loop_group\":" << tune.loop_group << "}";
+        tune_report = r.str();
+        return;
+      }
+    }
+    hipEvent_t a, b;
+    HIP_OK(hipEventCreate(&a));
+    HIP_OK(hipEventCreate(&b));
+    const auto measure = [&](auto launch) {
+      float best = 1e30f;
+      for (int rep = 0; rep < 2; ++rep) {
+        HIP_OK(hipEventRecord(a, stream));
+        launch();
+        HIP_OK(hipEventRecord(b, stream));
+        HIP_OK(hipEventSynchronize(b));
+        float ms = 0;
+        HIP_OK(hipEventElapsedTime(&ms, a, b));
+        best = std::min(best, ms);
+      }
+      return best;
+    };
+    std::ostringstream r;
+    r << "{\"key\":\"" << key << "\",\"cached\":false";
+    TuneChoice choice;
+    DeviceArena tmp;
+    // A candidate replaces the default only when at least 3% faster.
+    const auto pick = [&](const char* name, const std::vector<int>& values, auto time_of) {
+      r << ",\"" << name << "_ms\":{";
+      int chosen = values[0];
+      float chosen_ms = 0;
+      for (std::size_t i = 0; i < values.size(); ++i) {
+        const float ms = time_of(values[i]);
+        r << (i ? "," : "") << "\"" << values[i] << "\":" << ms;
+        if (i == 0) chosen_ms = ms;
+        else if (ms < 0.97f * chosen_ms) { chosen = values[i]; chosen_ms = ms; }
+      }
+      r << "}";
+      return chosen;
+    };
+    choice.fast_programs = pick("typed", {1, 2, 4}, [&](int k) {
+      tmp.release();
+      const Grid g = eval_grid(tmp, k);
+      return g.blocks ? measure([&] {
+        hipLaunchKernelGGL(k_eval<1>, dim3(g.blocks), dim3(kEvalThreads), eval_fast_lds, stream, d_all, g.map, 1);
+      }) : 0.f;
+    });
+    if (loop_kernel) {
+      choice.loop_group = pick("loop", {1, 2, 4}, [&](int group) {
+        tmp.release();
+        const Grid g = loop_grid(tmp, group);
+        return g.blocks ? measure([&] {
+          hipLaunchKernelGGL(k_eval_loop, dim3(g.blocks), dim3(kEvalThreads), eval_loop_lds, stream, d_all, g.map, 1, group);
+        }) : 0.f;
+      });
+    }
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(stream));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    r << ",\"fast_programs\":" << choice.fast_programs << ",\"loop_group\":" << choice.loop_group << "}";
+    tune_report = r.str();
+    {
+      std::lock_guard<std::mutex> lock(tune_mutex);
+      tune_cache[key] = choice;
+    }
+    apply_tuning(choice);
+  }
+  // New launch shapes: grids rebuilt, the round graph captured again.
+  void apply_tuning(const TuneChoice& c) {
+    tune = c;
+    tuned = true;
+    if (graph_exec) { (void)hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
+    graph_ready = false;
+    build_round_grids();
+  }
+
   void rebuild() {
     if (graph_exec) { (void)hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
     graph_ready = false;
+    tuned = false;
     if (d_all) { (void)hipFree(d_all); d_all = nullptr; }
     max_rounds = 0;
     if (!live.empty()) {
@@ -1850,11 +2025,20 @@ struct Batch::Impl {
     dirty = false;
   }
 
+  template <class Kernel, class... Args>
+  void launch(const char* name, const Grid& g, Kernel kernel, int threads, std::size_t lds, Args... args) {
+    if (g.blocks)
+      timer.time(name, stream, [&] { hipLaunchKernelGGL(kernel, dim3(g.blocks), dim3(threads), lds, stream, d_all, g.map, args...); });
+  }
   void enqueue_round() {
+    enqueue_front();
+    enqueue_eval();
+    enqueue_back();
+  }
+  // A round in three parts: reproduction up to the pending lists, the
+  // evaluation kernels, then tallies, placement and commit.
+  void enqueue_front() {
     const int n = int(live.size());
-    const auto launch = [&](const char* name, const Grid& g, auto kernel, int threads, std::size_t lds, auto... args) {
-      if (g.blocks) timer.time(name, stream, [&] { hipLaunchKernelGGL(kernel, dim3(g.blocks), dim3(threads), lds, stream, d_all, g.map, args...); });
-    };
     if (any_generational) timer.time("k_gen_begin", stream, [&] { hipLaunchKernelGGL(k_gen_begin, dim3(n), dim3(256), 0, stream, d_all); });
     timer.time("k_record", stream, [&] { hipLaunchKernelGGL(k_record, dim3(n), dim3(256), 0, stream, d_all); });
     launch("k_vary", g_vary_staged, k_vary<true, false>, 64 * kVaryWaves, vary_lds_bytes);
@@ -1864,9 +2048,14 @@ struct Batch::Impl {
     launch("k_clear", g_table, k_clear, 256, 0);
     launch("k_index", g_member, k_index, 128, 0);
     launch("k_reuse", g_child, k_reuse, 128, 0);
+  }
+  void enqueue_eval() {
     launch("k_eval", g_eval, k_eval<1>, kEvalThreads, eval_fast_lds, 1);
-    if (loop_kernel) launch("k_eval_loop", g_eval_loop, k_eval_loop, kEvalThreads, eval_loop_lds, 1);
-    launch("k_eval_general", g_eval, k_eval<0>, kEvalThreads, eval_general_lds, 1);
+    if (loop_kernel) launch("k_eval_loop", g_eval_loop, k_eval_loop, kEvalThreads, eval_loop_lds, 1, tune.loop_group);
+    launch("k_eval_general", g_eval_general, k_eval<0>, kEvalThreads, eval_general_lds, 1);
+  }
+  void enqueue_back() {
+    const int n = int(live.size());
     timer.time("k_tally", stream, [&] { hipLaunchKernelGGL(k_tally, dim3(kTallyBlocks, n), dim3(256), 0, stream, d_all, 1); });
     if (place_lds_bytes)
       timer.time("k_place", stream, [&] { hipLaunchKernelGGL(k_place<true>, dim3(n), dim3(kPlaceThreads), place_lds_bytes, stream, d_all); });
@@ -1882,6 +2071,13 @@ Batch::Batch() : impl_(std::make_unique<Impl>()) {}
 Batch::~Batch() = default;
 std::size_t Batch::size() const { return impl_->searches.size(); }
 void Batch::set_profile(bool on) { impl_->timer.on = on; }
+void Batch::set_autotune(int mode) {
+  if (mode < 0 || mode > 2) throw std::invalid_argument("autotune mode must be 0, 1 or 2");
+  impl_->autotune = mode;
+  if (!mode && impl_->tuned) impl_->apply_tuning(TuneChoice{});
+  impl_->tuned = false;
+}
+std::string Batch::tuning_report() const { return impl_->tune_report.empty() ? "{}" : impl_->tune_report; }
 int Batch::max_rounds() const { return impl_->max_rounds; }
 std::size_t Batch::device_bytes() const {
   std::size_t b = 0;
@@ -1983,6 +2179,21 @@ void Batch::step(int rounds, bool use_graph) {
   if (I.dirty) I.rebuild();
   if (I.live.empty() || rounds <= 0) return;
   const auto t0 = std::chrono::steady_clock::now();
+  if (I.autotune && !I.tuned) {
+    // The first round of a new table: its pending programs are measured
+    // under every candidate shape between reproduction and placement.
+    I.enqueue_front();
+    HIP_OK(hipStreamSynchronize(I.stream));
+    I.tune_launches();
+    I.enqueue_eval();
+    I.enqueue_back();
+    HIP_OK(hipStreamSynchronize(I.stream));
+    I.timer.collect();
+    if (--rounds == 0) {
+      I.run_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+      return;
+    }
+  }
   if (use_graph && !I.timer.on) {
     if (!I.graph_ready) {
       hipGraph_t graph;
@@ -2167,6 +2378,7 @@ SearchResult Batch::result(int index, bool save_all) {
       out.lineage.push_back(l);
     }
   }
+  out.grammar_hash = s.grammar.content_hash;
   // Best member (the strict order's front after the last round).
   const std::vector<int> bo = s.down(d.cur_origin + std::size_t(best) * s.R, std::size_t(cur_n[best]));
   const std::vector<Val> bc = s.down(d.cur_cval + std::size_t(best) * s.R, std::size_t(cur_nc[best]));
@@ -2207,6 +2419,39 @@ SearchResult Batch::result(int index, bool save_all) {
   out.result = r;
   out.search_ms = I.init_ms + I.run_ms;
   return out;
+}
+
+// ====================== VALU issue-rate probe ======================
+// Eight int32 VALU operations per iteration in four independent
+// add/xor chains, which no closed form removes; the final test keeps every
+// chain live without a store per iteration.
+__global__ __launch_bounds__(256) void k_valu_probe(unsigned* out, int iters, unsigned seed) {
+  unsigned a = threadIdx.x ^ seed, b = blockIdx.x, c = a + 1u, d = b + 3u, e = a ^ 5u, f = b ^ 7u, g = a + b, h = a - b;
+  for (int i = 0; i < iters; ++i) {
+    a += b; c += d; e += f; g += h;
+    b ^= a; d ^= c; f ^= e; h ^= g;
+  }
+  if ((a ^ c ^ e ^ g) == 0x9e3779b9u) out[blockIdx.x] = b ^ d ^ f ^ h;
+}
+
+float valu_probe_ms(int blocks, int iters) {
+  if (blocks <= 0 || iters <= 0) throw std::invalid_argument("probe needs positive blocks and iterations");
+  DeviceArena tmp;
+  unsigned* out = (unsigned*)tmp.alloc<unsigned>(std::size_t(blocks));
+  hipEvent_t a, b;
+  HIP_OK(hipEventCreate(&a));
+  HIP_OK(hipEventCreate(&b));
+  hipLaunchKernelGGL(k_valu_probe, dim3(blocks), dim3(256), 0, nullptr, out, iters, 1u);  // warm
+  HIP_OK(hipEventRecord(a, nullptr));
+  hipLaunchKernelGGL(k_valu_probe, dim3(blocks), dim3(256), 0, nullptr, out, iters, 2u);
+  HIP_OK(hipEventRecord(b, nullptr));
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipEventSynchronize(b));
+  float ms = 0;
+  HIP_OK(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return ms;
 }
 
 // ====================== evaluation session ======================

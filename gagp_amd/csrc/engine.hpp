@@ -37,6 +37,7 @@ struct SearchResult {
   json::Value result;                  // result.json document
   std::vector<json::Value> population; // ranked final population (ast JSON)
   std::vector<json::Value> lineage;    // per-generation best (history.json)
+  std::string grammar_hash;            // content hash of the search's grammar
   std::uint64_t evaluated_program_cases = 0;
   std::uint64_t evaluated_programs = 0;
   double search_ms = 0;
@@ -69,6 +70,14 @@ class Batch {
   // Per-kernel device time (ms) accumulated with HIP events, by kernel name.
   std::vector<std::pair<std::string, double>> kernel_times() const;
   void set_profile(bool on);
+  // Evaluation launch autotuning: in the first round of a new round table,
+  // the evaluation kernels are timed over that round's pending programs
+  // under each candidate launch shape and the fastest shapes are kept for
+  // the following rounds (cached per batch shape for the process). Results
+  // are identical under every shape. Modes: 0 off (fixed shapes), 1 (the
+  // default) when a round evaluates at least 2^22 program-cases, 2 always.
+  void set_autotune(int mode);
+  std::string tuning_report() const;  // JSON of the last tuning pass ("{}" if none)
   // Generation budget of the batch (max over searches).
   int max_rounds() const;
 
@@ -93,6 +102,10 @@ class Batch {
   struct Impl;
   std::unique_ptr<Impl> impl_;
 };
+
+// Device milliseconds of the int32 VALU probe: `blocks` blocks of 256
+// threads, each running `iters` iterations of 8 VALU operations.
+float valu_probe_ms(int blocks, int iters);
 
 // Evaluation session: the FitnessSessionGpu boundary (fixed case set,
 // arbitrary verified bytecode programs).

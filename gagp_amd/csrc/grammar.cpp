@@ -1,6 +1,7 @@
 // Grammar front end (host). See grammar.hpp for the reference behaviour
 // each step follows.
 #include "grammar.hpp"
+#include "identity.hpp"
 
 #include <algorithm>
 #include <fstream>
@@ -526,7 +527,12 @@ const char* type_name(std::uint8_t t) {
   return t < 8 ? names[t] : "Invalid";
 }
 
-Grammar load_grammar_text(const std::string& text) { return Compiler{}.compile(json::parse(text)); }
+Grammar load_grammar_text(const std::string& text) {
+  const json::Value doc = json::parse(text);
+  Grammar g = Compiler{}.compile(doc);
+  g.content_hash = grammar_content_hash(doc);
+  return g;
+}
 Grammar load_grammar_file(const std::string& path) { return load_grammar_text(read_file(path)); }
 
 PhaseGrammar compile_phase(const Grammar& g, unsigned root, unsigned max_depth, unsigned site_depth,

@@ -93,6 +93,7 @@ struct Grammar {
   std::vector<CExpr> exprs;
   std::vector<CDomain> domains;
   std::uint32_t entry = 0;
+  std::string content_hash;  // population.json grammar_hash (identity.hpp)
 };
 
 Grammar load_grammar_file(const std::string& path);

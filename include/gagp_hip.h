@@ -68,6 +68,13 @@ typedef struct gagp_policy {
   double summary_cap;
 } gagp_policy;
 
+/* int32 VALU issue-rate probe: device milliseconds of one launch of
+ * `blocks` blocks of 256 threads, each thread running `iterations`
+ * iterations of 8 dependent-pair int32 add/xor operations in four
+ * independent chains (8 * iterations * 256 * blocks lane operations).
+ * Calibrates the VALU roofline; no reference counterpart. */
+int gagp_valu_probe(int32_t blocks, int32_t iterations, float* out_ms);
+
 /* Message of the calling thread's last failure ("" if none). */
 const char* gagp_last_error(void);
 /* Number of visible HIP devices (0 without a GPU). */
@@ -169,6 +176,23 @@ int gagp_batch_evaluated_program_cases(gagp_batch* b, int32_t index, uint64_t* o
  * {"k_eval": ms, ...}. */
 int gagp_batch_set_profile(gagp_batch* b, int32_t on);
 int gagp_batch_kernel_times_json(gagp_batch* b, char* buf, size_t cap, size_t* needed);
+
+/* Evaluation launch autotuner. In the first round of a new round table,
+ * between reproduction and placement, the evaluation kernels are timed over
+ * that round's pending programs under each candidate launch shape -- 1, 2
+ * or 4 pending programs per wavefront of the typed straight-line VM; 1, 2
+ * or 4 64-case tiles per work item of the looping VM -- and the fastest
+ * shapes (cached per batch shape for the process) are captured into the
+ * round graph. Results are identical under every shape. mode: 0 off (fixed
+ * shapes: one program per wavefront, one tile per item), 1 (the default)
+ * when a round evaluates at least 2^22 program-cases, 2 always.
+ * gagp_batch_tuning_json returns the last pass: {"key", "cached",
+ * "typed_ms", "loop_ms", "fast_programs", "loop_group"}, with "skipped"
+ * when the round was too small ("{}" before the first). No reference
+ * counterpart: the reference picks its launch profile by heuristics,
+ * cpp/src/runtime/gpu/fitness_gpu.cu:754-980. */
+int gagp_batch_set_autotune(gagp_batch* b, int32_t mode);
+int gagp_batch_tuning_json(gagp_batch* b, char* buf, size_t cap, size_t* needed);
 void gagp_batch_destroy(gagp_batch* b);
 
 /* ---- serving: a request loop with continuous batching ----

@@ -216,7 +216,7 @@ def build_driver(tmp):
     build = os.path.join(ROOT, "gagp_amd", "_build")
     subprocess.run(["g++", "-O1", "-std=c++17", "-I" + os.path.join(ROOT, "gagp_amd", "csrc"),
                     os.path.join(ROOT, "tests", "host_lowering.cpp"), os.path.join(build, "grammar.o"),
-                    os.path.join(build, "task.o"), os.path.join(build, "export.o"),
+                    os.path.join(build, "task.o"), os.path.join(build, "export.o"), os.path.join(build, "identity.o"),
                     os.path.join(ROOT, "oracle", "liboracle.so"), "-Wl,-rpath," + os.path.join(ROOT, "oracle"),
                     "-o", driver], check=True)
     return driver
@@ -230,9 +230,6 @@ def run_reference(task, population, generations, seed, out, save_all, search=Non
     if save_all:
         cmd.append("--save-all")
     subprocess.run(cmd, check=True)
-    if save_all:
-        # history.json and population.json are large; keep the population only.
-        os.remove(os.path.join(out, "history.json"))
 
 
 def main():

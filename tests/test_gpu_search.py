@@ -57,6 +57,33 @@ def test_batched_searches_match_alone():
         assert strip_timing(b.result(i)) == strip_timing(golden(name)), name
 
 
+def test_autotuned_launch_shapes_leave_results_unchanged():
+    # The evaluation launch autotuner (on by default) times candidate shapes
+    # by re-running the evaluation kernels over a round's pending programs;
+    # with it off the fixed shapes run. Both publish every search's
+    # reference result, and a second batch of the same shape reuses the
+    # cached choice.
+    names = ["mixedl_0", "mixedl_1", "mixed_2", "list_count_gt_rev_p300_g8_s6", "stmt_count_gt_p300_g8_s6", "quintic_p512_g16_s3"]
+    print("autotune batch of", len(names), flush=True)
+    results = {}
+    for on in (0, 2, 2):
+        b = gagp_amd.Batch()
+        b.set_autotune(on)
+        index = {name: add(b, name) for name in names}
+        b.run()
+        for name, i in index.items():
+            assert strip_timing(b.result(i)) == strip_timing(golden(name)), (on, name)
+        results.setdefault(on, []).append(b.tuning())
+    assert results[0] == [{}]
+    first, second = results[2]
+    assert not first["cached"] and second["cached"]
+    assert first["key"] == second["key"]
+    assert first["fast_programs"] in (1, 2, 4) and first["loop_group"] in (1, 2, 4)
+    assert set(first["typed_ms"]) == {"1", "2", "4"}
+    assert (second["fast_programs"], second["loop_group"]) == (first["fast_programs"], first["loop_group"])
+    assert set(first["loop_ms"]) == {"1", "2", "4"}
+
+
 def test_graph_and_stream_execution_agree():
     name = "polynomial_p256_g20_s123"
     results = []
@@ -82,9 +109,12 @@ def test_cli_drop_in(tmp_path, name):
     subprocess.run(cmd, check=True, timeout=300)
     mine = json.load(open(out / "result.json"))
     assert strip_timing(mine) == strip_timing(golden(name))
-    ref_pop = json.load(open(os.path.join(GOLDEN, "runs", name, "population.json")))
-    my_pop = json.load(open(out / "population.json"))
-    assert my_pop["programs"] == ref_pop["programs"]
+    # population.json (format, grammar content hash, ranked programs) and
+    # history.json (every generation's best program and fitness) as the
+    # reference engine wrote them, whole
+    for doc in ("population.json", "history.json"):
+        ref = json.load(open(os.path.join(GOLDEN, "runs", name, doc)))
+        assert json.load(open(out / doc)) == ref, doc
 
 
 def test_cli_rejects_unsupported_with_exit_2(tmp_path):

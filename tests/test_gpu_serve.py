@@ -59,14 +59,17 @@ def planned(names):
 
 
 @pytest.mark.parametrize("limit", ["footprint", "max_live"])
-def test_budget_admits_one_search_at_a_time(limit):
-    # A budget that holds the largest search but no two of them (admission
-    # by planned footprint), or a limit of one live search, serializes
-    # admission; results are unchanged.
-    print("serve: one-at-a-time", limit, flush=True)
+def test_admission_limits(limit):
+    # Admission by planned footprint: a budget equal to the largest request's
+    # footprint never lets the live searches' planned bytes exceed it, so the
+    # largest search runs alone (the two small ones may share the device).
+    # A limit of one live search serializes admission. Results are unchanged.
+    print("serve: admission limit", limit, flush=True)
     names = ["mixed_1", "mixed_2", "polynomial_p100_g10_s5"]
+    sizes = planned(names)
+    budget = max(sizes)
     if limit == "footprint":
-        server = gagp_amd.Server(budget_bytes=max(planned(names)), poll_rounds=3)
+        server = gagp_amd.Server(budget_bytes=budget, poll_rounds=3)
     else:
         server = gagp_amd.Server(budget_bytes=8 << 30, poll_rounds=3, max_live=1)
     tickets = [submit(server, n, 0.0) for n in names]
@@ -75,9 +78,19 @@ def test_budget_admits_one_search_at_a_time(limit):
     for name, t, o in zip(names, tickets, outcomes):
         assert o["status"] == 0
         assert strip_timing(server.result(t)) == golden(name), name
-    # admission waits for the previous search to retire
-    for before, after in zip(outcomes, outcomes[1:]):
-        assert after["admit"] >= before["done"]
+    if limit == "footprint":
+        # at every admission, the searches live at that moment fit the budget
+        for o in outcomes:
+            live = [s for s, x in zip(sizes, outcomes) if x["admit"] <= o["admit"] < x["done"]]
+            assert sum(live) <= budget, (o, outcomes, sizes)
+        big = outcomes[sizes.index(budget)]
+        for o in outcomes:
+            if o is not big:
+                assert o["done"] <= big["admit"] or o["admit"] >= big["done"], (big, o)
+    else:
+        # admission waits for the previous search to retire
+        for before, after in zip(outcomes, outcomes[1:]):
+            assert after["admit"] >= before["done"]
 
 
 def test_unsupported_request_fails_alone():
@@ -120,5 +133,5 @@ def test_retired_searches_return_their_device_memory():
         assert server.outcome(t)["status"] == 0, name
         assert strip_timing(server.result(t)) == golden(name), name
     held = sum(planned(names))
-    assert held > 64 << 20
+    assert held >= 32 << 20  # the searches held far more than the tolerance below
     assert before - after < 8 << 20, (before, after, held)

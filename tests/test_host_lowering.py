@@ -25,7 +25,7 @@ STATEMENT_TASKS = sorted(f[:-5] for f in os.listdir(os.path.join(GOLDEN, "defini
 
 @pytest.fixture(scope="module")
 def driver(tmp_path_factory):
-    objs = [os.path.join(BUILD, n) for n in ("grammar.o", "task.o", "export.o")]
+    objs = [os.path.join(BUILD, n) for n in ("grammar.o", "task.o", "export.o", "identity.o")]
     missing = [p for p in objs + [ORACLE] if not os.path.exists(p)]
     assert not missing, "build the engine and the oracle first (__graft_entry__.build()): missing %s" % missing
     exe = str(tmp_path_factory.mktemp("driver") / "host_lowering")
