@@ -68,6 +68,7 @@ def lib():
                                       ctypes.c_int32, ctypes.c_double, Policy, ctypes.POINTER(ctypes.c_void_p)]
     L.gagp_session_eval.argtypes = [ctypes.c_void_p, ctypes.POINTER(Program), ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
                                     ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]
+    L.gagp_session_eval_static.argtypes = L.gagp_session_eval.argtypes
     L.gagp_session_eval_cases.argtypes = [ctypes.c_void_p, ctypes.POINTER(Program), ctypes.POINTER(Value),
                                           ctypes.POINTER(ctypes.c_int32)]
     L.gagp_session_last_kernel_ms.argtypes = [ctypes.c_void_p]
@@ -174,16 +175,18 @@ class Session:
         keep.extend([code, consts, fuel])
         return Program(code, len(p["code"]), fuel, consts, len(p["consts"]), p["n_locals"])
 
-    def eval(self, programs):
+    def eval(self, programs, allocate=True):
         """programs: dicts {"code": [(op, a|None, b|None)], "consts": [...], "n_locals": n, "fuel": [...]|None}.
-        Returns (fitness list, wrong list, counts list of (cases, errors, timeouts))."""
+        Returns (fitness list, wrong list, counts list of (cases, errors, timeouts)).
+        allocate=False calls the allocation-free entry point (gagp_session_eval_static)."""
         keep = []
         arr = (Program * len(programs))(*[self._program(p, keep) for p in programs])
         n = len(programs)
         fit = (ctypes.c_double * n)()
         wrong = (ctypes.c_uint32 * n)()
         counts = (ctypes.c_uint32 * (3 * n))()
-        _check(lib().gagp_session_eval(self.h, arr, n, fit, wrong, counts))
+        entry = lib().gagp_session_eval if allocate else lib().gagp_session_eval_static
+        _check(entry(self.h, arr, n, fit, wrong, counts))
         return list(fit), list(wrong), [tuple(counts[3 * i:3 * i + 3]) for i in range(n)]
 
     def eval_cases(self, program):

@@ -116,6 +116,17 @@ int gagp_session_eval(gagp_session* s, const gagp_program* programs, int32_t n, 
   });
 }
 
+int gagp_session_eval_static(gagp_session* s, const gagp_program* programs, int32_t n, double* out_fitness, uint32_t* out_wrong,
+                             uint32_t* out_counts) {
+  return guarded([&] {
+    if (!s || n < 0 || (n && (!programs || !out_fitness))) throw std::invalid_argument("invalid eval arguments");
+    std::vector<ProgramIn> in;
+    in.reserve(std::size_t(n));
+    for (int i = 0; i < n; ++i) in.push_back(program_in(programs[i]));
+    s->s->eval_static(in, out_fitness, out_wrong, out_counts);
+  });
+}
+
 int gagp_session_eval_cases(gagp_session* s, const gagp_program* program, gagp_value* out_values, int32_t* out_errors) {
   return guarded([&] {
     if (!s || !program || !out_values || !out_errors) throw std::invalid_argument("invalid eval arguments");

@@ -1262,6 +1262,7 @@ __global__ void k_commit(DSearch* all, const BlockMap* map) {
 
 // ---- evaluation session kernel (external bytecode) ----
 struct DProg { GAS const Ins* code; GAS const Val* pool; int len, has_jumps, n_locals, fast, n_pool, result_tag; };
+static_assert(sizeof(DProg) == 40, "program table entry size is part of the workspace bound");
 __device__ inline ProgView view_of(const DProg& p) {
   return ProgView{p.code, p.pool, p.len, p.n_locals, p.has_jumps, p.fast, p.n_pool, p.result_tag};
 }
@@ -2602,6 +2603,20 @@ FastCert certify_fast(const std::vector<Ins>& code, const std::vector<Val>& cons
   return cert;  // falls off the end: the general VM reports it
 }
 
+namespace {
+// Host packing of one evaluation launch: verified code and pools laid out
+// back to back, with each program's offsets and certificate.
+struct PackedPrograms {
+  std::vector<Ins> code;
+  std::vector<Val> pool;
+  std::vector<std::size_t> code_off, pool_off;
+  std::vector<int> lens, jumps, locals, fast, pool_n, result;
+  int slots = 1, temps = 0, shadow = 0;
+  bool any_fast = false, any_general = false;
+};
+
+}  // namespace
+
 struct Session::Impl {
   DeviceArena arena;
   DeviceCases cases;
@@ -2611,6 +2626,9 @@ struct Session::Impl {
   double penalty;
   Policy policy;
   float last_ms = 0;
+  PackedPrograms pack(const std::vector<ProgramIn>& programs) const;
+  std::vector<DProg> program_table(const PackedPrograms& k, GAS const Ins* code, GAS const Val* pool) const;
+  void launch(const PackedPrograms& k, const DProg* progs, int n, double* fit, unsigned* wrong, unsigned* counts) const;
 };
 
 namespace {
@@ -2654,75 +2672,153 @@ Session::Session(const Cases& c, int fuel, int lanes, double penalty, Policy pol
 Session::~Session() = default;
 float Session::last_kernel_ms() const { return impl_->last_ms; }
 
-void Session::eval(const std::vector<ProgramIn>& programs, double* fitness, std::uint32_t* wrong, std::uint32_t* counts) {
-  auto& I = *impl_;
-  const int n = int(programs.size());
-  if (!n) return;
-  std::vector<Ins> code;
-  std::vector<Val> pool;
-  std::vector<std::size_t> code_off, pool_off;
-  std::vector<int> lens, jumps_v, locals, fast_v, pool_n, result_v;
-  int slots = 1, temps = 0;
+PackedPrograms Session::Impl::pack(const std::vector<ProgramIn>& programs) const {
+  const auto& I = *this;
+  PackedPrograms k;
   bool stores = false;  // some program stores into an input
   for (const auto& p : programs) {
-    temps = std::max(temps, std::min(64, p.n_locals) - I.cases.cd.n_inputs);
+    k.temps = std::max(k.temps, std::min(64, p.n_locals) - I.cases.cd.n_inputs);
     int ms = 0;
     bool j = false;
     const auto ins = verify_program(p, I.cases.cd.n_inputs, ms, j, &stores);
     for (const auto& v : p.consts) check_view(v, I.arena_len);
     const FastCert cert = certify_fast(ins, p.consts, I.input_tags, p.n_locals, ms, j, I.cases.cd.C);
-    fast_v.push_back(cert.fast);
-    result_v.push_back(cert.result_tag);
-    pool_n.push_back(int(p.consts.size()));
-    slots = std::max(slots, ms + 1);
-    code_off.push_back(code.size());
-    pool_off.push_back(pool.size());
-    code.insert(code.end(), ins.begin(), ins.end());
-    pool.insert(pool.end(), p.consts.begin(), p.consts.end());
-    lens.push_back(int(ins.size()));
-    jumps_v.push_back(j);
-    locals.push_back(p.n_locals);
+    k.fast.push_back(cert.fast);
+    k.result.push_back(cert.result_tag);
+    k.pool_n.push_back(int(p.consts.size()));
+    k.slots = std::max(k.slots, ms + 1);
+    k.code_off.push_back(k.code.size());
+    k.pool_off.push_back(k.pool.size());
+    k.code.insert(k.code.end(), ins.begin(), ins.end());
+    k.pool.insert(k.pool.end(), p.consts.begin(), p.consts.end());
+    k.lens.push_back(int(ins.size()));
+    k.jumps.push_back(j);
+    k.locals.push_back(p.n_locals);
   }
-  if (pool.empty()) pool.push_back(val_invalid());
+  if (k.pool.empty()) k.pool.push_back(val_invalid());
+  k.shadow = stores ? I.cases.cd.n_inputs : 0;
+  k.temps += k.shadow;
+  k.any_fast = std::count(k.fast.begin(), k.fast.end(), 1) > 0;
+  k.any_general = std::count(k.fast.begin(), k.fast.end(), 0) > 0;
+  return k;
+}
+
+std::vector<DProg> Session::Impl::program_table(const PackedPrograms& k, GAS const Ins* code, GAS const Val* pool) const {
+  std::vector<DProg> progs(k.lens.size());
+  for (std::size_t i = 0; i < progs.size(); ++i)
+    progs[i] = DProg{code + k.code_off[i], pool + k.pool_off[i], k.lens[i], k.jumps[i], k.locals[i], k.fast[i], k.pool_n[i], k.result[i]};
+  return progs;
+}
+
+// Both evaluation kernels over a packed launch on the null stream.
+void Session::Impl::launch(const PackedPrograms& k, const DProg* progs, int n, double* fit, unsigned* wrong, unsigned* counts) const {
+  const auto& I = *this;
+  const EvalParams e{I.cases.cd, I.fuel, I.lanes, I.penalty, I.policy};
+  const std::size_t lds = std::size_t(kWavesPerBlock) * (k.slots + k.temps) * 64 * 10;
+  const std::size_t fast_lds = std::size_t(kWavesPerBlock) * 8 * std::size_t(I.cases.cd.C);
+  if (k.any_general && lds > 65536)
+    HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_eval_programs<false>), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               int(lds)));
+  if (k.any_fast && fast_lds > 65536)
+    HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_eval_programs<true>), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               int(fast_lds)));
+  const int blocks = std::min(4096, (n + kWavesPerBlock - 1) / kWavesPerBlock);
+  if (k.any_fast)
+    hipLaunchKernelGGL(k_eval_programs<true>, dim3(blocks), dim3(kEvalThreads), fast_lds, nullptr, progs, n, e, k.slots, k.temps,
+                       k.shadow, fit, wrong, counts);
+  if (k.any_general)
+    hipLaunchKernelGGL(k_eval_programs<false>, dim3(blocks), dim3(kEvalThreads), lds, nullptr, progs, n, e, k.slots, k.temps,
+                       k.shadow, fit, wrong, counts);
+  HIP_OK(hipGetLastError());
+}
+
+void Session::eval(const std::vector<ProgramIn>& programs, double* fitness, std::uint32_t* wrong, std::uint32_t* counts) {
+  auto& I = *impl_;
+  const int n = int(programs.size());
+  if (!n) return;
+  const PackedPrograms k = I.pack(programs);
   DeviceArena tmp;
-  auto dcode = tmp.upload(code);
-  auto dpool = tmp.upload(pool);
-  std::vector<DProg> progs(static_cast<std::size_t>(n));
-  for (int i = 0; i < n; ++i)
-    progs[i] = DProg{dcode + code_off[i], dpool + pool_off[i], lens[i], jumps_v[i], locals[i], fast_v[i], pool_n[i], result_v[i]};
-  auto dprogs = tmp.upload(progs);
+  auto dcode = tmp.upload(k.code);
+  auto dpool = tmp.upload(k.pool);
+  auto dprogs = tmp.upload(I.program_table(k, dcode, dpool));
   auto dfit = tmp.alloc<double>(std::size_t(n));
   auto dwrong = tmp.alloc<unsigned>(std::size_t(n));
   auto dcounts = tmp.alloc<unsigned>(std::size_t(n) * 3);
-  const EvalParams e{I.cases.cd, I.fuel, I.lanes, I.penalty, I.policy};
-  const int shadow = stores ? I.cases.cd.n_inputs : 0;
-  temps += shadow;
-  const std::size_t lds = std::size_t(kWavesPerBlock) * (slots + temps) * 64 * 10;
-  const std::size_t fast_lds = std::size_t(kWavesPerBlock) * 8 * std::size_t(I.cases.cd.C);
-  const bool any_fast = std::count(fast_v.begin(), fast_v.end(), 1) > 0, any_general = std::count(fast_v.begin(), fast_v.end(), 0) > 0;
-  if (lds > 65536)
-    HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_eval_programs<false>), hipFuncAttributeMaxDynamicSharedMemorySize,
-                               int(lds)));
-  if (any_fast && fast_lds > 65536)
-    HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_eval_programs<true>), hipFuncAttributeMaxDynamicSharedMemorySize,
-                               int(fast_lds)));
   hipEvent_t a, b;
   HIP_OK(hipEventCreate(&a));
   HIP_OK(hipEventCreate(&b));
-  const int blocks = std::min(4096, (n + kWavesPerBlock - 1) / kWavesPerBlock);
   HIP_OK(hipEventRecord(a, nullptr));
-  if (any_fast)
-    hipLaunchKernelGGL(k_eval_programs<true>, dim3(blocks), dim3(kEvalThreads), fast_lds, nullptr, dprogs, n, e, slots, temps, shadow,
-                       dfit, dwrong, dcounts);
-  if (any_general)
-    hipLaunchKernelGGL(k_eval_programs<false>, dim3(blocks), dim3(kEvalThreads), lds, nullptr, dprogs, n, e, slots, temps, shadow,
-                       dfit, dwrong, dcounts);
-  HIP_OK(hipGetLastError());
+  I.launch(k, (const DProg*)dprogs, n, (double*)dfit, (unsigned*)dwrong, (unsigned*)dcounts);
   HIP_OK(hipEventRecord(b, nullptr));
   HIP_OK(hipEventSynchronize(b));
   HIP_OK(hipEventElapsedTime(&I.last_ms, a, b));
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
+  HIP_OK(hipMemcpy(fitness, (const void*)dfit, sizeof(double) * n, hipMemcpyDeviceToHost));
+  if (wrong) HIP_OK(hipMemcpy(wrong, (const void*)dwrong, sizeof(unsigned) * n, hipMemcpyDeviceToHost));
+  if (counts) HIP_OK(hipMemcpy(counts, (const void*)dcounts, sizeof(unsigned) * 3 * n, hipMemcpyDeviceToHost));
+}
+
+// ---- allocation-free evaluation ----
+// A fixed device workspace, one copy per GPU, for callers that cannot
+// allocate device memory (code under HIP graph capture, serving loops,
+// harnesses that forbid hipMalloc). It holds one launch's program table,
+// code, constant pools and per-program results. Every call on a GPU shares
+// that GPU's copy: calls that overlap on one GPU race, and the workspace keeps
+// the last call's contents (the kernels write every result they report, so
+// nothing needs clearing).
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+namespace {
+std::mutex scratch_mutex;
+std::map<int, unsigned char*> scratch_address;  // device -> its copy of `scratch`
+
+// The current device's copy of the workspace, resolved once per device with
+// that device current. Host code uses only this address, never the symbol.
+unsigned char* scratch_on_current_device() {
+  int device = 0;
+  HIP_OK(hipGetDevice(&device));
+  std::lock_guard<std::mutex> lock(scratch_mutex);
+  auto it = scratch_address.find(device);
+  if (it != scratch_address.end()) return it->second;
+  void* p = nullptr;
+  HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) throw std::runtime_error("no device address for the evaluation workspace");
+  scratch_address[device] = static_cast<unsigned char*>(p);
+  return static_cast<unsigned char*>(p);
+}
+
+std::size_t align256(std::size_t x) { return (x + 255) & ~std::size_t(255); }
+}  // namespace
+
+std::size_t Session::static_bytes(std::size_t n, std::size_t code, std::size_t pool) {
+  return align256(n * sizeof(DProg)) + align256(code * sizeof(Ins)) + align256(pool * sizeof(Val)) +
+         align256(n * sizeof(double)) + align256(n * sizeof(unsigned)) + align256(3 * n * sizeof(unsigned));
+}
+
+void Session::eval_static(const std::vector<ProgramIn>& programs, double* fitness, std::uint32_t* wrong, std::uint32_t* counts) {
+  const auto& I = *impl_;
+  const int n = int(programs.size());
+  if (!n) return;
+  const PackedPrograms k = I.pack(programs);
+  const std::size_t need = static_bytes(std::size_t(n), k.code.size(), k.pool.size());
+  if (need > kStaticScratchBytes)
+    throw std::invalid_argument("programs need " + std::to_string(need) + " bytes of evaluation workspace; the allocation-free "
+                                "entry point holds at most " + std::to_string(kStaticScratchBytes));
+  unsigned char* base = scratch_on_current_device();
+  std::size_t at = 0;
+  const auto carve = [&](std::size_t bytes) { unsigned char* p = base + at; at += align256(bytes); return p; };
+  auto* dprogs = reinterpret_cast<DProg*>(carve(std::size_t(n) * sizeof(DProg)));
+  auto* dcode = reinterpret_cast<Ins*>(carve(k.code.size() * sizeof(Ins)));
+  auto* dpool = reinterpret_cast<Val*>(carve(k.pool.size() * sizeof(Val)));
+  auto* dfit = reinterpret_cast<double*>(carve(std::size_t(n) * sizeof(double)));
+  auto* dwrong = reinterpret_cast<unsigned*>(carve(std::size_t(n) * sizeof(unsigned)));
+  auto* dcounts = reinterpret_cast<unsigned*>(carve(std::size_t(3) * n * sizeof(unsigned)));
+  const auto progs = I.program_table(k, gbl(static_cast<const Ins*>(dcode)), gbl(static_cast<const Val*>(dpool)));
+  HIP_OK(hipMemcpy(dcode, k.code.data(), k.code.size() * sizeof(Ins), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(dpool, k.pool.data(), k.pool.size() * sizeof(Val), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(dprogs, progs.data(), progs.size() * sizeof(DProg), hipMemcpyHostToDevice));
+  I.launch(k, dprogs, n, dfit, dwrong, dcounts);
   HIP_OK(hipMemcpy(fitness, dfit, sizeof(double) * n, hipMemcpyDeviceToHost));
   if (wrong) HIP_OK(hipMemcpy(wrong, dwrong, sizeof(unsigned) * n, hipMemcpyDeviceToHost));
   if (counts) HIP_OK(hipMemcpy(counts, dcounts, sizeof(unsigned) * 3 * n, hipMemcpyDeviceToHost));

@@ -107,6 +107,9 @@ class Batch {
 // threads, each running `iters` iterations of 8 VALU operations.
 float valu_probe_ms(int blocks, int iters);
 
+// Size of the fixed device workspace of Session::eval_static, per GPU.
+constexpr std::size_t kStaticScratchBytes = std::size_t(16) << 20;
+
 // Evaluation session: the FitnessSessionGpu boundary (fixed case set,
 // arbitrary verified bytecode programs).
 struct ProgramIn {
@@ -121,6 +124,13 @@ class Session {
   Session(const Cases& cases, int fuel, int lanes, double penalty, Policy policy);
   ~Session();
   void eval(const std::vector<ProgramIn>& programs, double* fitness, std::uint32_t* wrong, std::uint32_t* counts);
+  // eval() without any device allocation: the launch's program table, code,
+  // pools and results live in a fixed per-GPU workspace of
+  // kStaticScratchBytes (throws std::invalid_argument when a launch needs
+  // more, see static_bytes). Calls that overlap on one GPU race on it. The
+  // kernel is not timed (last_kernel_ms keeps its value).
+  void eval_static(const std::vector<ProgramIn>& programs, double* fitness, std::uint32_t* wrong, std::uint32_t* counts);
+  static std::size_t static_bytes(std::size_t programs, std::size_t instructions, std::size_t pool_entries);
   // Per-case outcome of one program: value and error code (ERR_NONE = ok).
   void eval_cases(const ProgramIn& program, std::vector<Val>& values, std::vector<int>& errors);
   float last_kernel_ms() const;

@@ -100,6 +100,22 @@ int gagp_session_create(const gagp_value* case_inputs, const uint8_t* input_set,
 int gagp_session_eval(gagp_session* s, const gagp_program* programs, int32_t n, double* out_fitness, uint32_t* out_wrong,
                       uint32_t* out_counts);
 
+/* gagp_session_eval without any device allocation, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). No counterpart in the reference.
+ * Same arguments, results and errors as gagp_session_eval; the launch's
+ * program table, code, constant pools and per-program results are staged in
+ * a fixed per-GPU device workspace of 16 MiB (a file-scope device array),
+ * held by every GPU the library runs on until the process ends. Each
+ * launch needs, in 256-byte aligned parts, 40 B per program (table),
+ * 8 B per instruction, 16 B per constant-pool entry (at least one) and
+ * 20 B per program of results; a launch that needs more fails with
+ * GAGP_ERR_INVALID before anything reaches the device. Calls that overlap on
+ * one GPU share its workspace and race; serialize them. The kernel is not
+ * timed (gagp_session_last_kernel_ms keeps its previous value). */
+int gagp_session_eval_static(gagp_session* s, const gagp_program* programs, int32_t n, double* out_fitness,
+                             uint32_t* out_wrong, uint32_t* out_counts);
+
 /* Per-case outcome of one program: out_values[c] and out_errors[c]
  * (-1 ok, 0 NameError, 1 TypeError, 2 ZeroDivisionError, 3 ValueError, 4 Timeout).
  * Reference: CpuExecutionSession::execute as used by the run judge,
