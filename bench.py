@@ -262,7 +262,7 @@ def percentile(values, q):
     return v[min(len(v) - 1, int(round(q / 100.0 * (len(v) - 1))))]
 
 
-def batching(n, generations):
+def batching(n, generations, run_alone=True):
     """Searches/s and completion latency of a mixed workload run one search at
     a time and as one batch: n seeded searches, half polynomials of degree 2-5
     (coefficients in [-5,5], x in [-50,50]) and half list reductions (sum,
@@ -281,7 +281,7 @@ def batching(n, generations):
         tasks = workloads.mixed_searches(tmp, n=n, seed=4, p_range=(256, 4096), c_range=(16, 1024), lists=True)
         alone, lat_alone, cases = [], [], 0
         t0 = time.perf_counter()
-        for i, (task, population) in enumerate(tasks):
+        for i, (task, population) in enumerate(tasks if run_alone else []):
             b = gagp_amd.Batch()
             b.add(task, None, population, generations, 11 + i)
             b.run(use_graph=GRAPH)
@@ -298,6 +298,8 @@ def batching(n, generations):
         f.run(use_graph=GRAPH)
         fixed = [strip(f.result(i)) for i in range(n)]
         fixed_s = time.perf_counter() - t0
+        if not run_alone:
+            alone = fixed  # identity checks against the fixed-shape batch instead
         t0 = time.perf_counter()
         b = gagp_amd.Batch()
         for i, (task, population) in enumerate(tasks):
@@ -327,14 +329,15 @@ def batching(n, generations):
         kernel_ms = {k: round(v, 3) for k, v in sorted(p.kernel_times().items(), key=lambda kv: -kv[1])}
         profiled_same = sum(strip(p.result(i)) == x for i, x in enumerate(alone))
     same = sum(x == y for x, y in zip(alone, together))
+    one = None if not run_alone else {"searches_per_s": round(n / alone_s, 3), "seconds": round(alone_s, 3),
+                                      "p50_latency_s": round(percentile(lat_alone, 50), 3),
+                                      "p99_latency_s": round(percentile(lat_alone, 99), 3)}
     return {"searches": n, "generations": generations,
-            "one_at_a_time": {"searches_per_s": round(n / alone_s, 3), "seconds": round(alone_s, 3),
-                              "p50_latency_s": round(percentile(lat_alone, 50), 3),
-                              "p99_latency_s": round(percentile(lat_alone, 99), 3)},
+            "one_at_a_time": one,
             "batched": {"searches_per_s": round(n / batched_s, 3), "seconds": round(batched_s, 3),
                         "p50_latency_s": round(batched_s, 3), "p99_latency_s": round(batched_s, 3),
                         "program_case_evals_per_s": round(cases / batched_s, 1), "phases": phases},
-            "speedup": round(alone_s / batched_s, 2), "identical_to_alone": "%d/%d" % (same, n),
+            "speedup": round(alone_s / batched_s, 2) if run_alone else None, "identical_to_alone": "%d/%d" % (same, n),
             "batched_kernel_ms": kernel_ms, "profiled_identical_to_alone": "%d/%d" % (profiled_same, n),
             "autotune": {"tuning": tuning, "fixed_shapes_s": round(fixed_s, 3), "tuned_s": round(batched_s, 3),
                          "gain": round(fixed_s / batched_s, 3),
@@ -478,6 +481,8 @@ def main():
     ap.add_argument("--no-loop-heavy", dest="loop_heavy", action="store_false", help="skip the loop-heavy search")
     ap.add_argument("--no-graph", action="store_true", help="stream launches instead of graph replays everywhere")
     ap.add_argument("--no-whole", dest="whole", action="store_false", help="skip the whole-search timings")
+    ap.add_argument("--no-alone", dest="alone", action="store_false",
+                    help="batching: skip the one-at-a-time leg (for tools that trace every dispatch)")
     args = ap.parse_args()
     global GRAPH
     GRAPH = not args.no_graph
@@ -515,7 +520,7 @@ def main():
                          os.path.join(ROOT, "tests", "golden", "tasks", "poly_c1", "task.json"), 1024, 64, 0, not args.no_cpu),
     }
     loop = {form: loop_heavy(form, 0 if args.no_cpu else 2) for form in LOOP_FORMS} if args.loop_heavy else None
-    batch = batching(args.batch_searches, 64) if args.batch_searches else None
+    batch = batching(args.batch_searches, 64, args.alone) if args.batch_searches else None
     serve = serving(args.serve_requests, 20.0, 60.0, not args.no_cpu) if args.serve_requests else None
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_generations)
     line = {

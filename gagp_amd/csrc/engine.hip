@@ -21,6 +21,7 @@
 #include <map>
 #include <mutex>
 #include <fstream>
+#include <functional>
 #include <numeric>
 #include <sstream>
 #include <stdexcept>
@@ -1698,11 +1699,43 @@ struct Batch::Impl {
   double run_ms = 0, init_ms = 0;
   bool graph_ready = false;
   hipGraphExec_t graph_exec = nullptr;
+  // Side streams of a round's parallel branches (default priority, like
+  // `stream`) and the events that fork them from `stream` and join them
+  // back; captured, they become parallel branches of the round graph.
+  static constexpr int kSides = 2;
+  hipStream_t side[kSides] = {};
+  hipEvent_t fork_ev = nullptr, join_ev[kSides] = {};
   ~Impl() {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (d_all) (void)hipFree(d_all);
     if (d_status) (void)hipFree(d_status);
+    for (int i = 0; i < kSides; ++i) {
+      if (side[i]) (void)hipStreamDestroy(side[i]);
+      if (join_ev[i]) (void)hipEventDestroy(join_ev[i]);
+    }
+    if (fork_ev) (void)hipEventDestroy(fork_ev);
     if (stream) (void)hipStreamDestroy(stream);
+  }
+  void make_streams() {
+    if (!stream) HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    if (fork_ev) return;
+    HIP_OK(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
+    for (int i = 0; i < kSides; ++i) {
+      HIP_OK(hipStreamCreateWithFlags(&side[i], hipStreamNonBlocking));
+      HIP_OK(hipEventCreateWithFlags(&join_ev[i], hipEventDisableTiming));
+    }
+  }
+  // Runs parts[0] on `stream` and parts[1..] on side streams forked from it,
+  // then joins them: the parts must touch disjoint data.
+  void branches(const std::vector<std::function<void(hipStream_t)>>& parts) {
+    HIP_OK(hipEventRecord(fork_ev, stream));
+    for (std::size_t i = 1; i < parts.size(); ++i) HIP_OK(hipStreamWaitEvent(side[i - 1], fork_ev, 0));
+    parts[0](stream);
+    for (std::size_t i = 1; i < parts.size(); ++i) {
+      parts[i](side[i - 1]);
+      HIP_OK(hipEventRecord(join_ev[i - 1], side[i - 1]));
+      HIP_OK(hipStreamWaitEvent(stream, join_ev[i - 1], 0));
+    }
   }
   std::size_t eval_fast_lds = 0, eval_general_lds = 0;  // dynamic LDS of the evaluation kernels
   static constexpr std::size_t kLoopLds = std::size_t(kWavesPerBlock) * kLoopSlots * 64 * 8;  // the looping slots
@@ -1932,9 +1965,12 @@ struct Batch::Impl {
     hipEvent_t a, b;
     HIP_OK(hipEventCreate(&a));
     HIP_OK(hipEventCreate(&b));
+    // The first candidate of each kernel runs twice (its first launch warms
+    // code and caches), every other candidate once.
+    int reps = 2;
     const auto measure = [&](auto launch) {
       float best = 1e30f;
-      for (int rep = 0; rep < 2; ++rep) {
+      for (int rep = 0; rep < reps; ++rep) {
         HIP_OK(hipEventRecord(a, stream));
         launch();
         HIP_OK(hipEventRecord(b, stream));
@@ -1949,16 +1985,20 @@ struct Batch::Impl {
     r << "{\"key\":\"" << key << "\",\"cached\":false";
     TuneChoice choice;
     DeviceArena tmp;
-    // A candidate replaces the default only when at least 3% faster.
+    // A candidate replaces the default only when at least 3% faster; the
+    // candidates grow the work per wavefront, so once one is 15% slower
+    // than the best the larger ones are not tried.
     const auto pick = [&](const char* name, const std::vector<int>& values, auto time_of) {
       r << ",\"" << name << "_ms\":{";
       int chosen = values[0];
       float chosen_ms = 0;
       for (std::size_t i = 0; i < values.size(); ++i) {
+        reps = i == 0 ? 2 : 1;
         const float ms = time_of(values[i]);
         r << (i ? "," : "") << "\"" << values[i] << "\":" << ms;
         if (i == 0) chosen_ms = ms;
         else if (ms < 0.97f * chosen_ms) { chosen = values[i]; chosen_ms = ms; }
+        else if (ms > 1.15f * chosen_ms) break;
       }
       r << "}";
       return chosen;
@@ -2027,9 +2067,13 @@ struct Batch::Impl {
   }
 
   template <class Kernel, class... Args>
-  void launch(const char* name, const Grid& g, Kernel kernel, int threads, std::size_t lds, Args... args) {
+  void launch_on(hipStream_t s, const char* name, const Grid& g, Kernel kernel, int threads, std::size_t lds, Args... args) {
     if (g.blocks)
-      timer.time(name, stream, [&] { hipLaunchKernelGGL(kernel, dim3(g.blocks), dim3(threads), lds, stream, d_all, g.map, args...); });
+      timer.time(name, s, [&] { hipLaunchKernelGGL(kernel, dim3(g.blocks), dim3(threads), lds, s, d_all, g.map, args...); });
+  }
+  template <class Kernel, class... Args>
+  void launch(const char* name, const Grid& g, Kernel kernel, int threads, std::size_t lds, Args... args) {
+    launch_on(stream, name, g, kernel, threads, lds, args...);
   }
   void enqueue_round() {
     enqueue_front();
@@ -2042,18 +2086,32 @@ struct Batch::Impl {
     const int n = int(live.size());
     if (any_generational) timer.time("k_gen_begin", stream, [&] { hipLaunchKernelGGL(k_gen_begin, dim3(n), dim3(256), 0, stream, d_all); });
     timer.time("k_record", stream, [&] { hipLaunchKernelGGL(k_record, dim3(n), dim3(256), 0, stream, d_all); });
-    launch("k_vary", g_vary_staged, k_vary<true, false>, 64 * kVaryWaves, vary_lds_bytes);
-    launch("k_vary_global", g_vary_global, k_vary<false, false>, 64 * kVaryWaves, 0);
-    launch("k_vary_walks", g_vary_staged_w, k_vary<true, true>, 64 * kVaryWaves, vary_lds_bytes);
-    launch("k_vary_global_walks", g_vary_global_w, k_vary<false, true>, 64 * kVaryWaves, 0);
-    launch("k_clear", g_table, k_clear, 256, 0);
-    launch("k_index", g_member, k_index, 128, 0);
+    // Three independent branches: the reproduction of searches without and
+    // with traversal walks (disjoint searches) and the genotype index of the
+    // current members (read by nothing reproduction writes); the pending
+    // lists need all three.
+    branches({[&](hipStream_t s) {
+                launch_on(s, "k_vary", g_vary_staged, k_vary<true, false>, 64 * kVaryWaves, vary_lds_bytes);
+                launch_on(s, "k_vary_global", g_vary_global, k_vary<false, false>, 64 * kVaryWaves, 0);
+              },
+              [&](hipStream_t s) {
+                launch_on(s, "k_vary_walks", g_vary_staged_w, k_vary<true, true>, 64 * kVaryWaves, vary_lds_bytes);
+                launch_on(s, "k_vary_global_walks", g_vary_global_w, k_vary<false, true>, 64 * kVaryWaves, 0);
+              },
+              [&](hipStream_t s) {
+                launch_on(s, "k_clear", g_table, k_clear, 256, 0);
+                launch_on(s, "k_index", g_member, k_index, 128, 0);
+              }});
     launch("k_reuse", g_child, k_reuse, 128, 0);
   }
   void enqueue_eval() {
-    launch("k_eval", g_eval, k_eval<1>, kEvalThreads, eval_fast_lds, 1);
-    if (loop_kernel) launch("k_eval_loop", g_eval_loop, k_eval_loop, kEvalThreads, eval_loop_lds, 1, tune.loop_group);
-    launch("k_eval_general", g_eval_general, k_eval<0>, kEvalThreads, eval_general_lds, 1);
+    // The three evaluation kernels score disjoint pending programs.
+    branches({[&](hipStream_t s) {
+                if (loop_kernel)
+                  launch_on(s, "k_eval_loop", g_eval_loop, k_eval_loop, kEvalThreads, eval_loop_lds, 1, tune.loop_group);
+              },
+              [&](hipStream_t s) { launch_on(s, "k_eval", g_eval, k_eval<1>, kEvalThreads, eval_fast_lds, 1); },
+              [&](hipStream_t s) { launch_on(s, "k_eval_general", g_eval_general, k_eval<0>, kEvalThreads, eval_general_lds, 1); }});
   }
   void enqueue_back() {
     const int n = int(live.size());
@@ -2098,7 +2156,7 @@ int Batch::add(const SearchSpec& spec) {
 void Batch::prepare() {
   auto& I = *impl_;
   const auto t0 = std::chrono::steady_clock::now();
-  if (!I.stream) HIP_OK(hipStreamCreateWithFlags(&I.stream, hipStreamNonBlocking));
+  I.make_streams();
   std::vector<int> fresh;
   for (std::size_t i = 0; i < I.searches.size(); ++i) {
     if (I.searches[i]->configured) continue;
@@ -2117,7 +2175,7 @@ void Batch::admit(int index) {
   auto& I = *impl_;
   SearchState& s = *I.searches.at(std::size_t(index));
   if (s.configured || s.retired) throw std::invalid_argument("search already admitted");
-  if (!I.stream) HIP_OK(hipStreamCreateWithFlags(&I.stream, hipStreamNonBlocking));
+  I.make_streams();
   try {
     I.configure(s);
     I.init_searches({index});
