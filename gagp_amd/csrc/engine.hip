@@ -602,7 +602,11 @@ __host__ __device__ inline int loop_blocks(int C, int n, int group = 1) {
 // had a non-integral score. It then clears the record for the program's next
 // evaluation. LDS: per wavefront kLoopSlots slots of 64 lanes, independent of
 // the case count.
-__global__ __launch_bounds__(kEvalThreads) void k_eval_loop(DSearch* all, const BlockMap* map, int mode, int group) {
+// kGrouped false is the one-tile instantiation (group 1): its loop-carried
+// totals fold away, which keeps five wavefronts per SIMD.
+template <bool kGrouped>
+__global__ __launch_bounds__(kEvalThreads) void k_eval_loop(DSearch* all, const BlockMap* map, int mode, int group_arg) {
+  const int group = kGrouped ? group_arg : 1;
   const BlockMap bm = map[blockIdx.x];
   const DSearch& d = all[bm.search];
   if (mode == 1 ? !search_live(d) : (!d.active || *d.error)) return;
@@ -627,10 +631,11 @@ __global__ __launch_bounds__(kEvalThreads) void k_eval_loop(DSearch* all, const 
     bool frac = false;
     unsigned tw = 0, te = 0, tt = 0;
     unsigned long long tf = 0;
-    for (int tile = first; tile < last; ++tile) {
+    const int n_tiles = kGrouped ? last - first : 1;  // first < tiles: one tile when ungrouped
+    for (int q = 0; q < n_tiles; ++q) {
       // Cases in the search's visiting order (similar loop lengths per
       // tile); scores are kept by case index, so the totals are unchanged.
-      const int at = tile * 64 + lane;
+      const int at = (first + q) * 64 + lane;
       const bool active = at < C;
       const int c = active ? e.cd.order[at] : 0;
       const LaneResult r = vm_loop(lp, fast_inputs(e.cd, pv.n_locals, c, active), n_in, e.cd.arena, slots, lane, active, e.fuel,
@@ -1463,6 +1468,7 @@ struct SearchState {
   Layout layout;
   PhaseGrammar phase;
   int P = 0, B = 0, R = 0, G = 0, rounds = 0, per_epoch = 0;
+  int stepped = 0;  // rounds the host has stepped this search (it idles past `rounds`)
   int fuel = 0;
   double penalty = 0;
   bool may_jump = false;   // programs may go to the looping path: if/and/or, traversals, statements, lets
@@ -1855,7 +1861,8 @@ struct Batch::Impl {
     attr(reinterpret_cast<const void*>(&k_vary<true, true>), vary_lds_bytes);
     attr(reinterpret_cast<const void*>(&k_eval<0>), eval_general_lds);
     attr(reinterpret_cast<const void*>(&k_eval<1>), eval_fast_lds);
-    attr(reinterpret_cast<const void*>(&k_eval_loop), eval_loop_lds);
+    attr(reinterpret_cast<const void*>(&k_eval_loop<false>), eval_loop_lds);
+    attr(reinterpret_cast<const void*>(&k_eval_loop<true>), eval_loop_lds);
     attr(reinterpret_cast<const void*>(&k_place<true>), place_lds_bytes);
     attr(reinterpret_cast<const void*>(&k_place<false>), place_hbm_bytes);
   }
@@ -1885,7 +1892,7 @@ struct Batch::Impl {
     if (any_jumps(ids))
       timer.time("k_eval_init_loop", stream, [&] {
         const Grid gp = build_grid(tmp, ids, [&](const SearchState& x) { return loop_blocks(x.d.cd.C, x.P); });
-        hipLaunchKernelGGL(k_eval_loop, dim3(gp.blocks), dim3(kEvalThreads), eval_loop_lds, stream, t, gp.map, 0, 1);
+        hipLaunchKernelGGL(k_eval_loop<false>, dim3(gp.blocks), dim3(kEvalThreads), eval_loop_lds, stream, t, gp.map, 0, 1);
       });
     timer.time("k_eval_init_general", stream, [&] {
       hipLaunchKernelGGL(k_eval<0>, dim3(ge.blocks), dim3(kEvalThreads), eval_general_lds, stream, t, ge.map, 0);
@@ -1925,9 +1932,16 @@ struct Batch::Impl {
     return k.str();
   }
   // Evaluation work of one round (program-cases over the live searches):
-  // below kTuneWork the tuning pass would cost more than it can gain, and
-  // the fixed shapes run.
+  // below kTuneWork, or with fewer than kTuneRounds rounds left to run,
+  // the tuning pass would cost more than it can gain, and the fixed shapes
+  // run.
   static constexpr double kTuneWork = double(1 << 22);
+  static constexpr int kTuneRounds = 128;
+  int rounds_left() const {
+    int left = 0;
+    for (int id : live) left = std::max(left, searches[std::size_t(id)]->rounds - searches[std::size_t(id)]->stepped);
+    return left;
+  }
   double round_work() const {
     double w = 0;
     for (int id : live) w += double(searches[std::size_t(id)]->B) * searches[std::size_t(id)]->d.cd.C;
@@ -1942,11 +1956,11 @@ struct Batch::Impl {
   // enqueue_eval of a round, with the stream idle.
   void tune_launches() {
     const std::string key = shape_key();
-    if (autotune == 1 && round_work() < kTuneWork) {
+    if (autotune == 1 && (round_work() < kTuneWork || rounds_left() < kTuneRounds)) {
       apply_tuning(TuneChoice{});
       std::ostringstream r;
-      r << "{\"key\":\"" << key << "\",\"cached\":false,\"skipped\":\"round work below " << kTuneWork
-        << " program-cases\",\"fast_programs\":1,\"loop_group\":1}";
+      r << "{\"key\":\"" << key << "\",\"cached\":false,\"skipped\":\"fewer than " << kTuneWork
+        << " program-cases per round or " << kTuneRounds << " rounds left\",\"fast_programs\":1,\"loop_group\":1}";
       tune_report = r.str();
       return;
     }
@@ -2015,7 +2029,10 @@ struct Batch::Impl {
         tmp.release();
         const Grid g = loop_grid(tmp, group);
         return g.blocks ? measure([&] {
-          hipLaunchKernelGGL(k_eval_loop, dim3(g.blocks), dim3(kEvalThreads), eval_loop_lds, stream, d_all, g.map, 1, group);
+          if (group == 1)
+            hipLaunchKernelGGL(k_eval_loop<false>, dim3(g.blocks), dim3(kEvalThreads), eval_loop_lds, stream, d_all, g.map, 1, 1);
+          else
+            hipLaunchKernelGGL(k_eval_loop<true>, dim3(g.blocks), dim3(kEvalThreads), eval_loop_lds, stream, d_all, g.map, 1, group);
         }) : 0.f;
       });
     }
@@ -2040,10 +2057,13 @@ struct Batch::Impl {
     build_round_grids();
   }
 
+  static constexpr int kCaptureAfter = 8;
+  int table_rounds = 0;  // rounds stepped since the round table was formed
   void rebuild() {
     if (graph_exec) { (void)hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
     graph_ready = false;
     tuned = false;
+    table_rounds = 0;
     if (d_all) { (void)hipFree(d_all); d_all = nullptr; }
     max_rounds = 0;
     if (!live.empty()) {
@@ -2107,8 +2127,10 @@ struct Batch::Impl {
   void enqueue_eval() {
     // The three evaluation kernels score disjoint pending programs.
     branches({[&](hipStream_t s) {
-                if (loop_kernel)
-                  launch_on(s, "k_eval_loop", g_eval_loop, k_eval_loop, kEvalThreads, eval_loop_lds, 1, tune.loop_group);
+                if (loop_kernel && tune.loop_group == 1)
+                  launch_on(s, "k_eval_loop", g_eval_loop, k_eval_loop<false>, kEvalThreads, eval_loop_lds, 1, 1);
+                else if (loop_kernel)
+                  launch_on(s, "k_eval_loop", g_eval_loop, k_eval_loop<true>, kEvalThreads, eval_loop_lds, 1, tune.loop_group);
               },
               [&](hipStream_t s) { launch_on(s, "k_eval", g_eval, k_eval<1>, kEvalThreads, eval_fast_lds, 1); },
               [&](hipStream_t s) { launch_on(s, "k_eval_general", g_eval_general, k_eval<0>, kEvalThreads, eval_general_lds, 1); }});
@@ -2192,6 +2214,43 @@ void Batch::admit(int index) {
   I.dirty = true;
 }
 
+std::vector<std::pair<int, std::string>> Batch::admit_many(const std::vector<int>& indices) {
+  auto& I = *impl_;
+  std::vector<std::pair<int, std::string>> failed;
+  std::vector<int> ready;
+  I.make_streams();
+  const auto undo = [&](SearchState& s) {
+    s.arena.release();
+    s.d.active = 0;
+    s.retired = true;
+  };
+  for (int index : indices) {
+    SearchState& s = *I.searches.at(std::size_t(index));
+    if (s.configured || s.retired) throw std::invalid_argument("search already admitted");
+    try {
+      I.configure(s);  // allocation: a failure here is this search's alone
+      ready.push_back(index);
+    } catch (const std::exception& e) {
+      (void)hipStreamSynchronize(I.stream);
+      undo(s);
+      failed.push_back({index, e.what()});
+    }
+  }
+  try {
+    I.init_searches(ready);  // one launch sequence for all of them
+  } catch (const std::exception& e) {
+    (void)hipStreamSynchronize(I.stream);
+    for (int index : ready) {
+      undo(*I.searches[std::size_t(index)]);
+      failed.push_back({index, e.what()});
+    }
+    ready.clear();
+  }
+  for (int index : ready) I.live.push_back(index);
+  if (!ready.empty()) I.dirty = true;
+  return failed;
+}
+
 void Batch::drop(int index) {
   SearchState& s = *impl_->searches.at(std::size_t(index));
   if (s.configured && !s.retired) throw std::invalid_argument("search is admitted: retire it");
@@ -2237,6 +2296,15 @@ void Batch::step(int rounds, bool use_graph) {
   auto& I = *impl_;
   if (I.dirty) I.rebuild();
   if (I.live.empty() || rounds <= 0) return;
+  // Every live search has been stepped `rounds` more when this returns (the
+  // tuning rule below reads the count from before this step).
+  struct Stepped {
+    Impl& impl;
+    int n;
+    ~Stepped() {
+      for (int id : impl.live) impl.searches[std::size_t(id)]->stepped += n;
+    }
+  } stepped{I, rounds};
   const auto t0 = std::chrono::steady_clock::now();
   if (I.autotune && !I.tuned) {
     // The first round of a new table: its pending programs are measured
@@ -2253,7 +2321,12 @@ void Batch::step(int rounds, bool use_graph) {
       return;
     }
   }
-  if (use_graph && !I.timer.on) {
+  // A new round table runs on stream launches until it has stepped
+  // kCaptureAfter rounds: capturing and instantiating a graph costs more
+  // than the launches of a table that changes every few rounds (serving).
+  const bool replay = I.graph_ready || I.table_rounds + rounds >= Impl::kCaptureAfter;
+  I.table_rounds += rounds;
+  if (use_graph && !I.timer.on && replay) {
     if (!I.graph_ready) {
       hipGraph_t graph;
       HIP_OK(hipStreamBeginCapture(I.stream, hipStreamCaptureModeThreadLocal));

@@ -93,6 +93,11 @@ class Batch {
   struct Status { int index, round, stopped_at, error; bool done; };
   std::size_t planned_footprint(int index) const;
   void admit(int index);
+  // admit() for several searches with one initialization launch sequence.
+  // A search whose allocation fails is undone alone; a device error in the
+  // shared initialization undoes them all. Returns (index, message) of each
+  // search not admitted.
+  std::vector<std::pair<int, std::string>> admit_many(const std::vector<int>& indices);
   void drop(int index);
   std::vector<Status> status();
   void retire(int index);

@@ -31,10 +31,14 @@ void Server::run() {
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return requests_[a].arrival_s < requests_[b].arrival_s; });
   Batch batch;
+  // The round table changes at every admission and retirement; measuring
+  // launch shapes for each new table would cost more than it returns.
+  batch.set_autotune(0);
   std::map<int, int> ticket_of;  // batch index -> ticket of a live search
   std::map<int, std::size_t> bytes_of;
   std::size_t used = 0, next = 0, finished = 0;
   int pending = -1;              // batch index of the head request, parsed and planned
+  std::vector<int> admitting;    // reserved in this pass, initialized together
   std::size_t pending_bytes = 0;
   const auto t0 = Clock::now();
   const auto now = [&] { return std::chrono::duration<double>(Clock::now() - t0).count(); };
@@ -83,14 +87,32 @@ void Server::run() {
       }
       if (used + pending_bytes > budget_) break;
       if (max_live_ > 0 && int(ticket_of.size()) >= max_live_) break;
-      const int index = pending;
+      // Reserved now, initialized below with every other request admitted
+      // in this pass.
+      ticket_of[pending] = ticket;
+      bytes_of[pending] = pending_bytes;
+      used += pending_bytes;
+      admitting.push_back(pending);
       pending = -1;
       ++next;
-      if (!guarded(o, [&] { batch.admit(index); })) continue;
-      o.admit_s = now();
-      ticket_of[index] = ticket;
-      bytes_of[index] = pending_bytes;
-      used += pending_bytes;
+    }
+    if (!admitting.empty()) {
+      std::vector<std::pair<int, std::string>> failed;
+      try {
+        failed = batch.admit_many(admitting);
+      } catch (const std::exception& e) {
+        for (int index : admitting) failed.push_back({index, e.what()});
+      }
+      for (const auto& f : failed) {
+        fail(outcomes_[std::size_t(ticket_of.at(f.first))], GAGP_ERR_DEVICE, f.second);
+        used -= bytes_of.at(f.first);
+        ticket_of.erase(f.first);
+        bytes_of.erase(f.first);
+      }
+      const double at = now();
+      for (int index : admitting)
+        if (ticket_of.count(index)) outcomes_[std::size_t(ticket_of.at(index))].admit_s = at;
+      admitting.clear();
       peak_bytes_ = std::max(peak_bytes_, used);
       peak_live_ = std::max(peak_live_, int(ticket_of.size()));
     }
