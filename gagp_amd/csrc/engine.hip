@@ -259,6 +259,16 @@ __global__ __launch_bounds__(64 * kVaryWaves) void k_vary(DSearch* all, const Bl
     if (st.error) raise(d, 300 + st.error);
   }
   if (st.error) return;
+  // A child identical to its parent takes the parent's completed result
+  // (k_reuse) and is never evaluated, so it is not lowered (about a fifth
+  // of the children: no crossover and no mutation).
+  if (st.reusable_parent >= 0) {
+    if (wlane() == 0) {
+      d.code_len[c] = 0;
+      d.pool_len[c] = 0;
+    }
+    return;
+  }
   __shared__ WalkStore<kWalks> walk_lds[kVaryWaves];
   lower_into_w<kWalks>(g, d, child, c, walk_lds[wave_index()]);
 }
@@ -1732,8 +1742,14 @@ struct Batch::Impl {
     }
   }
   // Runs parts[0] on `stream` and parts[1..] on side streams forked from it,
-  // then joins them: the parts must touch disjoint data.
+  // then joins them: the parts must touch disjoint data. With the kernel
+  // timer on, every part runs on `stream` in order, so each launch's events
+  // bracket that launch alone.
   void branches(const std::vector<std::function<void(hipStream_t)>>& parts) {
+    if (timer.on) {
+      for (const auto& part : parts) part(stream);
+      return;
+    }
     HIP_OK(hipEventRecord(fork_ev, stream));
     for (std::size_t i = 1; i < parts.size(); ++i) HIP_OK(hipStreamWaitEvent(side[i - 1], fork_ev, 0));
     parts[0](stream);

@@ -605,6 +605,10 @@ __device__ inline LaneResult vm_loop(const LoopProgram& f, FastInputs in, int n_
   const auto L = [&](int k) -> LAS std::int64_t& { return sl[(kLoopStack + (k < 0 ? 0 : k < kLoopLocals ? k : 0)) * 64]; };
   unsigned assigned = 0;  // hidden locals stored so far
   int fuel = fuel0, ip = 0, at = 0;
+  // The top of this lane's operand stack lives in a register; the LDS
+  // slots hold the elements below it (slot k: element k, for k < height - 1).
+  // Only the lanes that execute a step change either.
+  std::int64_t tos = 0;
   // Every step that finishes a lane, jumps or runs a fixed sequence sets
   // `resched`; any other step leaves the live lanes as they were, at + 1.
   bool resched = true;
@@ -720,13 +724,15 @@ __device__ inline LaneResult vm_loop(const LoopProgram& f, FastInputs in, int n_
         const int head = __builtin_amdgcn_readfirstlane(loop_pick(f.w1, at + 5));
         const bool starve = mine && fuel < 6;
         const bool run = mine && !starve;
+        const std::int64_t deeper = S(sp - 2);  // the top once the state is stored
         if (run) {
-          L(k_state) = S(sp - 1);
+          L(k_state) = tos;
           assigned |= 1u << k_state;
           const std::int64_t c = L(k_step);
           L(k_step) = reverse ? wrap_sub(c, 1) : wrap_add(c, 1);
           fuel -= 6;
           ip = head;
+          tos = deeper;
         }
         if (starve) {
           fuel = 0;
@@ -759,7 +765,7 @@ __device__ inline LaneResult vm_loop(const LoopProgram& f, FastInputs in, int n_
           bound = (assigned >> k) & 1u;
         }
         bool zero = false;
-        const std::int64_t v = loop_binary(op2, S(sp - 1), y, zero);
+        const std::int64_t v = loop_binary(op2, tos, y, zero);
         int err2 = ERR_NONE;
         if (mine) {
           if (!bound) {
@@ -768,7 +774,7 @@ __device__ inline LaneResult vm_loop(const LoopProgram& f, FastInputs in, int n_
           } else {
             fuel -= cost + cost2;
             if (zero) err2 = ERR_ZERO_DIV;
-            S(sp - 1) = v;
+            tos = v;
             ip = at + 2;
           }
         }
@@ -793,7 +799,10 @@ __device__ inline LaneResult vm_loop(const LoopProgram& f, FastInputs in, int n_
     switch (op) {
       case OP_PUSH_CONST: {
         const std::uint32_t lo = std::uint32_t(loop_pick(f.c0, a)), hi = std::uint32_t(loop_pick(f.c1, a));
-        if (run) S(push) = std::int64_t(std::uint64_t(hi) << 32 | lo);
+        if (run) {
+          if (push > 0) S(push - 1) = tos;
+          tos = std::int64_t(std::uint64_t(hi) << 32 | lo);
+        }
         break;
       }
       case OP_LOAD: {
@@ -808,46 +817,55 @@ __device__ inline LaneResult vm_loop(const LoopProgram& f, FastInputs in, int n_
           bound = (assigned >> k) & 1u;
         }
         if (run && !bound) err = ERR_NAME;
-        if (run) S(push) = v;
+        if (run) {
+          if (push > 0) S(push - 1) = tos;
+          tos = v;
+        }
         break;
       }
       case OP_STORE: {
+        const std::int64_t deeper = S(below);  // the top after the pop
         if (a < n_inputs) {  // an input target (uniform branch)
-          if (run) in.set(a, S(top));
+          if (run) { in.set(a, tos); tos = deeper; }
           break;
         }
         const int k = a - n_inputs < kLoopLocals ? a - n_inputs : 0;
-        if (run) { L(k) = S(top); assigned |= 1u << k; }
+        if (run) { L(k) = tos; assigned |= 1u << k; tos = deeper; }
         break;
       }
       case OP_JMP: next = a; break;
-      case OP_JMP_IF_FALSE: next = S(top) == 0 ? a : ip + 1; break;
-      case OP_JMP_IF_TRUE: next = S(top) != 0 ? a : ip + 1; break;
+      case OP_JMP_IF_FALSE: case OP_JMP_IF_TRUE: {
+        const std::int64_t deeper = S(below);  // the condition is popped
+        next = (tos != 0) == (op == OP_JMP_IF_TRUE) ? a : ip + 1;
+        if (run) tos = deeper;
+        break;
+      }
       case OP_CHECK_INT: break;
       // argc 1 marks a NEG of a Bool (the for_range bound check): TypeError.
       case OP_NEG:
         if (run && argc == 1) err = ERR_TYPE;
-        else if (run) S(top) = wrap_neg(S(top));
+        else if (run) tos = wrap_neg(tos);
         break;
-      case OP_NOT: if (run) S(top) = std::int64_t(S(top) == 0); break;
+      case OP_NOT: if (run) tos = std::int64_t(tos == 0); break;
       case OP_ADD: case OP_SUB: case OP_MUL: case OP_MOD: case OP_LT: case OP_LE: case OP_GT: case OP_GE: case OP_EQ:
       case OP_NE: {
         bool zero = false;
-        const std::int64_t v = loop_binary(op, S(below), S(top), zero);
+        const std::int64_t v = loop_binary(op, S(below), tos, zero);
         if (run && zero) err = ERR_ZERO_DIV;
-        if (run) S(below) = v;
+        if (run) tos = v;
         break;
       }
       case OP_CALL_BUILTIN: {
+        // The last argument is the register top; the others are below it.
         const int base = sp - argc;
-        const std::int64_t x = S(base), y = S(base + 1);
+        const std::int64_t x = argc == 1 ? tos : S(base), y = argc == 2 ? tos : S(base + 1);
         std::int64_t out = 0;
         switch (a) {
           case B_ABS: out = x < 0 ? wrap_neg(x) : x; break;
           case B_MIN: out = x < y ? x : y; break;
           case B_MAX: out = x > y ? x : y; break;
           case B_CLIP: {
-            const std::int64_t hi = S(base + 2), m = x > y ? x : y;
+            const std::int64_t hi = argc == 3 ? tos : S(base + 2), m = x > y ? x : y;
             out = m < hi ? m : hi;
             break;
           }
@@ -887,7 +905,7 @@ __device__ inline LaneResult vm_loop(const LoopProgram& f, FastInputs in, int n_
           }
           default: if (run) err = ERR_NAME; break;
         }
-        if (run) S(base) = out;
+        if (run) tos = out;
         break;
       }
       case OP_RETURN: ret = true; break;
@@ -901,7 +919,7 @@ __device__ inline LaneResult vm_loop(const LoopProgram& f, FastInputs in, int n_
     if (finished) {
       r.err = err;
       if (err == ERR_NONE) {
-        const std::int64_t v = S(top);
+        const std::int64_t v = tos;
         if (result_tag == TAG_BOOL) { r.t = TAG_BOOL; r.b = v != 0; r.v = 0; }
         else { r.t = TAG_INT; r.v = v; }
       }
