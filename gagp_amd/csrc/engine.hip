@@ -132,6 +132,7 @@ struct DSearch {
   GAS double* loop_scores;  // the looping kernel's case scores: C per program slot
   GAS LoopAcc* loop_acc;    // the looping kernel's per-program tile totals (zero between evaluations)
   int stage_cases;     // the typed evaluation kernel stages the case set in LDS
+  int typed_tiles;     // typed programs go to k_eval_tiles (more than one 64-case tile, C <= kScoreCases)
 };
 
 __device__ inline CRow member_row(const DSearch& d, int m) {
@@ -735,6 +736,7 @@ __global__ __launch_bounds__(kEvalThreads) void k_eval(DSearch* all, const Block
   const BlockMap bm = map[blockIdx.x];
   const DSearch& d = all[bm.search];
   if (mode == 1 ? !search_live(d) : (!d.active || *d.error)) return;
+  if (kFast && d.typed_tiles) return;  // k_eval_tiles scores them
   extern __shared__ unsigned char smem[];
   const int wave = wave_index(), lane = threadIdx.x % 64;
   WaveStack st = stack_of(smem, kFast ? 0 : d.stack_slots, kFast ? 0 : d.temp_slots, wave, kFast ? 0 : d.input_shadow);
@@ -768,6 +770,95 @@ __global__ __launch_bounds__(kEvalThreads) void k_eval(DSearch* all, const Block
       d.e_timeouts[slot] = r.timeouts;
       d.e_fuel[slot] = r.fuel;
     }
+  }
+}
+
+// The typed straight-line VM with one block per program: the block's four
+// wavefronts take the program's 64-case tiles in turn (tile w, w + 4, ...),
+// so a program waits for a quarter of the tile chain one wavefront would
+// run. Each wavefront reads its tiles' inputs from the case arrays. The
+// case scores go to the block's LDS; the totals of the four wavefronts are
+// combined in LDS -- exactly, when every score is an integer of magnitude
+// <= 2^36 (all partial sums are integers below 2^48) -- and otherwise the
+// canonical replay reads the scores, as in eval_program. mode as k_eval.
+__global__ __launch_bounds__(kEvalThreads) void k_eval_tiles(DSearch* all, const BlockMap* map, int mode) {
+  const BlockMap bm = map[blockIdx.x];
+  const DSearch& d = all[bm.search];
+  if (mode == 1 ? !search_live(d) : (!d.active || *d.error)) return;
+  if (!d.typed_tiles) return;  // uniform per block
+  extern __shared__ unsigned char smem[];
+  double* const scores = reinterpret_cast<double*>(smem);  // [C]
+  __shared__ double part_sum[kWavesPerBlock];
+  __shared__ unsigned part_count[kWavesPerBlock][4];  // wrong, errors, timeouts, integral
+  __shared__ unsigned long long part_fuel[kWavesPerBlock];
+  const int wave = wave_index(), lane = threadIdx.x % 64;
+  const EvalParams e{d.cd, d.fuel, d.lanes, d.penalty, d.policy};
+  const int C = d.cd.C;
+  const int count = mode == 1 ? d.n_pending[1] : d.P;
+  for (int i = bm.block; i < count; i += bm.blocks) {  // one program per block, uniform
+    const int slot = mode == 1 ? d.pending[pending_at(d.B, 1, i)] : i;
+    const int pmode = d.code_jumps[slot];
+    if (mode == 0 && eval_class(pmode, C) != 1) continue;
+    const ProgView pv{d.code + std::size_t(slot) * d.code_cap, d.pool + std::size_t(slot) * d.pool_cap, d.code_len[slot],
+                      d.n_locals, pmode & 1, (pmode >> 1) & 1, d.pool_len[slot], (pmode & 4) ? TAG_BOOL : TAG_INT};
+    const FastProgram fp = fast_load(pv, lane);
+    double isum = 0;
+    unsigned w = 0, errs = 0, touts = 0;
+    unsigned long long fuel = 0;
+    bool integral = true;  // C <= kScoreCases <= 4096
+    for (int tile = wave * 64; tile < C; tile += kWavesPerBlock * 64) {
+      const int c = tile + lane;
+      const bool active = c < C;
+      const LaneResult r = vm_fast(fp, fast_inputs(e.cd, pv.n_locals, c, active), active, e.fuel, pv.result_tag);
+      if (active) {
+        bool exact = false;
+        const double sc = lane_score(e, r, expected_of(e.cd, c), exact);
+        w += !exact;
+        errs += r.err != ERR_NONE;
+        touts += r.err == ERR_TIMEOUT;
+        fuel += unsigned(r.fuel_used);
+        integral = integral && sc == double(std::int64_t(sc)) && sc >= -68719476736.0 && sc <= 68719476736.0;
+        isum += sc;
+        scores[c] = sc;
+      }
+    }
+    const double ws = wave_sum_d(isum);
+    const unsigned ww = wave_sum_u(w), we = wave_sum_u(errs), wt = wave_sum_u(touts);
+    for (int off = 32; off > 0; off >>= 1) fuel += __shfl_xor(fuel, off, 64);
+    const bool wint = __ballot(!integral) == 0;
+    if (lane == 0) {
+      part_sum[wave] = ws;
+      part_count[wave][0] = ww;
+      part_count[wave][1] = we;
+      part_count[wave][2] = wt;
+      part_count[wave][3] = wint;
+      part_fuel[wave] = fuel;
+    }
+    __syncthreads();  // every wavefront's scores and totals
+    if (wave == 0) {
+      double total = 0;
+      unsigned tw = 0, te = 0, tt = 0;
+      unsigned long long tf = 0;
+      bool all_integral = true;
+      for (int k = 0; k < kWavesPerBlock; ++k) {
+        total += part_sum[k];
+        tw += part_count[k][0];
+        te += part_count[k][1];
+        tt += part_count[k][2];
+        all_integral = all_integral && part_count[k][3] != 0;
+        tf += part_fuel[k];
+      }
+      const double fitness = all_integral ? total : canonical_from_scores(e, C, scores, lane);
+      if (lane == 0) {
+        // Ranking rounds every published fitness to a 40-bit mantissa.
+        if (mode == 1) { d.c_fit[slot] = canon40(fitness); d.c_wrong[slot] = tw; }
+        else { d.fitness[slot] = canon40(fitness); d.wrong[slot] = tw; }
+        d.e_errors[slot] = te;
+        d.e_timeouts[slot] = tt;
+        d.e_fuel[slot] = tf;
+      }
+    }
+    __syncthreads();  // before the next program reuses the scores
   }
 }
 
@@ -1759,7 +1850,7 @@ struct Batch::Impl {
       HIP_OK(hipStreamWaitEvent(stream, join_ev[i - 1], 0));
     }
   }
-  std::size_t eval_fast_lds = 0, eval_general_lds = 0;  // dynamic LDS of the evaluation kernels
+  std::size_t eval_fast_lds = 0, eval_general_lds = 0, eval_tiles_lds = 0;  // dynamic LDS of the evaluation kernels
   static constexpr std::size_t kLoopLds = std::size_t(kWavesPerBlock) * kLoopSlots * 64 * 8;  // the looping slots
   std::size_t eval_loop_lds = 0;
   std::size_t vary_lds_bytes = 0;  // phase-table copy of the staged reproduction kernel
@@ -1770,7 +1861,7 @@ struct Batch::Impl {
   struct Grid { GAS BlockMap* map = nullptr; int blocks = 0; };
   DeviceArena map_arena;
   Grid g_init, g_lower, g_eval_init, g_vary_staged, g_vary_global, g_vary_staged_w, g_vary_global_w, g_table, g_member, g_child,
-      g_eval, g_eval_general, g_eval_loop;
+      g_eval, g_eval_tiles, g_eval_general, g_eval_loop;
   // Evaluation launch shapes of the round (the autotuner's choice, or the
   // fixed default: one program per wavefront, one 64-case tile per item).
   TuneChoice tune;
@@ -1817,7 +1908,8 @@ struct Batch::Impl {
     g_table = build_grid(map_arena, live, [&](const SearchState& s) { return up(s.d.table_mask + 1, 256); });
     g_member = build_grid(map_arena, live, [&](const SearchState& s) { return up(s.P, 128); });
     g_child = build_grid(map_arena, live, [&](const SearchState& s) { return up(s.B, 128); });
-    g_eval = eval_grid(map_arena, tune.fast_programs);
+    g_eval = typed_grid(map_arena, tune.fast_programs);
+    g_eval_tiles = tiles_grid(map_arena, live, false);
     g_eval_general = eval_grid(map_arena, 1);
     g_eval_loop = loop_grid(map_arena, tune.loop_group);
   }
@@ -1826,6 +1918,16 @@ struct Batch::Impl {
   // of the looping kernel.
   Grid eval_grid(DeviceArena& arena, int per_wave) {
     return build_grid(arena, live, [&](const SearchState& s) { return std::min(4096, up(s.B, kWavesPerBlock * per_wave)); });
+  }
+  // The typed kernels: k_eval<1> for searches whose cases fit one tile,
+  // k_eval_tiles (one block per program) for the rest.
+  Grid typed_grid(DeviceArena& arena, int per_wave) {
+    return build_grid(arena, live, [&](const SearchState& s) {
+      return s.d.typed_tiles ? 0 : std::min(4096, up(s.B, kWavesPerBlock * per_wave));
+    });
+  }
+  Grid tiles_grid(DeviceArena& arena, const std::vector<int>& table, bool members) {
+    return build_grid(arena, table, [&](const SearchState& s) { return s.d.typed_tiles ? std::min(4096, members ? s.P : s.B) : 0; });
   }
   Grid loop_grid(DeviceArena& arena, int group) {
     return build_grid(arena, live, [&](const SearchState& s) { return s.may_jump ? loop_blocks(s.d.cd.C, s.B, group) : 0; });
@@ -1843,10 +1945,11 @@ struct Batch::Impl {
     s.d.general_scores = s.d.cd.C <= kScoreCases && stacks + score_buf <= kLdsLimit;
     const std::size_t cases = staged_case_bytes(s.d.cd.C, staged_inputs_of(s.d.n_locals, s.d.cd.n_inputs));
     s.d.stage_cases = s.d.cd.C <= kScoreCases && cases <= 48 * 1024 && score_buf + cases + 16 <= kLdsLimit;
+    s.d.typed_tiles = s.d.cd.C > 64 && s.d.cd.C <= kScoreCases;
   }
   // LDS sizes and kernel attributes for a table of searches.
   void size_kernels(const std::vector<int>& table) {
-    eval_fast_lds = eval_general_lds = vary_lds_bytes = place_lds_bytes = place_hbm_bytes = 0;
+    eval_fast_lds = eval_general_lds = eval_tiles_lds = vary_lds_bytes = place_lds_bytes = place_hbm_bytes = 0;
     eval_loop_lds = kLoopLds;
     vary_global = any_generational = false;
     max_P = max_B = max_slots = max_table = 0;
@@ -1860,6 +1963,7 @@ struct Batch::Impl {
         eval_fast_lds = std::max(eval_fast_lds, score_buf + cases);
         eval_loop_lds = kLoopLds;
       }
+      if (x.d.typed_tiles) eval_tiles_lds = std::max(eval_tiles_lds, std::size_t(8) * x.d.cd.C);
       if (x.d.stage_tables) vary_lds_bytes = std::max(vary_lds_bytes, phase_tables_bytes(x.d.g));
       else vary_global = true;
       if (x.d.place_lds) place_lds_bytes = std::max(place_lds_bytes, std::size_t(16) * x.P);
@@ -1877,6 +1981,7 @@ struct Batch::Impl {
     attr(reinterpret_cast<const void*>(&k_vary<true, true>), vary_lds_bytes);
     attr(reinterpret_cast<const void*>(&k_eval<0>), eval_general_lds);
     attr(reinterpret_cast<const void*>(&k_eval<1>), eval_fast_lds);
+    attr(reinterpret_cast<const void*>(&k_eval_tiles), eval_tiles_lds);
     attr(reinterpret_cast<const void*>(&k_eval_loop<false>), eval_loop_lds);
     attr(reinterpret_cast<const void*>(&k_eval_loop<true>), eval_loop_lds);
     attr(reinterpret_cast<const void*>(&k_place<true>), place_lds_bytes);
@@ -1905,6 +2010,11 @@ struct Batch::Impl {
                  [&] { hipLaunchKernelGGL(k_lower_members<true>, dim3(gl.blocks), dim3(64 * kVaryWaves), 0, stream, t, gl.map); });
     timer.time("k_eval_init", stream,
                [&] { hipLaunchKernelGGL(k_eval<1>, dim3(ge.blocks), dim3(kEvalThreads), eval_fast_lds, stream, t, ge.map, 0); });
+    const Grid gt = tiles_grid(tmp, ids, true);
+    if (gt.blocks)
+      timer.time("k_eval_init_tiles", stream, [&] {
+        hipLaunchKernelGGL(k_eval_tiles, dim3(gt.blocks), dim3(kEvalThreads), eval_tiles_lds, stream, t, gt.map, 0);
+      });
     if (any_jumps(ids))
       timer.time("k_eval_init_loop", stream, [&] {
         const Grid gp = build_grid(tmp, ids, [&](const SearchState& x) { return loop_blocks(x.d.cd.C, x.P); });
@@ -2035,7 +2145,7 @@ struct Batch::Impl {
     };
     choice.fast_programs = pick("typed", {1, 2, 4}, [&](int k) {
       tmp.release();
-      const Grid g = eval_grid(tmp, k);
+      const Grid g = typed_grid(tmp, k);
       return g.blocks ? measure([&] {
         hipLaunchKernelGGL(k_eval<1>, dim3(g.blocks), dim3(kEvalThreads), eval_fast_lds, stream, d_all, g.map, 1);
       }) : 0.f;
@@ -2148,7 +2258,10 @@ struct Batch::Impl {
                 else if (loop_kernel)
                   launch_on(s, "k_eval_loop", g_eval_loop, k_eval_loop<true>, kEvalThreads, eval_loop_lds, 1, tune.loop_group);
               },
-              [&](hipStream_t s) { launch_on(s, "k_eval", g_eval, k_eval<1>, kEvalThreads, eval_fast_lds, 1); },
+              [&](hipStream_t s) {
+                launch_on(s, "k_eval", g_eval, k_eval<1>, kEvalThreads, eval_fast_lds, 1);
+                launch_on(s, "k_eval_tiles", g_eval_tiles, k_eval_tiles, kEvalThreads, eval_tiles_lds, 1);
+              },
               [&](hipStream_t s) { launch_on(s, "k_eval_general", g_eval_general, k_eval<0>, kEvalThreads, eval_general_lds, 1); }});
   }
   void enqueue_back() {
