@@ -310,11 +310,14 @@ __device__ inline int eval_class(int code_flags, int C) {
 // Pending list of a round: general children from the front of [0, B), the
 // straight-line ones from its back, the looping ones in [B, 2B).
 __device__ inline int pending_at(int B, int kind, int i) { return kind == 0 ? i : kind == 1 ? B - 1 - i : B + i; }
-__global__ void k_reuse(DSearch* all, const BlockMap* map) {
+// One wavefront per child: the table probe is uniform and the row
+// comparison runs across the lanes (rows_equal's test).
+__global__ __launch_bounds__(64 * kVaryWaves) void k_reuse(DSearch* all, const BlockMap* map) {
   const BlockMap bm = map[blockIdx.x];
   const DSearch& d = all[bm.search];
-  const int c = bm.block * blockDim.x + threadIdx.x;
+  const int c = __builtin_amdgcn_readfirstlane(bm.block * kVaryWaves + wave_index());
   if (!search_live(d) || c >= d.B) return;
+  const int lane = wlane();
   int r = d.stats[c].reusable_parent;
   const CRow child = child_row(d, c);
   if (r < 0) {
@@ -326,20 +329,44 @@ __global__ void k_reuse(DSearch* all, const BlockMap* map) {
       if (k == 0) break;
       if (k != key) continue;
       const int m = d.table[slot];
-      if (d.member_hash[m] == h && rows_equal(member_row(d, m), child)) r = m;
+      if (d.member_hash[m] == h) {
+        const CRow member = member_row(d, m);
+        bool differ = member.n != child.n || member.nc != child.nc;
+        if (!differ) {
+          bool mine = false;
+          for (int i = lane; i < child.n; i += 64) mine = mine || member.origin[i] != child.origin[i];
+          for (int i = lane; i < child.nc; i += 64) {
+            const Val x = member.cval[i], y = child.cval[i];
+            mine = mine || x.tag != y.tag || (x.tag == TAG_BOOL ? x.b != y.b : x.i != y.i);
+          }
+          differ = __ballot(mine) != 0;
+        }
+        if (!differ) r = m;
+      }
       break;
     }
   }
-  d.reuse[c] = r;
   if (r >= 0) {
-    // Results are fixed before any placement of the round.
-    d.c_fit[c] = d.fitness[r];
-    d.c_wrong[c] = d.wrong[r];
-    d.c_nodes[c] = d.nodes[r];
+    if (lane == 0) {
+      d.reuse[c] = r;
+      // Results are fixed before any placement of the round.
+      d.c_fit[c] = d.fitness[r];
+      d.c_wrong[c] = d.wrong[r];
+      d.c_nodes[c] = d.nodes[r];
+    }
+    return;
+  }
+  unsigned n;
+  if (d.grammar_sites) {
+    unsigned mine = 0;
+    for (int i = lane; i < child.n; i += 64) mine += child.site[i] >= 0;
+    for (int off = 32; off > 0; off >>= 1) mine += __shfl_xor(mine, off, 64);
+    n = mine;
   } else {
-    unsigned n = d.grammar_sites ? 0u : unsigned(d.fixed_nodes);
-    if (d.grammar_sites) { for (int i = 0; i < child.n; ++i) n += child.site[i] >= 0; }
-    else n += unsigned(child.n);
+    n = unsigned(d.fixed_nodes) + unsigned(child.n);
+  }
+  if (lane == 0) {
+    d.reuse[c] = r;
     d.c_nodes[c] = n;
     const int k = eval_class(d.code_jumps[c], d.cd.C);
     d.pending[pending_at(d.B, k, atomicAdd(d.n_pending + k, 1))] = c;
@@ -1352,19 +1379,27 @@ __global__ void k_gen_end(DSearch* all) {
   atomicAdd(d.round, 1);
 }
 
-__global__ void k_commit(DSearch* all, const BlockMap* map) {
+// One wavefront per member: a placed child's row is copied across the lanes.
+__global__ __launch_bounds__(64 * kVaryWaves) void k_commit(DSearch* all, const BlockMap* map) {
   const BlockMap bm = map[blockIdx.x];
   const DSearch& d = all[bm.search];
-  const int m = bm.block * blockDim.x + threadIdx.x;
+  const int m = __builtin_amdgcn_readfirstlane(bm.block * kVaryWaves + wave_index());
   if (!d.active || m >= d.P) return;
   const int c = d.placed[m];
   if (c < 0) return;
   const CRow src = child_row(d, c);
   const std::size_t o = std::size_t(m) * d.R;
-  Row dst{d.cur_origin + o, d.cur_site + o, d.cur_cval + o, 0, 0};
-  copy_row(src, dst, d.cur_n + m, d.cur_nc + m);
-  d.member_hash[m] = d.child_hash[c];
-  d.placed[m] = -1;
+  GAS int* const origin = d.cur_origin + o;
+  GAS int* const site = d.cur_site + o;
+  GAS Val* const cval = d.cur_cval + o;
+  for (int i = wlane(); i < src.n; i += 64) { origin[i] = src.origin[i]; site[i] = src.site[i]; }
+  for (int i = wlane(); i < src.nc; i += 64) cval[i] = src.cval[i];
+  if (wlane() == 0) {
+    d.cur_n[m] = src.n;
+    d.cur_nc[m] = src.nc;
+    d.member_hash[m] = d.child_hash[c];
+    d.placed[m] = -1;
+  }
 }
 
 // ---- evaluation session kernel (external bytecode) ----
@@ -1860,8 +1895,8 @@ struct Batch::Impl {
   // Flat grids: every search contributes exactly its own blocks.
   struct Grid { GAS BlockMap* map = nullptr; int blocks = 0; };
   DeviceArena map_arena;
-  Grid g_init, g_lower, g_eval_init, g_vary_staged, g_vary_global, g_vary_staged_w, g_vary_global_w, g_table, g_member, g_child,
-      g_eval, g_eval_tiles, g_eval_general, g_eval_loop;
+  Grid g_init, g_lower, g_eval_init, g_vary_staged, g_vary_global, g_vary_staged_w, g_vary_global_w, g_table, g_member,
+      g_eval, g_eval_tiles, g_eval_general, g_eval_loop, g_child_w, g_member_w;
   // Evaluation launch shapes of the round (the autotuner's choice, or the
   // fixed default: one program per wavefront, one 64-case tile per item).
   TuneChoice tune;
@@ -1907,7 +1942,9 @@ struct Batch::Impl {
     g_vary_global_w = vary(false, true);
     g_table = build_grid(map_arena, live, [&](const SearchState& s) { return up(s.d.table_mask + 1, 256); });
     g_member = build_grid(map_arena, live, [&](const SearchState& s) { return up(s.P, 128); });
-    g_child = build_grid(map_arena, live, [&](const SearchState& s) { return up(s.B, 128); });
+    // one wavefront per child / member
+    g_child_w = build_grid(map_arena, live, [&](const SearchState& s) { return up(s.B, kVaryWaves); });
+    g_member_w = build_grid(map_arena, live, [&](const SearchState& s) { return up(s.P, kVaryWaves); });
     g_eval = typed_grid(map_arena, tune.fast_programs);
     g_eval_tiles = tiles_grid(map_arena, live, false);
     g_eval_general = eval_grid(map_arena, 1);
@@ -2248,7 +2285,7 @@ struct Batch::Impl {
                 launch_on(s, "k_clear", g_table, k_clear, 256, 0);
                 launch_on(s, "k_index", g_member, k_index, 128, 0);
               }});
-    launch("k_reuse", g_child, k_reuse, 128, 0);
+    launch("k_reuse", g_child_w, k_reuse, 64 * kVaryWaves, 0);
   }
   void enqueue_eval() {
     // The three evaluation kernels score disjoint pending programs.
@@ -2272,7 +2309,7 @@ struct Batch::Impl {
     if (place_hbm_bytes)
       timer.time("k_place_hbm", stream, [&] { hipLaunchKernelGGL(k_place<false>, dim3(n), dim3(kPlaceThreads), place_hbm_bytes, stream, d_all); });
     if (any_generational) launch("k_gen_place", g_member, k_gen_place, 128, 0);
-    launch("k_commit", g_member, k_commit, 128, 0);
+    launch("k_commit", g_member_w, k_commit, 64 * kVaryWaves, 0);
     if (any_generational) timer.time("k_gen_end", stream, [&] { hipLaunchKernelGGL(k_gen_end, dim3(n), dim3(64), 0, stream, d_all); });
   }
 };
