@@ -77,6 +77,8 @@ def lib():
     L.gagp_batch_create.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
     L.gagp_batch_add.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32,
                                  ctypes.c_int32, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int32)]
+    L.gagp_batch_add_init.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32,
+                                      ctypes.c_int32, ctypes.c_uint64, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int32)]
     L.gagp_batch_run.argtypes = [ctypes.c_void_p, ctypes.c_int32]
     L.gagp_batch_planned_bytes.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_uint64)]
     L.gagp_batch_result_json.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_char_p, ctypes.c_size_t,
@@ -221,10 +223,13 @@ class Batch:
         self.h = ctypes.c_void_p()
         _check(lib().gagp_batch_create(ctypes.byref(self.h)))
 
-    def add(self, task, search=None, population=0, generations=0, seed=None):
+    def add(self, task, search=None, population=0, generations=0, seed=None, init=None):
+        """Admits one search; `init` is a population document (`gagp run
+        --init`) whose first programs are the initial population."""
         idx = ctypes.c_int32(-1)
-        _check(lib().gagp_batch_add(self.h, task.encode(), search.encode() if search else None, population, generations,
-                                    int(seed is not None), seed or 0, ctypes.byref(idx)))
+        _check(lib().gagp_batch_add_init(self.h, task.encode(), search.encode() if search else None, population,
+                                         generations, int(seed is not None), seed or 0,
+                                         init.encode() if init else None, ctypes.byref(idx)))
         return idx.value
 
     def planned_bytes(self, index):

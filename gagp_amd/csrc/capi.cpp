@@ -150,8 +150,9 @@ int gagp_batch_create(gagp_batch** out) {
   });
 }
 
-int gagp_batch_add(gagp_batch* b, const char* task_path, const char* search_path, int32_t population, int32_t generations,
-                   int32_t seed_is_set, uint64_t seed, int32_t* out_index) {
+int gagp_batch_add_init(gagp_batch* b, const char* task_path, const char* search_path, int32_t population,
+                        int32_t generations, int32_t seed_is_set, uint64_t seed, const char* init_path,
+                        int32_t* out_index) {
   return guarded([&] {
     if (!b || !task_path) throw std::invalid_argument("invalid batch arguments");
     if (b->prepared) throw std::invalid_argument("batch already prepared");
@@ -161,9 +162,15 @@ int gagp_batch_add(gagp_batch* b, const char* task_path, const char* search_path
     if (population > 0) spec.population = population;
     if (generations > 0) spec.generations = generations;
     if (seed_is_set) spec.seed = seed;
+    if (init_path) spec.init_path = init_path;
     const int idx = b->b.add(spec);
     if (out_index) *out_index = idx;
   });
+}
+
+int gagp_batch_add(gagp_batch* b, const char* task_path, const char* search_path, int32_t population, int32_t generations,
+                   int32_t seed_is_set, uint64_t seed, int32_t* out_index) {
+  return gagp_batch_add_init(b, task_path, search_path, population, generations, seed_is_set, seed, nullptr, out_index);
 }
 
 int gagp_batch_planned_bytes(gagp_batch* b, int32_t index, uint64_t* out) {

@@ -28,6 +28,8 @@ const char* kRunUsage =
     "  --population N      population size (default 1024, or the search file's)\n"
     "  --generations N     generations (default 64, or the search file's)\n"
     "  --seed N            random seed (default 0)\n"
+    "  --init FILE         initial population from `gagp run --save-all` (its first\n"
+    "                      N programs, made for the task's grammar)\n"
     "  --engine gpu        the device engine (the only engine of this command)\n"
     "  --out DIR           write DIR/result.json\n"
     "  --save-all          also write DIR/population.json and DIR/history.json\n"
@@ -69,7 +71,7 @@ int run_command(const std::vector<std::string>& args) {
     else if (a == "--generations") spec.generations = integer(1);
     else if (a == "--save-all") save_all = true;
     else if (a == "--quiet") quiet = true;
-    else if (a == "--init") throw Unsupported("--init (imported populations need the native membership reconstruction)");
+    else if (a == "--init") spec.init_path = value();
     else if (!a.empty() && a[0] == '-') usage_error("unknown option " + a);
     else if (spec.task_path.empty()) spec.task_path = a;
     else usage_error("unexpected argument " + a);

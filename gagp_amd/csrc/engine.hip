@@ -27,6 +27,7 @@
 #include <stdexcept>
 
 #include "engine.hpp"
+#include "population.hpp"
 #include "phase_wave.hpp"
 #include "vm.hpp"
 
@@ -128,6 +129,7 @@ struct DSearch {
   int temp_slots;      // hidden-local slots per lane of the general VM (temporaries, grammar locals, input shadows)
   int input_shadow;    // the last input_shadow of them shadow stored inputs (statement input targets), else 0
   int walks;           // the grammar has traversals: lowering runs the kWalks instantiations
+  int imported;        // the initial population is uploaded, not generated
   int general_scores;  // the general evaluation kernel has an LDS score buffer for this search
   GAS double* loop_scores;  // the looping kernel's case scores: C per program slot
   GAS LoopAcc* loop_acc;    // the looping kernel's per-program tile totals (zero between evaluations)
@@ -161,7 +163,7 @@ __global__ __launch_bounds__(64 * kVaryWaves) void k_init_pop(DSearch* all, cons
   const BlockMap bm = map[blockIdx.x];
   const DSearch& d = all[bm.search];
   const int m = bm.block * kVaryWaves + wave_index();
-  if (!d.active || m >= d.P) return;
+  if (!d.active || d.imported || m >= d.P) return;
   const int sd = d.site_depth;
   const int ramp = sd < 2 ? sd : 2 + m % (sd - 1);
   const std::size_t o = std::size_t(m) * d.R;
@@ -1610,8 +1612,14 @@ struct SearchState {
   bool may_jump = false;   // programs may go to the looping path: if/and/or, traversals, statements, lets
   bool statement = false;  // a Program entry: statement programs with grammar locals
   // The export name table: the names the first initial member uses, in
-  // declaration order (the layout prototype of phase_population_cpu.cpp:195).
+  // declaration order (the layout prototype of phase_population_cpu.cpp:195),
+  // or an imported first program's own table; and the first binder id of an
+  // exported program (-1: the generated population's numbering).
   std::vector<std::string> layout_names;
+  int binder_base = -1;
+  // Rows of an imported initial population, held until they are uploaded.
+  ImportedPopulation imported;
+  bool has_import = false;
   DeviceArena arena;
   DSearch d{};
   double init_ms = 0;
@@ -1635,7 +1643,8 @@ struct SearchState {
     if (!cfg.resident) throw Unsupported("native_ast representation (host reproduction)");
     if (cfg.schedule == Schedule::Generational && cfg.elitism > kMaxElites)
       throw Unsupported("generational elitism above " + std::to_string(kMaxElites));
-    if (cfg.site_depth < 1) throw Unsupported("site_depth 0 (initial population from the native generator)");
+    if (cfg.site_depth < 1 && s.init_path.empty())
+      throw Unsupported("site_depth 0 (initial population from the native generator)");
     if (cfg.population < 2 || cfg.population > 16384) throw Unsupported("population outside 2..16384");
     if (cfg.elitism < 0 || cfg.elitism >= cfg.population) throw std::invalid_argument("elite_count must be >= 0 and < population_size");
     P = cfg.population;
@@ -1672,6 +1681,12 @@ struct SearchState {
     for (const auto& x : phase.exprs) arity = std::max(arity, x.node.arity);
     may_jump = may_jump || statement || phase.has_let || R + 1 > kFastCode ||
                (arity - 1) * int(layout.slot_max_depth) + 1 > kFastStack;
+    if (!s.init_path.empty()) {
+      imported = load_population(s.init_path, grammar, phase, unsigned(cfg.site_depth), R, std::size_t(P));
+      has_import = true;
+      layout_names = imported.layout_names;
+      binder_base = imported.binder_base;
+    }
   }
 
   // Allocates every device array of the search in its arena.
@@ -1701,6 +1716,7 @@ struct SearchState {
     x.stack_slots = std::min(kMaxStack, 2 * int(layout.slot_max_depth) + 2);
     // Statement programs lower through the hidden-locals instantiation too.
     x.walks = walks > 0 || statement || phase.has_let;
+    x.imported = has_import ? 1 : 0;
     x.temp_slots = statement || phase.has_let ? std::min(64 - int(train.input_names.size()), int(hidden_local_bound(grammar)))
                    : walks  ? std::max(0, std::min(64 - int(train.input_names.size()), 9 * std::min(walks, 8)))
                             : 0;
@@ -2038,6 +2054,39 @@ struct Batch::Impl {
     const Grid gi = build_grid(tmp, ids, [&](const SearchState& x) { return up(x.P, kVaryWaves); });
     const Grid gl = build_grid(tmp, ids, [&](const SearchState& x) { return up(x.P, kVaryWaves); });
     const Grid ge = build_grid(tmp, ids, [&](const SearchState& x) { return std::min(4096, up(x.P, kWavesPerBlock)); });
+    // Imported populations: their rows go up in place of k_init_pop's
+    // (which skips those searches); the staging vectors live until the
+    // stream is synchronized below.
+    std::vector<std::vector<int>> up_i;
+    std::vector<std::vector<Val>> up_v;
+    for (int id : ids) {
+      SearchState& x = *searches[std::size_t(id)];
+      if (!x.has_import) continue;
+      const std::size_t P = std::size_t(x.P), R = std::size_t(x.R);
+      if (x.imported.rows.size() != P) throw std::logic_error("imported population size");
+      std::vector<int> origin(P * R, 0), site(P * R, -1), n(P), nc(P), placed(P, -1);
+      std::vector<Val> cval(P * R, Val{});
+      for (std::size_t m = 0; m < P; ++m) {
+        const PhaseRow& r = x.imported.rows[m];
+        if (r.origin.size() > R || r.cval.size() > R) throw std::logic_error("imported row capacity");
+        std::copy(r.origin.begin(), r.origin.end(), origin.begin() + std::ptrdiff_t(m * R));
+        std::copy(r.site.begin(), r.site.end(), site.begin() + std::ptrdiff_t(m * R));
+        std::copy(r.cval.begin(), r.cval.end(), cval.begin() + std::ptrdiff_t(m * R));
+        n[m] = int(r.origin.size());
+        nc[m] = int(r.cval.size());
+      }
+      const auto put = [&](GAS void* to, const void* from, std::size_t bytes) {
+        HIP_OK(hipMemcpyAsync((void*)to, from, bytes, hipMemcpyHostToDevice, stream));
+      };
+      put(x.d.cur_origin, origin.data(), origin.size() * sizeof(int));
+      put(x.d.cur_site, site.data(), site.size() * sizeof(int));
+      put(x.d.cur_cval, cval.data(), cval.size() * sizeof(Val));
+      put(x.d.cur_n, n.data(), n.size() * sizeof(int));
+      put(x.d.cur_nc, nc.data(), nc.size() * sizeof(int));
+      put(x.d.placed, placed.data(), placed.size() * sizeof(int));
+      for (auto* v : {&origin, &site, &n, &nc, &placed}) up_i.push_back(std::move(*v));
+      up_v.push_back(std::move(cval));
+    }
     timer.time("k_init_pop", stream,
                [&] { hipLaunchKernelGGL(k_init_pop, dim3(gi.blocks), dim3(64 * kVaryWaves), 0, stream, t, gi.map); });
     timer.time("k_lower_members", stream,
@@ -2068,6 +2117,11 @@ struct Batch::Impl {
     for (int id : ids) {
       SearchState& x = *searches[std::size_t(id)];
       if (const int e = x.device_error()) throw std::runtime_error("device initialization error " + std::to_string(e));
+      if (x.has_import) {
+        x.imported.rows.clear();
+        x.imported.rows.shrink_to_fit();
+        continue;  // the first imported program's names are the layout's
+      }
       const int n0 = x.down(x.d.cur_n, 1)[0];
       x.layout_names = names_used(x.grammar, x.down(x.d.cur_origin, std::size_t(n0)));
     }
@@ -2539,7 +2593,7 @@ double Batch::init_ms() const { return impl_->init_ms; }
 namespace {
 Ast export_row(const SearchState& s, const int* origin, const Val* cval, int n, int nc) {
   return export_whole_program(s.grammar, s.phase, s.layout_names, std::vector<int>(origin, origin + n),
-                              std::vector<Val>(cval, cval + nc));
+                              std::vector<Val>(cval, cval + nc), s.binder_base);
 }
 
 // The run's success criterion on held-out cases (cpp/src/cli/run.cpp:179-239).
@@ -2680,7 +2734,7 @@ SearchResult Batch::result(int index, bool save_all) {
   // Best member (the strict order's front after the last round).
   const std::vector<int> bo = s.down(d.cur_origin + std::size_t(best) * s.R, std::size_t(cur_n[best]));
   const std::vector<Val> bc = s.down(d.cur_cval + std::size_t(best) * s.R, std::size_t(cur_nc[best]));
-  const Ast best_ast = export_whole_program(s.grammar, s.phase, s.layout_names, bo, bc);
+  const Ast best_ast = export_whole_program(s.grammar, s.phase, s.layout_names, bo, bc, s.binder_base);
   json::Value bj = json::make_object();
   bj.obj["fitness"] = json::make_number(fitness[best]);
   bj.obj["wrong"] = json::make_number(wrong[best]);

@@ -25,6 +25,9 @@ struct SearchSpec {
   // Stop at the first generation boundary whose best member has no wrong
   // train case; the result is that of a run with that many generations.
   bool early_exit = false;
+  // Initial population: the first `population` programs of a
+  // `gagp run --save-all` population document (empty: generated).
+  std::string init_path;
 };
 
 struct HistoryRow {

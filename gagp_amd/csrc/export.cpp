@@ -37,7 +37,7 @@ std::vector<std::string> names_used(const Grammar& g, const std::vector<int>& or
 }
 
 Ast export_whole_program(const Grammar& g, const PhaseGrammar& phase, const std::vector<std::string>& layout_names,
-                         const std::vector<int>& origins, const std::vector<Val>& consts) {
+                         const std::vector<int>& origins, const std::vector<Val>& consts, int binder_base) {
   Ast ast;
   const bool envelope = g.nts[g.entry].cat == Cat::Expression;
   if (envelope) {
@@ -51,9 +51,10 @@ Ast export_whole_program(const Grammar& g, const PhaseGrammar& phase, const std:
   std::vector<std::string> table = layout_names;
   std::vector<int> name_of_node(origins.size(), -1);
   std::size_t next_const = 0, j = 0;
-  // Binder ids follow those of the first member and of the prototype it was
-  // exported against (phase_layout.hpp:334-342): twice the fixed count.
-  int next_binder = 2 * phase.binder_ids;
+  // Binder ids follow those of the layout prototype (phase_layout.hpp:334-342):
+  // a generated population's first member holds ids above those of the
+  // prototype it was exported against, twice the fixed count in all.
+  int next_binder = binder_base >= 0 ? binder_base : 2 * phase.binder_ids;
   // Prefix-order emission; `env` maps binder numbers to public ids.
   std::function<void(const std::vector<int>&)> emit = [&](const std::vector<int>& env) {
     if (j >= origins.size()) throw std::runtime_error("invalid gene shape");

@@ -47,9 +47,12 @@ std::vector<std::string> names_used(const Grammar& g, const std::vector<int>& or
 // program lists those it uses in that order, then its other names in
 // first-use order (phase_layout.hpp name_index + pack.cpp compaction). An
 // Expression entry is wrapped in the PROGRAM, BLOCK_CONS, RETURN ... BLOCK_NIL
-// envelope; a Program entry is the phase itself.
+// envelope; a Program entry is the phase itself. `binder_base` is the first
+// binder id the program's lets and traversals take: one above every id of the
+// layout prototype, which for a generated population is twice the grammar's
+// binder count (-1 selects that).
 Ast export_whole_program(const Grammar& g, const PhaseGrammar& phase, const std::vector<std::string>& layout_names,
-                         const std::vector<int>& origins, const std::vector<Val>& consts);
+                         const std::vector<int>& origins, const std::vector<Val>& consts, int binder_base = -1);
 json::Value ast_json(const Ast& ast);
 std::string program_text(const Ast& ast);
 

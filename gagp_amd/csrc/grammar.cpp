@@ -514,6 +514,10 @@ std::string read_file(const std::string& path) {
 
 const char* node_serialized_name(int kind) { return node_info(kind).name; }
 int node_prefix_arity(int kind) { return node_info(kind).arity; }
+bool node_known(int kind) {
+  for (const auto& n : kNodes) if (n.kind == kind) return true;
+  return false;
+}
 bool node_i0_is_name(int kind) { return node_info(kind).i0 == 1; }
 bool node_i0_is_constant(int kind) { return node_info(kind).i0 == 2; }
 

@@ -41,6 +41,7 @@ enum NodeKind : int {
 };
 const char* node_serialized_name(int kind);
 int node_prefix_arity(int kind);
+bool node_known(int kind);  // a kind of the table above
 bool node_i0_is_name(int kind);
 bool node_i0_is_constant(int kind);
 

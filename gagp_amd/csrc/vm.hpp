@@ -794,7 +794,7 @@ __device__ inline LaneResult vm_loop(const LoopProgram& f, FastInputs in, int n_
     int err = starve ? ERR_TIMEOUT : ERR_NONE;
     bool ret = false;
     int next = ip + 1;
-    const int top = sp - 1, below = sp - 2, push = sp;
+    const int below = sp - 2, push = sp;
     // Binary operations read both operands before any store.
     switch (op) {
       case OP_PUSH_CONST: {

@@ -142,6 +142,18 @@ int gagp_batch_create(gagp_batch** out);
 int gagp_batch_add(gagp_batch* b, const char* task_path, const char* search_path, int32_t population, int32_t generations,
                    int32_t seed_is_set, uint64_t seed, int32_t* out_index);
 
+/* As gagp_batch_add, with the initial population read from init_path: the
+ * first `population` programs of a `gagp run --save-all` population.json
+ * made for the task's grammar, each derived from the grammar and uploaded in
+ * place of the generated population (GAGP_ERR_INVALID when a program is not
+ * a member of the grammar, the grammar hash differs or the document has too
+ * few programs; GAGP_ERR_UNSUPPORTED for grammar-population-v2 replays).
+ * Reference: `gagp run --init`, cpp/src/cli/run.cpp:122,143-173,305-307;
+ * import_native_phase_layout, cpp/src/evolution/repro/gpu/phase_layout.hpp:92-260. */
+int gagp_batch_add_init(gagp_batch* b, const char* task_path, const char* search_path, int32_t population,
+                        int32_t generations, int32_t seed_is_set, uint64_t seed, const char* init_path,
+                        int32_t* out_index);
+
 /* Device bytes search `index` will hold once allocated (its arena's
  * chunks: population and child rows, bytecode, cases, history, keys),
  * computed on the host from its documents without allocating. This is the

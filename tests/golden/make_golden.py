@@ -82,6 +82,22 @@ RUNS = {
     "let_sum_p256_g8_s1": ("let-sum", 256, 8, 1, True),
     "let_sum_p512_g12_s4": ("let-sum", 512, 12, 4, False),
 }
+# Initial populations read back (`gagp run --init`): the first P programs of
+# a saved population.json of the same task, over straight-line, traversal,
+# let and statement grammars (input targets too), a generational schedule and
+# site_depth 0, where the search grows no initial population of its own.
+# name: (task dir, source run, population, generations, seed, save_all[, search])
+INIT_RUNS = {
+    "init_polynomial_p512_g16_s3": ("polynomial", "polynomial_p1024_g64_s0", 512, 16, 3, True),
+    "init_poly_c1_p1024_g12_s5": ("poly_c1", "poly_c1_p1024_g64_s0", 1024, 12, 5, False),
+    "init_polynomial_sd0_p512_g16_s3": ("polynomial", "polynomial_p1024_g64_s0", 512, 16, 3, False, "site_depth0.json"),
+    "init_poly_c1_gen_e3r_p512_g12_s4": ("poly_c1", "poly_c1_gen_e3r_p512_g24_s9", 512, 12, 4, False,
+                                         "generational_e3_reservoir.json"),
+    "init_list_sum_even_p512_g8_s2": ("list_sum_even", "list_sum_even_p512_g10_s5", 512, 8, 2, True),
+    "init_let_sum_p256_g8_s3": ("let-sum", "let_sum_p256_g8_s1", 256, 8, 3, True),
+    "init_stmt_sum_even_p400_g8_s1": ("stmt_sum_even", "stmt_sum_even_p512_g10_s5", 400, 8, 1, True),
+    "init_stmt_input_p300_g8_s2": ("stmt_input", "stmt_input_p300_g10_s3", 300, 8, 2, False),
+}
 # Non-integral fitness: discounted proximity (-e/(1+e) per case), a
 # fractional case cap and the exact metric, over straight-line, general-VM,
 # traversal and statement programs, including case counts above 64 (several
@@ -222,11 +238,13 @@ def build_driver(tmp):
     return driver
 
 
-def run_reference(task, population, generations, seed, out, save_all, search=None):
+def run_reference(task, population, generations, seed, out, save_all, search=None, init=None):
     cmd = [REF, "run", task, "--engine", "cpu", "--population", str(population), "--generations", str(generations),
            "--seed", str(seed), "--out", out, "--quiet"]
     if search:
         cmd += ["--search", search]
+    if init:
+        cmd += ["--init", init]
     if save_all:
         cmd.append("--save-all")
     subprocess.run(cmd, check=True)
@@ -264,6 +282,22 @@ def main():
         run_reference(task, population, generations, seed, out, save_all, search)
         manifest[name] = {"task": os.path.relpath(task, GOLDEN), "population": population, "generations": generations,
                           "seed": seed, "save_all": save_all}
+        if search:
+            manifest[name]["search"] = os.path.relpath(search, GOLDEN)
+        print("golden", name)
+    # after the runs whose population documents they read
+    for name, spec in INIT_RUNS.items():
+        if args.only and not re.search(args.only, name):
+            continue
+        task_dir, source, population, generations, seed, save_all = spec[:6]
+        search = os.path.join(GOLDEN, "search", spec[6]) if len(spec) > 6 else None
+        init = os.path.join(GOLDEN, "runs", source, "population.json")
+        out = os.path.join(GOLDEN, "runs", name)
+        os.makedirs(out, exist_ok=True)
+        task = os.path.join(TASKS, task_dir, "task.json")
+        run_reference(task, population, generations, seed, out, save_all, search, init)
+        manifest[name] = {"task": os.path.relpath(task, GOLDEN), "population": population, "generations": generations,
+                          "seed": seed, "save_all": save_all, "init": os.path.relpath(init, GOLDEN)}
         if search:
             manifest[name]["search"] = os.path.relpath(search, GOLDEN)
         print("golden", name)

@@ -25,7 +25,8 @@ def golden(name):
 def add(batch, name):
     m = MANIFEST[name]
     search = os.path.join(GOLDEN, m["search"]) if "search" in m else None
-    return batch.add(os.path.join(GOLDEN, m["task"]), search, m["population"], m["generations"], m["seed"])
+    init = os.path.join(GOLDEN, m["init"]) if "init" in m else None
+    return batch.add(os.path.join(GOLDEN, m["task"]), search, m["population"], m["generations"], m["seed"], init=init)
 
 
 @pytest.mark.parametrize("name", SMALL)
@@ -106,6 +107,8 @@ def test_cli_drop_in(tmp_path, name):
            "--generations", str(m["generations"]), "--seed", str(m["seed"]), "--out", str(out), "--save-all", "--quiet"]
     if "search" in m:
         cmd += ["--search", os.path.join(GOLDEN, m["search"])]
+    if "init" in m:
+        cmd += ["--init", os.path.join(GOLDEN, m["init"])]
     subprocess.run(cmd, check=True, timeout=300)
     mine = json.load(open(out / "result.json"))
     assert strip_timing(mine) == strip_timing(golden(name))
