@@ -1,6 +1,7 @@
-// `gagp run TASK.json [options]` on the MI355X engine: the drop-in command
-// line of the reference (cpp/src/cli/run.cpp:105-141,275-370,
-// cpp/src/cli/gagp_main.cpp:23-53). Errors print one line and exit 2.
+// `gagp run TASK.json [options]` and `gagp eval TASK.json PROGRAM.json` on
+// the MI355X engine: the drop-in command line of the reference
+// (cpp/src/cli/run.cpp:105-141,267-407, cpp/src/cli/gagp_main.cpp:23-53).
+// Errors print one line and exit 2.
 #include <cstdio>
 #include <cstdlib>
 #include <filesystem>
@@ -15,7 +16,9 @@
 using namespace gagp_amd;
 
 namespace {
-[[noreturn]] void usage_error(const std::string& m) { throw std::invalid_argument(m + " (see `gagp run --help`)"); }
+[[noreturn]] void usage_error(const std::string& m, const char* command = "run") {
+  throw std::invalid_argument(m + " (see `gagp " + command + " --help`)");
+}
 
 const char* kRunUsage =
     "usage: gagp run TASK.json [options]\n"
@@ -34,6 +37,31 @@ const char* kRunUsage =
     "  --out DIR           write DIR/result.json\n"
     "  --save-all          also write DIR/population.json and DIR/history.json\n"
     "  --quiet             no progress output\n";
+
+const char* kEvalUsage =
+    "usage: gagp eval TASK.json PROGRAM.json [--split train|test]\n"
+    "\n"
+    "Score a program on the task's test cases (default) or train cases on the MI355X\n"
+    "engine. PROGRAM.json is a result.json (its best program), a population.json (its\n"
+    "first program) or a single program.\n";
+
+// `gagp eval` (cpp/src/cli/run.cpp:372-407): one canonical verdict line.
+int eval_command(const std::vector<std::string>& args) {
+  std::vector<std::string> positional;
+  std::string split = "test";
+  for (std::size_t i = 0; i < args.size(); ++i) {
+    if (args[i] == "--help" || args[i] == "-h") { std::cout << kEvalUsage; return 0; }
+    if (args[i] == "--split") {
+      if (i + 1 >= args.size()) usage_error("--split requires a value", "eval");
+      split = args[++i];
+    } else if (!args[i].empty() && args[i][0] == '-') usage_error("unknown option " + args[i], "eval");
+    else positional.push_back(args[i]);
+  }
+  if (positional.size() != 2) usage_error("eval needs TASK.json and PROGRAM.json", "eval");
+  if (split != "train" && split != "test") usage_error("--split must be train or test", "eval");
+  std::cout << json::canonical(evaluate_program(positional[0], positional[1], split)) << '\n';
+  return 0;
+}
 
 void write_json(const std::filesystem::path& p, const json::Value& v) {
   std::ofstream out(p);
@@ -124,10 +152,13 @@ int main(int argc, char** argv) {
   std::vector<std::string> args(argv + 1, argv + argc);
   try {
     if (args.empty() || args[0] == "--help" || args[0] == "-h") {
-      std::cout << "gagp - grammar-guided genetic programming on MI355X\n\n  gagp run TASK.json [options]   search for a program\n";
+      std::cout << "gagp - grammar-guided genetic programming on MI355X\n\n"
+                   "  gagp run TASK.json [options]          search for a program\n"
+                   "  gagp eval TASK.json PROGRAM.json      score a program (see `gagp eval --help`)\n";
       return args.empty() ? 2 : 0;
     }
     if (args[0] == "run") return run_command(std::vector<std::string>(args.begin() + 1, args.end()));
+    if (args[0] == "eval") return eval_command(std::vector<std::string>(args.begin() + 1, args.end()));
     throw std::invalid_argument("unknown command " + args[0]);
   } catch (const std::exception& e) {
     std::fprintf(stderr, "gagp: %s\n", e.what());

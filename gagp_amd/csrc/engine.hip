@@ -2596,9 +2596,9 @@ Ast export_row(const SearchState& s, const int* origin, const Val* cval, int n, 
                               std::vector<Val>(cval, cval + nc), s.binder_base);
 }
 
-// The run's success criterion on held-out cases (cpp/src/cli/run.cpp:179-239).
-json::Value judge(const SearchState& s, const Ast& ast, const std::vector<int>& origin, const std::vector<Val>& cval) {
-  const Cases test = load_cases(s.task.test_path);
+// The run's success criterion on `test`, the held-out cases of a run or the
+// split `gagp eval` names, with `fuel` per case (cpp/src/cli/run.cpp:179-239).
+json::Value judge(const SearchState& s, const Cases& test, int fuel, const std::vector<int>& origin, const std::vector<Val>& cval) {
   // Lower the best phase exactly as the engine does, then score on the device.
   std::vector<Ins> code(std::size_t(s.d.code_cap));
   std::vector<Val> pool(std::size_t(s.d.pool_cap));
@@ -2629,7 +2629,7 @@ json::Value judge(const SearchState& s, const Ast& ast, const std::vector<int>& 
   }
   p.consts.assign(pool.begin(), pool.begin() + lr.constants);
   p.n_locals = lr.locals;
-  Session session(test, s.fuel, 512, s.penalty, s.task.policy);
+  Session session(test, fuel, 512, s.penalty, s.task.policy);
   std::vector<Val> values;
   std::vector<int> errors;
   session.eval_cases(p, values, errors);
@@ -2668,10 +2668,33 @@ json::Value judge(const SearchState& s, const Ast& ast, const std::vector<int>& 
   v.obj["runtime_errors"] = json::make_number(double(errs));
   v.obj["mean_loss"] = json::make_number(loss / n);
   v.obj["solved"] = json::make_bool(correct == std::size_t(test.n));
-  (void)ast;
   return v;
 }
 }  // namespace
+
+json::Value evaluate_program(const std::string& task_path, const std::string& program_path, const std::string& split) {
+  if (split != "train" && split != "test") throw std::invalid_argument("--split must be train or test");
+  SearchSpec spec;
+  spec.task_path = task_path;
+  SearchState s(spec);
+  const std::string& cases_path = split == "train" ? s.task.train_path : s.task.test_path;
+  if (cases_path.empty()) throw std::invalid_argument("task has no " + split + " split");
+  std::ifstream in(program_path, std::ios::binary);
+  if (!in) throw std::invalid_argument("cannot read " + program_path);
+  std::ostringstream text;
+  text << in.rdbuf();
+  const json::Value doc = json::parse(text.str());
+  // The program must be a member of the task's grammar; its row lowers as
+  // the engine lowers a member. Lowering capacities come from the search's
+  // shapes, computed without allocating.
+  const PhaseRow row = import_row(s.grammar, s.phase, decode_ast(program_of_document(doc)), 0, s.R);
+  DeviceArena shapes(true);
+  s.d = s.build(shapes);
+  json::Value v = judge(s, load_cases(cases_path), int(s.grammar.fuel), row.origin, row.cval);
+  v.obj["task"] = json::make_string(s.task.name);
+  v.obj["split"] = json::make_string(split);
+  return v;
+}
 
 SearchResult Batch::result(int index, bool save_all) {
   auto& I = *impl_;
@@ -2767,7 +2790,7 @@ SearchResult Batch::result(int index, bool save_all) {
   tm.obj["search"] = json::make_number(I.init_ms + I.run_ms);
   tm.obj["initialization"] = json::make_number(I.init_ms);
   r.obj["time_ms"] = tm;
-  if (!s.task.test_path.empty()) r.obj["test"] = judge(s, best_ast, bo, bc);
+  if (!s.task.test_path.empty()) r.obj["test"] = judge(s, load_cases(s.task.test_path), s.fuel, bo, bc);
   out.result = r;
   out.search_ms = I.init_ms + I.run_ms;
   return out;

@@ -148,4 +148,11 @@ class Session {
   std::unique_ptr<Impl> impl_;
 };
 
+// `gagp eval TASK.json PROGRAM.json --split train|test`: the verdict of one
+// program (a run result's best, a population's first, or a bare ast-prefix-v2
+// program, which must be a member of the task's grammar) on the split's
+// cases with the grammar's fuel: cases, correct, wrong, runtime_errors,
+// mean_loss, solved, task and split (cpp/src/cli/run.cpp:372-407).
+json::Value evaluate_program(const std::string& task_path, const std::string& program_path, const std::string& split);
+
 }  // namespace gagp_amd

@@ -171,11 +171,6 @@ class Matcher {
       }
     invalid("AST is missing the lexical region of node " + std::to_string(index));
   }
-  std::vector<int> mapped(const CExpr& e) const {
-    std::vector<int> out;
-    for (auto m : e.scope_mapping) out.push_back(env_.at(m));
-    return out;
-  }
   bool nonterminal(std::uint32_t nt, std::size_t index) {
     Key key{nt, index, env_};
     const auto found = decisions_.find(key);
@@ -401,7 +396,18 @@ PhaseRow import_row(const Grammar& g, const PhaseGrammar& phase, const Ast& ast,
     if (n.kind == NK_CONST) row.cval.push_back(ast.consts.at(std::size_t(n.i0)));
   }
   if (!annotate(phase, row.origin, row.site)) invalid("import site annotation failed");
-  if (phase.site_depth && site_height(phase, row.origin, row.site) > int(site_depth)) invalid("import site annotation failed");
+  if (site_depth && phase.site_depth && site_height(phase, row.origin, row.site) > int(site_depth))
+    invalid("import site annotation failed");
+  // The native verifier rejects a statement program that reads a local
+  // before every path assigns it (undefined_local).
+  if (g.nts[g.entry].cat == Cat::Program) {
+    const PhaseView gv{phase.exprs.data(), int(phase.exprs.size()), phase.prods.data(), int(phase.prods.size()),
+                       phase.nts.data(), int(phase.nts.size()), phase.domains.data(), int(phase.domains.size()),
+                       phase.values.data(), int(phase.values.size()), phase.minimum.data(), phase.depth_count,
+                       phase.logical_count, phase.site_depth ? 1 : 0, phase.root};
+    if (!locals_defined(gv, CRow{row.origin.data(), row.site.data(), row.cval.data(), int(row.origin.size()), int(row.cval.size())}))
+      invalid("invalid AST (undefined_local): a grammar local is read before it is assigned");
+  }
   return row;
 }
 

@@ -47,7 +47,8 @@ struct PhaseRow {
 // The phase row of a program: its derivation origins, the site of every node
 // a nonterminal choice starts (annotated from the origins), and its
 // constants in prefix order. `site_depth` 0 skips the site-depth bound;
-// `capacity` is the row capacity of the search.
+// `capacity` is the row capacity of the search. A statement program must
+// assign every local before it reads it (the verifier's undefined_local).
 PhaseRow import_row(const Grammar& g, const PhaseGrammar& phase, const Ast& ast, unsigned site_depth, int capacity);
 
 struct ImportedPopulation {

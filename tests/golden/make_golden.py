@@ -7,7 +7,8 @@ under tests/golden/runs/<name>/. It also copies the reference's bytecode
 fixture files (cpp/tests/fixtures/runtime/*.json: programs, cases and
 expected values or error names) into tests/golden/bytecode/.
 
-It also writes tests/golden/traversal/<task>.json: the reference's
+It also writes tests/golden/eval.json, the reference's `gagp eval` verdicts
+on golden run documents, and tests/golden/traversal/<task>.json: the reference's
 `gagp eval --split train` of the programs tests/host_lowering.cpp exports for
 the list tasks (seeded random traversal and statement programs), which
 tests/test_host_lowering.py compares with the engine's lowering run on the
@@ -227,6 +228,41 @@ def definite_assignment_goldens(n=150):
             print("definite assignment golden", task_dir, len(programs), sum(p["accepted"] for p in programs))
 
 
+# `gagp eval` of golden run documents: (run, document, split). A result.json
+# names its best program, a population.json its first.
+EVAL_CASES = [
+    ("polynomial_p1024_g64_s0", "result.json", "test"),
+    ("polynomial_p1024_g64_s0", "population.json", "train"),
+    ("quintic_p512_g16_s3", "result.json", "train"),
+    ("poly_c1_gen_e3r_p512_g24_s9", "population.json", "test"),
+    ("list_sum_even_p512_g10_s5", "result.json", "test"),
+    ("list_sum_even_p512_g10_s5", "population.json", "train"),
+    ("list_lowfuel_p400_g8_s8", "result.json", "train"),
+    ("list_range_nested_p256_g8_s7", "result.json", "test"),
+    ("let_sum_p256_g8_s1", "population.json", "test"),
+    ("stmt_sum_even_p512_g10_s5", "population.json", "test"),
+    ("stmt_input_p300_g10_s3", "result.json", "train"),
+    ("stmt_lowfuel_p400_g8_s8", "result.json", "test"),
+    ("fit_polynomial_exact_p256_g12_s4", "result.json", "test"),
+    ("fit_list_sum_even_disc_p512_g10_s5", "population.json", "train"),
+    ("fit_stmt_count_gt_cap7p25_p300_g8_s3", "result.json", "train"),
+]
+
+
+def eval_goldens():
+    """The reference's `gagp eval` verdict on each EVAL_CASES document."""
+    manifest = json.load(open(os.path.join(GOLDEN, "runs", "manifest.json")))
+    cases = []
+    for run, doc, split in EVAL_CASES:
+        task = os.path.join(GOLDEN, manifest[run]["task"])
+        path = os.path.join(GOLDEN, "runs", run, doc)
+        r = subprocess.run([REF, "eval", task, path, "--split", split], check=True, capture_output=True, text=True)
+        cases.append({"run": run, "document": doc, "split": split, "verdict": json.loads(r.stdout)})
+    with open(os.path.join(GOLDEN, "eval.json"), "w") as f:
+        json.dump({"cases": cases}, f, indent=1)
+    print("eval goldens", len(cases))
+
+
 def build_driver(tmp):
     driver = os.path.join(tmp, "host_lowering")
     build = os.path.join(ROOT, "gagp_amd", "_build")
@@ -307,6 +343,8 @@ def main():
         traversal_goldens()
     if not args.only or re.search(args.only, "definite"):
         definite_assignment_goldens()
+    if not args.only or re.search(args.only, "eval"):
+        eval_goldens()
     if os.path.isdir(REFERENCE_FIXTURES):
         dst = os.path.join(GOLDEN, "bytecode")
         os.makedirs(dst, exist_ok=True)

@@ -12,6 +12,8 @@
 //   import_driver TASK.json - N SITE_DEPTH generate
 //     N generated rows of the grammar with their sites, re-annotated from
 //     the origins: {"i", "nodes", "same_sites"}
+//   import_driver TASK.json POPULATION.json COUNT SITE_DEPTH each
+//     each program imported on its own: {"i", "error"} (null when admitted)
 // Errors print {"error": "..."} and exit 2.
 #include <algorithm>
 #include <cstdio>
@@ -74,6 +76,11 @@ int main(int argc, char** argv) {
         const GenResult r = generate_phase(gv, ph.root, 0, int(layout.slot_max_depth), capacity, 0x9e3779b97f4a7c15ull * (i + 1),
                                            origin.data(), cval.data(), site.data(), int(site_depth));
         if (r.status != ST_OK) continue;
+        // Statement rows that read an unassigned local are not members (the
+        // engine's generator draws again).
+        if (g.nts[g.entry].cat == Cat::Program &&
+            !locals_defined(gv, CRow{origin.data(), site.data(), cval.data(), r.nodes, r.constants}))
+          continue;
         // The sites the annotation recovers from the origins alone, through
         // the import path's export -> import round trip.
         const std::vector<int> o(origin.begin(), origin.begin() + r.nodes);
@@ -90,6 +97,18 @@ int main(int argc, char** argv) {
     text << in.rdbuf();
     const json::Value doc = json::parse(text.str());
     const auto& programs = doc.at("programs").as_array("programs");
+    if (mode == "each") {
+      for (int i = 0; i < count && std::size_t(i) < programs.size(); ++i) {
+        std::string error = "null";
+        try {
+          (void)import_row(g, ph, decode_ast(programs[std::size_t(i)]), site_depth, capacity);
+        } catch (const std::exception& e) {
+          error = quoted(e.what());
+        }
+        std::printf("{\"i\":%d,\"error\":%s}\n", i, error.c_str());
+      }
+      return 0;
+    }
     for (int i = 0; i < count && std::size_t(i) < programs.size(); ++i) {
       const Ast ast = decode_ast(programs[std::size_t(i)]);
       const PhaseRow row = import_row(g, ph, ast, site_depth, capacity);

@@ -63,9 +63,11 @@ def test_saved_population_round_trips(driver, name):
 def test_generated_rows_annotate_back(driver, task):
     # Rows from the device's generator (run on the host): exported, then
     # imported, they give back the same origins and the generator's sites.
-    rc, rows = run(driver, os.path.join(GOLDEN, "tasks", task, "task.json"), "-", 200, SITE_DEPTH, "generate")
+    # (statement rows that read an unassigned local are skipped, as the
+    # engine's generator skips them)
+    rc, rows = run(driver, os.path.join(GOLDEN, "tasks", task, "task.json"), "-", 1000, SITE_DEPTH, "generate")
     assert rc == 0, rows[-1:]
-    assert len(rows) > 150
+    assert len(rows) > 100
     assert all(r["same_sites"] for r in rows), [r["i"] for r in rows if not r["same_sites"]][:5]
 
 
@@ -125,3 +127,38 @@ def test_load_population_errors(driver, tmp_path):
     path.write_text(json.dumps({"format": "gagp-population-v1"}))
     rc, rows = run(driver, poly, path, 10, SITE_DEPTH, "load")
     assert rc == 2 and "programs array" in rows[0]["error"]
+
+
+@pytest.fixture(scope="module")
+def lowering_driver(tmp_path_factory):
+    objs = [os.path.join(BUILD, n) for n in ("grammar.o", "task.o", "export.o", "identity.o")]
+    oracle = os.path.join(ROOT, "oracle", "liboracle.so")
+    missing = [p for p in objs + [oracle] if not os.path.exists(p)]
+    assert not missing, "build the engine and the oracle first (__graft_entry__.build()): missing %s" % missing
+    exe = str(tmp_path_factory.mktemp("lowering") / "host_lowering")
+    subprocess.run(["g++", "-O1", "-std=c++17", "-I" + os.path.join(ROOT, "gagp_amd", "csrc"),
+                    os.path.join(ROOT, "tests", "host_lowering.cpp")] + objs +
+                   [oracle, "-Wl,-rpath," + os.path.dirname(oracle), "-o", exe], check=True)
+    return exe
+
+
+@pytest.mark.parametrize("task", sorted(f[:-5] for f in os.listdir(os.path.join(GOLDEN, "definite")) if f.endswith(".json")))
+def test_import_rejects_undefined_local_reads(driver, lowering_driver, task, tmp_path):
+    # Random statement programs, some reading a local before it is
+    # assigned: the import admits exactly those the reference's verifier
+    # accepts (tests/golden/definite/, from `gagp eval`).
+    golden = json.load(open(os.path.join(GOLDEN, "definite", task + ".json")))
+    task_path = os.path.join(GOLDEN, golden["task"])
+    lines = subprocess.run([lowering_driver, task_path, str(golden["n"]), "defs"], check=True, capture_output=True,
+                           text=True, cwd=ROOT).stdout.splitlines()
+    programs = [json.loads(line) for line in lines]
+    assert [p["i"] for p in programs] == [p["i"] for p in golden["programs"]]
+    doc = tmp_path / "programs.json"
+    doc.write_text(json.dumps({"programs": [p["program"] for p in programs]}))
+    rc, rows = run(driver, task_path, doc, len(programs), SITE_DEPTH, "each")
+    assert rc == 0 and len(rows) == len(programs)
+    for row, want in zip(rows, golden["programs"]):
+        if want["accepted"]:
+            assert row["error"] is None, (want["text"], row["error"])
+        else:
+            assert row["error"] and "undefined_local" in row["error"], (want["text"], row["error"])
