@@ -107,6 +107,21 @@ def lib():
     L.gagp_valu_probe.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_float)]
     L.gagp_batch_tuning_json.argtypes = L.gagp_batch_kernel_times_json.argtypes
     L.gagp_batch_destroy.argtypes = [ctypes.c_void_p]
+    vp, i32, u64, dbl = ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint64, ctypes.c_double
+    P = ctypes.POINTER
+    L.gagp_owner_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, i32, i32, u64, ctypes.c_char_p, P(vp)]
+    for name in ("population_size", "batch_size", "has_wrong"):
+        getattr(L, "gagp_owner_" + name).argtypes = [vp, P(i32)]
+    L.gagp_owner_evaluate.argtypes = [vp, P(dbl), P(ctypes.c_uint32), P(u64)]
+    L.gagp_owner_evaluate_batch.argtypes = L.gagp_owner_evaluate.argtypes
+    L.gagp_owner_node_counts.argtypes = [vp, i32, P(ctypes.c_uint32)]
+    L.gagp_owner_batch_nodes.argtypes = L.gagp_owner_node_counts.argtypes
+    L.gagp_owner_begin_steady.argtypes = [vp, i32]
+    L.gagp_owner_make_batch.argtypes = [vp, P(dbl), i32, dbl, u64, P(i32)]
+    L.gagp_owner_commit_batch.argtypes = [vp, P(i32), i32]
+    L.gagp_owner_export_member.argtypes = [vp, i32, ctypes.c_char_p, ctypes.c_size_t, P(ctypes.c_size_t)]
+    L.gagp_owner_destroy.argtypes = [vp]
+    L.gagp_owner_destroy.restype = None
     _lib = L
     return L
 
@@ -287,6 +302,70 @@ class Batch:
         if getattr(self, "h", None):
             lib().gagp_batch_destroy(self.h)
             self.h = None
+
+
+class Owner:
+    """The owner seam (gagp_owner_*): one steady-state search on the device
+    whose generation loop the caller runs, as the reference's
+    run_phase_population_steady drives its GpuPhaseOwner."""
+
+    def __init__(self, task, search=None, population=0, seed=None, init=None):
+        self.h = ctypes.c_void_p()
+        _check(lib().gagp_owner_create(task.encode(), search.encode() if search else None, population or 0,
+                                       int(seed is not None), ctypes.c_uint64(seed or 0), init.encode() if init else None,
+                                       ctypes.byref(self.h)))
+        self.P = self._int("population_size")
+        self.B = self._int("batch_size")
+        self.has_wrong = bool(self._int("has_wrong"))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().gagp_owner_destroy(self.h)
+            self.h = None
+
+    def _int(self, name):
+        out = ctypes.c_int32()
+        _check(getattr(lib(), "gagp_owner_" + name)(self.h, ctypes.byref(out)))
+        return out.value
+
+    def _scores(self, fn, n):
+        fit, wrong, counts = (ctypes.c_double * n)(), (ctypes.c_uint32 * n)(), (ctypes.c_uint64 * (5 * n))()
+        _check(fn(self.h, fit, wrong, counts))
+        return list(fit), list(wrong), [list(counts[5 * i:5 * i + 5]) for i in range(n)]
+
+    def evaluate(self):
+        """(fitness[P], wrong[P], counts[P][5]) of the initial population."""
+        return self._scores(lib().gagp_owner_evaluate, self.P)
+
+    def node_counts(self, grammar_sites=False):
+        out = (ctypes.c_uint32 * self.P)()
+        _check(lib().gagp_owner_node_counts(self.h, int(grammar_sites), out))
+        return list(out)
+
+    def begin_steady(self, batch):
+        _check(lib().gagp_owner_begin_steady(self.h, batch))
+
+    def make_batch(self, selection, pressure, p_mut, seed):
+        sel = (ctypes.c_double * self.P)(*selection)
+        reuse = (ctypes.c_int32 * self.B)()
+        _check(lib().gagp_owner_make_batch(self.h, sel, pressure, float(p_mut), ctypes.c_uint64(seed), reuse))
+        return list(reuse)
+
+    def evaluate_batch(self):
+        return self._scores(lib().gagp_owner_evaluate_batch, self.B)
+
+    def batch_nodes(self, grammar_sites=False):
+        out = (ctypes.c_uint32 * self.B)()
+        _check(lib().gagp_owner_batch_nodes(self.h, int(grammar_sites), out))
+        return list(out)
+
+    def commit_batch(self, placements):
+        flat = [x for pair in placements for x in pair]
+        arr = (ctypes.c_int32 * max(1, len(flat)))(*flat)
+        _check(lib().gagp_owner_commit_batch(self.h, arr, len(placements)))
+
+    def export_member(self, member):
+        return json.loads(_text(lib().gagp_owner_export_member, self.h, member))
 
 
 class Server:

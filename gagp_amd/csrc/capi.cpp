@@ -15,6 +15,10 @@ using namespace gagp_amd;
 struct gagp_session { std::unique_ptr<Session> s; };
 struct gagp_batch { Batch b; std::vector<SearchResult> results; bool prepared = false, ran = false; };
 struct gagp_server { Server s; bool ran = false; };
+struct gagp_owner {
+  explicit gagp_owner(const SearchSpec& spec) : o(spec) {}
+  Owner o;
+};
 
 namespace {
 thread_local std::string g_last_error;
@@ -354,5 +358,102 @@ int gagp_server_result_json(gagp_server* s, int32_t ticket, char* buf, size_t ca
 }
 
 void gagp_server_destroy(gagp_server* s) { delete s; }
+
+int gagp_owner_create(const char* task_path, const char* search_path, int32_t population, int32_t seed_is_set, uint64_t seed,
+                      const char* init_path, gagp_owner** out) {
+  return guarded([&] {
+    if (!task_path || !out) throw std::invalid_argument("invalid owner arguments");
+    SearchSpec spec;
+    spec.task_path = task_path;
+    if (search_path) spec.search_path = search_path;
+    if (population > 0) spec.population = population;
+    if (seed_is_set) spec.seed = seed;
+    if (init_path) spec.init_path = init_path;
+    *out = new gagp_owner(spec);
+  });
+}
+
+int gagp_owner_population_size(gagp_owner* o, int32_t* out) {
+  return guarded([&] {
+    if (!o || !out) throw std::invalid_argument("null owner");
+    *out = o->o.population_size();
+  });
+}
+
+int gagp_owner_batch_size(gagp_owner* o, int32_t* out) {
+  return guarded([&] {
+    if (!o || !out) throw std::invalid_argument("null owner");
+    *out = o->o.batch_size();
+  });
+}
+
+int gagp_owner_has_wrong(gagp_owner* o, int32_t* out) {
+  return guarded([&] {
+    if (!o || !out) throw std::invalid_argument("null owner");
+    *out = o->o.has_wrong() ? 1 : 0;
+  });
+}
+
+int gagp_owner_evaluate(gagp_owner* o, double* fitness, uint32_t* wrong, uint64_t* counts) {
+  return guarded([&] {
+    if (!o || !fitness || !wrong || !counts) throw std::invalid_argument("null owner argument");
+    o->o.evaluate(fitness, wrong, counts);
+  });
+}
+
+int gagp_owner_node_counts(gagp_owner* o, int32_t grammar_sites, uint32_t* out) {
+  return guarded([&] {
+    if (!o || !out) throw std::invalid_argument("null owner argument");
+    o->o.node_counts(grammar_sites != 0, out);
+  });
+}
+
+int gagp_owner_begin_steady(gagp_owner* o, int32_t batch) {
+  return guarded([&] {
+    if (!o) throw std::invalid_argument("null owner");
+    o->o.begin_steady(batch);
+  });
+}
+
+int gagp_owner_make_batch(gagp_owner* o, const double* selection, int32_t pressure, double p_mut, uint64_t seed,
+                          int32_t* reuse) {
+  return guarded([&] {
+    if (!o || !selection || !reuse) throw std::invalid_argument("null owner argument");
+    o->o.make_batch(selection, pressure, p_mut, seed, reuse);
+  });
+}
+
+int gagp_owner_evaluate_batch(gagp_owner* o, double* fitness, uint32_t* wrong, uint64_t* counts) {
+  return guarded([&] {
+    if (!o || !fitness || !wrong || !counts) throw std::invalid_argument("null owner argument");
+    o->o.evaluate_batch(fitness, wrong, counts);
+  });
+}
+
+int gagp_owner_batch_nodes(gagp_owner* o, int32_t grammar_sites, uint32_t* out) {
+  return guarded([&] {
+    if (!o || !out) throw std::invalid_argument("null owner argument");
+    o->o.batch_nodes(grammar_sites != 0, out);
+  });
+}
+
+int gagp_owner_commit_batch(gagp_owner* o, const int32_t* pairs, int32_t n) {
+  return guarded([&] {
+    if (!o || n < 0 || (n > 0 && !pairs)) throw std::invalid_argument("invalid placements");
+    o->o.commit_batch(pairs, n);
+  });
+}
+
+int gagp_owner_export_member(gagp_owner* o, int32_t member, char* buf, size_t cap, size_t* needed) {
+  std::string text;
+  const int st = guarded([&] {
+    if (!o) throw std::invalid_argument("null owner");
+    text = json::canonical(o->o.export_member(member));
+  });
+  if (st != GAGP_OK) return st;
+  return copy_out(text, buf, cap, needed);
+}
+
+void gagp_owner_destroy(gagp_owner* o) { delete o; }
 
 }  // extern "C"

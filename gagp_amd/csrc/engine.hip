@@ -130,6 +130,9 @@ struct DSearch {
   int input_shadow;    // the last input_shadow of them shadow stored inputs (statement input targets), else 0
   int walks;           // the grammar has traversals: lowering runs the kWalks instantiations
   int imported;        // the initial population is uploaded, not generated
+  // Selection values of a caller-driven round (the owner seam): tournaments
+  // compare them alone; null in the engine's own rounds.
+  GAS const double* select;
   int general_scores;  // the general evaluation kernel has an LDS score buffer for this search
   GAS double* loop_scores;  // the looping kernel's case scores: C per program slot
   GAS LoopAcc* loop_acc;    // the looping kernel's per-program tile totals (zero between evaluations)
@@ -231,7 +234,7 @@ __global__ __launch_bounds__(64 * kVaryWaves) void k_vary(DSearch* all, const Bl
   const int c = bm.block * kVaryWaves + wave_index();
   if (c >= d.B) return;
   const std::uint64_t seed = *d.round_seed;
-  const Keys keys{d.fitness, d.wrong, d.nodes, d.has_wrong};
+  const Keys keys = d.select ? Keys{d.select, nullptr, nullptr, 0} : Keys{d.fitness, d.wrong, d.nodes, d.has_wrong};
   const std::size_t o = std::size_t(c) * d.R;
   Row x{d.x_origin + o, d.x_site + o, d.x_cval + o, 0, 0};
   OpStats st;
@@ -2589,6 +2592,166 @@ std::vector<std::uint64_t> Batch::counters(int index) const {
           c.child_code_words, c.child_pool_words};
 }
 double Batch::init_ms() const { return impl_->init_ms; }
+
+// ---- the owner seam (caller-driven steady-state rounds) ----
+struct Owner::Impl {
+  Batch batch;
+  SearchState* s = nullptr;
+  double* select = nullptr;  // device selection values, P doubles
+  bool made = false;         // a batch is made and not yet committed
+  bool scored = false;
+  std::vector<int> reuse;
+  ~Impl() {
+    if (select) (void)hipFree(select);
+  }
+};
+
+Owner::Owner(const SearchSpec& spec) : impl_(std::make_unique<Impl>()) {
+  Impl& O = *impl_;
+  O.batch.add(spec);
+  O.s = O.batch.impl_->searches[0].get();
+  if (O.s->cfg.schedule == Schedule::Generational)
+    throw Unsupported("the owner seam drives the steady-state schedule (generational searches run through gagp_batch)");
+  O.batch.set_autotune(0);
+  O.batch.prepare();
+  HIP_OK(hipMalloc(&O.select, sizeof(double) * std::size_t(O.s->P)));
+  O.s->d.select = (GAS const double*)O.select;
+  O.batch.impl_->rebuild();  // the round table with the selection array
+  if (const int e = O.s->device_error()) throw std::runtime_error("device initialization error " + std::to_string(e));
+}
+Owner::~Owner() = default;
+int Owner::population_size() const { return impl_->s->P; }
+int Owner::batch_size() const { return impl_->s->B; }
+bool Owner::has_wrong() const { return impl_->s->d.has_wrong != 0; }
+
+namespace {
+// {program cases, errors, timeouts, fallbacks, unscored} of evaluated slots.
+void slot_counts(const SearchState& s, int n, const std::vector<int>* reuse, std::uint64_t* counts) {
+  const std::vector<unsigned> errors = s.down(s.d.e_errors, std::size_t(n)), timeouts = s.down(s.d.e_timeouts, std::size_t(n));
+  for (int i = 0; i < n; ++i) {
+    std::uint64_t* c = counts + 5 * std::size_t(i);
+    const bool scored = !reuse || (*reuse)[std::size_t(i)] < 0;
+    c[0] = scored ? std::uint64_t(s.d.cd.C) : 0;
+    c[1] = scored ? errors[std::size_t(i)] : 0;
+    c[2] = scored ? timeouts[std::size_t(i)] : 0;
+    c[3] = 0;
+    c[4] = 0;
+  }
+}
+// Node or grammar-site counts of n rows (counts, site rows of stride R).
+void row_counts(const SearchState& s, bool grammar_sites, GAS const int* n_rows, GAS const int* sites, int n, unsigned* out) {
+  const std::vector<int> len = s.down(n_rows, std::size_t(n));
+  if (!grammar_sites) {
+    for (int i = 0; i < n; ++i) out[i] = unsigned(s.d.fixed_nodes) + unsigned(len[std::size_t(i)]);
+    return;
+  }
+  const std::vector<int> site = s.down(sites, std::size_t(n) * std::size_t(s.R));
+  for (int i = 0; i < n; ++i) {
+    unsigned k = 0;
+    for (int j = 0; j < len[std::size_t(i)]; ++j) k += site[std::size_t(i) * std::size_t(s.R) + std::size_t(j)] >= 0;
+    out[i] = k;
+  }
+}
+}  // namespace
+
+void Owner::evaluate(double* fitness, unsigned* wrong, std::uint64_t* counts) {
+  const SearchState& s = *impl_->s;
+  const std::vector<double> f = s.down(s.d.fitness, std::size_t(s.P));
+  const std::vector<unsigned> w = s.down(s.d.wrong, std::size_t(s.P));
+  std::copy(f.begin(), f.end(), fitness);
+  std::copy(w.begin(), w.end(), wrong);
+  slot_counts(s, s.P, nullptr, counts);
+}
+
+void Owner::node_counts(bool grammar_sites, unsigned* out) {
+  const SearchState& s = *impl_->s;
+  row_counts(s, grammar_sites, s.d.cur_n, s.d.cur_site, s.P, out);
+}
+
+void Owner::begin_steady(int batch) {
+  if (batch != impl_->s->B)
+    throw std::invalid_argument("steady batch " + std::to_string(batch) + " differs from the search configuration's " +
+                                std::to_string(impl_->s->B));
+}
+
+void Owner::make_batch(const double* selection, int pressure, double p_mut, std::uint64_t seed, int* reuse) {
+  Impl& O = *impl_;
+  auto& I = *O.batch.impl_;
+  SearchState& s = *O.s;
+  if (O.made) throw std::invalid_argument("the previous batch is not committed");
+  if (pressure < 1 || pressure > s.P) throw std::invalid_argument("tournament pressure outside 1..P");
+  if (!(p_mut >= 0 && p_mut <= 1)) throw std::invalid_argument("mutation probability outside [0, 1]");
+  if (pressure != s.d.pressure || p_mut != s.d.p_mut) {
+    s.d.pressure = pressure;
+    s.d.p_mut = p_mut;
+    I.rebuild();
+  }
+  HIP_OK(hipMemcpyAsync(O.select, selection, sizeof(double) * std::size_t(s.P), hipMemcpyHostToDevice, I.stream));
+  HIP_OK(hipMemcpyAsync((void*)s.d.round_seed, &seed, sizeof(seed), hipMemcpyHostToDevice, I.stream));
+  I.enqueue_front();
+  HIP_OK(hipStreamSynchronize(I.stream));
+  if (const int e = s.device_error()) throw std::runtime_error("device reproduction error " + std::to_string(e));
+  O.reuse = s.down(s.d.reuse, std::size_t(s.B));
+  std::copy(O.reuse.begin(), O.reuse.end(), reuse);
+  O.made = true;
+  O.scored = false;
+}
+
+void Owner::evaluate_batch(double* fitness, unsigned* wrong, std::uint64_t* counts) {
+  Impl& O = *impl_;
+  auto& I = *O.batch.impl_;
+  const SearchState& s = *O.s;
+  if (!O.made || O.scored) throw std::invalid_argument("no batch to evaluate");
+  I.enqueue_eval();
+  HIP_OK(hipStreamSynchronize(I.stream));
+  if (const int e = s.device_error()) throw std::runtime_error("device evaluation error " + std::to_string(e));
+  const std::vector<double> f = s.down(s.d.c_fit, std::size_t(s.B));
+  const std::vector<unsigned> w = s.down(s.d.c_wrong, std::size_t(s.B));
+  std::copy(f.begin(), f.end(), fitness);
+  std::copy(w.begin(), w.end(), wrong);
+  slot_counts(s, s.B, &O.reuse, counts);
+  O.scored = true;
+}
+
+void Owner::batch_nodes(bool grammar_sites, unsigned* out) {
+  const SearchState& s = *impl_->s;
+  if (!impl_->made) throw std::invalid_argument("no batch made");
+  row_counts(s, grammar_sites, s.d.x_n, s.d.x_site, s.B, out);
+}
+
+void Owner::commit_batch(const int* pairs, int n) {
+  Impl& O = *impl_;
+  auto& I = *O.batch.impl_;
+  const SearchState& s = *O.s;
+  if (!O.made || !O.scored) throw std::invalid_argument("commit before the batch is made and evaluated");
+  std::vector<int> placed(std::size_t(s.P), -1);
+  for (int k = 0; k < n; ++k) {
+    const int m = pairs[2 * k], c = pairs[2 * k + 1];
+    if (m < 0 || m >= s.P || c < 0 || c >= s.B) throw std::invalid_argument("placement outside the population or batch");
+    placed[std::size_t(m)] = c;
+  }
+  const int zero[3] = {0, 0, 0};
+  HIP_OK(hipMemcpyAsync((void*)s.d.placed, placed.data(), sizeof(int) * placed.size(), hipMemcpyHostToDevice, I.stream));
+  I.launch("k_commit", I.g_member_w, k_commit, 64 * kVaryWaves, 0);
+  HIP_OK(hipMemcpyAsync((void*)s.d.n_pending, zero, sizeof(zero), hipMemcpyHostToDevice, I.stream));
+  HIP_OK(hipStreamSynchronize(I.stream));
+  if (const int e = s.device_error()) throw std::runtime_error("device commit error " + std::to_string(e));
+  O.made = O.scored = false;
+}
+
+namespace {
+Ast export_row(const SearchState& s, const int* origin, const Val* cval, int n, int nc);
+}
+
+json::Value Owner::export_member(int member) {
+  const SearchState& s = *impl_->s;
+  if (member < 0 || member >= s.P) throw std::invalid_argument("member outside the population");
+  const std::size_t o = std::size_t(member) * std::size_t(s.R);
+  const int n = s.down(s.d.cur_n + member, 1)[0], nc = s.down(s.d.cur_nc + member, 1)[0];
+  const std::vector<int> origin = s.down(s.d.cur_origin + o, std::size_t(n));
+  const std::vector<Val> cval = s.down(s.d.cur_cval + o, std::size_t(nc));
+  return ast_json(export_row(s, origin.data(), cval.data(), n, nc));
+}
 
 namespace {
 Ast export_row(const SearchState& s, const int* origin, const Val* cval, int n, int nc) {

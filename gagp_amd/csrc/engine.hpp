@@ -107,6 +107,49 @@ class Batch {
   std::size_t footprint(int index) const;  // device bytes one admitted search holds
 
  private:
+  friend class Owner;
+  struct Impl;
+  std::unique_ptr<Impl> impl_;
+};
+
+// The in-process seam of the reference's resident steady-state loop: the
+// owner adapter GpuPhaseOwner (cpp/src/evolution/repro/gpu/phase_population.hpp:15-59)
+// that run_phase_population_steady (cpp/src/evolution/evolve.cpp:331-441)
+// drives one round at a time. The caller keeps fitness, ranks, selection
+// values and the run generator and chooses the victims; the owner keeps the
+// population on the device, makes and scores each batch, and commits the
+// caller's placements. Steady-state schedule only.
+class Owner {
+ public:
+  // Builds the search, its initial population (generated, or the spec's
+  // init_path) lowered and scored on the device.
+  explicit Owner(const SearchSpec& spec);
+  ~Owner();
+  int population_size() const;
+  int batch_size() const;  // the search configuration's steady-state batch
+  // Whether scores carry wrong counts (every metric but legacy): the
+  // reference's evaluate returns none otherwise, and ranks by fitness alone.
+  bool has_wrong() const;
+  // Scores of the members: ranking-canonical fitness (canonicalizing again
+  // changes nothing), wrong train cases, and per member {program cases,
+  // errors, timeouts, fallbacks, unscored}.
+  void evaluate(double* fitness, unsigned* wrong, std::uint64_t* counts);
+  // AST node counts, or grammar-site counts (`grammar_sites`), per member.
+  void node_counts(bool grammar_sites, unsigned* out);
+  void begin_steady(int batch);
+  // Tournament parents drawn by `selection` (higher wins, earlier draw on
+  // ties), crossover and mutation from `seed`, lowering, and the member
+  // each child reuses (its row equals it) or -1.
+  void make_batch(const double* selection, int pressure, double p_mut, std::uint64_t seed, int* reuse);
+  // Scores of the batch's children (those with reuse -1; the others are
+  // left to the caller), as evaluate() per member.
+  void evaluate_batch(double* fitness, unsigned* wrong, std::uint64_t* counts);
+  void batch_nodes(bool grammar_sites, unsigned* out);
+  // (member, child) pairs: each member's row becomes that child's.
+  void commit_batch(const int* pairs, int n);
+  json::Value export_member(int member);
+
+ private:
   struct Impl;
   std::unique_ptr<Impl> impl_;
 };

@@ -261,6 +261,55 @@ int gagp_server_outcome(gagp_server* s, int32_t ticket, double* times, int32_t* 
 int gagp_server_result_json(gagp_server* s, int32_t ticket, char* buf, size_t cap, size_t* needed);
 void gagp_server_destroy(gagp_server* s);
 
+/* ---- Owner seam: the reference's generation loop drives the device ----
+ * One steady-state search whose population stays on the device while the
+ * caller runs the generation loop: the caller keeps fitness, ranks,
+ * selection values and the run generator, and picks each child's victim.
+ * Reference: the owner adapter GpuPhaseOwner and its NativePhasePopulation
+ * (cpp/src/evolution/repro/gpu/phase_population.hpp:15-59), driven by
+ * run_phase_population_steady (cpp/src/evolution/evolve.cpp:331-441). */
+typedef struct gagp_owner gagp_owner;
+
+/* Builds the search of a task (search_path, population, seed and init_path
+ * as gagp_batch_add_init), its initial population generated (or read),
+ * lowered and scored on the device. Declines (GAGP_ERR_UNSUPPORTED) as
+ * gagp_batch_add does, and for the generational schedule.
+ * Reference: GpuPhaseOwner construction, evolve.cpp:457-470. */
+int gagp_owner_create(const char* task_path, const char* search_path, int32_t population, int32_t seed_is_set, uint64_t seed,
+                      const char* init_path, gagp_owner** out);
+/* population_size(); the configuration's steady-state batch B. */
+int gagp_owner_population_size(gagp_owner* o, int32_t* out);
+int gagp_owner_batch_size(gagp_owner* o, int32_t* out);
+/* 1 when scores carry wrong counts (every metric but legacy; the reference's
+ * evaluate returns an empty wrong vector otherwise and ranks by fitness). */
+int gagp_owner_has_wrong(gagp_owner* o, int32_t* out);
+/* The initial population's scores (evaluate_owner, evolve.cpp:341-350):
+ * fitness[P] (ranking-canonical: canonicalize_fitness_for_ranking leaves
+ * them unchanged), wrong[P], counts[P][5] = {program cases, errors,
+ * timeouts, fallbacks, unscored}. */
+int gagp_owner_evaluate(gagp_owner* o, double* fitness, uint32_t* wrong, uint64_t* counts);
+/* node_counts(grammar_sites): AST nodes, or grammar-site nodes, per member. */
+int gagp_owner_node_counts(gagp_owner* o, int32_t grammar_sites, uint32_t* out);
+/* begin_steady(B): B must be the configuration's batch. */
+int gagp_owner_begin_steady(gagp_owner* o, int32_t batch);
+/* make_batch(selection[P], pressure, p_mut, seed) -> reuse[B]: tournaments
+ * on the selection values, crossover and mutation from the seed word,
+ * lowering; reuse[b] is the member child b equals, or -1. */
+int gagp_owner_make_batch(gagp_owner* o, const double* selection, int32_t pressure, double p_mut, uint64_t seed,
+                          int32_t* reuse);
+/* evaluate_batch: fitness[B], wrong[B], counts[B][5] of the children whose
+ * reuse is -1 (the others are the caller's: their source's scores). */
+int gagp_owner_evaluate_batch(gagp_owner* o, double* fitness, uint32_t* wrong, uint64_t* counts);
+/* batch_nodes(grammar_sites): node counts of the batch's children. */
+int gagp_owner_batch_nodes(gagp_owner* o, int32_t grammar_sites, uint32_t* out);
+/* commit_batch: pairs[n][2] = (member, child); each member's row becomes
+ * its child's. */
+int gagp_owner_commit_batch(gagp_owner* o, const int32_t* pairs, int32_t n);
+/* export_member(i): the member's program as ast-prefix-v2 JSON (buffer
+ * protocol as gagp_batch_result_json). */
+int gagp_owner_export_member(gagp_owner* o, int32_t member, char* buf, size_t cap, size_t* needed);
+void gagp_owner_destroy(gagp_owner* o);
+
 #ifdef __cplusplus
 }
 #endif
