@@ -118,6 +118,7 @@ def lib():
     L.gagp_owner_batch_nodes.argtypes = L.gagp_owner_node_counts.argtypes
     L.gagp_owner_begin_steady.argtypes = [vp, i32]
     L.gagp_owner_make_batch.argtypes = [vp, P(dbl), i32, dbl, u64, P(i32)]
+    L.gagp_owner_reproduce.argtypes = [vp, P(dbl), i32, dbl, u64, i32]
     L.gagp_owner_commit_batch.argtypes = [vp, P(i32), i32]
     L.gagp_owner_export_member.argtypes = [vp, i32, ctypes.c_char_p, ctypes.c_size_t, P(ctypes.c_size_t)]
     L.gagp_owner_destroy.argtypes = [vp]
@@ -350,6 +351,10 @@ class Owner:
         reuse = (ctypes.c_int32 * self.B)()
         _check(lib().gagp_owner_make_batch(self.h, sel, pressure, float(p_mut), ctypes.c_uint64(seed), reuse))
         return list(reuse)
+
+    def reproduce(self, selection, pressure, p_mut, seed, elites):
+        sel = (ctypes.c_double * self.P)(*selection)
+        _check(lib().gagp_owner_reproduce(self.h, sel, pressure, float(p_mut), ctypes.c_uint64(seed), elites))
 
     def evaluate_batch(self):
         return self._scores(lib().gagp_owner_evaluate_batch, self.B)

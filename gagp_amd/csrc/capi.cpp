@@ -423,6 +423,14 @@ int gagp_owner_make_batch(gagp_owner* o, const double* selection, int32_t pressu
   });
 }
 
+int gagp_owner_reproduce(gagp_owner* o, const double* selection, int32_t pressure, double p_mut, uint64_t seed,
+                         int32_t elites) {
+  return guarded([&] {
+    if (!o || !selection) throw std::invalid_argument("null owner argument");
+    o->o.reproduce(selection, pressure, p_mut, seed, elites);
+  });
+}
+
 int gagp_owner_evaluate_batch(gagp_owner* o, double* fitness, uint32_t* wrong, uint64_t* counts) {
   return guarded([&] {
     if (!o || !fitness || !wrong || !counts) throw std::invalid_argument("null owner argument");

@@ -1920,6 +1920,7 @@ struct Batch::Impl {
   // fixed default: one program per wavefront, one 64-case tile per item).
   TuneChoice tune;
   int autotune = 1;  // 0 off, 1 above kTuneWork, 2 always
+  bool seam = false;  // rounds driven through the owner seam
   bool tuned = false;
   std::string tune_report;  // JSON of the last tuning pass
   bool any_walks(const std::vector<int>& ids) const {
@@ -2324,8 +2325,11 @@ struct Batch::Impl {
   // evaluation kernels, then tallies, placement and commit.
   void enqueue_front() {
     const int n = int(live.size());
-    if (any_generational) timer.time("k_gen_begin", stream, [&] { hipLaunchKernelGGL(k_gen_begin, dim3(n), dim3(256), 0, stream, d_all); });
-    timer.time("k_record", stream, [&] { hipLaunchKernelGGL(k_record, dim3(n), dim3(256), 0, stream, d_all); });
+    // Caller-driven rounds (the owner seam) bring their own elites, seed and
+    // history.
+    if (any_generational && !seam)
+      timer.time("k_gen_begin", stream, [&] { hipLaunchKernelGGL(k_gen_begin, dim3(n), dim3(256), 0, stream, d_all); });
+    if (!seam) timer.time("k_record", stream, [&] { hipLaunchKernelGGL(k_record, dim3(n), dim3(256), 0, stream, d_all); });
     // Three independent branches: the reproduction of searches without and
     // with traversal walks (disjoint searches) and the genotype index of the
     // current members (read by nothing reproduction writes); the pending
@@ -2600,6 +2604,7 @@ struct Owner::Impl {
   double* select = nullptr;  // device selection values, P doubles
   bool made = false;         // a batch is made and not yet committed
   bool scored = false;
+  bool reproduced = false;   // the members are the children of reproduce()
   std::vector<int> reuse;
   ~Impl() {
     if (select) (void)hipFree(select);
@@ -2610,9 +2615,8 @@ Owner::Owner(const SearchSpec& spec) : impl_(std::make_unique<Impl>()) {
   Impl& O = *impl_;
   O.batch.add(spec);
   O.s = O.batch.impl_->searches[0].get();
-  if (O.s->cfg.schedule == Schedule::Generational)
-    throw Unsupported("the owner seam drives the steady-state schedule (generational searches run through gagp_batch)");
   O.batch.set_autotune(0);
+  O.batch.impl_->seam = true;
   O.batch.prepare();
   HIP_OK(hipMalloc(&O.select, sizeof(double) * std::size_t(O.s->P)));
   O.s->d.select = (GAS const double*)O.select;
@@ -2660,7 +2664,44 @@ void Owner::evaluate(double* fitness, unsigned* wrong, std::uint64_t* counts) {
   const std::vector<unsigned> w = s.down(s.d.wrong, std::size_t(s.P));
   std::copy(f.begin(), f.end(), fitness);
   std::copy(w.begin(), w.end(), wrong);
-  slot_counts(s, s.P, nullptr, counts);
+  // After reproduce(), member m is child m: reused children were not scored.
+  slot_counts(s, s.P, impl_->reproduced ? &impl_->reuse : nullptr, counts);
+}
+
+void Owner::reproduce(const double* selection, int pressure, double p_mut, std::uint64_t seed, int elites) {
+  Impl& O = *impl_;
+  auto& I = *O.batch.impl_;
+  SearchState& s = *O.s;
+  if (!s.d.generational) throw std::invalid_argument("reproduce() is the generational schedule's; steady-state searches make batches");
+  if (pressure < 1 || pressure > s.P) throw std::invalid_argument("tournament pressure outside 1..P");
+  if (!(p_mut >= 0 && p_mut <= 1)) throw std::invalid_argument("mutation probability outside [0, 1]");
+  if (elites < 0 || elites >= s.P) throw std::invalid_argument("elite count outside 0..P-1");
+  if (elites > kMaxElites) throw Unsupported("generational elitism above " + std::to_string(kMaxElites));
+  for (int m = 0; m < s.P; ++m)
+    if (std::isnan(selection[m])) throw std::invalid_argument("selection values must not be NaN");
+  if (pressure != s.d.pressure || p_mut != s.d.p_mut || elites != s.d.elites) {
+    s.d.pressure = pressure;
+    s.d.p_mut = p_mut;
+    s.d.elites = elites;
+    I.rebuild();
+  }
+  // Elites: highest selection value, ties to the lowest index.
+  std::vector<int> order(std::size_t(s.P));
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return selection[a] > selection[b]; });
+  HIP_OK(hipMemcpyAsync((void*)s.d.elite, order.data(), sizeof(int) * std::size_t(elites), hipMemcpyHostToDevice, I.stream));
+  HIP_OK(hipMemcpyAsync(O.select, selection, sizeof(double) * std::size_t(s.P), hipMemcpyHostToDevice, I.stream));
+  HIP_OK(hipMemcpyAsync((void*)s.d.round_seed, &seed, sizeof(seed), hipMemcpyHostToDevice, I.stream));
+  I.enqueue_front();
+  I.enqueue_eval();
+  I.launch("k_gen_place", I.g_member, k_gen_place, 128, 0);
+  I.launch("k_commit", I.g_member_w, k_commit, 64 * kVaryWaves, 0);
+  const int zero[3] = {0, 0, 0};
+  HIP_OK(hipMemcpyAsync((void*)s.d.n_pending, zero, sizeof(zero), hipMemcpyHostToDevice, I.stream));
+  HIP_OK(hipStreamSynchronize(I.stream));
+  if (const int e = s.device_error()) throw std::runtime_error("device reproduction error " + std::to_string(e));
+  O.reuse = s.down(s.d.reuse, std::size_t(s.P));
+  O.reproduced = true;
 }
 
 void Owner::node_counts(bool grammar_sites, unsigned* out) {
@@ -2678,6 +2719,7 @@ void Owner::make_batch(const double* selection, int pressure, double p_mut, std:
   Impl& O = *impl_;
   auto& I = *O.batch.impl_;
   SearchState& s = *O.s;
+  if (s.d.generational) throw std::invalid_argument("make_batch() is the steady-state schedule's; generational searches reproduce");
   if (O.made) throw std::invalid_argument("the previous batch is not committed");
   if (pressure < 1 || pressure > s.P) throw std::invalid_argument("tournament pressure outside 1..P");
   if (!(p_mut >= 0 && p_mut <= 1)) throw std::invalid_argument("mutation probability outside [0, 1]");

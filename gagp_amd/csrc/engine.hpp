@@ -115,10 +115,11 @@ class Batch {
 // The in-process seam of the reference's resident steady-state loop: the
 // owner adapter GpuPhaseOwner (cpp/src/evolution/repro/gpu/phase_population.hpp:15-59)
 // that run_phase_population_steady (cpp/src/evolution/evolve.cpp:331-441)
-// drives one round at a time. The caller keeps fitness, ranks, selection
+// drives one round at a time (and run_phase_population, evolve.cpp:253-322,
+// one generation at a time). The caller keeps fitness, ranks, selection
 // values and the run generator and chooses the victims; the owner keeps the
 // population on the device, makes and scores each batch, and commits the
-// caller's placements. Steady-state schedule only.
+// caller's placements, or replaces the whole population by reproduce().
 class Owner {
  public:
   // Builds the search, its initial population (generated, or the spec's
@@ -132,8 +133,15 @@ class Owner {
   bool has_wrong() const;
   // Scores of the members: ranking-canonical fitness (canonicalizing again
   // changes nothing), wrong train cases, and per member {program cases,
-  // errors, timeouts, fallbacks, unscored}.
+  // errors, timeouts, fallbacks, unscored} (zero for a member that reused
+  // its parent's result after reproduce()).
   void evaluate(double* fitness, unsigned* wrong, std::uint64_t* counts);
+  // Generational schedule: P children from tournaments on `selection`,
+  // crossover and mutation from `seed`, the `elites` members of highest
+  // selection value (ties to the lower index) copied into the first
+  // children, scored (a child equal to a member reuses its result), and the
+  // population replaced by them.
+  void reproduce(const double* selection, int pressure, double p_mut, std::uint64_t seed, int elites);
   // AST node counts, or grammar-site counts (`grammar_sites`), per member.
   void node_counts(bool grammar_sites, unsigned* out);
   void begin_steady(int batch);

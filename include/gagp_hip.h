@@ -273,7 +273,9 @@ typedef struct gagp_owner gagp_owner;
 /* Builds the search of a task (search_path, population, seed and init_path
  * as gagp_batch_add_init), its initial population generated (or read),
  * lowered and scored on the device. Declines (GAGP_ERR_UNSUPPORTED) as
- * gagp_batch_add does, and for the generational schedule.
+ * gagp_batch_add does. A steady-state search is driven by begin_steady,
+ * make_batch, evaluate_batch and commit_batch; a generational one by
+ * reproduce and evaluate.
  * Reference: GpuPhaseOwner construction, evolve.cpp:457-470. */
 int gagp_owner_create(const char* task_path, const char* search_path, int32_t population, int32_t seed_is_set, uint64_t seed,
                       const char* init_path, gagp_owner** out);
@@ -297,6 +299,16 @@ int gagp_owner_begin_steady(gagp_owner* o, int32_t batch);
  * lowering; reuse[b] is the member child b equals, or -1. */
 int gagp_owner_make_batch(gagp_owner* o, const double* selection, int32_t pressure, double p_mut, uint64_t seed,
                           int32_t* reuse);
+/* Generational schedule: reproduce(selection[P], pressure, p_mut, seed,
+ * elites) replaces the population by P children (tournaments on the
+ * selection values; the `elites` members of highest value, ties to the
+ * lower index, copied into the first children), scored on the device; a
+ * child equal to a member reuses its result. gagp_owner_evaluate then
+ * reports the new members (counts zero for reused ones).
+ * Reference: NativePhasePopulation::reproduce, driven by run_phase_population
+ * (cpp/src/evolution/evolve.cpp:253-322). */
+int gagp_owner_reproduce(gagp_owner* o, const double* selection, int32_t pressure, double p_mut, uint64_t seed,
+                         int32_t elites);
 /* evaluate_batch: fitness[B], wrong[B], counts[B][5] of the children whose
  * reuse is -1 (the others are the caller's: their source's scores). */
 int gagp_owner_evaluate_batch(gagp_owner* o, double* fitness, uint32_t* wrong, uint64_t* counts);

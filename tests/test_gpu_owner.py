@@ -1,11 +1,12 @@
-"""The owner seam on the device (gagp_owner_*): the reference's steady-state
-generation loop, restated here from run_phase_population_steady
-(cpp/src/evolution/evolve.cpp:331-441), drives the device population one
-round at a time -- host-side ranks, selection values, std::mt19937_64 seeds
-and victims, device-side batches and commits -- and must reproduce the
-reference CPU engine's golden runs: every generation's best fitness, mean
-fitness, best wrong count and best program size, and the final best
-program, byte for byte."""
+"""The owner seam on the device (gagp_owner_*): the reference's generation
+loops, restated here from run_phase_population_steady
+(cpp/src/evolution/evolve.cpp:331-441) and run_phase_population (:253-322),
+drive the device population one round or generation at a time -- host-side
+ranks, selection values, std::mt19937_64 seeds and victims, device-side
+batches, commits and reproduction -- and must reproduce the reference CPU
+engine's golden runs: every generation's best fitness, mean fitness, best
+wrong count and best program size, and the final best program, byte for
+byte."""
 import ctypes
 import functools
 import json
@@ -136,6 +137,81 @@ def drive(name):
     return history, {"fitness": fitness[b], "wrong": wrong[b] if has_wrong else -1, "program": owner.export_member(b)}
 
 
+GENERATIONAL = ["polynomial_gen_e1_p256_g12_s3", "poly_c1_gen_e3r_p512_g24_s9", "list_count_gt_rev_gen_e1_p256_g8_s2",
+                "stmt_count_gt_gen_e1_p256_g8_s2", "fit_poly_c1_disc_gen_e3r_p256_g10_s6", "init_poly_c1_gen_e3r_p512_g12_s4"]
+
+
+def mean(fitness):
+    total = np.longdouble(0)
+    for f in fitness:
+        total += np.longdouble(f)
+    return float(total / np.longdouble(len(fitness)))
+
+
+def drive_generational(name):
+    """run_phase_population over a gagp_amd.Owner."""
+    m = MANIFEST[name]
+    gold = json.load(open(os.path.join(GOLDEN, "runs", name, "result.json")))
+    search = gold["search"]
+    P, G, seed = m["population"], m["generations"], m["seed"]
+    owner = gagp_amd.Owner(os.path.join(GOLDEN, m["task"]), os.path.join(GOLDEN, m["search"]) if "search" in m else None,
+                           P, seed, os.path.join(GOLDEN, m["init"]) if "init" in m else None)
+    pressure = min(max(search["selection"]["size"], 1), P)
+    p_mut, elites = search["mutation"]["probability"], search["elitism"]
+    sites = search["ranking"]["tie_break"] == "grammar_sites"
+    rng = Words(seed, G + 16)
+    state = {}
+
+    def evaluate():
+        fitness, wrong, counts = owner.evaluate()
+        has_wrong = owner.has_wrong
+        nodes = owner.node_counts(sites) if has_wrong else []
+
+        def better(a, b):
+            if fitness[a] != fitness[b]:
+                return fitness[a] > fitness[b]
+            if has_wrong:
+                if wrong[a] != wrong[b]:
+                    return wrong[a] < wrong[b]
+                if nodes[a] != nodes[b]:
+                    return nodes[a] < nodes[b]
+            return a < b
+
+        ranking = sorted(range(P), key=functools.cmp_to_key(lambda a, b: -1 if better(a, b) else (1 if better(b, a) else 0)))
+        selection = list(fitness)
+        if has_wrong:
+            for r, member in enumerate(ranking):
+                selection[member] = float(P - r)
+        state.update(fitness=fitness, wrong=wrong if has_wrong else None, ranking=ranking, selection=selection,
+                     counts=counts)
+
+    history = []
+    for gen in range(G):
+        evaluate()
+        b = state["ranking"][0]
+        history.append({"best_fitness": state["fitness"][b], "mean_fitness": mean(state["fitness"]),
+                        "best_wrong": state["wrong"][b] if state["wrong"] is not None else -1,
+                        "best_nodes": len(owner.export_member(b)["nodes"])})
+        owner.reproduce(state["selection"], pressure, p_mut, rng(), elites)
+    evaluate()
+    # reused members (the elites at least) were not scored again
+    assert sum(1 for c in state["counts"] if c[0] == 0) >= elites
+    b = state["ranking"][0]
+    return history, {"fitness": state["fitness"][b], "wrong": state["wrong"][b] if state["wrong"] is not None else -1,
+                     "program": owner.export_member(b)}
+
+
+@pytest.mark.parametrize("name", GENERATIONAL)
+def test_owner_generational_loop_reproduces_reference_run(name):
+    print("owner generational", name, flush=True)
+    history, best = drive_generational(name)
+    gold = json.load(open(os.path.join(GOLDEN, "runs", name, "result.json")))
+    want = [{k: row[k] for k in ("best_fitness", "mean_fitness", "best_wrong", "best_nodes")} for row in gold["history"]]
+    assert history == want
+    assert best["fitness"] == gold["best"]["fitness"] and best["wrong"] == gold["best"]["wrong"]
+    assert best["program"] == gold["best"]["program"]
+
+
 @pytest.mark.parametrize("name", RUNS)
 def test_owner_loop_reproduces_reference_run(name):
     print("owner", name, flush=True)
@@ -161,3 +237,5 @@ def test_owner_rejects_out_of_order_calls():
     with pytest.raises(gagp_amd.GagpError):
         owner.commit_batch([(owner.P, 0)])  # member outside the population
     owner.commit_batch([(0, 0)])
+    with pytest.raises(gagp_amd.GagpError):
+        owner.reproduce([0.0] * owner.P, 2, 0.3, 1, 1)  # a steady-state search
