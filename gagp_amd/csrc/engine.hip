@@ -1188,17 +1188,18 @@ __global__ __launch_bounds__(kPlaceThreads) void k_place(DSearch* all) {
     ridx[i] = mod_p(x);
   }
   __syncthreads();
-  const auto refill = [&](int need) {  // every thread, uniformly
-    while (tail - head < need) {
-      mt_twist<kPlaceThreads>(mt);
-      for (int i = t; i < 312; i += kPlaceThreads) {
-        const std::uint64_t x = mt_temper(mt[i]);
-        ring[(tail + i) & (kRing - 1)] = x;
-        ridx[(tail + i) & (kRing - 1)] = mod_p(x);
-      }
-      tail += 312;
-      __syncthreads();
+  const auto twist = [&] {  // every thread, uniformly: 312 more words at the tail
+    mt_twist<kPlaceThreads>(mt);
+    for (int i = t; i < 312; i += kPlaceThreads) {
+      const std::uint64_t x = mt_temper(mt[i]);
+      ring[(tail + i) & (kRing - 1)] = x;
+      ridx[(tail + i) & (kRing - 1)] = mod_p(x);
     }
+    tail += 312;
+    __syncthreads();
+  };
+  const auto refill = [&](int need) {
+    while (tail - head < need) twist();
   };
   const int W = rs < kPlaceWords / 64 ? 64 : kPlaceWords / rs;
   const std::uint64_t below = (1ull << lane) - 1;
@@ -1214,9 +1215,14 @@ __global__ __launch_bounds__(kPlaceThreads) void k_place(DSearch* all) {
     if (64 + lane < B) { f1 = c_fit[64 + lane]; w1 = c_wrong[64 + lane]; n1 = c_nodes[64 + lane]; }
   }
   while (b < B) {
-    refill(W * rs);
-    if (leader) {
-      bool stop = ++chunk > chunk_limit;
+    // The ring is filled with whole twists (at least kRing - 311 unconsumed
+    // words, above one chunk's W * rs <= kPlaceWords), and the leader runs
+    // chunks until fewer words than a chunk's are left, so the block meets
+    // at a barrier once per fill rather than once per chunk.
+    while (tail - head + 312 <= kRing) twist();
+    bool stop = false;
+    while (leader && !stop && b < B && tail - head >= W * rs) {
+      stop = ++chunk > chunk_limit;
       if (stop && lane == 0) raise(d, 500);
       while (!stop && b >= win + 64) {  // slide the windows
         f0 = f1; w0 = w1; n0 = n1;
@@ -1278,8 +1284,8 @@ __global__ __launch_bounds__(kPlaceThreads) void k_place(DSearch* all) {
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
         __builtin_amdgcn_wave_barrier();
       }
-      if (lane == 0) { s_head = head; s_b = stop ? B : b; s_best = best; }
     }
+    if (leader && lane == 0) { s_head = head; s_b = stop ? B : b; s_best = best; }
     __syncthreads();
     head = s_head;
     b = s_b;
