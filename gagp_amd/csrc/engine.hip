@@ -1142,7 +1142,9 @@ __global__ __launch_bounds__(kPlaceThreads) void k_place(DSearch* all) {
   extern __shared__ __attribute__((aligned(16))) unsigned char place_smem[];
   __shared__ std::uint64_t ring[kRing];
   __shared__ unsigned ridx[kRing];
-  __shared__ std::uint64_t mt[312];
+  // Generator state, double-buffered: a twist reads one buffer and writes
+  // the other (`cur` is the buffer holding the current state).
+  __shared__ std::uint64_t mt[2][312];
   __shared__ int s_head, s_b, s_best;
   const int t = threadIdx.x, lane = t & 63;
   const bool leader = wave_index() == 0;
@@ -1180,7 +1182,8 @@ __global__ __launch_bounds__(kPlaceThreads) void k_place(DSearch* all) {
       }
     }
   }
-  for (int i = t; i < 312; i += kPlaceThreads) mt[i] = d.mt[i];
+  int cur = 0;
+  for (int i = t; i < 312; i += kPlaceThreads) mt[0][i] = d.mt[i];
   int head = 0, tail = *d.n_left;
   for (int i = t; i < tail; i += kPlaceThreads) {
     const std::uint64_t x = d.mt_left[i];
@@ -1188,15 +1191,24 @@ __global__ __launch_bounds__(kPlaceThreads) void k_place(DSearch* all) {
     ridx[i] = mod_p(x);
   }
   __syncthreads();
-  const auto twist = [&] {  // every thread, uniformly: 312 more words at the tail
-    mt_twist<kPlaceThreads>(mt);
-    for (int i = t; i < 312; i += kPlaceThreads) {
-      const std::uint64_t x = mt_temper(mt[i]);
+  // Every thread, uniformly: the next state into the other buffer and its
+  // 312 tempered words at the ring's tail. Words 0..155 read only the old
+  // state; 156..310 read the new word i-156, and 311 the new words 0 and 155.
+  const auto twist = [&] {
+    const int nxt = cur ^ 1;
+    const auto put = [&](int i, std::uint64_t w) {
+      mt[nxt][i] = w;
+      const std::uint64_t x = mt_temper(w);
       ring[(tail + i) & (kRing - 1)] = x;
       ridx[(tail + i) & (kRing - 1)] = mod_p(x);
-    }
-    tail += 312;
+    };
+    for (int i = t; i < 156; i += kPlaceThreads) put(i, mt_next_word(mt[cur][i], mt[cur][i + 1], mt[cur][i + 156]));
     __syncthreads();
+    for (int i = 156 + t; i < 312; i += kPlaceThreads)
+      put(i, mt_next_word(mt[cur][i], i == 311 ? mt[nxt][0] : mt[cur][i + 1], mt[nxt][i - 156]));
+    __syncthreads();
+    cur = nxt;
+    tail += 312;
   };
   const auto refill = [&](int need) {
     while (tail - head < need) twist();
@@ -1297,7 +1309,7 @@ __global__ __launch_bounds__(kPlaceThreads) void k_place(DSearch* all) {
   ++head;
   const int left = tail - head;
   for (int i = t; i < left; i += kPlaceThreads) d.mt_left[i] = ring[(head + i) & (kRing - 1)];
-  for (int i = t; i < 312; i += kPlaceThreads) d.mt[i] = mt[i];
+  for (int i = t; i < 312; i += kPlaceThreads) d.mt[i] = mt[cur][i];
   if (t == 0) {
     *d.n_left = left;
     *d.round_seed = next_seed;
