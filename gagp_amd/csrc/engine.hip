@@ -1216,6 +1216,8 @@ __global__ __launch_bounds__(kPlaceThreads) void k_place(DSearch* all) {
   const int W = rs < kPlaceWords / 64 ? 64 : kPlaceWords / rs;
   const std::uint64_t below = (1ull << lane) - 1;
   int b = 0, best = *d.best;
+  double best_f = keys.fit(best);
+  std::uint64_t best_o = has_wrong ? keys.order(best) : 0;
   unsigned chunk = 0;
   const unsigned chunk_limit = 4u * unsigned(B) + 4096u;
   // Child-key windows [win, win+64) and [win+64, win+128), lane l holding child win+l.
@@ -1247,10 +1249,17 @@ __global__ __launch_bounds__(kPlaceThreads) void k_place(DSearch* all) {
         const bool active = lane < W;
         const int base = head + lane * rs;
         int worst = -1;
+        // The first four draws stay in registers for the conflict test
+        // (named registers: a dynamically indexed array would spill).
+        int d0 = 0, d1 = 0, d2 = 0, d3 = 0;
         if (active) {
           worst = int(ridx[base & (kRing - 1)]);
+          d0 = worst;
           for (int i = 1; i < rs; ++i) {
             const int c = int(ridx[(base + i) & (kRing - 1)]);
+            d1 = i == 1 ? c : d1;
+            d2 = i == 2 ? c : d2;
+            d3 = i == 3 ? c : d3;
             if (keys.better(worst, c)) worst = c;
           }
         }
@@ -1274,10 +1283,15 @@ __global__ __launch_bounds__(kPlaceThreads) void k_place(DSearch* all) {
         bool conflict = false;
         if (active)
           for (int i = 0; i < rs; ++i) {
-            const unsigned m = mark[ridx[(base + i) & (kRing - 1)]];
+            const int drawn = i == 0 ? d0 : i == 1 ? d1 : i == 2 ? d2 : i == 3 ? d3 : int(ridx[(base + i) & (kRing - 1)]);
+            const unsigned m = mark[drawn];
             conflict = conflict || ((m >> 6) == chunk && int(63u - (m & 63u)) < lane);
           }
-        const bool raises = fills && keys.key_better(cf, keys.pack(cw, cn), worst, best);
+        // `better` of the child (at the victim's index) against the best
+        // member, whose keys stay in registers: the best is never a victim,
+        // so they change only when a child becomes the best.
+        const std::uint64_t co = keys.pack(cw, cn);
+        const bool raises = fills && (cf != best_f ? cf > best_f : has_wrong && co != best_o ? co < best_o : worst < best);
         int cut = W;
         const std::uint64_t last = __ballot(fills && child == B - 1);
         if (last) cut = min(cut, __ffsll((long long)last));
@@ -1290,7 +1304,11 @@ __global__ __launch_bounds__(kPlaceThreads) void k_place(DSearch* all) {
           keys.store(worst, cf, cw, cn);
           placed[worst] = child;
         }
-        if (lifter < cut) best = __shfl(worst, lifter, 64);
+        if (lifter < cut) {
+          best = __shfl(worst, lifter, 64);
+          best_f = __shfl(cf, lifter, 64);
+          best_o = __shfl(co, lifter, 64);
+        }
         b += __popcll(cut >= 64 ? pmask : pmask & ((1ull << cut) - 1));
         head += cut * rs;
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
