@@ -1278,7 +1278,10 @@ __global__ __launch_bounds__(kPlaceThreads) void k_place(DSearch* all) {
         const unsigned cw = unsigned(k < 64 ? b0 : b1);
         const unsigned cn = unsigned(k < 64 ? c0 : c1);
         if (place) atomicMax(&mark[worst], chunk * 64u + unsigned(63 - lane));
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        // LDS operations of one wavefront complete in issue order, so the
+        // mark reads below see every lane's maximum without a wait between
+        // them; keys in HBM need the fence.
+        if constexpr (!kLds) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
         __builtin_amdgcn_wave_barrier();
         bool conflict = false;
         if (active)
@@ -1311,7 +1314,7 @@ __global__ __launch_bounds__(kPlaceThreads) void k_place(DSearch* all) {
         }
         b += __popcll(cut >= 64 ? pmask : pmask & ((1ull << cut) - 1));
         head += cut * rs;
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        if constexpr (!kLds) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
         __builtin_amdgcn_wave_barrier();
       }
     }
