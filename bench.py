@@ -37,7 +37,7 @@ HBM_PEAK_GBS = 8000.0
 GRAPH = True
 REF_GAGP = os.path.join(ROOT, "oracle", "_ref", "gagp")
 TRAFFIC = os.path.join(ROOT, "profiles", "quintic_p4096_hbm_traffic.md")
-ROUND_KERNELS = ("k_record", "k_vary", "k_vary_global", "k_clear", "k_index", "k_reuse", "k_eval", "k_eval_tiles", "k_eval_general",
+ROUND_KERNELS = ("k_record", "k_vary", "k_vary_global", "k_clear", "k_eval", "k_eval_tiles", "k_eval_general",
                  "k_tally", "k_place", "k_place_hbm", "k_commit")
 
 
@@ -52,15 +52,17 @@ def algorithmic_bytes(c, steps):
     return {
         "k_eval": 8 * per["code_words"] + 16 * per["pool_words"] + case_bytes + 28 * per["programs"],
         "k_eval_tiles": 8 * per["code_words"] + 16 * per["pool_words"] + case_bytes + 28 * per["programs"],
+        # rows read and written, code, per-child keys, hash and stats, the
+        # member-table probe (key, member, member hash) and both rows of every
+        # child whose hash matches a member (compared in full before reuse)
         "k_vary": row(per["parent_nodes"], per["parent_consts"]) + row(per["child_nodes"], per["child_consts"])
-        + 8 * per["child_code_words"] + 16 * per["child_pool_words"] + 60 * per["children"],
-        "k_place": 16 * POPULATION + 16 * per["children"] + 20 * per["children"],
-        # hash and stats per child, plus both rows of every child whose hash
-        # matches a member (compared in full before its result is reused)
-        "k_reuse": 24 * per["children"] + 2 * max(0.0, per["children"] - per["programs"]) * (
+        + 8 * per["child_code_words"] + 16 * per["child_pool_words"] + 60 * per["children"] + 20 * per["children"]
+        + 2 * max(0.0, per["children"] - per["programs"]) * (
             (4 * per["child_nodes"] + 16 * per["child_consts"]) / max(1.0, per["children"])),
-        "k_index": 8 * POPULATION + 12 * POPULATION,
-        "k_commit": row(per["child_nodes"], per["child_consts"]) * 2 + 4 * POPULATION,
+        "k_place": 16 * POPULATION + 16 * per["children"] + 20 * per["children"],
+        # placed rows copied, placements, and every member's hash indexed
+        # (key and member) in the next round's table
+        "k_commit": row(per["child_nodes"], per["child_consts"]) * 2 + 4 * POPULATION + 8 * POPULATION + 12 * POPULATION,
     }
 
 

@@ -4,12 +4,14 @@
 // bodies of cpp/src/evolution/repro/gpu/phase_population_cpu.cpp:314-394) is
 // the kernel sequence
 //   k_record  history row of generation r/per_epoch (best row, fitness vector)
-//   k_vary    tournaments, crossover, mutation, compare, lowering (thread/child)
-//   k_clear + k_index   hash table of the current members
-//   k_reuse   completed-result reuse; pending list of children to score
+//   k_vary    tournaments, crossover, mutation, compare, completed-result
+//             reuse through the member table, lowering, pending lists
+//             (wavefront/child)
 //   k_eval    bytecode VM + canonical reduction (wavefront/program)
-//   k_place   mt19937_64 inverse-tournament placement (block/search)
-//   k_commit  placed children overwrite their members (thread/member)
+//   k_place   mt19937_64 inverse-tournament placement (block/search), with
+//             k_clear emptying the member table beside it
+//   k_commit  placed children overwrite their members; every member is
+//             indexed in the next round's table (wavefront/member)
 // over every search of a batch (grid.y = search). Nothing returns to the
 // host between rounds.
 #include <hip/hip_runtime.h>
@@ -195,21 +197,24 @@ __global__ __launch_bounds__(64 * kVaryWaves) void k_init_pop(DSearch* all, cons
 
 // ---- lowering: one wavefront per row ----
 // kWalks: the lowering instantiation with traversals (grammars that have them).
+// Returns the code flags it stores (code_jumps).
 template <bool kWalks>
-__device__ inline void lower_into_w(const PhaseView& g, const DSearch& d, const CRow& row, int slot, WalkStore<kWalks>& ws) {
+__device__ inline int lower_into_w(const PhaseView& g, const DSearch& d, const CRow& row, int slot, WalkStore<kWalks>& ws) {
   Ins* code = d.code + std::size_t(slot) * d.code_cap;
   Val* pool = d.pool + std::size_t(slot) * d.pool_cap;
   const LowerResult r =
       lower_phase_w<kWalks>(g, row, d.local_types, d.n_locals, d.expected_type, code, d.code_cap, pool, d.pool_cap, ws);
   const bool ok = r.status == ST_OK && r.stack_bound >= 1 && r.stack_bound <= d.stack_slots &&
                   r.locals - d.n_locals <= d.temp_slots - d.input_shadow;
+  // bit 0: jumps; bit 1: typed straight-line (vm_fast); bit 2: Bool result; bit 3: typed looping (vm_loop)
+  const int flags = r.jumps | (r.fast ? 2 : 0) | (r.result_type == TAG_BOOL ? 4 : 0) | (r.loop ? 8 : 0);
   if (wlane() == 0) {
     if (!ok) raise(d, 200 + r.status);
     d.code_len[slot] = ok ? r.code : 0;
     d.pool_len[slot] = ok ? r.constants : 0;
-    // bit 0: jumps; bit 1: typed straight-line (vm_fast); bit 2: Bool result; bit 3: typed looping (vm_loop)
-    d.code_jumps[slot] = r.jumps | (r.fast ? 2 : 0) | (r.result_type == TAG_BOOL ? 4 : 0) | (r.loop ? 8 : 0);
+    d.code_jumps[slot] = flags;
   }
+  return flags;
 }
 template <bool kWalks>
 __global__ __launch_bounds__(64 * kVaryWaves) void k_lower_members(DSearch* all, const BlockMap* map) {
@@ -220,6 +225,72 @@ __global__ __launch_bounds__(64 * kVaryWaves) void k_lower_members(DSearch* all,
   __shared__ WalkStore<kWalks> walk_lds[kVaryWaves];
   lower_into_w<kWalks>(view_of(d.g), d, member_row(d, m), m, walk_lds[wave_index()]);
 }
+
+// ---- member hash table (parent_fitness_cache.cuh:11-58) ----
+// One entry per distinct genotype hash: identical members share a hash, and
+// any one of them carries the result a child identical to them reuses (the
+// rows are compared in full before a result is reused). Keys are hash | 1,
+// 0 marks an empty slot. A round reads the table its previous round's
+// commit built (the initialization builds the first); it is cleared while
+// the previous round places its children.
+__global__ void k_clear(DSearch* all, const BlockMap* map) {
+  const BlockMap bm = map[blockIdx.x];
+  const DSearch& d = all[bm.search];
+  const int i = bm.block * blockDim.x + threadIdx.x;
+  if (!d.active || i > d.table_mask) return;
+  d.table_key[i] = 0;
+}
+// Indexes member m under hash h (one thread).
+__device__ inline void insert_member(const DSearch& d, int m, std::uint64_t h) {
+  const std::uint64_t key = h | 1;
+  for (int probe = 0; probe <= d.table_mask; ++probe) {
+    const int slot = int((h + std::uint64_t(probe)) & std::uint64_t(d.table_mask));
+    const unsigned long long seen = atomicCAS(reinterpret_cast<unsigned long long*>(d.table_key) + slot, 0ull, key);
+    if (seen == 0) { d.table[slot] = m; return; }
+    if (seen == key) return;  // a member with this hash is indexed
+  }
+  raise(d, 400);
+}
+__global__ void k_index(DSearch* all, const BlockMap* map) {
+  const BlockMap bm = map[blockIdx.x];
+  const DSearch& d = all[bm.search];
+  const int m = bm.block * blockDim.x + threadIdx.x;
+  if (!search_live(d) || m >= d.P) return;
+  insert_member(d, m, d.member_hash[m]);
+}
+// The member a child identical to it reuses, or -1: the table probe is
+// uniform and the row comparison runs across the lanes (rows_equal's test).
+__device__ inline int find_member_w(const DSearch& d, const CRow& child, std::uint64_t h) {
+  const std::uint64_t key = h | 1;
+  for (int probe = 0; probe <= d.table_mask; ++probe) {
+    const int slot = int((h + std::uint64_t(probe)) & std::uint64_t(d.table_mask));
+    const std::uint64_t k = d.table_key[slot];
+    if (k == 0) return -1;
+    if (k != key) continue;
+    const int m = d.table[slot];
+    if (d.member_hash[m] != h) return -1;
+    const CRow member = member_row(d, m);
+    if (member.n != child.n || member.nc != child.nc) return -1;
+    bool mine = false;
+    for (int i = wlane(); i < child.n; i += 64) mine = mine || member.origin[i] != child.origin[i];
+    for (int i = wlane(); i < child.nc; i += 64) {
+      const Val x = member.cval[i], y = child.cval[i];
+      mine = mine || x.tag != y.tag || (x.tag == TAG_BOOL ? x.b != y.b : x.i != y.i);
+    }
+    return __ballot(mine) != 0 ? -1 : m;
+  }
+  return -1;
+}
+// The evaluation kernel that scores a program: 1 the typed straight-line
+// path, 2 the typed looping path (both with an LDS score buffer, so only
+// where the cases fit it), 0 the general VM.
+__device__ inline int eval_class(int code_flags, int C) {
+  if (C > kScoreCases) return 0;
+  return (code_flags & 2) ? 1 : (code_flags & 8) ? 2 : 0;
+}
+// Pending list of a round: general children from the front of [0, B), the
+// straight-line ones from its back, the looping ones in [B, 2B).
+__device__ inline int pending_at(int B, int kind, int i) { return kind == 0 ? i : kind == 1 ? B - 1 - i : B + i; }
 
 // ---- reproduction: one wavefront per child (phase_operators.cuh:156-265) ----
 // kStaged: the block first copies the phase tables into LDS (searches whose
@@ -265,115 +336,36 @@ __global__ __launch_bounds__(64 * kVaryWaves) void k_vary(DSearch* all, const Bl
     if (st.error) raise(d, 300 + st.error);
   }
   if (st.error) return;
-  // A child identical to its parent takes the parent's completed result
-  // (k_reuse) and is never evaluated, so it is not lowered (about a fifth
-  // of the children: no crossover and no mutation).
-  if (st.reusable_parent >= 0) {
+  // A child identical to its parent, or to any member, takes that member's
+  // completed result and is neither lowered nor evaluated. Results are
+  // fixed before any placement of the round.
+  const int r = st.reusable_parent >= 0 ? st.reusable_parent : find_member_w(d, child, h);
+  if (r >= 0) {
     if (wlane() == 0) {
       d.code_len[c] = 0;
       d.pool_len[c] = 0;
-    }
-    return;
-  }
-  __shared__ WalkStore<kWalks> walk_lds[kVaryWaves];
-  lower_into_w<kWalks>(g, d, child, c, walk_lds[wave_index()]);
-}
-
-// ---- member hash table (parent_fitness_cache.cuh:11-58) ----
-// One entry per distinct genotype hash: identical members share a hash, and
-// any one of them carries the result a child identical to them reuses (the
-// rows are compared in full before a result is reused). Keys are hash | 1,
-// 0 marks an empty slot.
-__global__ void k_clear(DSearch* all, const BlockMap* map) {
-  const BlockMap bm = map[blockIdx.x];
-  const DSearch& d = all[bm.search];
-  const int i = bm.block * blockDim.x + threadIdx.x;
-  if (!search_live(d) || i > d.table_mask) return;
-  d.table_key[i] = 0;
-}
-__global__ void k_index(DSearch* all, const BlockMap* map) {
-  const BlockMap bm = map[blockIdx.x];
-  const DSearch& d = all[bm.search];
-  const int m = bm.block * blockDim.x + threadIdx.x;
-  if (!search_live(d) || m >= d.P) return;
-  const std::uint64_t h = d.member_hash[m], key = h | 1;
-  for (int probe = 0; probe <= d.table_mask; ++probe) {
-    const int slot = int((h + std::uint64_t(probe)) & std::uint64_t(d.table_mask));
-    const unsigned long long seen = atomicCAS(reinterpret_cast<unsigned long long*>(d.table_key) + slot, 0ull, key);
-    if (seen == 0) { d.table[slot] = m; return; }
-    if (seen == key) return;  // a member with this hash is indexed
-  }
-  raise(d, 400);
-}
-// The evaluation kernel that scores a program: 1 the typed straight-line
-// path, 2 the typed looping path (both with an LDS score buffer, so only
-// where the cases fit it), 0 the general VM.
-__device__ inline int eval_class(int code_flags, int C) {
-  if (C > kScoreCases) return 0;
-  return (code_flags & 2) ? 1 : (code_flags & 8) ? 2 : 0;
-}
-// Pending list of a round: general children from the front of [0, B), the
-// straight-line ones from its back, the looping ones in [B, 2B).
-__device__ inline int pending_at(int B, int kind, int i) { return kind == 0 ? i : kind == 1 ? B - 1 - i : B + i; }
-// One wavefront per child: the table probe is uniform and the row
-// comparison runs across the lanes (rows_equal's test).
-__global__ __launch_bounds__(64 * kVaryWaves) void k_reuse(DSearch* all, const BlockMap* map) {
-  const BlockMap bm = map[blockIdx.x];
-  const DSearch& d = all[bm.search];
-  const int c = __builtin_amdgcn_readfirstlane(bm.block * kVaryWaves + wave_index());
-  if (!search_live(d) || c >= d.B) return;
-  const int lane = wlane();
-  int r = d.stats[c].reusable_parent;
-  const CRow child = child_row(d, c);
-  if (r < 0) {
-    const std::uint64_t h = d.child_hash[c];
-    const std::uint64_t key = h | 1;
-    for (int probe = 0; probe <= d.table_mask; ++probe) {
-      const int slot = int((h + std::uint64_t(probe)) & std::uint64_t(d.table_mask));
-      const std::uint64_t k = d.table_key[slot];
-      if (k == 0) break;
-      if (k != key) continue;
-      const int m = d.table[slot];
-      if (d.member_hash[m] == h) {
-        const CRow member = member_row(d, m);
-        bool differ = member.n != child.n || member.nc != child.nc;
-        if (!differ) {
-          bool mine = false;
-          for (int i = lane; i < child.n; i += 64) mine = mine || member.origin[i] != child.origin[i];
-          for (int i = lane; i < child.nc; i += 64) {
-            const Val x = member.cval[i], y = child.cval[i];
-            mine = mine || x.tag != y.tag || (x.tag == TAG_BOOL ? x.b != y.b : x.i != y.i);
-          }
-          differ = __ballot(mine) != 0;
-        }
-        if (!differ) r = m;
-      }
-      break;
-    }
-  }
-  if (r >= 0) {
-    if (lane == 0) {
       d.reuse[c] = r;
-      // Results are fixed before any placement of the round.
       d.c_fit[c] = d.fitness[r];
       d.c_wrong[c] = d.wrong[r];
       d.c_nodes[c] = d.nodes[r];
     }
     return;
   }
+  __shared__ WalkStore<kWalks> walk_lds[kVaryWaves];
+  const int flags = lower_into_w<kWalks>(g, d, child, c, walk_lds[wave_index()]);
   unsigned n;
   if (d.grammar_sites) {
     unsigned mine = 0;
-    for (int i = lane; i < child.n; i += 64) mine += child.site[i] >= 0;
+    for (int i = wlane(); i < child.n; i += 64) mine += child.site[i] >= 0;
     for (int off = 32; off > 0; off >>= 1) mine += __shfl_xor(mine, off, 64);
     n = mine;
   } else {
     n = unsigned(d.fixed_nodes) + unsigned(child.n);
   }
-  if (lane == 0) {
-    d.reuse[c] = r;
+  if (wlane() == 0) {
+    d.reuse[c] = -1;
     d.c_nodes[c] = n;
-    const int k = eval_class(d.code_jumps[c], d.cd.C);
+    const int k = eval_class(flags, d.cd.C);
     d.pending[pending_at(d.B, k, atomicAdd(d.n_pending + k, 1))] = c;
   }
 }
@@ -1430,19 +1422,25 @@ __global__ __launch_bounds__(64 * kVaryWaves) void k_commit(DSearch* all, const 
   const int m = __builtin_amdgcn_readfirstlane(bm.block * kVaryWaves + wave_index());
   if (!d.active || m >= d.P) return;
   const int c = d.placed[m];
-  if (c < 0) return;
-  const CRow src = child_row(d, c);
-  const std::size_t o = std::size_t(m) * d.R;
-  GAS int* const origin = d.cur_origin + o;
-  GAS int* const site = d.cur_site + o;
-  GAS Val* const cval = d.cur_cval + o;
-  for (int i = wlane(); i < src.n; i += 64) { origin[i] = src.origin[i]; site[i] = src.site[i]; }
-  for (int i = wlane(); i < src.nc; i += 64) cval[i] = src.cval[i];
+  if (c >= 0) {
+    const CRow src = child_row(d, c);
+    const std::size_t o = std::size_t(m) * d.R;
+    GAS int* const origin = d.cur_origin + o;
+    GAS int* const site = d.cur_site + o;
+    GAS Val* const cval = d.cur_cval + o;
+    for (int i = wlane(); i < src.n; i += 64) { origin[i] = src.origin[i]; site[i] = src.site[i]; }
+    for (int i = wlane(); i < src.nc; i += 64) cval[i] = src.cval[i];
+  }
   if (wlane() == 0) {
-    d.cur_n[m] = src.n;
-    d.cur_nc[m] = src.nc;
-    d.member_hash[m] = d.child_hash[c];
-    d.placed[m] = -1;
+    const std::uint64_t h = c >= 0 ? d.child_hash[c] : d.member_hash[m];
+    if (c >= 0) {
+      d.cur_n[m] = d.x_n[c];
+      d.cur_nc[m] = d.x_nc[c];
+      d.member_hash[m] = h;
+      d.placed[m] = -1;
+    }
+    // The next round's member table (cleared during placement).
+    if (search_live(d)) insert_member(d, m, h);
   }
 }
 
@@ -1954,7 +1952,7 @@ struct Batch::Impl {
   struct Grid { GAS BlockMap* map = nullptr; int blocks = 0; };
   DeviceArena map_arena;
   Grid g_init, g_lower, g_eval_init, g_vary_staged, g_vary_global, g_vary_staged_w, g_vary_global_w, g_table, g_member,
-      g_eval, g_eval_tiles, g_eval_general, g_eval_loop, g_child_w, g_member_w;
+      g_eval, g_eval_tiles, g_eval_general, g_eval_loop, g_member_w;
   // Evaluation launch shapes of the round (the autotuner's choice, or the
   // fixed default: one program per wavefront, one 64-case tile per item).
   TuneChoice tune;
@@ -2002,7 +2000,6 @@ struct Batch::Impl {
     g_table = build_grid(map_arena, live, [&](const SearchState& s) { return up(s.d.table_mask + 1, 256); });
     g_member = build_grid(map_arena, live, [&](const SearchState& s) { return up(s.P, 128); });
     // one wavefront per child / member
-    g_child_w = build_grid(map_arena, live, [&](const SearchState& s) { return up(s.B, kVaryWaves); });
     g_member_w = build_grid(map_arena, live, [&](const SearchState& s) { return up(s.P, kVaryWaves); });
     g_eval = typed_grid(map_arena, tune.fast_programs);
     g_eval_tiles = tiles_grid(map_arena, live, false);
@@ -2155,6 +2152,11 @@ struct Batch::Impl {
     timer.time("k_tally_init", stream, [&] { hipLaunchKernelGGL(k_tally, dim3(kTallyBlocks, n), dim3(256), 0, stream, t, 0); });
     timer.time("k_init_keys", stream, [&] { hipLaunchKernelGGL(k_init_keys, dim3(n), dim3(256), 0, stream, t); });
     timer.time("k_seed", stream, [&] { hipLaunchKernelGGL(k_seed, dim3(n), dim3(256), 0, stream, t); });
+    // The first round's member table (later rounds' are built by k_commit).
+    const Grid gt_clear = build_grid(tmp, ids, [&](const SearchState& x) { return up(x.d.table_mask + 1, 256); });
+    const Grid gt_index = build_grid(tmp, ids, [&](const SearchState& x) { return up(x.P, 128); });
+    timer.time("k_clear_init", stream, [&] { hipLaunchKernelGGL(k_clear, dim3(gt_clear.blocks), dim3(256), 0, stream, t, gt_clear.map); });
+    timer.time("k_index_init", stream, [&] { hipLaunchKernelGGL(k_index, dim3(gt_index.blocks), dim3(128), 0, stream, t, gt_index.map); });
     HIP_OK(hipStreamSynchronize(stream));
     timer.collect();
     for (int id : ids) {
@@ -2380,12 +2382,7 @@ struct Batch::Impl {
               [&](hipStream_t s) {
                 launch_on(s, "k_vary_walks", g_vary_staged_w, k_vary<true, true>, 64 * kVaryWaves, vary_lds_bytes);
                 launch_on(s, "k_vary_global_walks", g_vary_global_w, k_vary<false, true>, 64 * kVaryWaves, 0);
-              },
-              [&](hipStream_t s) {
-                launch_on(s, "k_clear", g_table, k_clear, 256, 0);
-                launch_on(s, "k_index", g_member, k_index, 128, 0);
               }});
-    launch("k_reuse", g_child_w, k_reuse, 64 * kVaryWaves, 0);
   }
   void enqueue_eval() {
     // The three evaluation kernels score disjoint pending programs.
@@ -2404,10 +2401,19 @@ struct Batch::Impl {
   void enqueue_back() {
     const int n = int(live.size());
     timer.time("k_tally", stream, [&] { hipLaunchKernelGGL(k_tally, dim3(kTallyBlocks, n), dim3(256), 0, stream, d_all, 1); });
-    if (place_lds_bytes)
-      timer.time("k_place", stream, [&] { hipLaunchKernelGGL(k_place<true>, dim3(n), dim3(kPlaceThreads), place_lds_bytes, stream, d_all); });
-    if (place_hbm_bytes)
-      timer.time("k_place_hbm", stream, [&] { hipLaunchKernelGGL(k_place<false>, dim3(n), dim3(kPlaceThreads), place_hbm_bytes, stream, d_all); });
+    // Placement and the clearing of the member table k_commit refills are
+    // independent.
+    branches({[&](hipStream_t s) {
+                if (place_lds_bytes)
+                  timer.time("k_place", s, [&] {
+                    hipLaunchKernelGGL(k_place<true>, dim3(n), dim3(kPlaceThreads), place_lds_bytes, s, d_all);
+                  });
+                if (place_hbm_bytes)
+                  timer.time("k_place_hbm", s, [&] {
+                    hipLaunchKernelGGL(k_place<false>, dim3(n), dim3(kPlaceThreads), place_hbm_bytes, s, d_all);
+                  });
+              },
+              [&](hipStream_t s) { launch_on(s, "k_clear", g_table, k_clear, 256, 0); }});
     if (any_generational) launch("k_gen_place", g_member, k_gen_place, 128, 0);
     launch("k_commit", g_member_w, k_commit, 64 * kVaryWaves, 0);
     if (any_generational) timer.time("k_gen_end", stream, [&] { hipLaunchKernelGGL(k_gen_end, dim3(n), dim3(64), 0, stream, d_all); });
@@ -2734,6 +2740,7 @@ void Owner::reproduce(const double* selection, int pressure, double p_mut, std::
   I.enqueue_front();
   I.enqueue_eval();
   I.launch("k_gen_place", I.g_member, k_gen_place, 128, 0);
+  I.launch("k_clear", I.g_table, k_clear, 256, 0);
   I.launch("k_commit", I.g_member_w, k_commit, 64 * kVaryWaves, 0);
   const int zero[3] = {0, 0, 0};
   HIP_OK(hipMemcpyAsync((void*)s.d.n_pending, zero, sizeof(zero), hipMemcpyHostToDevice, I.stream));
@@ -2813,6 +2820,7 @@ void Owner::commit_batch(const int* pairs, int n) {
   }
   const int zero[3] = {0, 0, 0};
   HIP_OK(hipMemcpyAsync((void*)s.d.placed, placed.data(), sizeof(int) * placed.size(), hipMemcpyHostToDevice, I.stream));
+  I.launch("k_clear", I.g_table, k_clear, 256, 0);
   I.launch("k_commit", I.g_member_w, k_commit, 64 * kVaryWaves, 0);
   HIP_OK(hipMemcpyAsync((void*)s.d.n_pending, zero, sizeof(zero), hipMemcpyHostToDevice, I.stream));
   HIP_OK(hipStreamSynchronize(I.stream));
