@@ -1327,6 +1327,9 @@ __global__ __launch_bounds__(kPlaceThreads) void k_place(DSearch* all) {
     *d.n_left = left;
     *d.round_seed = next_seed;
     *d.best = best;
+    d.n_pending[0] = 0;
+    d.n_pending[1] = 0;
+    d.n_pending[2] = 0;
     atomicAdd(d.round, 1);
   }
 }
@@ -1429,11 +1432,6 @@ __global__ __launch_bounds__(64 * kVaryWaves) void k_commit(DSearch* all, const 
     for (int i = wlane(); i < src.nc; i += 64) cval[i] = src.cval[i];
   }
   if (wlane() == 0) {
-    if (m == 0) {  // the round's pending lists are consumed (tallies included)
-      d.n_pending[0] = 0;
-      d.n_pending[1] = 0;
-      d.n_pending[2] = 0;
-    }
     const std::uint64_t h = c >= 0 ? d.child_hash[c] : d.member_hash[m];
     if (c >= 0) {
       d.cur_n[m] = d.x_n[c];
@@ -2372,11 +2370,11 @@ struct Batch::Impl {
     // history.
     if (any_generational && !seam)
       timer.time("k_gen_begin", stream, [&] { hipLaunchKernelGGL(k_gen_begin, dim3(n), dim3(256), 0, stream, d_all); });
+    if (!seam) timer.time("k_record", stream, [&] { hipLaunchKernelGGL(k_record, dim3(n), dim3(256), 0, stream, d_all); });
     // Three independent branches: the reproduction of searches without and
-    // with traversal walks (disjoint searches), and the history row, which
-    // only reads the members. An early stop the row raises idles the rest of
-    // the round: the kernels after the join skip a stopped search, and its
-    // children are never placed.
+    // with traversal walks (disjoint searches) and the genotype index of the
+    // current members (read by nothing reproduction writes); the pending
+    // lists need all three.
     branches({[&](hipStream_t s) {
                 launch_on(s, "k_vary", g_vary_staged, k_vary<true, false>, 64 * kVaryWaves, vary_lds_bytes);
                 launch_on(s, "k_vary_global", g_vary_global, k_vary<false, false>, 64 * kVaryWaves, 0);
@@ -2384,9 +2382,6 @@ struct Batch::Impl {
               [&](hipStream_t s) {
                 launch_on(s, "k_vary_walks", g_vary_staged_w, k_vary<true, true>, 64 * kVaryWaves, vary_lds_bytes);
                 launch_on(s, "k_vary_global_walks", g_vary_global_w, k_vary<false, true>, 64 * kVaryWaves, 0);
-              },
-              [&](hipStream_t s) {
-                if (!seam) timer.time("k_record", s, [&] { hipLaunchKernelGGL(k_record, dim3(n), dim3(256), 0, s, d_all); });
               }});
   }
   void enqueue_eval() {
@@ -2405,8 +2400,8 @@ struct Batch::Impl {
   }
   void enqueue_back() {
     const int n = int(live.size());
-    // Placement, the clearing of the member table k_commit refills, and the
-    // round's tallies (which read the pending counts k_commit resets) are
+    timer.time("k_tally", stream, [&] { hipLaunchKernelGGL(k_tally, dim3(kTallyBlocks, n), dim3(256), 0, stream, d_all, 1); });
+    // Placement and the clearing of the member table k_commit refills are
     // independent.
     branches({[&](hipStream_t s) {
                 if (place_lds_bytes)
@@ -2418,10 +2413,7 @@ struct Batch::Impl {
                     hipLaunchKernelGGL(k_place<false>, dim3(n), dim3(kPlaceThreads), place_hbm_bytes, s, d_all);
                   });
               },
-              [&](hipStream_t s) { launch_on(s, "k_clear", g_table, k_clear, 256, 0); },
-              [&](hipStream_t s) {
-                timer.time("k_tally", s, [&] { hipLaunchKernelGGL(k_tally, dim3(kTallyBlocks, n), dim3(256), 0, s, d_all, 1); });
-              }});
+              [&](hipStream_t s) { launch_on(s, "k_clear", g_table, k_clear, 256, 0); }});
     if (any_generational) launch("k_gen_place", g_member, k_gen_place, 128, 0);
     launch("k_commit", g_member_w, k_commit, 64 * kVaryWaves, 0);
     if (any_generational) timer.time("k_gen_end", stream, [&] { hipLaunchKernelGGL(k_gen_end, dim3(n), dim3(64), 0, stream, d_all); });
