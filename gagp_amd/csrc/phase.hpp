@@ -103,6 +103,49 @@ struct LocalTasks {
   GHD void set(int k, const GenTask& x) { t[k] = x; }
 };
 
+// The production a reference expands to: among the nonterminal's
+// productions whose minimum cost at (depth d, logical depth l) fits
+// `available`, weights divided by the largest eligible weight unless the
+// table is normalized statically, one unit draw scaled by their sum and
+// scanned in production order. Returns the chosen expression, or one of the
+// codes below.
+constexpr int kChooseNone = -1;      // no production selected
+constexpr int kChooseInvalid = -2;   // an eligible production names no expression
+constexpr int kChooseCapacity = -3;  // no eligible positive weight
+GHD inline int choose_production_in_order(const PhaseView& g, const PNt& nt, int d, int l, unsigned available,
+                                          SplitMix& random) {
+  double maximum = nt.normalized ? 1 : 0, total = 0;
+  if (!nt.normalized)
+    for (int i = 0; i < nt.production_count; ++i) {
+      const PProd p = g.prod[nt.production_offset + i];
+      if (g_minimum(g, p.expression, d, l) <= available && p.weight > maximum) maximum = p.weight;
+    }
+  if (!(maximum > 0)) return kChooseCapacity;
+  for (int i = 0; i < nt.production_count; ++i) {
+    const PProd p = g.prod[nt.production_offset + i];
+    if (g_minimum(g, p.expression, d, l) <= available) total += nt.normalized ? p.weight : p.weight / maximum;
+  }
+  double choice = random.unit() * total;
+  int chosen = kChooseNone;
+  for (int i = 0; i < nt.production_count; ++i) {
+    const PProd p = g.prod[nt.production_offset + i];
+    if (g_minimum(g, p.expression, d, l) > available) continue;
+    if (p.expression < 0 || p.expression >= g.n_expr) return kChooseInvalid;
+    chosen = p.expression;
+    choice -= nt.normalized ? p.weight : p.weight / maximum;
+    if (choice < 0) break;
+  }
+  return chosen;
+}
+// A derivation's pending-stack type selects how it chooses productions: one
+// thread scans them in order; a wavefront's stack (WaveTasks) has its own
+// overload with one production per lane and the same result.
+template <class Tasks>
+GHD inline int choose_production(const Tasks&, const PhaseView& g, const PNt& nt, int d, int l, unsigned available,
+                                 SplitMix& random) {
+  return choose_production_in_order(g, nt, d, l, available, random);
+}
+
 // Top-down derivation with a minimum-cost reservation and a pending stack.
 // Writes origins (and sites when `sites` is non-null) and constants.
 template <class Tasks>
@@ -142,27 +185,10 @@ GHD inline GenResult generate_phase_with(const PhaseView& g, int root_expression
       GenTask next{unselected, task.d, x.scope_count, task.l - (g.site_depth ? 1 : 0)};
       if (reserved > unsigned(capacity - r.nodes)) { r.status = ST_CAPACITY; return r; }
       const unsigned available = unsigned(capacity - r.nodes) - reserved;
-      double maximum = nt.normalized ? 1 : 0, total = 0;
-      if (!nt.normalized)
-        for (int i = 0; i < nt.production_count; ++i) {
-          const PProd p = g.prod[nt.production_offset + i];
-          if (g_minimum(g, p.expression, task.d, next.l) <= available && p.weight > maximum) maximum = p.weight;
-        }
-      if (!(maximum > 0)) { r.status = ST_CAPACITY; return r; }
-      for (int i = 0; i < nt.production_count; ++i) {
-        const PProd p = g.prod[nt.production_offset + i];
-        if (g_minimum(g, p.expression, task.d, next.l) <= available) total += nt.normalized ? p.weight : p.weight / maximum;
-      }
-      double choice = random.unit() * total;
-      for (int i = 0; i < nt.production_count; ++i) {
-        const PProd p = g.prod[nt.production_offset + i];
-        if (g_minimum(g, p.expression, task.d, next.l) > available) continue;
-        if (p.expression < 0 || p.expression >= g.n_expr) return r;
-        next.e = p.expression;
-        choice -= nt.normalized ? p.weight : p.weight / maximum;
-        if (choice < 0) break;
-      }
-      if (next.e == unselected || count >= kPending) return r;
+      const int chosen = choose_production(pending, g, nt, task.d, next.l, available, random);
+      if (chosen == kChooseCapacity) { r.status = ST_CAPACITY; return r; }
+      if (chosen < 0 || count >= kPending) return r;
+      next.e = chosen;
       reserved += g_minimum(g, next.e, next.d, next.l);
       pending.set(count++, next);
       continue;

@@ -107,6 +107,50 @@ struct WaveTasks {
   }
 };
 // Open nodes of the definite-assignment check: frame k in lane k.
+__device__ inline double rdlane_d(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = unsigned(rdlane(int(b), lane)), hi = unsigned(rdlane(int(b >> 32), lane));
+  return __longlong_as_double((long long)(std::uint64_t(hi) << 32 | lo));
+}
+// choose_production for a wavefront's derivation: lane i reads production i
+// and its minimum cost in one step, the largest eligible weight is a
+// max-reduction (exact in any order), each lane divides its own weight, and
+// the sum and the draw's scan add the eligible weights in production order
+// from registers, so every rounding is the in-order loop's.
+__device__ inline int choose_production(const WaveTasks&, const PhaseView& g, const PNt& nt, int d, int l, unsigned available,
+                                        SplitMix& random) {
+  if (nt.production_count > 64) return choose_production_in_order(g, nt, d, l, available, random);
+  const int lane = wlane();
+  const bool in = lane < nt.production_count;
+  const PProd p = in ? g.prod[nt.production_offset + lane] : PProd{-1, 0.0};
+  const bool eligible = in && g_minimum(g, p.expression, d, l) <= available;
+  double maximum = 1;
+  if (!nt.normalized) {
+    double m = eligible && p.weight > 0 ? p.weight : 0;
+    for (int off = 32; off > 0; off >>= 1) {
+      const double o = __shfl_xor(m, off, 64);
+      m = o > m ? o : m;
+    }
+    maximum = rdlane_d(m, 0);
+  }
+  if (!(maximum > 0)) return kChooseCapacity;
+  const double w = nt.normalized ? p.weight : p.weight / maximum;
+  const std::uint64_t mask = __ballot(eligible);
+  double total = 0;
+  for (std::uint64_t m = mask; m; m &= m - 1) total += rdlane_d(w, wave_first(m));
+  double choice = random.unit() * total;
+  int chosen = kChooseNone;
+  for (std::uint64_t m = mask; m; m &= m - 1) {
+    const int i = wave_first(m);
+    const int e = rdlane(p.expression, i);
+    if (e < 0 || e >= g.n_expr) return kChooseInvalid;
+    chosen = e;
+    choice -= rdlane_d(w, i);
+    if (choice < 0) break;
+  }
+  return chosen;
+}
+
 struct WaveDefFrames {
   int origin = 0, seen = 0, saved = 0, branch = 0;
   __device__ DefFrame get(int k) const {
