@@ -92,18 +92,27 @@ __device__ inline PhaseView stage_phase_tables(const PhaseTables& t, unsigned ch
 // registers rather than arrays, so no access can fall back to memory.
 struct WaveTasks {
   static_assert(kPending <= 192, "three groups of 64 lanes");
-  int e0 = 0, e1 = 0, e2 = 0, d0 = 0, d1 = 0, d2 = 0, s0 = 0, s1 = 0, s2 = 0, l0 = 0, l1 = 0, l2 = 0;
+  // Task k is lane k mod 64 of group k / 64, packed in two registers:
+  // expression | scope count << 17, and depth | logical depth << 16 (the
+  // derivation keeps expressions below 2^17, scope counts and depths small
+  // and non-negative). Groups are chosen by value selects only: a branch per
+  // group lets the optimizer address the registers as an array in scratch
+  // memory, whose loads then wait for the derivation's row stores.
+  unsigned a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0;
   __device__ GenTask get(int k) const {
-    const int ln = k & 63;
-    if (k < 64) return GenTask{rdlane(e0, ln), rdlane(d0, ln), rdlane(s0, ln), rdlane(l0, ln)};
-    if (k < 128) return GenTask{rdlane(e1, ln), rdlane(d1, ln), rdlane(s1, ln), rdlane(l1, ln)};
-    return GenTask{rdlane(e2, ln), rdlane(d2, ln), rdlane(s2, ln), rdlane(l2, ln)};
+    const int ln = k & 63, grp = k >> 6;
+    const unsigned ra0 = unsigned(rdlane(int(a0), ln)), ra1 = unsigned(rdlane(int(a1), ln)), ra2 = unsigned(rdlane(int(a2), ln));
+    const unsigned rb0 = unsigned(rdlane(int(b0), ln)), rb1 = unsigned(rdlane(int(b1), ln)), rb2 = unsigned(rdlane(int(b2), ln));
+    const unsigned a = grp == 0 ? ra0 : grp == 1 ? ra1 : ra2, b = grp == 0 ? rb0 : grp == 1 ? rb1 : rb2;
+    return GenTask{int(a & 0x1ffffu), int(b & 0xffffu), int(a >> 17), int(b >> 16)};
   }
   __device__ void set(int k, const GenTask& x) {
-    const int ln = k & 63;
-    if (k < 64) { e0 = wrlane(x.e, ln, e0); d0 = wrlane(x.d, ln, d0); s0 = wrlane(x.sc, ln, s0); l0 = wrlane(x.l, ln, l0); }
-    else if (k < 128) { e1 = wrlane(x.e, ln, e1); d1 = wrlane(x.d, ln, d1); s1 = wrlane(x.sc, ln, s1); l1 = wrlane(x.l, ln, l1); }
-    else { e2 = wrlane(x.e, ln, e2); d2 = wrlane(x.d, ln, d2); s2 = wrlane(x.sc, ln, s2); l2 = wrlane(x.l, ln, l2); }
+    const unsigned a = unsigned(x.e) | unsigned(x.sc) << 17, b = unsigned(x.d) | unsigned(x.l) << 16;
+    const int lane = wlane();
+    const bool h0 = lane == k, h1 = lane + 64 == k, h2 = lane + 128 == k;
+    a0 = h0 ? a : a0; b0 = h0 ? b : b0;
+    a1 = h1 ? a : a1; b1 = h1 ? b : b1;
+    a2 = h2 ? a : a2; b2 = h2 ? b : b2;
   }
 };
 // Open nodes of the definite-assignment check: frame k in lane k.
