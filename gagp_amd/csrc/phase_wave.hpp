@@ -115,7 +115,6 @@ struct WaveTasks {
     a2 = h2 ? a : a2; b2 = h2 ? b : b2;
   }
 };
-// Open nodes of the definite-assignment check: frame k in lane k.
 __device__ inline double rdlane_d(double v, int lane) {
   const long long b = __double_as_longlong(v);
   const unsigned lo = unsigned(rdlane(int(b), lane)), hi = unsigned(rdlane(int(b >> 32), lane));
@@ -160,6 +159,7 @@ __device__ inline int choose_production(const WaveTasks&, const PhaseView& g, co
   return chosen;
 }
 
+// Open nodes of the definite-assignment check: frame k in lane k.
 struct WaveDefFrames {
   int origin = 0, seen = 0, saved = 0, branch = 0;
   __device__ DefFrame get(int k) const {
