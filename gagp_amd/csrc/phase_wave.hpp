@@ -290,8 +290,43 @@ struct WaveWalk {
 };
 
 // node_depth (phase_variation.cuh:93-110).
+// The same walk over the nodes [0, n) of one tile (n <= 64) at once: lane i
+// holds node i's arity `a` and site flag `f`. q_i = 1 + sum_{k<i} (a_k - 1)
+// is the number of open child slots before node i, and node j < i is an
+// ancestor of i exactly when min_{j<k<=i} q_k >= q_j (i lies in j's subtree,
+// which ends where the slot count first drops below q_j). Each lane i <= n
+// gets its depth (1 + ancestors: WaveWalk's depth before visiting i) and its
+// incoming site depth (the site flags of its ancestors); `ok` is false where
+// the walk would fail on one of the nodes [0, n): no open slot, an arity
+// outside [0, kMaxArity], or children below depth 64. `open` is q_n.
+struct TileWalk { int depth, inc, open; bool ok; };
+__device__ inline TileWalk walk_tile_w(int a, int f, int n) {
+  const int lane = wlane();
+  const int delta = lane < n ? a - 1 : 0;
+  const int q = 1 + wave_incl_scan(delta) - delta;
+  int depth = 1, inc = 0, m = q;
+  for (int j = (n < 63 ? n : 63) - 1; j >= 0; --j) {  // lanes 0..n, ancestors j < lane
+    const int qj = rdlane(q, j), qn = rdlane(q, j + 1), fj = rdlane(f, j);
+    if (j < lane) {
+      m = qn < m ? qn : m;
+      if (m >= qj) { ++depth; inc += fj; }
+    }
+  }
+  TileWalk w;
+  w.depth = depth;
+  w.inc = inc;
+  w.open = rdlane(q, n < 64 ? n : 63) + (n < 64 ? 0 : rdlane(delta, 63));
+  w.ok = __ballot(lane < n && (q < 1 || a < 0 || a > kMaxArity || (a > 0 && depth >= 64))) == 0;
+  return w;
+}
+
 __device__ inline int node_depth_w(const PhaseView& g, const CRow& t, int position) {
   if (position < 0 || position >= t.n || t.n > kPhaseCapacity) return -1;
+  if (position < 64) {
+    const int i = wlane();
+    const TileWalk w = walk_tile_w(i <= position ? arity_of(g, t.origin[i]) : 0, 0, position);
+    return w.ok && w.open >= 1 ? rdlane(w.depth, position) : -1;
+  }
   WaveWalk w;
   for (int tile = 0; tile <= position; tile += 64) {
     const int i = tile + wlane();
@@ -311,6 +346,12 @@ __device__ inline int node_depth_w(const PhaseView& g, const CRow& t, int positi
 // site_depth_of with a position: the incoming grammar-invocation depth there.
 __device__ inline int site_depth_at_w(const PhaseView& g, const CRow& t, int position) {
   if (t.n <= 0 || t.n > kPhaseCapacity || position < 0 || position >= t.n) return -1;
+  if (position < 64) {
+    const int i = wlane();
+    const bool in = i <= position;
+    const TileWalk w = walk_tile_w(in ? arity_of(g, t.origin[i]) : 0, in && t.site[i] >= 0 ? 1 : 0, position);
+    return w.ok && w.open >= 1 ? rdlane(w.inc, position) : -1;
+  }
   WaveWalk w;
   for (int tile = 0; tile <= position; tile += 64) {
     const int i = tile + wlane();
@@ -349,6 +390,10 @@ __device__ inline GenResult splice_w(const PhaseView& g, const CRow& base, int d
   int head_run = 0, in_run = 0, tail_run = 0;
   WaveWalk w;
   int maximum = 1, site_max = 0;
+  // One tile: the walk runs on all lanes at once (walk_tile_w).
+  const bool one_tile = length <= 64;
+  bool deep = false, site_deep = false;
+  int open = 0;
   for (int tile = 0; tile < length; tile += 64) {
     const int i = tile + wlane();
     const bool live = i < length;
@@ -374,6 +419,15 @@ __device__ inline GenResult splice_w(const PhaseView& g, const CRow& base, int d
     head_run += __popcll(m0);
     in_run += __popcll(m1);
     tail_run += __popcll(m2);
+    if (one_tile) {
+      const int f = st >= 0 ? 1 : 0;
+      const TileWalk tw = walk_tile_w(ar, f, length);
+      if (!tw.ok) return r;
+      open = tw.open;
+      deep = __ballot(live && tw.depth > max_depth) != 0;
+      site_deep = site_limit && __ballot(live && tw.inc + f > site_limit) != 0;
+      break;
+    }
     const int lanes = length - tile < 64 ? length - tile : 64;
     for (int j = 0; j < lanes; ++j) {
       const int aj = rdlane(ar, j);
@@ -386,9 +440,9 @@ __device__ inline GenResult splice_w(const PhaseView& g, const CRow& base, int d
     }
   }
   out.site[destination] = a;
-  if (w.pop_finished()) return r;
-  if (maximum > max_depth) { r.status = ST_CAPACITY; return r; }
-  if (site_limit && site_max > site_limit) { r.status = ST_CAPACITY; return r; }
+  if (one_tile ? open >= 1 : w.pop_finished()) return r;
+  if (one_tile ? deep : maximum > max_depth) { r.status = ST_CAPACITY; return r; }
+  if (one_tile ? site_deep : site_limit && site_max > site_limit) { r.status = ST_CAPACITY; return r; }
   r.nodes = length;
   r.constants = head_run + in_run + tail_run;
   r.status = ST_OK;
