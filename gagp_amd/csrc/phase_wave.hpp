@@ -1,12 +1,15 @@
 // Wavefront forms of the per-child operators: one wavefront (64 lanes) works
 // on one child. Every argument and result is uniform across the wavefront.
 // Row passes (copies, site counts, subtree ends, constant ranges,
-// comparisons, hashes) run lane-parallel over 64-node tiles; the prefix-order
-// tree walks keep their pending-children stack one entry per lane (entry k in
-// lane k, read with readlane and written with a lane select), so a walk step is a few
-// VALU/SALU instructions with no memory traffic. Derivation and lowering run
-// the shared bodies of phase.hpp uniformly with their stacks in lane
-// registers.
+// comparisons, hashes) run lane-parallel over 64-node tiles. The prefix-order
+// tree walks (depths, site depths, the splice's bounds) take all the nodes of
+// a row of at most 64 at once from open-slot prefix sums (walk_tile_w);
+// longer rows keep a pending-children stack one entry per lane (entry k in
+// lane k, read with readlane and written with a lane select), so a walk step
+// is a few VALU/SALU instructions with no memory traffic. Derivation and
+// lowering run the shared bodies of phase.hpp uniformly with their stacks in
+// lane registers; a derivation chooses each production with one lane per
+// production.
 //
 // Every function returns exactly what its one-thread counterpart in phase.hpp
 // returns and consumes the same random draws in the same order; those
