@@ -475,14 +475,57 @@ __device__ inline bool locals_defined_w(const PhaseView& g, const CRow& t) {
   return locals_defined_with(g, t.n, origin_at, frames);
 }
 
+// The parent and donor tournaments of crossover_child on all lanes at once.
+// SplitMix64 is counter-based: draw k of the stream is the mix of the state
+// plus (k + 1) increments, so lane k computes draw k directly and loads its
+// candidate's keys; the selection then replays tournament()'s comparisons in
+// draw order from registers. When a draw would be rejected by bounded()
+// (below (2^64 - P) mod P), or the draws exceed the lanes, the in-order
+// tournaments run from the unchanged state instead.
+__device__ inline void tournaments_w(const Keys& k, int population, int pressure, SplitMix& random, int& parent, int& donor) {
+  const int draws = 2 * pressure;
+  const std::uint64_t n = std::uint64_t(population), threshold = (0ULL - n) % n;
+  const int lane = wlane();
+  const bool mine = lane < draws;
+  const std::uint64_t v = splitmix_mix(random.s + std::uint64_t(lane) * 0x9e3779b97f4a7c15ULL);
+  if (draws > 64 || __ballot(mine && v < threshold) != 0) {
+    parent = tournament(k, population, pressure, random);
+    donor = tournament(k, population, pressure, random);
+    return;
+  }
+  const int c = mine ? int(v % n) : 0;
+  const double f = mine ? k.fitness[c] : 0.0;
+  const int w = mine && k.has_wrong ? int(k.wrong[c]) : 0, m = mine && k.has_wrong ? int(k.nodes[c]) : 0;
+  // better() / selected_over() of draw i over the running best, in registers.
+  const auto select = [&](int first) {
+    int best = rdlane(c, first), bw = rdlane(w, first), bn = rdlane(m, first);
+    double bf = rdlane_d(f, first);
+    for (int i = first + 1; i < first + pressure; ++i) {
+      const int ci = rdlane(c, i), wi = rdlane(w, i), ni = rdlane(m, i);
+      const double fi = rdlane_d(f, i);
+      bool over;
+      if (fi != bf) over = fi > bf;
+      else if (!k.has_wrong) over = false;
+      else if (wi != bw) over = unsigned(wi) < unsigned(bw);
+      else if (ni != bn) over = unsigned(ni) < unsigned(bn);
+      else over = ci < best;
+      if (over) { best = ci; bf = fi; bw = wi; bn = ni; }
+    }
+    return best;
+  };
+  parent = select(0);
+  donor = select(pressure);
+  random.s += std::uint64_t(draws) * 0x9e3779b97f4a7c15ULL;
+}
+
 // crossover_child (phase_operators.cuh:156-199).
 template <class MemberRow>
 __device__ inline void crossover_w(const PhaseView& g, const SlotInfo& s, int child, MemberRow in, Row& out, const Keys& keys,
                                    int population, int pressure, std::uint64_t seed, const OpPolicy& pol, OpStats& st) {
   SplitMix random(seed ^ (0x617089e168ULL + std::uint64_t(child)));
   st = OpStats{-1, 0, 0, 0, 0, 0, 0, -1, -1};
-  const int parent = tournament(keys, population, pressure, random);
-  const int donor = tournament(keys, population, pressure, random);
+  int parent, donor;
+  tournaments_w(keys, population, pressure, random, parent, donor);
   st.parent = parent;
   st.donor = donor;
   const CRow a = in(parent);
