@@ -387,17 +387,20 @@ std::vector<int> derive_origins(const Grammar& g, const Ast& ast) { return Match
 PhaseRow import_row(const Grammar& g, const PhaseGrammar& phase, const Ast& ast, unsigned site_depth, int capacity) {
   PhaseRow row;
   row.origin = derive_origins(g, ast);
+  // The reference's resident layout declines these two (its native
+  // representation then runs them, phase_population_cpu.cpp:229-235), so
+  // they are declines here too, not invalid input.
   if (int(row.origin.size()) > capacity)
-    invalid("imported program has " + std::to_string(row.origin.size()) + " phase nodes, above the row capacity " +
-            std::to_string(capacity));
+    throw Unsupported("imported program has " + std::to_string(row.origin.size()) +
+                      " phase nodes, above the row capacity " + std::to_string(capacity));
   const std::size_t start = g.nts[g.entry].cat == Cat::Expression ? 3 : 0;
   for (std::size_t i = 0; i < row.origin.size(); ++i) {
     const AstNode& n = ast.nodes[start + i];
     if (n.kind == NK_CONST) row.cval.push_back(ast.consts.at(std::size_t(n.i0)));
   }
-  if (!annotate(phase, row.origin, row.site)) invalid("import site annotation failed");
-  if (site_depth && phase.site_depth && site_height(phase, row.origin, row.site) > int(site_depth))
-    invalid("import site annotation failed");
+  if (!annotate(phase, row.origin, row.site) ||
+      (site_depth && phase.site_depth && site_height(phase, row.origin, row.site) > int(site_depth)))
+    throw Unsupported("import site annotation failed");
   // The native verifier rejects a statement program that reads a local
   // before every path assigns it (undefined_local).
   if (g.nts[g.entry].cat == Cat::Program) {

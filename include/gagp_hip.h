@@ -147,9 +147,13 @@ int gagp_batch_add(gagp_batch* b, const char* task_path, const char* search_path
  * made for the task's grammar, each derived from the grammar and uploaded in
  * place of the generated population (GAGP_ERR_INVALID when a program is not
  * a member of the grammar, the grammar hash differs or the document has too
- * few programs; GAGP_ERR_UNSUPPORTED for grammar-population-v2 replays).
+ * few programs; GAGP_ERR_UNSUPPORTED for grammar-population-v2 replays and
+ * for programs the resident layout declines -- a site height above the
+ * search's site_depth or more phase nodes than a row holds -- which the
+ * reference then runs in its native representation).
  * Reference: `gagp run --init`, cpp/src/cli/run.cpp:122,143-173,305-307;
- * import_native_phase_layout, cpp/src/evolution/repro/gpu/phase_layout.hpp:92-260. */
+ * import_native_phase_layout, cpp/src/evolution/repro/gpu/phase_layout.hpp:92-260;
+ * the resident layout's import checks, phase_population_cpu.cpp:229-235. */
 int gagp_batch_add_init(gagp_batch* b, const char* task_path, const char* search_path, int32_t population,
                         int32_t generations, int32_t seed_is_set, uint64_t seed, const char* init_path,
                         int32_t* out_index);

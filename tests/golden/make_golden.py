@@ -22,6 +22,7 @@ locals.
 import argparse
 import json
 import os
+import random
 import re
 import shutil
 import subprocess
@@ -46,6 +47,10 @@ RUNS = {
     "poly_c1_p1024_g64_s0": ("poly_c1", 1024, 64, 0, True),
     "quintic_p512_g16_s3": ("quintic", 512, 16, 3, False),
     "quintic_p4096_g128_s0": ("quintic", 4096, 128, 0, False),
+    # rows up to 128 nodes (walks over more than one 64-node tile) and
+    # tournaments of 40 (more draws than a wavefront's lanes)
+    "quintic_wide_p512_g16_s5": ("quintic_wide", 512, 16, 5, True),
+    "polynomial_t40_p256_g10_s8": ("polynomial", 256, 10, 8, False, "tournament40.json"),
     # populations above 8192 keep the placement keys in HBM
     "quintic_p16384_g3_s11": ("quintic", 16384, 3, 11, False),
     "polynomial_p12000_g4_s2": ("polynomial", 12000, 4, 2, False),
@@ -98,6 +103,16 @@ INIT_RUNS = {
     "init_let_sum_p256_g8_s3": ("let-sum", "let_sum_p256_g8_s1", 256, 8, 3, True),
     "init_stmt_sum_even_p400_g8_s1": ("stmt_sum_even", "stmt_sum_even_p512_g10_s5", 400, 8, 1, True),
     "init_stmt_input_p300_g8_s2": ("stmt_input", "stmt_input_p300_g10_s3", 300, 8, 2, False),
+    # rows above one 64-node tile: programs of 70-124 expression nodes
+    # (init/quintic_wide_big.json, from wide_population())
+    "init_quintic_wide_big_p512_g8_s6": ("quintic_wide", "init/quintic_wide_big.json", 512, 8, 6, False),
+}
+# Populations the reference's resident layout declines (it then runs them in
+# its native representation, which this engine does not take): the engine
+# must decline them too, for the reason the reference records.
+# name: (task dir, population document, population, generations, seed)
+DECLINED_RUNS = {
+    "quintic_wide_deep": ("quintic_wide", "init/quintic_wide_deep.json", 512, 2, 6),
 }
 # Non-integral fitness: discounted proximity (-e/(1+e) per case), a
 # fractional case cap and the exact metric, over straight-line, general-VM,
@@ -129,6 +144,71 @@ FITNESS_VARIANTS = [
     ("stmt_count_gt_cap7p25", "stmt_count_gt", {"metric": "absolute", "case_cap": 7.25}),
     ("let_sum_disc", "let-sum", {"metric": "absolute", "proximity": "discounted"}),
 ]
+
+
+def wide_task():
+    """The quintic task with a 128-node, depth-14 budget (grammar
+    search_limits), so rows exceed one 64-node tile."""
+    src, dst = os.path.join(TASKS, "quintic"), os.path.join(TASKS, "quintic_wide")
+    os.makedirs(dst, exist_ok=True)
+    for f in os.listdir(src):  # the case files
+        shutil.copy(os.path.join(src, f), os.path.join(dst, f))
+    task = json.load(open(os.path.join(src, "task.json")))
+    task["name"] = "quintic_wide"
+    grammar = json.load(open(os.path.join(src, task["grammar"])))
+    grammar["search_limits"] = {"max_nodes": 128, "max_depth": 14}
+    json.dump(task, open(os.path.join(dst, "task.json"), "w"), indent=1)
+    json.dump(grammar, open(os.path.join(dst, task["grammar"]), "w"), indent=1)
+
+
+def wide_population():
+    """512 programs of 70-124 expression nodes for the wide task, so rows
+    span two 64-node tiles: random add/sub/mul trees over x and constants of
+    the wide run's saved programs, at most 7 levels deep (every node of this
+    grammar is a site, and the resident layout admits a site height up to the
+    search's site_depth, 7). The reference's own --init run decides whether
+    its resident layout takes them (its result records no resident_declined)."""
+    src = json.load(open(os.path.join(GOLDEN, "runs", "quintic_wide_p512_g16_s5", "population.json")))
+    progs = src["programs"]
+    values = sorted({c["value"] for p in progs for c in p["consts"]})
+    rnd = random.Random(12)
+
+    def tree(level, nodes, consts):
+        if level < 7 and rnd.random() < 0.88:
+            nodes.append({"i0": 0, "i1": 0, "kind": rnd.choice([11, 12, 13])})  # add, sub, mul
+            tree(level + 1, nodes, consts)
+            tree(level + 1, nodes, consts)
+        elif rnd.random() < 0.5:
+            nodes.append({"i0": 0, "i1": 0, "kind": 8})  # x
+        else:
+            nodes.append({"i0": len(consts), "i1": 0, "kind": 7})
+            consts.append({"type": "int", "value": rnd.choice(values)})
+
+    out = []
+    while len(out) < 512:
+        body, consts = [], []
+        tree(1, body, consts)
+        if 70 <= len(body) <= 124:
+            nodes = progs[0]["nodes"][:3] + body + progs[0]["nodes"][-1:]  # PROGRAM, BLOCK_CONS, RETURN ... BLOCK_NIL
+            out.append({"consts": consts, "names": progs[0]["names"], "nodes": nodes, "version": progs[0]["version"]})
+    doc = {"format": src["format"], "grammar_hash": src["grammar_hash"], "programs": out}
+    os.makedirs(os.path.join(GOLDEN, "init"), exist_ok=True)
+    json.dump(doc, open(os.path.join(GOLDEN, "init", "quintic_wide_big.json"), "w"), separators=(",", ":"))
+
+
+def deep_population():
+    """The wide programs one level deeper (add(program, x)): a site height
+    of 8, above the search's site_depth of 7."""
+    doc = json.load(open(os.path.join(GOLDEN, "init", "quintic_wide_big.json")))
+    fits = [p for p in doc["programs"] if len(p["nodes"]) - 4 <= 118]  # room in the 128-node budget
+    programs = []
+    for k in range(len(doc["programs"])):
+        p = json.loads(json.dumps(fits[k % len(fits)]))
+        body = p["nodes"][3:-1]
+        p["nodes"] = p["nodes"][:3] + [{"i0": 0, "i1": 0, "kind": 11}] + body + [{"i0": 0, "i1": 0, "kind": 8}] + p["nodes"][-1:]
+        programs.append(p)
+    doc["programs"] = programs
+    json.dump(doc, open(os.path.join(GOLDEN, "init", "quintic_wide_deep.json"), "w"), separators=(",", ":"))
 
 
 def fitness_tasks():
@@ -294,6 +374,7 @@ def main():
         sys.exit("build the reference first: make -f oracle/ref.mk -j8")
     workloads.poly_c1(os.path.join(TASKS, "poly_c1"))
     workloads.quintic(os.path.join(TASKS, "quintic"))
+    wide_task()
     list_tasks()
     fitness_tasks()
     mixed = workloads.mixed_searches(os.path.join(TASKS, "mixed"), n=8, seed=4, p_range=(64, 512), c_range=(16, 256))
@@ -327,7 +408,12 @@ def main():
             continue
         task_dir, source, population, generations, seed, save_all = spec[:6]
         search = os.path.join(GOLDEN, "search", spec[6]) if len(spec) > 6 else None
-        init = os.path.join(GOLDEN, "runs", source, "population.json")
+        if source.endswith(".json"):  # a population document of its own
+            if source == "init/quintic_wide_big.json":
+                wide_population()
+            init = os.path.join(GOLDEN, source)
+        else:
+            init = os.path.join(GOLDEN, "runs", source, "population.json")
         out = os.path.join(GOLDEN, "runs", name)
         os.makedirs(out, exist_ok=True)
         task = os.path.join(TASKS, task_dir, "task.json")
@@ -339,6 +425,22 @@ def main():
         print("golden", name)
     with open(manifest_path, "w") as f:
         json.dump(manifest, f, indent=1, sort_keys=True)
+    declined_path = os.path.join(GOLDEN, "declined.json")
+    declined = json.load(open(declined_path)) if os.path.exists(declined_path) else {}
+    for name, (task_dir, source, population, generations, seed) in DECLINED_RUNS.items():
+        if args.only and not re.search(args.only, name):
+            continue
+        if source == "init/quintic_wide_deep.json":
+            deep_population()
+        task = os.path.join(TASKS, task_dir, "task.json")
+        with tempfile.TemporaryDirectory() as out:
+            run_reference(task, population, generations, seed, out, False, None, os.path.join(GOLDEN, source))
+            result = json.load(open(os.path.join(out, "result.json")))
+        declined[name] = {"task": os.path.relpath(task, GOLDEN), "init": source, "population": population,
+                          "resident_declined": result.get("resident_declined")}
+        print("declined", name, result.get("resident_declined"))
+    with open(declined_path, "w") as f:
+        json.dump(declined, f, indent=1, sort_keys=True)
     if not args.only or re.search(args.only, "traversal"):
         traversal_goldens()
     if not args.only or re.search(args.only, "definite"):

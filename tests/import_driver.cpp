@@ -123,7 +123,8 @@ int main(int argc, char** argv) {
                   ints(row.origin).c_str(), ints(row.site).c_str(), same ? "true" : "false");
     }
   } catch (const std::exception& e) {
-    std::printf("{\"error\":%s}\n", quoted(e.what()).c_str());
+    std::printf("{\"error\":%s,\"unsupported\":%s}\n", quoted(e.what()).c_str(),
+                dynamic_cast<const Unsupported*>(&e) ? "true" : "false");
     return 2;
   }
   return 0;

@@ -124,3 +124,18 @@ def test_cli_rejects_unsupported_with_exit_2(tmp_path):
     r = subprocess.run([gagp_amd.CLI_PATH, "run", os.path.join(GOLDEN, "tasks", "fuel-profile", "task.json"), "--quiet"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2 and "gagp:" in r.stderr
+
+
+def test_init_that_the_resident_layout_declines_is_unsupported():
+    # The reference's resident layout declines this population (a site height
+    # above site_depth) and runs it in its native representation; through the
+    # C ABI the engine declines it as GAGP_ERR_UNSUPPORTED, for that reason.
+    declined = json.load(open(os.path.join(GOLDEN, "declined.json")))["quintic_wide_deep"]
+    print("declined init", declined["init"], flush=True)
+    b = gagp_amd.Batch()
+    with pytest.raises(gagp_amd.GagpError) as e:
+        b.add(os.path.join(GOLDEN, declined["task"]), None, declined["population"], 2, 6,
+              os.path.join(GOLDEN, declined["init"]))
+        b.prepare()
+    assert e.value.status == 2 and "import site annotation failed" in str(e.value)
+    assert "import site annotation failed" in declined["resident_declined"]

@@ -18,6 +18,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BUILD = os.path.join(ROOT, "gagp_amd", "_build")
 MANIFEST = json.load(open(os.path.join(GOLDEN, "runs", "manifest.json")))
 SAVED = sorted(n for n in MANIFEST if os.path.exists(os.path.join(GOLDEN, "runs", n, "population.json")))
+# Populations the reference's resident layout declines, with its recorded reason
+DECLINED = json.load(open(os.path.join(GOLDEN, "declined.json")))
 SITE_DEPTH = 7  # the default search's limits.site_depth
 
 
@@ -162,3 +164,17 @@ def test_import_rejects_undefined_local_reads(driver, lowering_driver, task, tmp
             assert row["error"] is None, (want["text"], row["error"])
         else:
             assert row["error"] and "undefined_local" in row["error"], (want["text"], row["error"])
+
+
+@pytest.mark.parametrize("name", sorted(DECLINED))
+def test_import_declines_what_the_reference_resident_layout_declines(driver, name):
+    # Programs of a site height above the search's site_depth: the
+    # reference's resident layout declines them (and runs them in its native
+    # representation); the engine declines them as unsupported, for the same
+    # reason, rather than as invalid input.
+    d = DECLINED[name]
+    assert d["resident_declined"], name
+    rc, rows = run(driver, os.path.join(GOLDEN, d["task"]), os.path.join(GOLDEN, d["init"]), d["population"], SITE_DEPTH,
+                   "load")
+    assert rc == 2 and rows[0]["unsupported"] is True, rows
+    assert rows[0]["error"] in d["resident_declined"], (rows[0]["error"], d["resident_declined"])
