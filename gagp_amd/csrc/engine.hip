@@ -1335,7 +1335,6 @@ __global__ __launch_bounds__(kPlaceThreads) void k_place(DSearch* all) {
 }
 
 // ---- generational schedule (evolve.cpp:253-322; phase_population_cpu.cpp:269-312) ----
-constexpr int kMaxElites = 32;
 // Elites (the top `elites` members by the ranking order, best first; the
 // best alone when elitism is 0) and the generation's seed, one generator
 // word drawn after the history row as in the reference's loop.
@@ -1681,8 +1680,6 @@ struct SearchState {
     if (s.generations) cfg.generations = *s.generations;
     if (s.seed) cfg.seed = *s.seed;
     if (!cfg.resident) throw Unsupported("native_ast representation (host reproduction)");
-    if (cfg.schedule == Schedule::Generational && cfg.elitism > kMaxElites)
-      throw Unsupported("generational elitism above " + std::to_string(kMaxElites));
     if (cfg.site_depth < 1 && s.init_path.empty())
       throw Unsupported("site_depth 0 (initial population from the native generator)");
     if (cfg.population < 2 || cfg.population > 16384) throw Unsupported("population outside 2..16384");
@@ -1819,7 +1816,7 @@ struct SearchState {
     x.counters = a.alloc<Counters>(1);
     x.generational = cfg.schedule == Schedule::Generational;
     x.elites = x.generational ? cfg.elitism : 0;
-    x.elite = a.alloc<int>(kMaxElites);
+    x.elite = a.alloc<int>(x.generational ? std::size_t(P) : 1);  // any elite count below P
     x.early_exit = spec.early_exit ? 1 : 0;
     x.stop = a.alloc<int>(1);
     x.e_errors = a.alloc<unsigned>(code_rows); x.e_timeouts = a.alloc<unsigned>(code_rows);
@@ -2721,7 +2718,6 @@ void Owner::reproduce(const double* selection, int pressure, double p_mut, std::
   if (pressure < 1 || pressure > s.P) throw std::invalid_argument("tournament pressure outside 1..P");
   if (!(p_mut >= 0 && p_mut <= 1)) throw std::invalid_argument("mutation probability outside [0, 1]");
   if (elites < 0 || elites >= s.P) throw std::invalid_argument("elite count outside 0..P-1");
-  if (elites > kMaxElites) throw Unsupported("generational elitism above " + std::to_string(kMaxElites));
   for (int m = 0; m < s.P; ++m)
     if (std::isnan(selection[m])) throw std::invalid_argument("selection values must not be NaN");
   if (pressure != s.d.pressure || p_mut != s.d.p_mut || elites != s.d.elites) {

@@ -51,6 +51,8 @@ RUNS = {
     # tournaments of 40 (more draws than a wavefront's lanes)
     "quintic_wide_p512_g16_s5": ("quintic_wide", 512, 16, 5, True),
     "polynomial_t40_p256_g10_s8": ("polynomial", 256, 10, 8, False, "tournament40.json"),
+    # generational elitism of 40 elites
+    "polynomial_gen_e40_p256_g10_s7": ("polynomial", 256, 10, 7, False, "generational_e40.json"),
     # populations above 8192 keep the placement keys in HBM
     "quintic_p16384_g3_s11": ("quintic", 16384, 3, 11, False),
     "polynomial_p12000_g4_s2": ("polynomial", 12000, 4, 2, False),

@@ -138,7 +138,8 @@ def drive(name):
 
 
 GENERATIONAL = ["polynomial_gen_e1_p256_g12_s3", "poly_c1_gen_e3r_p512_g24_s9", "list_count_gt_rev_gen_e1_p256_g8_s2",
-                "stmt_count_gt_gen_e1_p256_g8_s2", "fit_poly_c1_disc_gen_e3r_p256_g10_s6", "init_poly_c1_gen_e3r_p512_g12_s4"]
+                "stmt_count_gt_gen_e1_p256_g8_s2", "fit_poly_c1_disc_gen_e3r_p256_g10_s6", "init_poly_c1_gen_e3r_p512_g12_s4",
+                "polynomial_gen_e40_p256_g10_s7"]
 
 
 def mean(fitness):
